@@ -1,0 +1,198 @@
+#!/usr/bin/env python3
+"""Throughput of the 2D EM-PIC step on one MI355X.
+
+Workload: the two-stream configuration at scale (2048x2048 periodic cells,
+dx = 0.5 d_e, dt = 0.2/w_pe, 100 electrons per cell in two counter-streaming
+beams at +-0.2c with vth = 0.02c, immobile ions), i.e. 419,430,400 particles,
+in both forms: FP32 split precision (the headline value) and FP64.  A step is
+the reference time-loop body (pic2d.f90:171-188): zero J, push (gather + Boris
++ move + zigzag deposit + wrap), field solve, and the counting sort every 50
+steps.  Particles start resident in HBM (device-side seeded loader), sorted
+once as the reference does at start-up (:161).
+
+Prints one JSON line.  --gpus N > 1 is not supported on this single-GPU path.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "particle-steps/s FP32-split vs FP64 on 1 MI355X; adjoint/forward time ratio"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def run_form(precision, cfg, steps, warmup, sort_every, seed):
+    import torch
+    import pbpic_amd
+
+    sp = cfg["species"][0]
+    ctx = pbpic_amd.Context(cfg["nx"], cfg["ny"], cfg["dx"], cfg["dy"], cfg["dt"], cfg["cva"],
+                            cfg["mime"], cfg["in0"], precision=precision,
+                            sort_every=sort_every, deposit_rho=False,
+                            rho_background=cfg["rho_background"])
+    npart = cfg["nx"] * cfg["ny"] * sp["ppc"]
+    k = ctx.add_species(sp["q"], npart)
+    ctx.init_maxwellian(k, sp["ppc"], sp["vth"], sp["vdrift"], counter_stream=sp["counter"],
+                        seed=seed, stream=0)
+    ctx.countingsort()                                  # pic2d.f90:161
+    ctx.step(warmup)
+    d0 = ctx.diag()
+    torch.cuda.synchronize()
+    ctx.profile(True)
+    t0 = time.perf_counter()
+    ctx.step(steps)
+    ctx.sync()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    prof = ctx.profile_read()
+    ctx.profile(False)
+    d1 = ctx.diag()
+    ctx.close()
+    wall = t1 - t0
+    return dict(npart=npart, wall=wall, ps=npart * steps / wall, ms_per_step=1e3 * wall / steps,
+                prof=prof, diag0=d0, diag1=d1)
+
+
+def roofline(precision, res, cfg):
+    """Push kernel: algorithmic bytes per launch = particles x (read + write of
+    one particle record) + cells x (6 staggered field planes read, FP64, + 3
+    current planes flushed)."""
+    sp = 44 if precision == 0 else 24          # i32 cell + 5 x f64 | 5 x f32
+    wj = 8 if precision == 0 else 4
+    ncell = cfg["nx"] * cfg["ny"]
+    bytes_launch = res["npart"] * 2 * sp + ncell * (6 * 8 + 3 * wj)
+    ms, cnt = res["prof"]["push"]
+    avg_s = ms / cnt / 1e3
+    achieved = bytes_launch / avg_s / 1e9
+    return dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s",
+                frac=achieved / HBM_PEAK_GBS, traffic=None, kernel="push_kernel",
+                bytes_per_launch=bytes_launch, avg_ms=avg_s * 1e3)
+
+
+def step_roofline(precision, res, cfg, sort_every):
+    """Whole step against B_alg = 2 S_p + B_cell/ppc + B_sort/K_sort."""
+    ppc = cfg["species"][0]["ppc"]
+    sp, bcell, bsort = (44, 192, 96) if precision == 0 else (24, 96, 56)
+    balg = 2 * sp + bcell / ppc + bsort / sort_every
+    gbs = balg * res["ps"] / 1e9
+    return dict(bytes_per_particle_step=balg, achieved=gbs, peak=HBM_PEAK_GBS, unit="GB/s",
+                frac=gbs / HBM_PEAK_GBS)
+
+
+def cpu_baseline(budget_s, seed):
+    """The oracle (CPU restatement of pic2d.f90, AoS `particle` records in the
+    reference's default FP32) on a bounded sample of the same workload: the
+    two-stream physics on a 512x512 grid with 100 electrons per cell."""
+    import numpy as np
+    import oracle
+    from pbpic_amd import configs
+
+    threads = min(16, os.cpu_count() or 1)
+    cfg = configs.two_stream_c2(512, 512, 100)
+    sp = cfg["species"][0]
+    p = oracle.init_maxwellian(cfg["nx"], cfg["ny"], sp["ppc"], sp["vth"], sp["vdrift"],
+                               counter_stream=True, seed=seed)
+    n = len(p["grid"])
+    o = oracle.Oracle(cfg["nx"], cfg["ny"], cfg["dx"], cfg["dy"], cfg["dt"], cfg["cva"],
+                      cfg["mime"], cfg["in0"], prec=4, threads=threads)
+    o.set_particles(p["grid"], p["dx"], p["dy"], p["ux"], p["uy"], p["uz"],
+                    np.full(n, -1, np.int32))
+    del p
+    o.countingsort()
+    o.step(1, sort_every=50)
+    steps, t0 = 0, time.perf_counter()
+    while time.perf_counter() - t0 < budget_s and steps < 200:
+        o.step(1, sort_every=50)
+        steps += 1
+    wall = time.perf_counter() - t0
+    o.close()
+    return dict(value=n * steps / wall, unit="particle-steps/s", cores=threads, kind="port",
+                sample=f"two-stream 512x512 x 100 ppc ({n} electrons), {steps} steps, FP32 "
+                       f"(reference default real), sort every 50, {threads} OpenMP threads")
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--nx", type=int, default=2048)
+    ap.add_argument("--ny", type=int, default=2048)
+    ap.add_argument("--ppc", type=int, default=100)
+    ap.add_argument("--sort-every", type=int, default=50)
+    ap.add_argument("--forms", default="split,fp64")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--seed", type=int, default=20260101)
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(json.dumps({"error": "this benchmark runs on exactly one GPU; the PIC step "
+                                   "is not sharded across GPUs in this build"}))
+        sys.exit(2)
+
+    import torch
+    from pbpic_amd import configs
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    cfg = configs.two_stream_c2(args.nx, args.ny, args.ppc)
+    forms = [f.strip() for f in args.forms.split(",") if f.strip()]
+    res = {}
+    for f in forms:
+        prec = 1 if f == "split" else 0
+        log(f"[bench] form={f} nx={args.nx} ny={args.ny} ppc={args.ppc} steps={args.steps} "
+            f"warmup={args.warmup}")
+        res[f] = run_form(prec, cfg, args.steps, args.warmup, args.sort_every, args.seed)
+        r = res[f]
+        log(f"[bench] form={f}: {r['ps']:.4e} particle-steps/s, {r['ms_per_step']:.3f} ms/step, "
+            f"kernels(ms,count)={r['prof']}")
+    head = "split" if "split" in res else forms[0]
+    hp = 1 if head == "split" else 0
+    out = {
+        "metric": METRIC,
+        "value": res[head]["ps"],
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": res[head]["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if head == "split" else "f64",
+        "data": "synthetic (device-side seeded loader)",
+        "config": {"workload": f"two-stream {args.nx}x{args.ny}, {args.ppc} electrons/cell "
+                               f"({res[head]['npart']} particles), K_sort={args.sort_every}, "
+                               f"form={head}",
+                   "nx": args.nx, "ny": args.ny, "ppc": args.ppc,
+                   "sort_every": args.sort_every, "parallelism": "single GPU"},
+        "roofline": roofline(hp, res[head], cfg),
+        "step_roofline": step_roofline(hp, res[head], cfg, args.sort_every),
+        "kernel_ms": {k: (v[0] / max(v[1], 1), v[1]) for k, v in res[head]["prof"].items()},
+    }
+    if "fp64" in res and head != "fp64":
+        r = res["fp64"]
+        out["fp64"] = {"value": r["ps"], "ms_per_step": r["ms_per_step"],
+                       "roofline": roofline(0, r, cfg), "step_roofline": step_roofline(0, r, cfg, args.sort_every),
+                       "kernel_ms": {k: (v[0] / max(v[1], 1), v[1]) for k, v in r["prof"].items()}}
+        out["ratio_split_vs_fp64"] = res[head]["ps"] / r["ps"]
+    out["adjoint_forward_ratio"] = None
+    out["diag"] = {f: {k: res[f]["diag1"][k] for k in ("ke_e", "w_e", "w_b", "je_int", "gauss_dlinf")}
+                   for f in res}
+    if not args.no_cpu:
+        log(f"[bench] cpu baseline ({args.cpu_seconds:.0f} s budget)")
+        out["cpu_baseline"] = cpu_baseline(args.cpu_seconds, args.seed)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,272 @@
+/*
+ * pbpic.h -- C ABI of the MI355X-native 2D electromagnetic PIC step.
+ *
+ * This library replaces the hot path of xiaocanli/pbpic's Fortran program
+ * (src/programs/pic2d.f90): the per-step subroutines `pusher`,
+ * `currentdeposit`, `fieldsolver`, `fieldsenergy` and `countingsort`, plus the
+ * MPI particle / current / field exchange they contain, which becomes an
+ * in-device periodic wrap on one GPU.
+ *
+ * Conventions
+ *   - Every entry point returns an int status: 0 = PBPIC_OK, negative = error
+ *     (see PBPIC_ERR_*). The library never aborts; pbpic_last_error() gives
+ *     a message.  The reference ignores every MPI/HDF5 error code and aborts
+ *     through MPI's default handler (pic2d.f90, all `ierr` arguments).
+ *   - Only plain C types cross the boundary.  Host arrays are borrowed for the
+ *     duration of the call and copied in or out; device buffers are owned by
+ *     the context.
+ *   - Work is stream-ordered on the context's HIP stream.  Calls that return
+ *     data to the host synchronize; pbpic_sync() waits and reports device-side
+ *     error flags (CFL violation, non-finite values, bad cell index).
+ *   - One context per host thread.  Two contexts never share buffers.
+ *
+ * Index conventions (shared with the reference)
+ *   - Cells: nx x ny, periodic in x and y.  The reference's particle cell id is
+ *     `grid = my*(ix-1) + iy` with 1-based ix, iy (pic2d.f90:422, 938-939):
+ *     y runs fastest.  The same 1-based `grid` is used at this boundary.
+ *   - In-cell offsets dx, dy lie in (-1/2, 1/2] (pic2d.f90:420-421, 1008-1023).
+ *   - Field planes at this boundary are nx*ny arrays, x fastest, i.e. the
+ *     owned region emf(1:mx,1:my) of the reference's Fortran arrays
+ *     (pic2d.f90:593, 221).  Staggering: Ex,Bx,Jx at (i-1/2, j-1);
+ *     Ey,By,Jy at (i-1, j-1/2); Ez,Bz,Jz at (i-1/2, j-1/2); rho and the
+ *     node-averaged emfgrid at (i-1, j-1), in units of cells with node (1,1)
+ *     at the origin.
+ */
+#ifndef PBPIC_H
+#define PBPIC_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- status */
+#define PBPIC_OK              0
+#define PBPIC_ERR_ARG        -1  /* invalid argument */
+#define PBPIC_ERR_HIP        -2  /* HIP runtime error */
+#define PBPIC_ERR_CAPACITY   -3  /* particle count exceeds species capacity */
+#define PBPIC_ERR_CFL        -4  /* a particle moved more than one cell in a step */
+#define PBPIC_ERR_NONFINITE  -5  /* NaN/Inf in a particle momentum */
+#define PBPIC_ERR_CELL       -6  /* particle cell index out of range */
+#define PBPIC_ERR_STATE      -7  /* call not valid in the current state */
+#define PBPIC_ERR_NOMEM      -8  /* device allocation failed */
+#define PBPIC_ERR_SPIN       -9  /* sort look-back exceeded its spin limit */
+
+/* ------------------------------------------------------------- precision */
+#define PBPIC_PREC_FP64     0  /* FP64 particles, fields, currents */
+#define PBPIC_PREC_SPLIT32  1  /* int cell + FP32 offset/momenta, FP32 push,
+                                  deposit and FDTD increments; FP64 field
+                                  state and FP64 reductions */
+
+/* -------------------------------------------------- reference behaviours
+ * Each bit reproduces one behaviour of pic2d.f90 that a textbook scheme would
+ * not have.  PBPIC_COMPAT_REFERENCE sets all of them (the parity default).  */
+#define PBPIC_COMPAT_BORIS_OLD_U  0x1  /* Boris rotation uses the pre-kick u
+                                          (pic2d.f90:985-987) instead of u-  */
+#define PBPIC_COMPAT_RHO_MIRROR   0x2  /* rho CIC weights mirrored
+                                          (pic2d.f90:1285-1295 vs 948-951)   */
+#define PBPIC_COMPAT_JY_DROP      0x4  /* jy deposited at column mx+2 is lost
+                                          in the x fold (pic2d.f90:1156,1159)*/
+#define PBPIC_COMPAT_REFERENCE    0x7
+
+/* ------------------------------------------------------------------ flags */
+#define PBPIC_FLAG_DEPOSIT_RHO  0x1  /* deposit rho at time n during the push
+                                        (pic2d.f90:1283-1295)               */
+#define PBPIC_FLAG_TRACK_PID    0x2  /* keep a per-particle id array that the
+                                        sort permutes (ptl%pid, :47-50)     */
+
+/* ------------------------------------------------------------ field names */
+#define PBPIC_EX   0
+#define PBPIC_EY   1
+#define PBPIC_EZ   2
+#define PBPIC_BX   3
+#define PBPIC_BY   4
+#define PBPIC_BZ   5
+#define PBPIC_JX   6   /* normalized current (x in0), as after pusher :1066 */
+#define PBPIC_JY   7
+#define PBPIC_JZ   8
+#define PBPIC_RHO  9   /* normalized charge density (x in0)                 */
+#define PBPIC_GEX  10  /* node-averaged emfgrid (pic2d.f90:837-845), get only */
+#define PBPIC_GEY  11
+#define PBPIC_GEZ  12
+#define PBPIC_GBX  13
+#define PBPIC_GBY  14
+#define PBPIC_GBZ  15
+
+/* ----------------------------------------------------- diagnostics slots */
+#define PBPIC_D_KE_E        0  /* electron kinetic energy of the last push,
+                                  sum (gamma-1)*ms  (totene(1,:), :1061)    */
+#define PBPIC_D_KE_I        1  /* ion kinetic energy (totene(2,:))          */
+#define PBPIC_D_WE          2  /* electric energy (totene(3,:), :653-673)    */
+#define PBPIC_D_WB          3  /* magnetic energy (totene(4,:))              */
+#define PBPIC_D_JE_STEP     4  /* (mime/c^2) sum J.E_avg of the last field
+                                  solve; energy per unit time               */
+#define PBPIC_D_JE_INT      5  /* time integral of D_JE_STEP since creation */
+#define PBPIC_D_GAUSS_L2    6  /* || divE - c^2 in0 (rho_cic + rho_bg) ||_2  */
+#define PBPIC_D_GAUSS_LINF  7  /* same, max norm                            */
+#define PBPIC_D_GAUSS_DL2   8  /* L2 of the residual's change since the first
+                                  pbpic_diag call                           */
+#define PBPIC_D_GAUSS_DLINF 9  /* max norm of that change                   */
+#define PBPIC_D_STEPS       10 /* steps taken                               */
+#define PBPIC_NDIAG         16
+
+typedef struct pbpic_ctx pbpic_ctx;
+
+/* Run parameters: the runtime replacement for the compile-time constants of
+ * `module parameters` (pic2d.f90:4-142).                                     */
+typedef struct pbpic_params {
+    int32_t nx, ny;          /* mx, my: cells, both periodic (:12, :206-212) */
+    int32_t precision;       /* PBPIC_PREC_*                                 */
+    int32_t compat;          /* PBPIC_COMPAT_* bits                          */
+    double  dx, dy;          /* cell size (:15)                              */
+    double  dt;              /* time step (:42)                              */
+    double  cva;             /* c / v_A (:20)                                */
+    double  mime;            /* m_i / m_e (:19)                              */
+    double  in0;             /* 1/(n0 dx dy) (:31)                           */
+    double  rho_background;  /* uniform charge per cell of an immobile
+                                background, used only by the Gauss residual */
+    int32_t sort_every;      /* countingsort cadence in pbpic_step (:180);
+                                0 = never                                    */
+    int32_t flags;           /* PBPIC_FLAG_* bits                            */
+    int32_t reserved[8];     /* must be zero                                 */
+} pbpic_params;
+
+/* Fill *p with the reference's own constants (pic2d.f90:12-42): 256x256,
+ * dx=dy=0.04, cva=15, mime=25, in0=1/291, dt=0.045/mime, sort every 50,
+ * PBPIC_COMPAT_REFERENCE, FP64.                                              */
+void pbpic_params_default(pbpic_params *p);
+
+/* ---------------------------------------------------------- context life
+ * Replaces the allocation half of `init` (pic2d.f90:197-308) and
+ * `releasememory` (:2557-2582).                                              */
+int pbpic_ctx_create(const pbpic_params *p, pbpic_ctx **out);
+int pbpic_ctx_destroy(pbpic_ctx *ctx);
+const char *pbpic_last_error(const pbpic_ctx *ctx);
+int pbpic_sync(pbpic_ctx *ctx);
+
+/* Add a particle species.  q = +1 (ion, mass mime) or -1 (electron, mass 1),
+ * the reference's per-particle charge `ptl%q` and mass rule (:964-969).
+ * capacity = maximum number of particles.  *species receives its index.     */
+int pbpic_add_species(pbpic_ctx *ctx, int32_t q, int64_t capacity,
+                      int32_t *species);
+int pbpic_num_particles(pbpic_ctx *ctx, int32_t species, int64_t *n);
+
+/* Host particle I/O, structure of arrays, FP64 or FP32 host values (the
+ * device form follows params.precision).  grid is the reference's 1-based
+ * cell id; pid may be NULL (then ids 1..n are assigned if PID tracking is on).
+ * Mirrors the fields of `type particle` (pic2d.f90:47-50).                   */
+int pbpic_set_particles(pbpic_ctx *ctx, int32_t species, int64_t n,
+                        const int32_t *grid, const double *dx,
+                        const double *dy, const double *ux, const double *uy,
+                        const double *uz, const int32_t *pid);
+int pbpic_get_particles(pbpic_ctx *ctx, int32_t species, int64_t cap,
+                        int64_t *n, int32_t *grid, double *dx, double *dy,
+                        double *ux, double *uy, double *uz, int32_t *pid);
+int pbpic_set_particles_f32(pbpic_ctx *ctx, int32_t species, int64_t n,
+                            const int32_t *grid, const float *dx,
+                            const float *dy, const float *ux, const float *uy,
+                            const float *uz, const int32_t *pid);
+int pbpic_get_particles_f32(pbpic_ctx *ctx, int32_t species, int64_t cap,
+                            int64_t *n, int32_t *grid, float *dx, float *dy,
+                            float *ux, float *uy, float *uz, int32_t *pid);
+
+/* Seeded device-side loader: ppc particles per cell, uniform in each cell,
+ * momenta u = gamma v / c drawn from a Maxwellian of thermal speed vth (in
+ * units of c, converted as vth/sqrt(1-vth^2) like pic2d.f90:380-384) plus a
+ * drift u_d = vd/sqrt(1-vd^2) along x; if counter_stream is non-zero the
+ * drift sign alternates between consecutive particles of a cell (two beams).
+ * Philox4x32-10 keyed by (seed, stream).  Replaces the /dev/urandom-seeded
+ * initparticles (:357-578) for the benchmark configurations; results are in
+ * cell order.                                                                */
+int pbpic_init_maxwellian(pbpic_ctx *ctx, int32_t species, int32_t ppc,
+                          double vth, double vdrift_x, int32_t counter_stream,
+                          uint64_t seed, uint32_t stream);
+
+/* Host field I/O, nx*ny planes, x fastest, FP64 host values.  Setting J or
+ * RHO replaces the accumulated current (normalized values, as after the
+ * in0 scaling at pic2d.f90:1066-1069).                                       */
+int pbpic_set_field(pbpic_ctx *ctx, int32_t which, const double *plane);
+int pbpic_get_field(pbpic_ctx *ctx, int32_t which, double *plane);
+
+/* ------------------------------------------------------- the step kernels */
+
+/* denjr = 0 at the top of the time loop (pic2d.f90:173-176).                */
+int pbpic_zero_current(pbpic_ctx *ctx);
+
+/* `pusher` (pic2d.f90:912-1217): for every particle, CIC gather from the
+ * node-averaged fields, relativistic Boris push, move (at most one cell),
+ * Umeda zigzag deposit of Jx, Jy, CIC Jz (and rho if enabled) through the
+ * same arithmetic as `currentdeposit` (:1226-1296), periodic wrap in x and y
+ * (the MPI exchange of :1085-1216 with the rank as its own neighbour), and
+ * the per-species kinetic-energy tally (:1061-1062).  Accumulates into J.   */
+int pbpic_push(pbpic_ctx *ctx);
+
+/* `currentdeposit` (pic2d.f90:1226-1296), batched over n host-array entries.
+ * i1,j1 are a particle's old 1-based cell, i2,j2 its new unwrapped cell
+ * (|i2-i1|<=1, |j2-j1|<=1), dx1..dy2 its old/new offsets, q its charge and
+ * vz = uz*c/gamma.  Accumulates into the context's J (and rho if enabled).  */
+int pbpic_deposit(pbpic_ctx *ctx, int64_t n, const int32_t *i1,
+                  const int32_t *i2, const int32_t *j1, const int32_t *j2,
+                  const double *dx1, const double *dx2, const double *dy1,
+                  const double *dy2, const int32_t *q, const double *vz);
+
+/* `fieldsolver` (pic2d.f90:678-860): half-step B, full-step E with -J,
+ * half-step B, all on the periodic Yee lattice.  Also accumulates
+ * J.E (PBPIC_D_JE_STEP / PBPIC_D_JE_INT).                                    */
+int pbpic_fieldsolve(pbpic_ctx *ctx);
+
+/* `fieldsenergy` (pic2d.f90:653-673): out[0] = electric, out[1] = magnetic
+ * energy of the node-averaged fields, FP64 reduction.                        */
+int pbpic_fieldsenergy(pbpic_ctx *ctx, double out[2]);
+
+/* `countingsort` (pic2d.f90:1303-1329) of every species: stable sort of the
+ * particles by cell id.                                                      */
+int pbpic_countingsort(pbpic_ctx *ctx);
+
+/* The same for one species, optionally returning the per-cell counts
+ * (ptlalloc after :1310-1313; nx*ny entries indexed by grid-1) and the
+ * permutation (perm[k] = 0-based index before the sort of the particle now at
+ * position k).  Either output may be NULL.                                   */
+int pbpic_countingsort_species(pbpic_ctx *ctx, int32_t species,
+                               int32_t *counts, int32_t *perm);
+
+/* The driver loop body (pic2d.f90:171-188) nsteps times: zero J, push,
+ * field solve, and countingsort whenever the step count is a multiple of
+ * params.sort_every.                                                         */
+int pbpic_step(pbpic_ctx *ctx, int32_t nsteps);
+
+/* FP64 diagnostics, PBPIC_NDIAG slots (see PBPIC_D_*).                       */
+int pbpic_diag(pbpic_ctx *ctx, double out[PBPIC_NDIAG]);
+
+/* Device buffers of a species (for the PyTorch binding).  ptrs receives 7
+ * device pointers: cell (int32, 0-based compact id), dx, dy, ux, uy, uz
+ * (double or float by precision), pid (int32 or NULL).  Valid until the next
+ * countingsort or destroy.                                                   */
+int pbpic_species_device_ptrs(pbpic_ctx *ctx, int32_t species, void **ptrs);
+
+/* Device field planes: ptrs receives 10 pointers: ex, ey, ez, bx, by, bz
+ * (double), jx, jy, jz, rho (double or float by precision, unnormalized).   */
+int pbpic_field_device_ptrs(pbpic_ctx *ctx, void **ptrs);
+
+/* The HIP stream (hipStream_t) the context orders its work on.              */
+void *pbpic_stream(pbpic_ctx *ctx);
+
+/* Per-kernel timing with HIP events on the context's stream (measurement
+ * support; the reference has no timers, pic2d.f90:172 only prints itime).
+ * enable != 0 starts collecting (and clears earlier samples), 0 stops.
+ * pbpic_profile_read waits for the stream and returns, per PBPIC_PROF_* slot,
+ * the summed milliseconds and the number of launches timed.                 */
+#define PBPIC_PROF_PUSH   0   /* push kernel (gather+Boris+move+deposit)      */
+#define PBPIC_PROF_FIELD  1   /* FDTD kernel                                  */
+#define PBPIC_PROF_SORT   2   /* counting sort incl. payload gather           */
+#define PBPIC_PROF_ZERO   3   /* current clear                                */
+#define PBPIC_NPROF       4
+int pbpic_profile(pbpic_ctx *ctx, int32_t enable);
+int pbpic_profile_read(pbpic_ctx *ctx, double ms[PBPIC_NPROF],
+                       int64_t count[PBPIC_NPROF]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PBPIC_H */

@@ -1,0 +1,65 @@
+"""Builds libpbpic.so (hipcc, gfx950) in the package directory.
+
+Run as ``python -m pbpic_amd.build`` or through ``__graft_entry__.build()``.
+"""
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+LIB = os.path.join(HERE, "libpbpic.so")
+
+SOURCES = ["capi.hip", "k_push.hip", "k_field.hip", "k_sort.hip"]
+
+HIPCC_FLAGS = [
+    "--offload-arch=gfx950",
+    "-O3",
+    "-std=c++17",
+    "-fPIC",
+    "-shared",
+    "-munsafe-fp-atomics",
+    # packed FP32 instructions are kept out of the generated code
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wno-unused-command-line-argument",
+]
+
+
+def hipcc():
+    for cand in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", "hipcc"):
+        if cand and (os.path.sep not in cand or os.path.exists(cand)):
+            return cand
+    raise RuntimeError("hipcc not found")
+
+
+def build_library(verbose=True):
+    cmd = [hipcc(), *HIPCC_FLAGS, "-I", os.path.join(ROOT, "include"),
+           *[os.path.join(CSRC, s) for s in SOURCES], "-o", LIB + ".tmp"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(LIB + ".tmp", LIB)
+    return LIB
+
+
+def build_fortran(verbose=True):
+    """Fortran driver mirroring `program pic2d`, linked against libpbpic.so."""
+    fdir = os.path.join(HERE, "fortran")
+    fc = "/opt/rocm/bin/amdflang"
+    if not os.path.exists(fc):
+        return None
+    out = os.path.join(fdir, "pic2d_gpu")
+    cmd = [fc, "-O2", "-J", fdir, os.path.join(fdir, "pbpic_mod.f90"),
+           os.path.join(fdir, "pic2d_driver.f90"), "-o", out,
+           "-L" + HERE, "-lpbpic", "-Wl,-rpath,$ORIGIN/.."]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    return out
+
+
+if __name__ == "__main__":
+    build_library()
+    if len(sys.argv) > 1 and sys.argv[1] == "--fortran":
+        build_fortran()

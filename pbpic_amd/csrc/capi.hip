@@ -1,0 +1,755 @@
+// capi.hip -- the C ABI of include/pbpic.h: context, buffers, host I/O and
+// the step entry points that mirror pic2d.f90's subroutines.
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "pbpic_impl.h"
+
+using namespace pbpic;
+
+namespace {
+
+struct Species {
+    int32_t q = 0;
+    int64_t cap = 0, n = 0;
+    double ms = 1, c0 = 0, c1 = 0, qx = 0, qy = 0;
+    int32_t *cell[2] = {nullptr, nullptr};
+    void *ox[2] = {nullptr, nullptr}, *oy[2] = {nullptr, nullptr};
+    void *ux[2] = {nullptr, nullptr}, *uy[2] = {nullptr, nullptr};
+    void *uz[2] = {nullptr, nullptr};
+    int32_t *pid[2] = {nullptr, nullptr};
+    int cur = 0;
+};
+
+}  // namespace
+
+struct pbpic_ctx {
+    pbpic_params p{};
+    int rsz = 8;
+    int dev = 0;
+    hipStream_t st = nullptr;
+    double *E[3] = {nullptr, nullptr, nullptr};
+    double *B[3] = {nullptr, nullptr, nullptr};
+    void *J[4] = {nullptr, nullptr, nullptr, nullptr};
+    double *acc = nullptr;
+    int *err = nullptr;
+    double *rho_scratch = nullptr, *gauss0 = nullptr;
+    bool gauss_first = true;
+    std::vector<Species> sp;
+    SortScratch sort;
+    int64_t steps = 0;
+    int maxw = 0;
+    double idx = 0, idy = 0, dth = 0, dtc2 = 0, dtcva = 0;
+    std::string last_error;
+    // event timing
+    bool prof = false;
+    std::vector<hipEvent_t> ev_pool;
+    size_t ev_used = 0;
+    std::vector<std::pair<int, size_t>> ev_marks;   // (slot, index of start event)
+};
+
+namespace {
+
+int fail(pbpic_ctx *c, int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (c) c->last_error = buf;
+    return code;
+}
+
+#define HIPCHK(c, x)                                                             \
+    do {                                                                         \
+        hipError_t _e = (x);                                                     \
+        if (_e != hipSuccess)                                                    \
+            return fail((c), PBPIC_ERR_HIP, "%s: %s", #x, hipGetErrorString(_e)); \
+    } while (0)
+
+int64_t ncell(const pbpic_ctx *c) { return (int64_t)c->p.nx * c->p.ny; }
+
+template <typename T>
+hipError_t dalloc(T **p, size_t bytes)
+{
+    void *q = nullptr;
+    hipError_t e = hipMalloc(&q, bytes > 0 ? bytes : 16);
+    *p = (T *)q;
+    return e;
+}
+
+SpeciesDev species_dev(pbpic_ctx *c, int s, int buf)
+{
+    Species &S = c->sp[s];
+    SpeciesDev d{};
+    d.cell = S.cell[buf];
+    d.ox = S.ox[buf]; d.oy = S.oy[buf];
+    d.ux = S.ux[buf]; d.uy = S.uy[buf]; d.uz = S.uz[buf];
+    d.pid = S.pid[buf];
+    d.n = S.n;
+    d.q = S.q;
+    d.ms = S.ms; d.c0 = S.c0; d.c1 = S.c1; d.qx = S.qx; d.qy = S.qy;
+    return d;
+}
+
+PushParams push_params(pbpic_ctx *c)
+{
+    PushParams P{};
+    P.nx = c->p.nx;
+    P.ny = c->p.ny;
+    P.dny.init((uint32_t)c->p.ny);
+    P.compat = c->p.compat;
+    P.deposit_rho = (c->p.flags & PBPIC_FLAG_DEPOSIT_RHO) ? 1 : 0;
+    P.maxw = c->maxw;
+    P.dt = c->p.dt;
+    P.cva = c->p.cva;
+    P.idx = c->idx;
+    P.idy = c->idy;
+    P.dtcva = c->dtcva;
+    P.ex = c->E[0]; P.ey = c->E[1]; P.ez = c->E[2];
+    P.bx = c->B[0]; P.by = c->B[1]; P.bz = c->B[2];
+    P.jx = c->J[0]; P.jy = c->J[1]; P.jz = c->J[2]; P.rho = c->J[3];
+    P.acc = c->acc;
+    P.err = c->err;
+    P.nspecies = (int)c->sp.size();
+    for (int s = 0; s < P.nspecies; s++) P.sp[s] = species_dev(c, s, c->sp[s].cur);
+    return P;
+}
+
+FdtdParams fdtd_params(pbpic_ctx *c)
+{
+    FdtdParams F{};
+    F.nx = c->p.nx; F.ny = c->p.ny;
+    F.dth = c->dth; F.dtc2 = c->dtc2; F.idx = c->idx; F.idy = c->idy; F.in0 = c->p.in0;
+    F.ex = c->E[0]; F.ey = c->E[1]; F.ez = c->E[2];
+    F.bx = c->B[0]; F.by = c->B[1]; F.bz = c->B[2];
+    F.jx = c->J[0]; F.jy = c->J[1]; F.jz = c->J[2];
+    F.acc = c->acc;
+    return F;
+}
+
+// Translates and clears the device error flags.
+int check_flags(pbpic_ctx *c)
+{
+    int h = 0;
+    HIPCHK(c, hipMemcpyAsync(&h, c->err, sizeof(int), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    if (!h) return PBPIC_OK;
+    HIPCHK(c, launch_fill_u32((uint32_t *)c->err, 1, 0u, c->st));
+    if (h & ERRF_CELL) return fail(c, PBPIC_ERR_CELL, "particle cell index out of range");
+    if (h & ERRF_CFL) return fail(c, PBPIC_ERR_CFL, "a particle moved more than one cell in one step");
+    if (h & ERRF_NONFINITE) return fail(c, PBPIC_ERR_NONFINITE, "non-finite particle momentum");
+    if (h & ERRF_SPIN) return fail(c, PBPIC_ERR_SPIN, "sort look-back exceeded its spin limit");
+    return fail(c, PBPIC_ERR_STATE, "unknown device error flag %d", h);
+}
+
+int valid_species(pbpic_ctx *c, int32_t s)
+{
+    if (!c) return PBPIC_ERR_ARG;
+    if (s < 0 || s >= (int32_t)c->sp.size()) return fail(c, PBPIC_ERR_ARG, "bad species %d", s);
+    return PBPIC_OK;
+}
+
+hipError_t prof_mark(pbpic_ctx *c, int slot, bool begin);
+
+int sort_one(pbpic_ctx *c, int s, int32_t *counts, int32_t *perm)
+{
+    Species &S = c->sp[s];
+    SpeciesDev src = species_dev(c, s, S.cur), dst = species_dev(c, s, S.cur ^ 1);
+    HIPCHK(c, prof_mark(c, PBPIC_PROF_SORT, true));
+    HIPCHK(c, sort_species(c->sort, src, c->p.precision, ncell(c), dst, c->err, counts != nullptr, c->st));
+    HIPCHK(c, prof_mark(c, PBPIC_PROF_SORT, false));
+    S.cur ^= 1;
+    if (counts)
+        HIPCHK(c, hipMemcpyAsync(counts, c->sort.counts, sizeof(int32_t) * ncell(c),
+                                 hipMemcpyDeviceToHost, c->st));
+    if (perm && S.n > 0)
+        HIPCHK(c, hipMemcpyAsync(perm, c->sort.perm, sizeof(int32_t) * S.n, hipMemcpyDeviceToHost, c->st));
+    if (counts || perm) HIPCHK(c, hipStreamSynchronize(c->st));
+    return PBPIC_OK;
+}
+
+// Records a timing event pair around a launch when profiling is on.
+hipError_t prof_mark(pbpic_ctx *c, int slot, bool begin)
+{
+    if (!c->prof) return hipSuccess;
+    if (c->ev_used == c->ev_pool.size()) {
+        hipEvent_t e;
+        hipError_t r = hipEventCreate(&e);
+        if (r != hipSuccess) return r;
+        c->ev_pool.push_back(e);
+    }
+    if (begin) c->ev_marks.push_back({slot, c->ev_used});
+    return hipEventRecord(c->ev_pool[c->ev_used++], c->st);
+}
+
+int push_impl(pbpic_ctx *c, bool reset_ke)
+{
+    if (reset_ke) HIPCHK(c, launch_acc_op(c->acc, 0, 0.0, c->st));
+    HIPCHK(c, prof_mark(c, PBPIC_PROF_PUSH, true));
+    HIPCHK(c, launch_push(push_params(c), c->p.precision, c->st));
+    HIPCHK(c, prof_mark(c, PBPIC_PROF_PUSH, false));
+    return PBPIC_OK;
+}
+
+int fieldsolve_impl(pbpic_ctx *c)
+{
+    HIPCHK(c, launch_acc_op(c->acc, 1, c->p.dt * c->p.mime / (c->p.cva * c->p.cva), c->st));
+    HIPCHK(c, prof_mark(c, PBPIC_PROF_FIELD, true));
+    HIPCHK(c, launch_fdtd(fdtd_params(c), c->p.precision, c->st));
+    HIPCHK(c, prof_mark(c, PBPIC_PROF_FIELD, false));
+    c->steps++;
+    return PBPIC_OK;
+}
+
+template <typename H>
+int set_particles_t(pbpic_ctx *c, int32_t s, int64_t n, const int32_t *grid, const H *dx,
+                    const H *dy, const H *ux, const H *uy, const H *uz, const int32_t *pid)
+{
+    int rc = valid_species(c, s);
+    if (rc) return rc;
+    Species &S = c->sp[s];
+    if (n < 0 || n > S.cap) return fail(c, PBPIC_ERR_CAPACITY, "n=%lld exceeds capacity %lld", (long long)n, (long long)S.cap);
+    if (n > 0 && (!grid || !dx || !dy || !ux || !uy || !uz)) return fail(c, PBPIC_ERR_ARG, "null particle array");
+    const int64_t nc = ncell(c);
+    std::vector<int32_t> cell((size_t)n);
+    for (int64_t i = 0; i < n; i++) {
+        if (grid[i] < 1 || grid[i] > nc) return fail(c, PBPIC_ERR_CELL, "grid[%lld]=%d out of 1..%lld", (long long)i, grid[i], (long long)nc);
+        cell[(size_t)i] = grid[i] - 1;
+    }
+    const int b = S.cur;
+    HIPCHK(c, hipMemcpyAsync(S.cell[b], cell.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, c->st));
+    const H *src[5] = {dx, dy, ux, uy, uz};
+    void *dst[5] = {S.ox[b], S.oy[b], S.ux[b], S.uy[b], S.uz[b]};
+    std::vector<double> vd;
+    std::vector<float> vf;
+    for (int k = 0; k < 5; k++) {
+        if (c->rsz == 8) {
+            vd.assign(src[k], src[k] + n);
+            HIPCHK(c, hipMemcpyAsync(dst[k], vd.data(), 8 * n, hipMemcpyHostToDevice, c->st));
+        } else {
+            vf.assign(src[k], src[k] + n);
+            HIPCHK(c, hipMemcpyAsync(dst[k], vf.data(), 4 * n, hipMemcpyHostToDevice, c->st));
+        }
+        HIPCHK(c, hipStreamSynchronize(c->st));
+    }
+    if (S.pid[b]) {
+        std::vector<int32_t> ids((size_t)n);
+        for (int64_t i = 0; i < n; i++) ids[(size_t)i] = pid ? pid[i] : (int32_t)(i + 1);
+        HIPCHK(c, hipMemcpyAsync(S.pid[b], ids.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, c->st));
+    }
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    S.n = n;
+    return PBPIC_OK;
+}
+
+template <typename H>
+int get_particles_t(pbpic_ctx *c, int32_t s, int64_t cap, int64_t *nout, int32_t *grid, H *dx,
+                    H *dy, H *ux, H *uy, H *uz, int32_t *pid)
+{
+    int rc = valid_species(c, s);
+    if (rc) return rc;
+    if ((rc = check_flags(c))) return rc;
+    Species &S = c->sp[s];
+    if (nout) *nout = S.n;
+    const int64_t n = S.n;
+    if (cap < n) return fail(c, PBPIC_ERR_CAPACITY, "host capacity %lld < %lld particles", (long long)cap, (long long)n);
+    const int b = S.cur;
+    if (grid) {
+        HIPCHK(c, hipMemcpy(grid, S.cell[b], sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+        for (int64_t i = 0; i < n; i++) grid[i] += 1;
+    }
+    H *dst[5] = {dx, dy, ux, uy, uz};
+    void *src[5] = {S.ox[b], S.oy[b], S.ux[b], S.uy[b], S.uz[b]};
+    std::vector<double> vd;
+    std::vector<float> vf;
+    for (int k = 0; k < 5; k++) {
+        if (!dst[k]) continue;
+        if (c->rsz == 8) {
+            vd.resize((size_t)n);
+            HIPCHK(c, hipMemcpy(vd.data(), src[k], 8 * n, hipMemcpyDeviceToHost));
+            for (int64_t i = 0; i < n; i++) dst[k][i] = (H)vd[(size_t)i];
+        } else {
+            vf.resize((size_t)n);
+            HIPCHK(c, hipMemcpy(vf.data(), src[k], 4 * n, hipMemcpyDeviceToHost));
+            for (int64_t i = 0; i < n; i++) dst[k][i] = (H)vf[(size_t)i];
+        }
+    }
+    if (pid) {
+        if (S.pid[b]) HIPCHK(c, hipMemcpy(pid, S.pid[b], sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+        else for (int64_t i = 0; i < n; i++) pid[i] = 0;
+    }
+    return PBPIC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void pbpic_params_default(pbpic_params *p)
+{
+    if (!p) return;
+    memset(p, 0, sizeof *p);
+    p->nx = 256;                 // pic2d.f90:12
+    p->ny = 256;
+    p->precision = PBPIC_PREC_FP64;
+    p->compat = PBPIC_COMPAT_REFERENCE;
+    p->dx = 0.04;                // :15
+    p->dy = 0.04;
+    p->mime = 25.0;              // :19
+    p->cva = 15.0;               // :20
+    p->in0 = 1.0 / 291.0;        // :31
+    p->dt = 0.045 / 25.0;        // :42
+    p->sort_every = 50;          // :180
+    p->flags = PBPIC_FLAG_DEPOSIT_RHO;
+}
+
+int pbpic_ctx_create(const pbpic_params *pp, pbpic_ctx **out)
+{
+    if (!pp || !out) return PBPIC_ERR_ARG;
+    *out = nullptr;
+    const pbpic_params &p = *pp;
+    if (p.nx < 2 || p.ny < 2 || (int64_t)p.nx * p.ny >= (1ll << 30)) return PBPIC_ERR_ARG;
+    if (!(p.dx > 0) || !(p.dy > 0) || !(p.dt > 0) || !(p.cva > 0) || !(p.mime > 0) || !(p.in0 > 0))
+        return PBPIC_ERR_ARG;
+    if (p.precision != PBPIC_PREC_FP64 && p.precision != PBPIC_PREC_SPLIT32) return PBPIC_ERR_ARG;
+    if (p.compat & ~PBPIC_COMPAT_REFERENCE) return PBPIC_ERR_ARG;
+    for (int k = 0; k < 8; k++)
+        if (p.reserved[k]) return PBPIC_ERR_ARG;
+    pbpic_ctx *c = new pbpic_ctx();
+    c->p = p;
+    c->rsz = p.precision == PBPIC_PREC_FP64 ? 8 : 4;
+    c->maxw = p.precision == PBPIC_PREC_FP64 ? 800 : 1280;
+    c->idx = 1.0 / p.dx;                   // pic2d.f90:16
+    c->idy = 1.0 / p.dy;
+    c->dth = p.dt / 2.0;                   // :43
+    c->dtc2 = p.dt * (p.cva * p.cva);      // dt*cva2 (:742)
+    c->dtcva = p.dt * p.cva;               // (:1003)
+    int rc = PBPIC_OK;
+    auto chk = [&](hipError_t e, const char *what) {
+        if (e != hipSuccess && rc == PBPIC_OK) rc = fail(c, e == hipErrorOutOfMemory ? PBPIC_ERR_NOMEM : PBPIC_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
+    };
+    chk(hipGetDevice(&c->dev), "hipGetDevice");
+    chk(hipStreamCreate(&c->st), "hipStreamCreate");
+    const size_t nc = (size_t)p.nx * p.ny;
+    for (int k = 0; k < 3; k++) {
+        chk(dalloc(&c->E[k], nc * 8), "alloc E");
+        chk(dalloc(&c->B[k], nc * 8), "alloc B");
+    }
+    for (int k = 0; k < 4; k++) chk(dalloc(&c->J[k], nc * c->rsz), "alloc J");
+    chk(dalloc(&c->acc, ACC_N * sizeof(double)), "alloc acc");
+    chk(dalloc(&c->err, sizeof(int) * 4), "alloc err");
+    chk(dalloc(&c->sort.counts, nc * sizeof(int32_t)), "alloc counts");
+    chk(dalloc(&c->sort.dhist, 4 * 256 * sizeof(uint32_t)), "alloc dhist");
+    chk(dalloc(&c->sort.ticket, 8 * sizeof(uint32_t)), "alloc ticket");
+    if (rc == PBPIC_OK) {
+        for (int k = 0; k < 3; k++) {
+            chk(hipMemsetAsync(c->E[k], 0, nc * 8, c->st), "zero E");
+            chk(hipMemsetAsync(c->B[k], 0, nc * 8, c->st), "zero B");
+        }
+        for (int k = 0; k < 4; k++) chk(hipMemsetAsync(c->J[k], 0, nc * c->rsz, c->st), "zero J");
+        chk(hipMemsetAsync(c->acc, 0, ACC_N * sizeof(double), c->st), "zero acc");
+        chk(hipMemsetAsync(c->err, 0, sizeof(int) * 4, c->st), "zero err");
+        chk(hipStreamSynchronize(c->st), "sync");
+    }
+    if (rc != PBPIC_OK) {
+        std::string msg = c->last_error;
+        pbpic_ctx_destroy(c);
+        fprintf(stderr, "pbpic_ctx_create: %s\n", msg.c_str());
+        return rc;
+    }
+    *out = c;
+    return PBPIC_OK;
+}
+
+int pbpic_ctx_destroy(pbpic_ctx *c)
+{
+    if (!c) return PBPIC_ERR_ARG;
+    if (c->st) (void)hipStreamSynchronize(c->st);
+    auto f = [](void *p) { if (p) (void)hipFree(p); };
+    for (int k = 0; k < 3; k++) { f(c->E[k]); f(c->B[k]); }
+    for (int k = 0; k < 4; k++) f(c->J[k]);
+    f(c->acc); f(c->err); f(c->rho_scratch); f(c->gauss0);
+    for (auto &S : c->sp)
+        for (int b = 0; b < 2; b++) {
+            f(S.cell[b]); f(S.ox[b]); f(S.oy[b]); f(S.ux[b]); f(S.uy[b]); f(S.uz[b]); f(S.pid[b]);
+        }
+    for (int b = 0; b < 2; b++) { f(c->sort.keys[b]); f(c->sort.vals[b]); }
+    f(c->sort.status); f(c->sort.dhist); f(c->sort.ticket); f(c->sort.counts);
+    for (auto e : c->ev_pool) (void)hipEventDestroy(e);
+    if (c->st) (void)hipStreamDestroy(c->st);
+    delete c;
+    return PBPIC_OK;
+}
+
+const char *pbpic_last_error(const pbpic_ctx *c) { return c ? c->last_error.c_str() : "null context"; }
+
+int pbpic_sync(pbpic_ctx *c)
+{
+    if (!c) return PBPIC_ERR_ARG;
+    return check_flags(c);
+}
+
+int pbpic_add_species(pbpic_ctx *c, int32_t q, int64_t capacity, int32_t *species)
+{
+    if (!c) return PBPIC_ERR_ARG;
+    if (q != 1 && q != -1) return fail(c, PBPIC_ERR_ARG, "species charge must be +1 or -1, got %d", q);
+    if (capacity < 0 || capacity >= (1ll << 30)) return fail(c, PBPIC_ERR_CAPACITY, "capacity %lld out of range", (long long)capacity);
+    if ((int)c->sp.size() >= MAX_SPECIES) return fail(c, PBPIC_ERR_ARG, "at most %d species", MAX_SPECIES);
+    Species S;
+    S.q = q;
+    S.cap = capacity;
+    const pbpic_params &p = c->p;
+    S.ms = q > 0 ? p.mime : 1.0;                         // pic2d.f90:964-968
+    S.c0 = (double)q * p.dt * p.mime / S.ms / 2.0;       // :969
+    S.c1 = S.c0 / p.cva;                                 // :970
+    S.qx = (double)q * p.dx / p.dt;                      // :1237
+    S.qy = (double)q * p.dy / p.dt;
+    const size_t n = (size_t)capacity;
+    for (int b = 0; b < 2; b++) {
+        HIPCHK(c, dalloc(&S.cell[b], n * 4));
+        HIPCHK(c, dalloc(&S.ox[b], n * c->rsz));
+        HIPCHK(c, dalloc(&S.oy[b], n * c->rsz));
+        HIPCHK(c, dalloc(&S.ux[b], n * c->rsz));
+        HIPCHK(c, dalloc(&S.uy[b], n * c->rsz));
+        HIPCHK(c, dalloc(&S.uz[b], n * c->rsz));
+        if (p.flags & PBPIC_FLAG_TRACK_PID) HIPCHK(c, dalloc(&S.pid[b], n * 4));
+    }
+    // sort scratch sized for the largest species
+    if (capacity > c->sort.cap) {
+        for (int b = 0; b < 2; b++) {
+            if (c->sort.keys[b]) (void)hipFree(c->sort.keys[b]);
+            if (c->sort.vals[b]) (void)hipFree(c->sort.vals[b]);
+            HIPCHK(c, dalloc(&c->sort.keys[b], n * 4));
+            HIPCHK(c, dalloc(&c->sort.vals[b], n * 4));
+        }
+        if (c->sort.status) (void)hipFree(c->sort.status);
+        const int64_t ntile = (capacity + 4095) / 4096 + 1;
+        HIPCHK(c, dalloc(&c->sort.status, (size_t)ntile * 256 * 4));
+        c->sort.cap = capacity;
+        c->sort.status_cap = ntile;
+    }
+    c->sp.push_back(S);
+    if (species) *species = (int32_t)c->sp.size() - 1;
+    return PBPIC_OK;
+}
+
+int pbpic_num_particles(pbpic_ctx *c, int32_t s, int64_t *n)
+{
+    int rc = valid_species(c, s);
+    if (rc) return rc;
+    if (n) *n = c->sp[s].n;
+    return PBPIC_OK;
+}
+
+int pbpic_set_particles(pbpic_ctx *c, int32_t s, int64_t n, const int32_t *grid, const double *dx,
+                        const double *dy, const double *ux, const double *uy, const double *uz,
+                        const int32_t *pid)
+{
+    return set_particles_t<double>(c, s, n, grid, dx, dy, ux, uy, uz, pid);
+}
+
+int pbpic_set_particles_f32(pbpic_ctx *c, int32_t s, int64_t n, const int32_t *grid,
+                            const float *dx, const float *dy, const float *ux, const float *uy,
+                            const float *uz, const int32_t *pid)
+{
+    return set_particles_t<float>(c, s, n, grid, dx, dy, ux, uy, uz, pid);
+}
+
+int pbpic_get_particles(pbpic_ctx *c, int32_t s, int64_t cap, int64_t *n, int32_t *grid, double *dx,
+                        double *dy, double *ux, double *uy, double *uz, int32_t *pid)
+{
+    return get_particles_t<double>(c, s, cap, n, grid, dx, dy, ux, uy, uz, pid);
+}
+
+int pbpic_get_particles_f32(pbpic_ctx *c, int32_t s, int64_t cap, int64_t *n, int32_t *grid,
+                            float *dx, float *dy, float *ux, float *uy, float *uz, int32_t *pid)
+{
+    return get_particles_t<float>(c, s, cap, n, grid, dx, dy, ux, uy, uz, pid);
+}
+
+int pbpic_init_maxwellian(pbpic_ctx *c, int32_t s, int32_t ppc, double vth, double vdrift_x,
+                          int32_t counter_stream, uint64_t seed, uint32_t stream)
+{
+    int rc = valid_species(c, s);
+    if (rc) return rc;
+    if (ppc < 1 || !(vth >= 0 && vth < 1) || !(fabs(vdrift_x) < 1))
+        return fail(c, PBPIC_ERR_ARG, "bad loader arguments");
+    Species &S = c->sp[s];
+    const int64_t n = ncell(c) * (int64_t)ppc;
+    if (n > S.cap) return fail(c, PBPIC_ERR_CAPACITY, "ppc*cells=%lld exceeds capacity %lld", (long long)n, (long long)S.cap);
+    S.n = n;
+    SpeciesDev d = species_dev(c, s, S.cur);
+    HIPCHK(c, launch_init_maxwellian(d, c->p.precision, c->p.nx, c->p.ny, ppc, vth, vdrift_x,
+                                     counter_stream, seed, stream, c->st));
+    return PBPIC_OK;
+}
+
+int pbpic_set_field(pbpic_ctx *c, int32_t which, const double *plane)
+{
+    if (!c || !plane) return PBPIC_ERR_ARG;
+    const size_t nc = (size_t)ncell(c);
+    if (which >= PBPIC_EX && which <= PBPIC_EZ) {
+        HIPCHK(c, hipMemcpy(c->E[which - PBPIC_EX], plane, nc * 8, hipMemcpyHostToDevice));
+    } else if (which >= PBPIC_BX && which <= PBPIC_BZ) {
+        HIPCHK(c, hipMemcpy(c->B[which - PBPIC_BX], plane, nc * 8, hipMemcpyHostToDevice));
+    } else if (which >= PBPIC_JX && which <= PBPIC_RHO) {
+        // the device holds the unnormalized sum; the boundary value is x in0
+        std::vector<double> vd(nc);
+        std::vector<float> vf;
+        for (size_t k = 0; k < nc; k++) vd[k] = plane[k] / c->p.in0;
+        void *dst = c->J[which - PBPIC_JX];
+        if (c->rsz == 8) {
+            HIPCHK(c, hipMemcpy(dst, vd.data(), nc * 8, hipMemcpyHostToDevice));
+        } else {
+            vf.assign(vd.begin(), vd.end());
+            HIPCHK(c, hipMemcpy(dst, vf.data(), nc * 4, hipMemcpyHostToDevice));
+        }
+    } else {
+        return fail(c, PBPIC_ERR_ARG, "field %d cannot be set", which);
+    }
+    return PBPIC_OK;
+}
+
+int pbpic_get_field(pbpic_ctx *c, int32_t which, double *plane)
+{
+    if (!c || !plane) return PBPIC_ERR_ARG;
+    int rc = check_flags(c);
+    if (rc) return rc;
+    const int nx = c->p.nx, ny = c->p.ny;
+    const size_t nc = (size_t)ncell(c);
+    if (which >= PBPIC_EX && which <= PBPIC_EZ) {
+        HIPCHK(c, hipMemcpy(plane, c->E[which - PBPIC_EX], nc * 8, hipMemcpyDeviceToHost));
+    } else if (which >= PBPIC_BX && which <= PBPIC_BZ) {
+        HIPCHK(c, hipMemcpy(plane, c->B[which - PBPIC_BX], nc * 8, hipMemcpyDeviceToHost));
+    } else if (which >= PBPIC_JX && which <= PBPIC_RHO) {
+        void *src = c->J[which - PBPIC_JX];
+        if (c->rsz == 8) {
+            HIPCHK(c, hipMemcpy(plane, src, nc * 8, hipMemcpyDeviceToHost));
+            for (size_t k = 0; k < nc; k++) plane[k] = plane[k] * c->p.in0;   // :1066
+        } else {
+            std::vector<float> vf(nc);
+            HIPCHK(c, hipMemcpy(vf.data(), src, nc * 4, hipMemcpyDeviceToHost));
+            for (size_t k = 0; k < nc; k++) plane[k] = (double)(vf[k] * (float)c->p.in0);
+        }
+    } else if (which >= PBPIC_GEX && which <= PBPIC_GBZ) {
+        // emfgrid node averages (pic2d.f90:837-845)
+        const int comp = (which - PBPIC_GEX) % 3;
+        const bool isb = which >= PBPIC_GBX;
+        std::vector<double> f(nc);
+        HIPCHK(c, hipMemcpy(f.data(), isb ? c->B[comp] : c->E[comp], nc * 8, hipMemcpyDeviceToHost));
+        for (int gy = 0; gy < ny; gy++)
+            for (int gx = 0; gx < nx; gx++) {
+                const int gxm = gx == 0 ? nx - 1 : gx - 1, gym = gy == 0 ? ny - 1 : gy - 1;
+                const double a = f[(size_t)gy * nx + gx], bx = f[(size_t)gy * nx + gxm];
+                const double by = f[(size_t)gym * nx + gx], bxy = f[(size_t)gym * nx + gxm];
+                double v;
+                if (comp == 0) v = (a + bx) * 0.5;
+                else if (comp == 1) v = (a + by) * 0.5;
+                else v = (a + bx + by + bxy) * 0.25;
+                plane[(size_t)gy * nx + gx] = v;
+            }
+    } else {
+        return fail(c, PBPIC_ERR_ARG, "unknown field %d", which);
+    }
+    return PBPIC_OK;
+}
+
+int pbpic_zero_current(pbpic_ctx *c)
+{
+    if (!c) return PBPIC_ERR_ARG;
+    HIPCHK(c, launch_zero_current(c->J[0], c->J[1], c->J[2], c->J[3], ncell(c), c->rsz, c->acc, 0, c->st));
+    return PBPIC_OK;
+}
+
+int pbpic_push(pbpic_ctx *c)
+{
+    if (!c) return PBPIC_ERR_ARG;
+    return push_impl(c, true);
+}
+
+int pbpic_deposit(pbpic_ctx *c, int64_t n, const int32_t *i1, const int32_t *i2, const int32_t *j1,
+                  const int32_t *j2, const double *dx1, const double *dx2, const double *dy1,
+                  const double *dy2, const int32_t *q, const double *vz)
+{
+    if (!c) return PBPIC_ERR_ARG;
+    if (n <= 0) return PBPIC_OK;
+    if (!i1 || !i2 || !j1 || !j2 || !dx1 || !dx2 || !dy1 || !dy2 || !q || !vz) return fail(c, PBPIC_ERR_ARG, "null deposit array");
+    const int nx = c->p.nx, ny = c->p.ny;
+    for (int64_t k = 0; k < n; k++) {
+        if (i1[k] < 1 || i1[k] > nx || j1[k] < 1 || j1[k] > ny || std::abs(i2[k] - i1[k]) > 1 ||
+            std::abs(j2[k] - j1[k]) > 1 || (q[k] != 1 && q[k] != -1))
+            return fail(c, PBPIC_ERR_ARG, "deposit entry %lld out of range", (long long)k);
+    }
+    void *buf = nullptr;
+    const size_t ib = (size_t)n * 4, db = (size_t)n * 8;
+    HIPCHK(c, hipMalloc(&buf, 5 * ib + 5 * db));
+    char *b = (char *)buf;
+    int32_t *d_i1 = (int32_t *)b, *d_i2 = (int32_t *)(b + ib), *d_j1 = (int32_t *)(b + 2 * ib),
+            *d_j2 = (int32_t *)(b + 3 * ib), *d_q = (int32_t *)(b + 4 * ib);
+    double *d_dx1 = (double *)(b + 5 * ib), *d_dx2 = d_dx1 + n, *d_dy1 = d_dx2 + n, *d_dy2 = d_dy1 + n,
+           *d_vz = d_dy2 + n;
+    int rc = PBPIC_OK;
+    const void *src[10] = {i1, i2, j1, j2, q, dx1, dx2, dy1, dy2, vz};
+    void *dst[10] = {d_i1, d_i2, d_j1, d_j2, d_q, d_dx1, d_dx2, d_dy1, d_dy2, d_vz};
+    for (int k = 0; k < 10 && rc == PBPIC_OK; k++) {
+        hipError_t e = hipMemcpyAsync(dst[k], src[k], k < 5 ? ib : db, hipMemcpyHostToDevice, c->st);
+        if (e != hipSuccess) rc = fail(c, PBPIC_ERR_HIP, "deposit upload: %s", hipGetErrorString(e));
+    }
+    if (rc == PBPIC_OK) {
+        PushParams P = push_params(c);
+        P.sp[0].qx = c->p.dx / c->p.dt;
+        P.sp[0].qy = c->p.dy / c->p.dt;
+        hipError_t e = launch_deposit_batch(P, c->p.precision, n, d_i1, d_i2, d_j1, d_j2, d_dx1,
+                                            d_dx2, d_dy1, d_dy2, d_q, d_vz, c->st);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+        if (e != hipSuccess) rc = fail(c, PBPIC_ERR_HIP, "deposit: %s", hipGetErrorString(e));
+    }
+    (void)hipFree(buf);
+    return rc;
+}
+
+int pbpic_fieldsolve(pbpic_ctx *c)
+{
+    if (!c) return PBPIC_ERR_ARG;
+    return fieldsolve_impl(c);
+}
+
+int pbpic_fieldsenergy(pbpic_ctx *c, double out[2])
+{
+    if (!c || !out) return PBPIC_ERR_ARG;
+    HIPCHK(c, launch_acc_op(c->acc, 2, 0.0, c->st));
+    HIPCHK(c, launch_energy(fdtd_params(c), c->st));
+    double a[ACC_N];
+    HIPCHK(c, hipMemcpyAsync(a, c->acc, sizeof a, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    const double cva2 = c->p.cva * c->p.cva;
+    out[0] = 0.5 * a[ACC_WE] * c->p.mime / (cva2 * cva2 * c->p.in0);   // :671
+    out[1] = 0.5 * a[ACC_WB] * c->p.mime / (cva2 * c->p.in0);          // :672
+    return PBPIC_OK;
+}
+
+int pbpic_countingsort_species(pbpic_ctx *c, int32_t s, int32_t *counts, int32_t *perm)
+{
+    int rc = valid_species(c, s);
+    if (rc) return rc;
+    return sort_one(c, s, counts, perm);
+}
+
+int pbpic_countingsort(pbpic_ctx *c)
+{
+    if (!c) return PBPIC_ERR_ARG;
+    for (int s = 0; s < (int)c->sp.size(); s++) {
+        int rc = sort_one(c, s, nullptr, nullptr);
+        if (rc) return rc;
+    }
+    return PBPIC_OK;
+}
+
+int pbpic_step(pbpic_ctx *c, int32_t nsteps)
+{
+    if (!c || nsteps < 0) return PBPIC_ERR_ARG;
+    for (int32_t k = 0; k < nsteps; k++) {
+        HIPCHK(c, prof_mark(c, PBPIC_PROF_ZERO, true));
+        HIPCHK(c, launch_zero_current(c->J[0], c->J[1], c->J[2], c->J[3], ncell(c), c->rsz, c->acc, 1, c->st));
+        HIPCHK(c, prof_mark(c, PBPIC_PROF_ZERO, false));
+        int rc = push_impl(c, false);
+        if (rc) return rc;
+        if ((rc = fieldsolve_impl(c))) return rc;
+        if (c->p.sort_every > 0 && c->steps % c->p.sort_every == 0)
+            if ((rc = pbpic_countingsort(c))) return rc;
+    }
+    return PBPIC_OK;
+}
+
+int pbpic_diag(pbpic_ctx *c, double out[PBPIC_NDIAG])
+{
+    if (!c || !out) return PBPIC_ERR_ARG;
+    int rc = check_flags(c);
+    if (rc) return rc;
+    for (int k = 0; k < PBPIC_NDIAG; k++) out[k] = 0;
+    const size_t nc = (size_t)ncell(c);
+    if (!c->rho_scratch) {
+        HIPCHK(c, dalloc(&c->rho_scratch, nc * 8));
+        HIPCHK(c, dalloc(&c->gauss0, nc * 8));
+        c->gauss_first = true;
+    }
+    HIPCHK(c, launch_acc_op(c->acc, 2, 0.0, c->st));
+    HIPCHK(c, launch_energy(fdtd_params(c), c->st));
+    const double cva2 = c->p.cva * c->p.cva;
+    HIPCHK(c, launch_gauss(push_params(c), c->p.precision, c->rho_scratch, c->gauss0,
+                           c->gauss_first ? 1 : 0, c->p.rho_background, cva2 * c->p.in0, c->st));
+    double a[ACC_N];
+    HIPCHK(c, hipMemcpyAsync(a, c->acc, sizeof a, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    c->gauss_first = false;
+    const double scale = c->p.mime / cva2;
+    out[PBPIC_D_KE_E] = a[ACC_KE_E];
+    out[PBPIC_D_KE_I] = a[ACC_KE_I];
+    out[PBPIC_D_WE] = 0.5 * a[ACC_WE] * c->p.mime / (cva2 * cva2 * c->p.in0);
+    out[PBPIC_D_WB] = 0.5 * a[ACC_WB] * c->p.mime / (cva2 * c->p.in0);
+    out[PBPIC_D_JE_STEP] = a[ACC_JE_STEP] * scale;
+    out[PBPIC_D_JE_INT] = a[ACC_JE_INT] + a[ACC_JE_STEP] * scale * c->p.dt;
+    out[PBPIC_D_GAUSS_L2] = sqrt(a[ACC_G_L2]);
+    out[PBPIC_D_GAUSS_LINF] = a[ACC_G_LINF];
+    out[PBPIC_D_GAUSS_DL2] = sqrt(a[ACC_G_DL2]);
+    out[PBPIC_D_GAUSS_DLINF] = a[ACC_G_DLINF];
+    out[PBPIC_D_STEPS] = (double)c->steps;
+    return PBPIC_OK;
+}
+
+int pbpic_species_device_ptrs(pbpic_ctx *c, int32_t s, void **ptrs)
+{
+    int rc = valid_species(c, s);
+    if (rc) return rc;
+    if (!ptrs) return PBPIC_ERR_ARG;
+    Species &S = c->sp[s];
+    const int b = S.cur;
+    ptrs[0] = S.cell[b]; ptrs[1] = S.ox[b]; ptrs[2] = S.oy[b];
+    ptrs[3] = S.ux[b]; ptrs[4] = S.uy[b]; ptrs[5] = S.uz[b]; ptrs[6] = S.pid[b];
+    return PBPIC_OK;
+}
+
+int pbpic_field_device_ptrs(pbpic_ctx *c, void **ptrs)
+{
+    if (!c || !ptrs) return PBPIC_ERR_ARG;
+    for (int k = 0; k < 3; k++) { ptrs[k] = c->E[k]; ptrs[3 + k] = c->B[k]; }
+    for (int k = 0; k < 4; k++) ptrs[6 + k] = c->J[k];
+    return PBPIC_OK;
+}
+
+void *pbpic_stream(pbpic_ctx *c) { return c ? (void *)c->st : nullptr; }
+
+int pbpic_profile(pbpic_ctx *c, int32_t enable)
+{
+    if (!c) return PBPIC_ERR_ARG;
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    c->prof = enable != 0;
+    if (c->prof) {
+        c->ev_used = 0;
+        c->ev_marks.clear();
+    }
+    return PBPIC_OK;
+}
+
+int pbpic_profile_read(pbpic_ctx *c, double ms[PBPIC_NPROF], int64_t count[PBPIC_NPROF])
+{
+    if (!c || !ms || !count) return PBPIC_ERR_ARG;
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    for (int k = 0; k < PBPIC_NPROF; k++) { ms[k] = 0; count[k] = 0; }
+    for (auto &m : c->ev_marks) {
+        float t = 0;
+        HIPCHK(c, hipEventElapsedTime(&t, c->ev_pool[m.second], c->ev_pool[m.second + 1]));
+        ms[m.first] += t;
+        count[m.first] += 1;
+    }
+    return PBPIC_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,446 @@
+// k_field.hip -- Yee/FDTD field solve and field-side reductions for gfx950.
+//
+// fieldsolver (pic2d.f90:678-860) on the periodic compact lattice: one fused,
+// LDS-staged stencil per 32x16 output tile.  E^n is staged with a 3-cell halo
+// and B^n with the same halo; the block then computes in LDS
+//   B^{n+1/2} on tile+2  (:688-690)
+//   E^{n+1}   on tile+1  (:742-744, current normalized by in0 as at :1066)
+//   B^{n+1}   on tile    (:801-803)
+// and writes E^{n+1}, B^{n+1} once: 9 planes read (E, B, J), 6 written.  The
+// curl terms the reference stores between steps are recomputed from E and B,
+// which yields the same values (they are formed from the same fields by the
+// same expressions, :733-736, :790-793).
+#include "pbpic_impl.h"
+
+#pragma clang fp contract(off)
+
+namespace pbpic {
+
+constexpr int FT_X = 32, FT_Y = 16, FT_H = 3;
+constexpr int FS_X = FT_X + 2 * FT_H, FS_Y = FT_Y + 2 * FT_H;   // 38 x 22
+constexpr int FS_N = FS_X * FS_Y;
+constexpr int FT_THREADS = 256;
+
+__device__ __forceinline__ int fwrap(int g, int n)
+{
+    while (g < 0) g += n;
+    while (g >= n) g -= n;
+    return g;
+}
+
+__device__ __forceinline__ int fxcd_remap(int b, int nb)
+{
+    const int q = nb >> 3, r = nb & 7, x = b & 7, k = b >> 3;
+    return x * q + (x < r ? x : r) + k;
+}
+
+// SPLIT: the stencil differences are taken on the FP64 state, rounded to FP32,
+// and the update increments are formed in FP32 and accumulated into FP64.
+template <typename RJ, bool SPLIT>
+__global__ __launch_bounds__(FT_THREADS) void fdtd_kernel(FdtdParams P)
+{
+    __shared__ double se[3][FS_N];
+    __shared__ double sb[3][FS_N];
+    __shared__ double jred[FT_THREADS / 64];
+    const int tid = threadIdx.x;
+    const int nx = P.nx, ny = P.ny;
+    const int ntx = (nx + FT_X - 1) / FT_X;
+    const int t = fxcd_remap(blockIdx.x, gridDim.x);
+    const int tx = t % ntx, ty = t / ntx;
+    const int x0 = tx * FT_X - FT_H, y0 = ty * FT_Y - FT_H;
+
+    for (int k = tid; k < FS_N; k += FT_THREADS) {
+        const int v = k / FS_X, u = k - v * FS_X;
+        const size_t g = (size_t)fwrap(y0 + v, ny) * nx + fwrap(x0 + u, nx);
+        se[0][k] = P.ex[g]; se[1][k] = P.ey[g]; se[2][k] = P.ez[g];
+        sb[0][k] = P.bx[g]; sb[1][k] = P.by[g]; sb[2][k] = P.bz[g];
+    }
+    __syncthreads();
+
+    const double dth = P.dth, idx = P.idx, idy = P.idy;
+    const float dthf = (float)P.dth, idxf = (float)P.idx, idyf = (float)P.idy;
+    // B^{n+1/2} on u in [1, FS_X-1), v in [1, FS_Y-1)
+    for (int k = tid; k < (FS_X - 2) * (FS_Y - 2); k += FT_THREADS) {
+        const int v = k / (FS_X - 2) + 1, u = k - (v - 1) * (FS_X - 2) + 1;
+        const int c = v * FS_X + u;
+        if (!SPLIT) {
+            sb[0][c] = sb[0][c] - dth * ((se[2][c] - se[2][c - FS_X]) * idy);
+            sb[1][c] = sb[1][c] + dth * ((se[2][c] - se[2][c - 1]) * idx);
+            sb[2][c] = sb[2][c] + dth * (((se[0][c + FS_X] - se[0][c]) * idy) -
+                                         ((se[1][c + 1] - se[1][c]) * idx));
+        } else {
+            const float dzy = (float)(se[2][c] - se[2][c - FS_X]);
+            const float dzx = (float)(se[2][c] - se[2][c - 1]);
+            const float dxy = (float)(se[0][c + FS_X] - se[0][c]);
+            const float dyx = (float)(se[1][c + 1] - se[1][c]);
+            sb[0][c] += (double)(-(dthf * (dzy * idyf)));
+            sb[1][c] += (double)(dthf * (dzx * idxf));
+            sb[2][c] += (double)(dthf * ((dxy * idyf) - (dyx * idxf)));
+        }
+    }
+    __syncthreads();
+
+    // E^{n+1} on u in [2, FS_X-2), v in [2, FS_Y-2); J.E on the owned tile
+    const double dtc2 = P.dtc2, in0 = P.in0;
+    const float dtc2f = (float)P.dtc2, in0f = (float)P.in0;
+    const RJ *jx = (const RJ *)P.jx, *jy = (const RJ *)P.jy, *jz = (const RJ *)P.jz;
+    double je = 0;
+    for (int k = tid; k < (FS_X - 4) * (FS_Y - 4); k += FT_THREADS) {
+        const int v = k / (FS_X - 4) + 2, u = k - (v - 2) * (FS_X - 4) + 2;
+        const int c = v * FS_X + u;
+        const int gx = fwrap(x0 + u, nx), gy = fwrap(y0 + v, ny);
+        const size_t g = (size_t)gy * nx + gx;
+        const double jxr = (double)jx[g], jyr = (double)jy[g], jzr = (double)jz[g];
+        const double exo = se[0][c], eyo = se[1][c], ezo = se[2][c];
+        double exn, eyn, ezn;
+        if (!SPLIT) {
+            exn = exo + dtc2 * (((sb[2][c] - sb[2][c - FS_X]) * idy) - jxr * in0);
+            eyn = eyo - dtc2 * (((sb[2][c] - sb[2][c - 1]) * idx) + jyr * in0);
+            ezn = ezo + dtc2 * ((((sb[1][c + 1] - sb[1][c]) * idx) -
+                                 ((sb[0][c + FS_X] - sb[0][c]) * idy)) - jzr * in0);
+        } else {
+            const float bzy = (float)(sb[2][c] - sb[2][c - FS_X]);
+            const float bzx = (float)(sb[2][c] - sb[2][c - 1]);
+            const float byx = (float)(sb[1][c + 1] - sb[1][c]);
+            const float bxy = (float)(sb[0][c + FS_X] - sb[0][c]);
+            exn = exo + (double)(dtc2f * ((bzy * idyf) - (float)jxr * in0f));
+            eyn = eyo + (double)(-(dtc2f * ((bzx * idxf) + (float)jyr * in0f)));
+            ezn = ezo + (double)(dtc2f * (((byx * idxf) - (bxy * idyf)) - (float)jzr * in0f));
+        }
+        se[0][c] = exn; se[1][c] = eyn; se[2][c] = ezn;
+        const bool own = u >= FT_H && u < FT_H + FT_X && v >= FT_H && v < FT_H + FT_Y &&
+                         x0 + u < nx && y0 + v < ny;
+        if (own)
+            je += jxr * 0.5 * (exo + exn) + jyr * 0.5 * (eyo + eyn) + jzr * 0.5 * (ezo + ezn);
+    }
+    __syncthreads();
+
+    // B^{n+1} and write-out on the owned tile
+    for (int k = tid; k < FT_X * FT_Y; k += FT_THREADS) {
+        const int vv = k / FT_X, uu = k - vv * FT_X;
+        const int u = uu + FT_H, v = vv + FT_H;
+        if (x0 + u >= nx || y0 + v >= ny) continue;
+        const int c = v * FS_X + u;
+        double bxn, byn, bzn;
+        if (!SPLIT) {
+            bxn = sb[0][c] - dth * ((se[2][c] - se[2][c - FS_X]) * idy);
+            byn = sb[1][c] + dth * ((se[2][c] - se[2][c - 1]) * idx);
+            bzn = sb[2][c] + dth * (((se[0][c + FS_X] - se[0][c]) * idy) -
+                                    ((se[1][c + 1] - se[1][c]) * idx));
+        } else {
+            const float dzy = (float)(se[2][c] - se[2][c - FS_X]);
+            const float dzx = (float)(se[2][c] - se[2][c - 1]);
+            const float dxy = (float)(se[0][c + FS_X] - se[0][c]);
+            const float dyx = (float)(se[1][c + 1] - se[1][c]);
+            bxn = sb[0][c] + (double)(-(dthf * (dzy * idyf)));
+            byn = sb[1][c] + (double)(dthf * (dzx * idxf));
+            bzn = sb[2][c] + (double)(dthf * ((dxy * idyf) - (dyx * idxf)));
+        }
+        const size_t g = (size_t)(y0 + v) * nx + (x0 + u);
+        P.ex[g] = se[0][c]; P.ey[g] = se[1][c]; P.ez[g] = se[2][c];
+        P.bx[g] = bxn; P.by[g] = byn; P.bz[g] = bzn;
+    }
+
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) je += __shfl_xor(je, o);
+    if ((tid & 63) == 0) jred[tid >> 6] = je;
+    __syncthreads();
+    if (tid == 0) unsafeAtomicAdd(P.acc + ACC_JE_STEP, (jred[0] + jred[1]) + (jred[2] + jred[3]));
+}
+
+hipError_t launch_fdtd(const FdtdParams &p, int precision, hipStream_t st)
+{
+    const int nb = ((p.nx + FT_X - 1) / FT_X) * ((p.ny + FT_Y - 1) / FT_Y);
+    if (precision == PBPIC_PREC_FP64)
+        hipLaunchKernelGGL((fdtd_kernel<double, false>), dim3(nb), dim3(FT_THREADS), 0, st, p);
+    else
+        hipLaunchKernelGGL((fdtd_kernel<float, true>), dim3(nb), dim3(FT_THREADS), 0, st, p);
+    return hipGetLastError();
+}
+
+// fieldsenergy (pic2d.f90:653-673): sums of squared node-averaged E and B
+// (emfgrid, :837-845) over all nodes, FP64.
+__global__ __launch_bounds__(256) void energy_kernel(FdtdParams P)
+{
+    __shared__ double red[2][4];
+    const int nx = P.nx, ny = P.ny;
+    const int64_t n = (int64_t)nx * ny;
+    double se = 0, sbv = 0;
+    for (int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x; k < n;
+         k += (int64_t)gridDim.x * 256) {
+        const int gy = (int)(k / nx), gx = (int)(k - (int64_t)gy * nx);
+        const int gxm = gx == 0 ? nx - 1 : gx - 1, gym = gy == 0 ? ny - 1 : gy - 1;
+        const size_t c = k, xm = (size_t)gy * nx + gxm, ym = (size_t)gym * nx + gx,
+                     xym = (size_t)gym * nx + gxm;
+        const double ex = (P.ex[c] + P.ex[xm]) * 0.5, ey = (P.ey[c] + P.ey[ym]) * 0.5;
+        const double ez = (P.ez[c] + P.ez[xm] + P.ez[ym] + P.ez[xym]) * 0.25;
+        const double bx = (P.bx[c] + P.bx[xm]) * 0.5, by = (P.by[c] + P.by[ym]) * 0.5;
+        const double bz = (P.bz[c] + P.bz[xm] + P.bz[ym] + P.bz[xym]) * 0.25;
+        se += ex * ex + ey * ey + ez * ez;
+        sbv += bx * bx + by * by + bz * bz;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        se += __shfl_xor(se, o);
+        sbv += __shfl_xor(sbv, o);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        red[0][threadIdx.x >> 6] = se;
+        red[1][threadIdx.x >> 6] = sbv;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsafeAtomicAdd(P.acc + ACC_WE, (red[0][0] + red[0][1]) + (red[0][2] + red[0][3]));
+        unsafeAtomicAdd(P.acc + ACC_WB, (red[1][0] + red[1][1]) + (red[1][2] + red[1][3]));
+    }
+}
+
+hipError_t launch_energy(const FdtdParams &p, hipStream_t st)
+{
+    const int64_t n = (int64_t)p.nx * p.ny;
+    int nb = (int)((n + 255) / 256);
+    if (nb > 4096) nb = 4096;
+    hipLaunchKernelGGL(energy_kernel, dim3(nb), dim3(256), 0, st, p);
+    return hipGetLastError();
+}
+
+// Gauss residual: rho by CIC with the gather's weights (:948-951) from the
+// current positions, then divE - c^2 in0 (rho + rho_bg) at every node.
+template <typename R>
+__global__ void rho_cic_kernel(PushParams P, SpeciesDev S, double *rho)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= S.n) return;
+    const int nx = P.nx, ny = P.ny;
+    const int c = S.cell[i];
+    if ((unsigned)c >= (unsigned)(nx * ny)) return;
+    const int cx = c / ny, cy = c - cx * ny;
+    const int cx1 = cx + 1 == nx ? 0 : cx + 1, cy1 = cy + 1 == ny ? 0 : cy + 1;
+    const double wx = 0.5 + (double)((const R *)S.ox)[i];
+    const double wy = 0.5 + (double)((const R *)S.oy)[i];
+    const double q = (double)S.q;
+    unsafeAtomicAdd(rho + (size_t)cy * nx + cx, q * (1 - wx) * (1 - wy));
+    unsafeAtomicAdd(rho + (size_t)cy * nx + cx1, q * wx * (1 - wy));
+    unsafeAtomicAdd(rho + (size_t)cy1 * nx + cx, q * (1 - wx) * wy);
+    unsafeAtomicAdd(rho + (size_t)cy1 * nx + cx1, q * wx * wy);
+}
+
+__global__ __launch_bounds__(256) void gauss_kernel(PushParams P, const double *rho,
+                                                    double *g0, int first, double rho_bg,
+                                                    double cva2in0)
+{
+    __shared__ double red[2][4];
+    __shared__ unsigned long long mx[2][4];
+    const int nx = P.nx, ny = P.ny;
+    const int64_t n = (int64_t)nx * ny;
+    double l2 = 0, dl2 = 0, li = 0, dli = 0;
+    for (int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x; k < n;
+         k += (int64_t)gridDim.x * 256) {
+        const int gy = (int)(k / nx), gx = (int)(k - (int64_t)gy * nx);
+        const int gxm = gx == 0 ? nx - 1 : gx - 1, gym = gy == 0 ? ny - 1 : gy - 1;
+        const double div = (P.ex[k] - P.ex[(size_t)gy * nx + gxm]) * P.idx +
+                           (P.ey[k] - P.ey[(size_t)gym * nx + gx]) * P.idy;
+        const double res = div - cva2in0 * (rho[k] + rho_bg);
+        if (first) g0[k] = res;
+        const double d = res - g0[k];
+        l2 += res * res;
+        dl2 += d * d;
+        li = fmax(li, fabs(res));
+        dli = fmax(dli, fabs(d));
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        l2 += __shfl_xor(l2, o);
+        dl2 += __shfl_xor(dl2, o);
+        li = fmax(li, __shfl_xor(li, o));
+        dli = fmax(dli, __shfl_xor(dli, o));
+    }
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        red[0][w] = l2; red[1][w] = dl2;
+        mx[0][w] = (unsigned long long)__double_as_longlong(li);
+        mx[1][w] = (unsigned long long)__double_as_longlong(dli);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsafeAtomicAdd(P.acc + ACC_G_L2, (red[0][0] + red[0][1]) + (red[0][2] + red[0][3]));
+        unsafeAtomicAdd(P.acc + ACC_G_DL2, (red[1][0] + red[1][1]) + (red[1][2] + red[1][3]));
+        unsigned long long a = mx[0][0], b = mx[1][0];
+        for (int i = 1; i < 4; i++) { a = a > mx[0][i] ? a : mx[0][i]; b = b > mx[1][i] ? b : mx[1][i]; }
+        atomicMax((unsigned long long *)(P.acc + ACC_G_LINF), a);
+        atomicMax((unsigned long long *)(P.acc + ACC_G_DLINF), b);
+    }
+}
+
+__global__ void zero_f64_kernel(double *p, int64_t n)
+{
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
+         k += (int64_t)gridDim.x * blockDim.x)
+        p[k] = 0.0;
+}
+
+hipError_t launch_gauss(const PushParams &p, int precision, double *rho_scratch,
+                        double *gauss0, int first, double rho_bg, double cva2in0,
+                        hipStream_t st)
+{
+    const int64_t n = (int64_t)p.nx * p.ny;
+    hipLaunchKernelGGL(zero_f64_kernel, dim3(1024), dim3(256), 0, st, rho_scratch, n);
+    for (int s = 0; s < p.nspecies; s++) {
+        if (p.sp[s].n == 0) continue;
+        const int nb = (int)((p.sp[s].n + 255) / 256);
+        if (precision == PBPIC_PREC_FP64)
+            hipLaunchKernelGGL(rho_cic_kernel<double>, dim3(nb), dim3(256), 0, st, p, p.sp[s], rho_scratch);
+        else
+            hipLaunchKernelGGL(rho_cic_kernel<float>, dim3(nb), dim3(256), 0, st, p, p.sp[s], rho_scratch);
+    }
+    int nb = (int)((n + 255) / 256);
+    if (nb > 4096) nb = 4096;
+    hipLaunchKernelGGL(gauss_kernel, dim3(nb), dim3(256), 0, st, p, rho_scratch, gauss0,
+                       first, rho_bg, cva2in0);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- utilities
+__global__ void fill_u32_kernel(uint32_t *p, int64_t n, uint32_t v)
+{
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
+         k += (int64_t)gridDim.x * blockDim.x)
+        p[k] = v;
+}
+
+hipError_t launch_fill_u32(uint32_t *p, int64_t n, uint32_t v, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    int64_t nb = (n + 255) / 256;
+    if (nb > 8192) nb = 8192;
+    hipLaunchKernelGGL(fill_u32_kernel, dim3((unsigned)nb), dim3(256), 0, st, p, n, v);
+    return hipGetLastError();
+}
+
+// denjr = 0 (pic2d.f90:173-176); optionally resets the kinetic-energy tally
+// (totene(1:2,itime) starts from zero each step).
+template <typename R>
+__global__ void zero_current_kernel(R *a, R *b, R *c, R *d, int64_t n, double *acc, int reset)
+{
+    const int64_t k0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k0 == 0 && reset) { acc[ACC_KE_E] = 0; acc[ACC_KE_I] = 0; }
+    for (int64_t k = k0; k < n; k += (int64_t)gridDim.x * blockDim.x) {
+        a[k] = (R)0; b[k] = (R)0; c[k] = (R)0;
+        if (d) d[k] = (R)0;
+    }
+}
+
+hipError_t launch_zero_current(void *jx, void *jy, void *jz, void *rho, int64_t ncell,
+                               int elem_bytes, double *acc, int reset_ke, hipStream_t st)
+{
+    int64_t nb = (ncell + 255) / 256;
+    if (nb > 8192) nb = 8192;
+    if (nb < 1) nb = 1;
+    if (elem_bytes == 8)
+        hipLaunchKernelGGL(zero_current_kernel<double>, dim3((unsigned)nb), dim3(256), 0, st,
+                           (double *)jx, (double *)jy, (double *)jz, (double *)rho, ncell, acc, reset_ke);
+    else
+        hipLaunchKernelGGL(zero_current_kernel<float>, dim3((unsigned)nb), dim3(256), 0, st,
+                           (float *)jx, (float *)jy, (float *)jz, (float *)rho, ncell, acc, reset_ke);
+    return hipGetLastError();
+}
+
+// Accumulator maintenance.  op 0: reset the kinetic-energy tally.  op 1: fold
+// the last J.E sum into its time integral (scale a = dt*mime/cva^2) and clear
+// it.  op 2: clear the energy and Gauss slots.
+__global__ void acc_op_kernel(double *acc, int op, double a)
+{
+    if (threadIdx.x != 0) return;
+    if (op == 0) { acc[ACC_KE_E] = 0; acc[ACC_KE_I] = 0; }
+    if (op == 1) { acc[ACC_JE_INT] += acc[ACC_JE_STEP] * a; acc[ACC_JE_STEP] = 0; }
+    if (op == 2) {
+        acc[ACC_WE] = 0; acc[ACC_WB] = 0; acc[ACC_G_L2] = 0; acc[ACC_G_LINF] = 0;
+        acc[ACC_G_DL2] = 0; acc[ACC_G_DLINF] = 0;
+    }
+}
+
+hipError_t launch_acc_op(double *acc, int op, double a, hipStream_t st)
+{
+    hipLaunchKernelGGL(acc_op_kernel, dim3(1), dim3(64), 0, st, acc, op, a);
+    return hipGetLastError();
+}
+
+// Seeded Maxwellian loader: Philox4x32-10 (Salmon et al., SC'11) counter
+// (p, stream, 0|1) under key seed, the same words as the oracle's loader.
+__device__ __forceinline__ void philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
+                                       uint32_t k0, uint32_t k1, uint32_t o[4])
+{
+    const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u, W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
+#pragma unroll
+    for (int r = 0; r < 10; r++) {
+        const uint64_t p0 = (uint64_t)M0 * c0, p1 = (uint64_t)M1 * c2;
+        const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+        const uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+        const uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
+        c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+        k0 += W0; k1 += W1;
+    }
+    o[0] = c0; o[1] = c1; o[2] = c2; o[3] = c3;
+}
+
+__device__ __forceinline__ double u01(uint32_t u) { return ((double)u + 0.5) * 2.3283064365386963e-10; }
+
+template <typename R>
+__global__ void init_maxwellian_kernel(SpeciesDev S, int ppc, double uth, double ud,
+                                       int counter_stream, uint32_t k0, uint32_t k1,
+                                       uint32_t stream)
+{
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= S.n) return;
+    uint32_t a[4], b[4];
+    philox((uint32_t)p, (uint32_t)((uint64_t)p >> 32), stream, 0u, k0, k1, a);
+    philox((uint32_t)p, (uint32_t)((uint64_t)p >> 32), stream, 1u, k0, k1, b);
+    const double twopi = 6.283185307179586;
+    const double r1 = sqrt(-2.0 * log(u01(a[2]))), t1 = twopi * u01(a[3]);
+    const double r2 = sqrt(-2.0 * log(u01(b[0]))), t2 = twopi * u01(b[1]);
+    const double drift = (counter_stream && ((p % ppc) & 1)) ? -ud : ud;
+    S.cell[p] = (int32_t)(p / ppc);
+    ((R *)S.ox)[p] = (R)(u01(a[0]) - 0.5);
+    ((R *)S.oy)[p] = (R)(u01(a[1]) - 0.5);
+    ((R *)S.ux)[p] = (R)(uth * (r1 * cos(t1)) + drift);
+    ((R *)S.uy)[p] = (R)(uth * (r1 * sin(t1)));
+    ((R *)S.uz)[p] = (R)(uth * (r2 * cos(t2)));
+    if (S.pid) S.pid[p] = (int32_t)(p + 1);
+}
+
+hipError_t launch_init_maxwellian(const SpeciesDev &s, int precision, int nx, int ny, int ppc,
+                                  double vth, double vdrift, int counter_stream, uint64_t seed,
+                                  uint32_t stream, hipStream_t st)
+{
+    (void)nx; (void)ny;
+    if (s.n <= 0) return hipSuccess;
+    const double uth = vth / sqrt(1.0 - vth * vth);
+    const double ud = vdrift / sqrt(1.0 - vdrift * vdrift);
+    const int64_t nb = (s.n + 255) / 256;
+    if (precision == PBPIC_PREC_FP64)
+        hipLaunchKernelGGL(init_maxwellian_kernel<double>, dim3((unsigned)nb), dim3(256), 0, st,
+                           s, ppc, uth, ud, counter_stream, (uint32_t)seed,
+                           (uint32_t)(seed >> 32), stream);
+    else
+        hipLaunchKernelGGL(init_maxwellian_kernel<float>, dim3((unsigned)nb), dim3(256), 0, st,
+                           s, ppc, uth, ud, counter_stream, (uint32_t)seed,
+                           (uint32_t)(seed >> 32), stream);
+    return hipGetLastError();
+}
+
+__global__ void check_cells_kernel(const int32_t *cell, int64_t n, int64_t ncell, int *err)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && (cell[i] < 0 || cell[i] >= ncell)) atomicOr(err, ERRF_CELL);
+}
+
+hipError_t launch_check_cells(const int32_t *cell, int64_t n, int64_t ncell, int *err,
+                              hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(check_cells_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
+                       cell, n, ncell, err);
+    return hipGetLastError();
+}
+
+}  // namespace pbpic

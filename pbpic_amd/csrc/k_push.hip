@@ -1,0 +1,576 @@
+// k_push.hip -- fused gather + Boris push + move + charge-conserving deposit
+// for gfx950 (the `pusher` loop body of pic2d.f90:934-1062 with the
+// `currentdeposit` call of :1029 and the wrap/fold/halo of :1032-1216).
+//
+// Work decomposition.  Particles are kept sorted by cell (countingsort), so a
+// contiguous tile of PUSH_TILE particles covers a compact patch of cells.  A
+// block of 256 threads takes one tile:
+//   A. loads the tile's cell ids and reduces their periodic-unwrapped bounding
+//      box;
+//   B. stages the node-averaged E and B of that box (+1 node) into LDS and
+//      zeroes an LDS current window (box grown by 1 cell below and 2 above,
+//      which holds every deposit target of a particle that moves <= 1 cell);
+//   C. pushes every particle, gathering from LDS and depositing with LDS
+//      atomics;
+//   D. flushes the LDS window to the global current with one atomic per
+//      non-zero cell value.
+// Tiles whose box does not fit the LDS window (unsorted input) fall back to
+// global gathers and global atomics.
+//
+// Lane mapping: the 256 threads form 32 groups of 8 lanes; group G owns the
+// run [G*128, G*128+128) of the tile and walks it 8 particles per round.  The
+// 8 groups of a wave therefore sit 128 particles apart (usually in different
+// cells), which keeps same-address LDS atomic conflicts at about 8-way, while
+// each group still reads 8 consecutive elements (64 B in FP64) per load.
+//
+// Precision.  FP64 reproduces the reference arithmetic operation for
+// operation (no FMA contraction), including the absolute-index form of the
+// zigzag relay point (:1239-1252), so one step agrees with the oracle to
+// rounding of the current sums.  SPLIT32 keeps int cell + float offset and
+// evaluates the same scheme in offsets relative to the old cell.
+#include "pbpic_impl.h"
+
+#pragma clang fp contract(off)
+
+namespace pbpic {
+
+constexpr int PUSH_THREADS = 256;
+constexpr int PUSH_ROUNDS = 16;
+constexpr int PUSH_GROUP = 8;
+constexpr int PUSH_RUN = PUSH_ROUNDS * PUSH_GROUP;        // 128
+constexpr int PUSH_TILE = PUSH_THREADS * PUSH_ROUNDS;     // 4096
+
+__device__ __forceinline__ int xcd_remap(int b, int nb)
+{
+    // Workgroups are dispatched round-robin over the 8 XCDs; give each XCD a
+    // contiguous range of tiles so halo atomics stay in one L2.
+    const int q = nb >> 3, r = nb & 7, x = b & 7, k = b >> 3;
+    return x * q + (x < r ? x : r) + k;
+}
+
+__device__ __forceinline__ int wrapi(int g, int n)
+{
+    while (g < 0) g += n;
+    while (g >= n) g -= n;
+    return g;
+}
+
+template <typename R>
+__device__ __forceinline__ void lds_add(R *p, R v)
+{
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ void gadd(double *p, double v) { unsafeAtomicAdd(p, v); }
+__device__ __forceinline__ void gadd(float *p, float v) { unsafeAtomicAdd(p, v); }
+
+// Node-averaged field at compact node (gx, gy): the emfgrid formulas of
+// pic2d.f90:837-845, evaluated in FP64 in the reference's order.
+__device__ __forceinline__ void node_fields(const PushParams &P, int gx, int gy,
+                                            double f[6])
+{
+    const int nx = P.nx, ny = P.ny;
+    const int gxm = gx == 0 ? nx - 1 : gx - 1;
+    const int gym = gy == 0 ? ny - 1 : gy - 1;
+    const size_t c = (size_t)gy * nx + gx, xm = (size_t)gy * nx + gxm;
+    const size_t ym = (size_t)gym * nx + gx, xym = (size_t)gym * nx + gxm;
+    f[0] = (P.ex[c] + P.ex[xm]) * 0.5;
+    f[1] = (P.ey[c] + P.ey[ym]) * 0.5;
+    f[2] = (P.ez[c] + P.ez[xm] + P.ez[ym] + P.ez[xym]) * 0.25;
+    f[3] = (P.bx[c] + P.bx[xm]) * 0.5;
+    f[4] = (P.by[c] + P.by[ym]) * 0.5;
+    f[5] = (P.bz[c] + P.bz[xm] + P.bz[ym] + P.bz[xym]) * 0.25;
+}
+
+// Deposit contributions of one particle, relative to its old cell (0,0).
+template <typename R>
+struct Dep {
+    R jxa0, jxa1, jya0, jya1;  // segment 1: jx(0,0),(0,1); jy(0,0),(1,0)
+    R jxb0, jxb1, jyb0, jyb1;  // segment 2: jx(a,b),(a,b+1); jy(a,b),(a+1,b)
+    R jz0, jz1, jz2, jz3;      // jz(0,0),(di,0),(0,dj),(di,dj)
+    R r0, r1, r2, r3;          // rho(0,0),(1,0),(0,1),(1,1)
+    int a, b, di, dj;
+};
+
+__device__ __forceinline__ double sgn1(double x) { return signbit(x) ? -1.0 : 1.0; }
+__device__ __forceinline__ float sgn1(float x) { return signbit(x) ? -1.0f : 1.0f; }
+template <typename R> __device__ __forceinline__ R rmin(R a, R b) { return b < a ? b : a; }
+template <typename R> __device__ __forceinline__ R rmax(R a, R b) { return b > a ? b : a; }
+
+// currentdeposit (pic2d.f90:1226-1296).  i1, j1 = old 1-based cell, a, b =
+// cell displacement (i2 = i1 + a).  SPLIT: offsets relative to i1, j1.
+template <typename R, bool SPLIT>
+__device__ __forceinline__ void deposit_terms(int i1, int j1, int a, int b,
+                                              R dx1, R dx2, R dy1, R dy2, R qx,
+                                              R qy, R qr, R vz, int compat,
+                                              bool want_rho, Dep<R> &d)
+{
+    const R H = (R)0.5, ONE = (R)1;
+    R irr, jrr, fx1, fy1, fx2, fy2, wx1, wy1, wx2, wy2, dimid, djmid;
+    if (!SPLIT) {
+        const int i2 = i1 + a, j2 = j1 + b;
+        const R im = ((R)(i1 + i2) + dx1 + dx2 + ONE) * H;                 // :1239
+        const R jm = ((R)(j1 + j2) + dy1 + dy2 + ONE) * H;
+        const R ir = rmin((R)(i1 < i2 ? i1 : i2) + ONE,
+                          rmax((R)(i1 > i2 ? i1 : i2) + (R)0, im));       // :1241
+        const R jr = rmin((R)(j1 < j2 ? j1 : j2) + ONE,
+                          rmax((R)(j1 > j2 ? j1 : j2) + (R)0, jm));
+        fx1 = qx * (ir - (R)i1 - dx1 - H);                                   // :1244
+        fy1 = qy * (jr - (R)j1 - dy1 - H);
+        fx2 = qx * ((R)i2 + dx2 + H - ir);
+        fy2 = qy * ((R)j2 + dy2 + H - jr);
+        wx1 = (ir - (R)i1 + dx1 + H) * H;                                    // :1249
+        wy1 = (jr - (R)j1 + dy1 + H) * H;
+        wx2 = (ir - (R)i2 + dx2 + H) * H;
+        wy2 = (jr - (R)j2 + dy2 + H) * H;
+        dimid = im - (R)i1 - H;                                              // :1263
+        djmid = jm - (R)j1 - H;
+    } else {
+        const R imr = ((R)a + dx1 + dx2 + ONE) * H;      // im - i1
+        const R jmr = ((R)b + dy1 + dy2 + ONE) * H;
+        irr = rmin((R)(a < 0 ? a : 0) + ONE, rmax((R)(a > 0 ? a : 0), imr));
+        jrr = rmin((R)(b < 0 ? b : 0) + ONE, rmax((R)(b > 0 ? b : 0), jmr));
+        fx1 = qx * (irr - dx1 - H);
+        fy1 = qy * (jrr - dy1 - H);
+        fx2 = qx * ((R)a + dx2 + H - irr);
+        fy2 = qy * ((R)b + dy2 + H - jrr);
+        wx1 = (irr + dx1 + H) * H;
+        wy1 = (jrr + dy1 + H) * H;
+        wx2 = (irr - (R)a + dx2 + H) * H;
+        wy2 = (jrr - (R)b + dy2 + H) * H;
+        dimid = imr - H;
+        djmid = jmr - H;
+    }
+    d.a = a;
+    d.b = b;
+    d.jxa0 = fx1 * (ONE - wy1);                                              // :1254
+    d.jxa1 = fx1 * wy1;
+    d.jya0 = fy1 * (ONE - wx1);
+    d.jya1 = fy1 * wx1;
+    d.jxb0 = fx2 * (ONE - wy2);
+    d.jxb1 = fx2 * wy2;
+    d.jyb0 = fy2 * (ONE - wx2);
+    d.jyb1 = fy2 * wx2;
+    d.di = (int)sgn1(dimid);                                                 // :1266
+    d.dj = (int)sgn1(djmid);
+    dimid = fabs(dimid);
+    djmid = fabs(djmid);
+    R s1 = dimid * djmid;                                                    // :1272
+    R s2 = dimid - s1;
+    R s3 = djmid - s1;
+    R s4 = ONE - s1 - s2 - s3;
+    const R quz = qr * vz;                                                   // :1277
+    d.jz0 = quz * s4;
+    d.jz1 = quz * s2;
+    d.jz2 = quz * s3;
+    d.jz3 = quz * s1;
+    if (want_rho) {
+        if (compat & PBPIC_COMPAT_RHO_MIRROR) {                              // :1285
+            s1 = (H - dx1) * (H - dy1);
+            s2 = H - dx1 - s1;
+            s3 = H - dy1 - s1;
+        } else {                      // the gather's CIC weights (:948-951)
+            s1 = (H + dx1) * (H + dy1);
+            s2 = H + dx1 - s1;
+            s3 = H + dy1 - s1;
+        }
+        s4 = ONE - s1 - s2 - s3;
+        d.r0 = s4 * qr;
+        d.r1 = s2 * qr;
+        d.r2 = s3 * qr;
+        d.r3 = s1 * qr;
+    }
+}
+
+// Gather + Boris + move for one particle (pic2d.f90:948-1028).  n00..n11 are
+// the 6-component node fields at (ix,iy), (ix+1,iy), (ix,iy+1), (ix+1,iy+1).
+template <typename R>
+__device__ __forceinline__ void push_math(const PushParams &P, R c0, R c1,
+                                          R &ox, R &oy, R &ux, R &uy, R &uz,
+                                          const R *n00, const R *n10,
+                                          const R *n01, const R *n11, int &dcx,
+                                          int &dcy, R &vz, R &gam, int &err)
+{
+    const R H = (R)0.5, ONE = (R)1;
+    const R s1 = (H + ox) * (H + oy);                                        // :948
+    const R s2 = H + ox - s1;
+    const R s3 = H + oy - s1;
+    const R s4 = ONE - s1 - s2 - s3;
+    R f[6];
+#pragma unroll
+    for (int k = 0; k < 6; k++)
+        f[k] = n00[k] * s4 + n10[k] * s2 + n01[k] * s3 + n11[k] * s1;        // :952
+    const R umx = ux + c1 * f[0];                                            // :971
+    const R umy = uy + c1 * f[1];
+    const R umz = uz + c1 * f[2];
+    R gama = sqrt(ONE + umx * umx + umy * umy + umz * umz);
+    const R c2 = c0 / gama;                                                  // :976
+    const R tx = c2 * f[3], ty = c2 * f[4], tz = c2 * f[5];
+    const R t2 = (R)2 / (ONE + tx * tx + ty * ty + tz * tz);
+    const R sx = t2 * tx, sy = t2 * ty, sz = t2 * tz;
+    R u0x, u0y, u0z;
+    if (P.compat & PBPIC_COMPAT_BORIS_OLD_U) {                              // :985
+        u0x = umx + (uy * tz - uz * ty);
+        u0y = umy + (uz * tx - ux * tz);
+        u0z = umz + (ux * ty - uy * tx);
+    } else {
+        u0x = umx + (umy * tz - umz * ty);
+        u0y = umy + (umz * tx - umx * tz);
+        u0z = umz + (umx * ty - umy * tx);
+    }
+    const R upx = umx + (u0y * sz - u0z * sy);                               // :989
+    const R upy = umy + (u0z * sx - u0x * sz);
+    const R upz = umz + (u0x * sy - u0y * sx);
+    ux = upx + c1 * f[0];                                                    // :993
+    uy = upy + c1 * f[1];
+    uz = upz + c1 * f[2];
+    gama = sqrt(ONE + ux * ux + uy * uy + uz * uz);
+    const R c3 = (R)P.dtcva / gama;                                          // :1003
+    const R dx1 = ox + c3 * ux * (R)P.idx;
+    const R dy1 = oy + c3 * uy * (R)P.idy;
+    if (fabs(dx1) > H) {                                                     // :1008
+        dcx = (int)sgn1(dx1);
+        ox = dx1 + sgn1(-dx1);
+    } else {
+        dcx = 0;
+        ox = dx1;
+    }
+    if (fabs(dy1) > H) {                                                     // :1017
+        dcy = (int)sgn1(dy1);
+        oy = dy1 + sgn1(-dy1);
+    } else {
+        dcy = 0;
+        oy = dy1;
+    }
+    if (fabs(ox) > H || fabs(oy) > H) err |= ERRF_CFL;
+    if (!isfinite(gama)) err |= ERRF_NONFINITE;
+    vz = uz * (R)P.cva / gama;                                               // :1028
+    gam = gama;
+}
+
+template <typename R>
+__device__ __forceinline__ R ldg_r(const void *p, int64_t i) { return ((const R *)p)[i]; }
+
+template <typename R, bool SPLIT>
+__global__ __launch_bounds__(PUSH_THREADS, 2) void push_kernel(PushParams P)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ int red[4][4];
+    __shared__ double kred[4];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int t = xcd_remap(blockIdx.x, gridDim.x);
+    int s = 0;
+    while (s + 1 < P.nspecies && t >= P.sp[s + 1].tile0) s++;
+    const SpeciesDev S = P.sp[s];
+    const int64_t tstart = (int64_t)(t - S.tile0) * PUSH_TILE;
+    const int G = wave * 8 + (lane >> 3), gp = lane & 7;
+    const int64_t pbase = tstart + (int64_t)G * PUSH_RUN + gp;
+    const int nx = P.nx, ny = P.ny, nxh = P.nx >> 1, nyh = P.ny >> 1;
+
+    // ---- A: bounding box of the tile's cells, unwrapped around its first
+    const int c00 = S.cell[tstart];
+    const bool c00ok = (unsigned)c00 < (unsigned)(nx * ny);
+    const int x0 = c00ok ? (int)P.dny.div((uint32_t)c00) : 0;
+    const int y0 = c00ok ? c00 - x0 * ny : 0;
+    int cl[PUSH_ROUNDS];
+    int mnx = 1 << 30, mxx = -(1 << 30), mny = 1 << 30, mxy = -(1 << 30);
+    int err = 0;
+#pragma unroll
+    for (int r = 0; r < PUSH_ROUNDS; r++) {
+        const int64_t i = pbase + r * PUSH_GROUP;
+        cl[r] = -1;
+        if (i < S.n) {
+            int c = S.cell[i];
+            if ((unsigned)c >= (unsigned)(nx * ny)) { err |= ERRF_CELL; continue; }
+            cl[r] = c;
+            const int cx = (int)P.dny.div((uint32_t)c), cy = c - cx * ny;
+            int rx = cx - x0, ry = cy - y0;
+            if (rx > nxh) rx -= nx; else if (rx < -nxh) rx += nx;
+            if (ry > nyh) ry -= ny; else if (ry < -nyh) ry += ny;
+            mnx = min(mnx, rx); mxx = max(mxx, rx);
+            mny = min(mny, ry); mxy = max(mxy, ry);
+        }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        mnx = min(mnx, __shfl_xor(mnx, o));
+        mxx = max(mxx, __shfl_xor(mxx, o));
+        mny = min(mny, __shfl_xor(mny, o));
+        mxy = max(mxy, __shfl_xor(mxy, o));
+    }
+    if (lane == 0) { red[wave][0] = mnx; red[wave][1] = mxx; red[wave][2] = mny; red[wave][3] = mxy; }
+    __syncthreads();
+    mnx = min(min(red[0][0], red[1][0]), min(red[2][0], red[3][0]));
+    mxx = max(max(red[0][1], red[1][1]), max(red[2][1], red[3][1]));
+    mny = min(min(red[0][2], red[1][2]), min(red[2][2], red[3][2]));
+    mxy = max(max(red[0][3], red[1][3]), max(red[2][3], red[3][3]));
+    const int wx0 = mnx - 1, wy0 = mny - 1;
+    const int W = mxx - mnx + 4, H = mxy - mny + 4;
+    const bool inlds = mnx <= mxx && (int64_t)W * H <= P.maxw;
+    const int WH = inlds ? W * H : 0;
+    const int njc = P.deposit_rho ? 4 : 3;
+    R *nf = (R *)smem;                       // [WH][6] node fields
+    R *jw = nf + (size_t)6 * P.maxw;         // [njc][WH] current window
+
+    // ---- B: stage node fields, zero the current window
+    if (inlds) {
+        for (int k = tid; k < WH; k += PUSH_THREADS) {
+            const int wj = k / W, wi = k - wj * W;
+            const int gx = wrapi(x0 + wx0 + wi, nx), gy = wrapi(y0 + wy0 + wj, ny);
+            double f[6];
+            node_fields(P, gx, gy, f);
+#pragma unroll
+            for (int q = 0; q < 6; q++) nf[(size_t)k * 6 + q] = (R)f[q];
+        }
+        for (int k = tid; k < njc * WH; k += PUSH_THREADS) jw[k] = (R)0;
+    }
+    __syncthreads();
+
+    // ---- C: push
+    const R c0 = (R)S.c0, c1 = (R)S.c1, qx = (R)S.qx, qy = (R)S.qy, qr = (R)S.q;
+    R *ox_p = (R *)S.ox, *oy_p = (R *)S.oy, *ux_p = (R *)S.ux, *uy_p = (R *)S.uy,
+      *uz_p = (R *)S.uz;
+    double ke = 0;
+    R *gjx = (R *)P.jx, *gjy = (R *)P.jy, *gjz = (R *)P.jz, *grho = (R *)P.rho;
+    const bool want_rho = P.deposit_rho != 0;
+#pragma unroll 1
+    for (int r = 0; r < PUSH_ROUNDS; r++) {
+        const int c = cl[r];
+        if (c < 0) continue;
+        const int64_t i = pbase + r * PUSH_GROUP;
+        R ox = ox_p[i], oy = oy_p[i], ux = ux_p[i], uy = uy_p[i], uz = uz_p[i];
+        const int cx = (int)P.dny.div((uint32_t)c), cy = c - cx * ny;
+        int rx = cx - x0, ry = cy - y0;
+        if (rx > nxh) rx -= nx; else if (rx < -nxh) rx += nx;
+        if (ry > nyh) ry -= ny; else if (ry < -nyh) ry += ny;
+        const int lx = rx - wx0, ly = ry - wy0;
+        R n00[6], n10[6], n01[6], n11[6];
+        if (inlds) {
+            const R *b = nf + (size_t)(ly * W + lx) * 6;
+#pragma unroll
+            for (int q = 0; q < 6; q++) {
+                n00[q] = b[q];
+                n10[q] = b[6 + q];
+                n01[q] = b[(size_t)W * 6 + q];
+                n11[q] = b[(size_t)W * 6 + 6 + q];
+            }
+        } else {
+            const int cx1 = cx + 1 == nx ? 0 : cx + 1, cy1 = cy + 1 == ny ? 0 : cy + 1;
+            double f[6];
+            node_fields(P, cx, cy, f);
+            for (int q = 0; q < 6; q++) n00[q] = (R)f[q];
+            node_fields(P, cx1, cy, f);
+            for (int q = 0; q < 6; q++) n10[q] = (R)f[q];
+            node_fields(P, cx, cy1, f);
+            for (int q = 0; q < 6; q++) n01[q] = (R)f[q];
+            node_fields(P, cx1, cy1, f);
+            for (int q = 0; q < 6; q++) n11[q] = (R)f[q];
+        }
+        const R dx0 = ox, dy0 = oy;
+        int dcx, dcy;
+        R vz, gam;
+        push_math<R>(P, c0, c1, ox, oy, ux, uy, uz, n00, n10, n01, n11, dcx, dcy,
+                     vz, gam, err);
+        Dep<R> d;
+        deposit_terms<R, SPLIT>(cx + 1, cy + 1, dcx, dcy, dx0, ox, dy0, oy, qx, qy,
+                                qr, vz, P.compat, want_rho, d);
+        // jy at column mx+2 is lost by the reference's fold (:1156, :1159)
+        if ((P.compat & PBPIC_COMPAT_JY_DROP) && cx == nx - 1 && dcx == 1) d.jyb1 = (R)0;
+        if (d.a == 0 && d.b == 0) {
+            d.jxa0 += d.jxb0; d.jxa1 += d.jxb1; d.jya0 += d.jyb0; d.jya1 += d.jyb1;
+        }
+        if (inlds) {
+            const int base = ly * W + lx;
+            R *wx = jw, *wy = jw + WH, *wz = jw + 2 * WH;
+            lds_add(wx + base, d.jxa0);
+            lds_add(wx + base + W, d.jxa1);
+            lds_add(wy + base, d.jya0);
+            lds_add(wy + base + 1, d.jya1);
+            if (d.a != 0 || d.b != 0) {
+                const int bb = base + d.b * W + d.a;
+                lds_add(wx + bb, d.jxb0);
+                lds_add(wx + bb + W, d.jxb1);
+                lds_add(wy + bb, d.jyb0);
+                lds_add(wy + bb + 1, d.jyb1);
+            }
+            lds_add(wz + base, d.jz0);
+            lds_add(wz + base + d.di, d.jz1);
+            lds_add(wz + base + d.dj * W, d.jz2);
+            lds_add(wz + base + d.dj * W + d.di, d.jz3);
+            if (want_rho) {
+                R *wr = jw + 3 * WH;
+                lds_add(wr + base, d.r0);
+                lds_add(wr + base + 1, d.r1);
+                lds_add(wr + base + W, d.r2);
+                lds_add(wr + base + W + 1, d.r3);
+            }
+        } else {
+            const int xm = cx == 0 ? nx - 1 : cx - 1, xp = cx + 1 == nx ? 0 : cx + 1;
+            const int ym = cy == 0 ? ny - 1 : cy - 1, yp = cy + 1 == ny ? 0 : cy + 1;
+            const int xs[3] = {xm, cx, xp}, ys[3] = {ym, cy, yp};
+            auto at = [&](int ddx, int ddy) -> size_t {
+                int gx = ddx == 2 ? (xp + 1 == nx ? 0 : xp + 1) : xs[ddx + 1];
+                int gy = ddy == 2 ? (yp + 1 == ny ? 0 : yp + 1) : ys[ddy + 1];
+                return (size_t)gy * nx + gx;
+            };
+            gadd(gjx + at(0, 0), d.jxa0);
+            gadd(gjx + at(0, 1), d.jxa1);
+            gadd(gjy + at(0, 0), d.jya0);
+            gadd(gjy + at(1, 0), d.jya1);
+            if (d.a != 0 || d.b != 0) {
+                gadd(gjx + at(d.a, d.b), d.jxb0);
+                gadd(gjx + at(d.a, d.b + 1), d.jxb1);
+                gadd(gjy + at(d.a, d.b), d.jyb0);
+                gadd(gjy + at(d.a + 1, d.b), d.jyb1);
+            }
+            gadd(gjz + at(0, 0), d.jz0);
+            gadd(gjz + at(d.di, 0), d.jz1);
+            gadd(gjz + at(0, d.dj), d.jz2);
+            gadd(gjz + at(d.di, d.dj), d.jz3);
+            if (want_rho) {
+                gadd(grho + at(0, 0), d.r0);
+                gadd(grho + at(1, 0), d.r1);
+                gadd(grho + at(0, 1), d.r2);
+                gadd(grho + at(1, 1), d.r3);
+            }
+        }
+        // periodic wrap (:1032-1052) and write-back
+        int ncx = cx + dcx, ncy = cy + dcy;
+        ncx = ncx < 0 ? ncx + nx : (ncx >= nx ? ncx - nx : ncx);
+        ncy = ncy < 0 ? ncy + ny : (ncy >= ny ? ncy - ny : ncy);
+        S.cell[i] = ncx * ny + ncy;
+        ox_p[i] = ox; oy_p[i] = oy; ux_p[i] = ux; uy_p[i] = uy; uz_p[i] = uz;
+        ke += (double)((gam - (R)1) * (R)S.ms);                                // :1062
+    }
+
+    // ---- D: flush the window, reduce the kinetic energy and error flags
+    __syncthreads();
+    if (inlds) {
+        for (int k = tid; k < WH; k += PUSH_THREADS) {
+            const int wj = k / W, wi = k - wj * W;
+            const int gx = wrapi(x0 + wx0 + wi, nx), gy = wrapi(y0 + wy0 + wj, ny);
+            const size_t g = (size_t)gy * nx + gx;
+            R v;
+            if ((v = jw[k]) != (R)0) gadd(gjx + g, v);
+            if ((v = jw[WH + k]) != (R)0) gadd(gjy + g, v);
+            if ((v = jw[2 * WH + k]) != (R)0) gadd(gjz + g, v);
+            if (want_rho && (v = jw[3 * WH + k]) != (R)0) gadd(grho + g, v);
+        }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        ke += __shfl_xor(ke, o);
+        err |= __shfl_xor(err, o);
+    }
+    if (lane == 0) {
+        kred[wave] = ke;
+        if (err) atomicOr(P.err, err);
+    }
+    __syncthreads();
+    if (tid == 0)
+        unsafeAtomicAdd(P.acc + (S.q > 0 ? ACC_KE_I : ACC_KE_E),
+                        (kred[0] + kred[1]) + (kred[2] + kred[3]));
+}
+
+// Batched currentdeposit with global atomics (unit-test entry point).
+template <typename R, bool SPLIT>
+__global__ void deposit_batch_kernel(PushParams P, int64_t n, const int32_t *i1,
+                                     const int32_t *i2, const int32_t *j1,
+                                     const int32_t *j2, const double *dx1,
+                                     const double *dx2, const double *dy1,
+                                     const double *dy2, const int32_t *q,
+                                     const double *vz)
+{
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int nx = P.nx, ny = P.ny;
+    const R qr = (R)q[k];
+    // qx = q*dx/dt (:1237): sp[0].qx holds dx/dt; q is +-1 (checked on the
+    // host), so q*(dx/dt) equals (q*dx)/dt exactly.
+    const R qxx = (R)((double)q[k] * P.sp[0].qx), qyy = (R)((double)q[k] * P.sp[0].qy);
+    Dep<R> d;
+    const int a = i2[k] - i1[k], b = j2[k] - j1[k];
+    deposit_terms<R, SPLIT>(i1[k], j1[k], a, b, (R)dx1[k], (R)dx2[k], (R)dy1[k],
+                            (R)dy2[k], qxx, qyy, qr, (R)vz[k], P.compat,
+                            P.deposit_rho != 0, d);
+    if ((P.compat & PBPIC_COMPAT_JY_DROP) && i2[k] == nx + 1) d.jyb1 = (R)0;
+    const int cx = i1[k] - 1, cy = j1[k] - 1;
+    auto at = [&](int ddx, int ddy) -> size_t {
+        return (size_t)wrapi(cy + ddy, ny) * nx + wrapi(cx + ddx, nx);
+    };
+    R *gjx = (R *)P.jx, *gjy = (R *)P.jy, *gjz = (R *)P.jz, *grho = (R *)P.rho;
+    gadd(gjx + at(0, 0), d.jxa0);
+    gadd(gjx + at(0, 1), d.jxa1);
+    gadd(gjy + at(0, 0), d.jya0);
+    gadd(gjy + at(1, 0), d.jya1);
+    gadd(gjx + at(a, b), d.jxb0);
+    gadd(gjx + at(a, b + 1), d.jxb1);
+    gadd(gjy + at(a, b), d.jyb0);
+    gadd(gjy + at(a + 1, b), d.jyb1);
+    gadd(gjz + at(0, 0), d.jz0);
+    gadd(gjz + at(d.di, 0), d.jz1);
+    gadd(gjz + at(0, d.dj), d.jz2);
+    gadd(gjz + at(d.di, d.dj), d.jz3);
+    if (P.deposit_rho) {
+        gadd(grho + at(0, 0), d.r0);
+        gadd(grho + at(1, 0), d.r1);
+        gadd(grho + at(0, 1), d.r2);
+        gadd(grho + at(1, 1), d.r3);
+    }
+}
+
+static int push_tiles(const PushParams &p, PushParams &q)
+{
+    q = p;
+    int t = 0;
+    for (int s = 0; s < p.nspecies; s++) {
+        q.sp[s].tile0 = t;
+        t += (int)((p.sp[s].n + PUSH_TILE - 1) / PUSH_TILE);
+    }
+    q.ntiles = t;
+    return t;
+}
+
+size_t push_lds_bytes(int precision, int maxw, int deposit_rho)
+{
+    const size_t r = precision == PBPIC_PREC_FP64 ? 8 : 4;
+    return (size_t)maxw * (6 + (deposit_rho ? 4 : 3)) * r;
+}
+
+hipError_t launch_push(const PushParams &p, int precision, hipStream_t st)
+{
+    PushParams q;
+    const int nt = push_tiles(p, q);
+    if (nt == 0) return hipSuccess;
+    const size_t lds = push_lds_bytes(precision, p.maxw, p.deposit_rho);
+    if (precision == PBPIC_PREC_FP64) {
+        (void)hipFuncSetAttribute((const void *)push_kernel<double, false>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        hipLaunchKernelGGL((push_kernel<double, false>), dim3(nt), dim3(PUSH_THREADS), lds, st, q);
+    } else {
+        (void)hipFuncSetAttribute((const void *)push_kernel<float, true>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        hipLaunchKernelGGL((push_kernel<float, true>), dim3(nt), dim3(PUSH_THREADS), lds, st, q);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_deposit_batch(const PushParams &p, int precision, int64_t n,
+                                const int32_t *i1, const int32_t *i2,
+                                const int32_t *j1, const int32_t *j2,
+                                const double *dx1, const double *dx2,
+                                const double *dy1, const double *dy2,
+                                const int32_t *q, const double *vz,
+                                hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    const int nb = (int)((n + 255) / 256);
+    if (precision == PBPIC_PREC_FP64)
+        hipLaunchKernelGGL((deposit_batch_kernel<double, false>), dim3(nb), dim3(256), 0, st,
+                           p, n, i1, i2, j1, j2, dx1, dx2, dy1, dy2, q, vz);
+    else
+        hipLaunchKernelGGL((deposit_batch_kernel<float, true>), dim3(nb), dim3(256), 0, st,
+                           p, n, i1, i2, j1, j2, dx1, dx2, dy1, dy2, q, vz);
+    return hipGetLastError();
+}
+
+}  // namespace pbpic

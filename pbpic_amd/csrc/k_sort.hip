@@ -1,0 +1,249 @@
+// k_sort.hip -- countingsort (pic2d.f90:1303-1329) for gfx950.
+//
+// The reference is a stable counting sort of the particles by cell id:
+// histogram (:1310-1313), exclusive scan (:1315-1320), stable scatter
+// (:1322-1327), copy back (:1328).  With millions of cells a per-block x
+// per-cell count matrix does not fit, so the stable scatter is done as an LSD
+// radix sort of (cell, index) pairs over 8-bit digits; each digit pass is
+// itself a counting sort (a per-wave ballot histogram with in-order ranks,
+// then a decoupled look-back prefix scan across tiles).  A final gather moves
+// the particle payload once.  The result (permutation and per-cell counts) is
+// identical to the reference's: both are the unique stable sort by cell.
+#include "pbpic_impl.h"
+
+namespace pbpic {
+
+constexpr int ST_THREADS = 256;
+constexpr int ST_ROUNDS = 16;                    // 16 x 64 items per wave
+constexpr int ST_TILE = ST_THREADS * ST_ROUNDS;  // 4096
+constexpr uint32_t FLAG_AGG = 1u << 30, FLAG_INC = 2u << 30, CNT_MASK = (1u << 30) - 1;
+constexpr int SPIN_LIMIT = 1 << 24;
+
+// Per-cell counts with run-length aggregation inside a wave: consecutive
+// equal cell ids (the common case for an almost sorted array) cost one atomic.
+__global__ __launch_bounds__(256) void cell_hist_kernel(const int32_t *cell, int64_t n,
+                                                        int64_t ncell, int32_t *counts, int *err)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    bool valid = i < n;
+    if (valid && (cell[i] < 0 || cell[i] >= ncell)) {
+        atomicOr(err, ERRF_CELL);
+        valid = false;
+    }
+    const int k = valid ? cell[i] : -1;
+    const int prev = __shfl_up(k, 1);
+    // a run ends at the next lane whose key differs (invalid lanes carry -1)
+    const bool boundary = lane == 0 || k != prev;
+    const uint64_t heads = __ballot(boundary);
+    if (valid && boundary) {
+        const uint64_t above = lane == 63 ? 0ull : heads & ~((2ull << lane) - 1);
+        const int end = above ? __ffsll((unsigned long long)above) - 1 : 64;
+        atomicAdd(counts + k, end - lane);
+    }
+}
+
+// Digit histograms of all passes from the per-cell counts.
+__global__ __launch_bounds__(256) void digit_hist_kernel(const int32_t *counts, int64_t ncell,
+                                                         int npass, uint32_t *dhist)
+{
+    __shared__ uint32_t h[4][256];
+    for (int k = threadIdx.x; k < 4 * 256; k += 256) h[k >> 8][k & 255] = 0;
+    __syncthreads();
+    const int64_t c0 = ((int64_t)blockIdx.x * 256 + threadIdx.x) * 64;
+    const int64_t c1 = c0 + 64 < ncell ? c0 + 64 : ncell;
+    for (int64_t c = c0; c < c1; c++) {
+        const int32_t v = counts[c];
+        if (v) atomicAdd(&h[0][c & 255], (uint32_t)v);
+    }
+    for (int p = 1; p < npass; p++) {
+        uint32_t cur = 0xFFFFFFFFu, acc = 0;
+        for (int64_t c = c0; c < c1; c++) {
+            const uint32_t d = (uint32_t)(c >> (8 * p)) & 255u;
+            if (d != cur) {
+                if (acc) atomicAdd(&h[p][cur], acc);
+                cur = d;
+                acc = 0;
+            }
+            acc += (uint32_t)counts[c];
+        }
+        if (acc) atomicAdd(&h[p][cur], acc);
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < npass * 256; k += 256)
+        if (h[k >> 8][k & 255]) atomicAdd(dhist + k, h[k >> 8][k & 255]);
+}
+
+__global__ void digit_scan_kernel(uint32_t *dhist, int npass)
+{
+    if (threadIdx.x >= npass) return;
+    uint32_t *h = dhist + threadIdx.x * 256;
+    uint32_t run = 0;
+    for (int d = 0; d < 256; d++) {
+        const uint32_t v = h[d];
+        h[d] = run;
+        run += v;
+    }
+}
+
+template <bool FIRST, bool LAST>
+__global__ __launch_bounds__(ST_THREADS) void radix_pass_kernel(
+    const uint32_t *kin, const uint32_t *vin, uint32_t *kout, uint32_t *vout, int64_t n,
+    int shift, const uint32_t *dbase, uint32_t *status, uint32_t *ticket, int *err)
+{
+    __shared__ uint32_t whist[4][256];
+    __shared__ uint32_t woff[4][256];
+    __shared__ uint32_t gbase[256];
+    __shared__ int tile_s;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    for (int k = tid; k < 4 * 256; k += ST_THREADS) whist[k >> 8][k & 255] = 0;
+    if (tid == 0) tile_s = (int)atomicAdd(ticket, 1u);
+    __syncthreads();
+    const int tile = tile_s;
+    const int64_t wbase = (int64_t)tile * ST_TILE + (int64_t)wave * (ST_ROUNDS * 64);
+    const uint64_t lt = (1ull << lane) - 1;
+    uint32_t key[ST_ROUNDS], val[ST_ROUNDS], rank[ST_ROUNDS], dig[ST_ROUNDS];
+#pragma unroll
+    for (int r = 0; r < ST_ROUNDS; r++) {
+        const int64_t i = wbase + r * 64 + lane;
+        const bool valid = i < n;
+        const uint32_t k = valid ? kin[i] : 0u;
+        const uint32_t v = FIRST ? (uint32_t)i : (valid ? vin[i] : 0u);
+        const uint32_t d = (k >> shift) & 255u;
+        uint64_t m = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; b++) {
+            const uint64_t bb = __ballot((d >> b) & 1u);
+            m &= ((d >> b) & 1u) ? bb : ~bb;
+        }
+        const uint32_t base = whist[wave][d];
+        rank[r] = base + (uint32_t)__popcll(m & lt);
+        if (valid && (m & lt) == 0) whist[wave][d] = base + (uint32_t)__popcll(m);
+        __builtin_amdgcn_wave_barrier();
+        key[r] = k;
+        val[r] = v;
+        dig[r] = valid ? d : 0xFFFFFFFFu;
+    }
+    __syncthreads();
+    {
+        const int d = tid;
+        const uint32_t c0 = whist[0][d], c1 = whist[1][d], c2 = whist[2][d], c3 = whist[3][d];
+        woff[0][d] = 0;
+        woff[1][d] = c0;
+        woff[2][d] = c0 + c1;
+        woff[3][d] = c0 + c1 + c2;
+        const uint32_t total = c0 + c1 + c2 + c3;
+        uint32_t excl = 0;
+        uint32_t *mine = status + (size_t)tile * 256 + d;
+        if (tile == 0) {
+            __hip_atomic_store(mine, FLAG_INC | total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            __hip_atomic_store(mine, FLAG_AGG | total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            int j = tile - 1, spins = 0;
+            while (true) {
+                const uint32_t s = __hip_atomic_load(status + (size_t)j * 256 + d, __ATOMIC_RELAXED,
+                                                     __HIP_MEMORY_SCOPE_AGENT);
+                if (s == 0) {
+                    if (++spins > SPIN_LIMIT) { atomicOr(err, ERRF_SPIN); break; }
+                    __builtin_amdgcn_s_sleep(1);
+                    continue;
+                }
+                excl += s & CNT_MASK;
+                if ((s & FLAG_INC) || j == 0) break;
+                j--;
+            }
+            __hip_atomic_store(mine, FLAG_INC | (excl + total), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        }
+        gbase[d] = dbase[d] + excl;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < ST_ROUNDS; r++) {
+        const uint32_t d = dig[r];
+        if (d == 0xFFFFFFFFu) continue;
+        const uint32_t pos = gbase[d] + woff[wave][d] + rank[r];
+        if ((int64_t)pos >= n) { atomicOr(err, ERRF_SPIN); continue; }
+        if (!LAST) kout[pos] = key[r];
+        vout[pos] = val[r];
+    }
+}
+
+template <typename R>
+__global__ void gather_kernel(const uint32_t *perm, int64_t n, SpeciesDev src, SpeciesDev dst)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t p = perm[i];
+    dst.cell[i] = src.cell[p];
+    ((R *)dst.ox)[i] = ((const R *)src.ox)[p];
+    ((R *)dst.oy)[i] = ((const R *)src.oy)[p];
+    ((R *)dst.ux)[i] = ((const R *)src.ux)[p];
+    ((R *)dst.uy)[i] = ((const R *)src.uy)[p];
+    ((R *)dst.uz)[i] = ((const R *)src.uz)[p];
+    if (src.pid) dst.pid[i] = src.pid[p];
+}
+
+// Sorts species sp into dst (a second buffer set of the same capacity).
+hipError_t sort_species(SortScratch &s, const SpeciesDev &sp, int precision, int64_t ncell,
+                        SpeciesDev &dst, int *err, int want_counts, hipStream_t st)
+{
+    (void)want_counts;
+    const int64_t n = sp.n;
+    hipError_t e;
+    dst.n = n;
+    if (n == 0) return hipSuccess;
+    int bits = 0;
+    while ((1ll << bits) < ncell) bits++;
+    int npass = (bits + 7) / 8;
+    if (npass < 1) npass = 1;
+    const int64_t ntile = (n + ST_TILE - 1) / ST_TILE;
+
+    if ((e = launch_fill_u32((uint32_t *)s.counts, ncell, 0u, st)) != hipSuccess) return e;
+    if ((e = launch_fill_u32(s.dhist, 4 * 256, 0u, st)) != hipSuccess) return e;
+    hipLaunchKernelGGL(cell_hist_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
+                       sp.cell, n, ncell, s.counts, err);
+    const int64_t nhb = (ncell + 256 * 64 - 1) / (256 * 64);
+    hipLaunchKernelGGL(digit_hist_kernel, dim3((unsigned)nhb), dim3(256), 0, st, s.counts, ncell,
+                       npass, s.dhist);
+    hipLaunchKernelGGL(digit_scan_kernel, dim3(1), dim3(64), 0, st, s.dhist, npass);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+
+    const uint32_t *kin = (const uint32_t *)sp.cell;
+    const uint32_t *vin = nullptr;
+    int pb = 0;
+    for (int p = 0; p < npass; p++) {
+        if ((e = launch_fill_u32(s.status, ntile * 256, 0u, st)) != hipSuccess) return e;
+        if ((e = launch_fill_u32(s.ticket + p, 1, 0u, st)) != hipSuccess) return e;
+        const bool first = p == 0, last = p == npass - 1;
+        uint32_t *ko = s.keys[pb], *vo = s.vals[pb];
+        const uint32_t *db = s.dhist + p * 256;
+        dim3 g((unsigned)ntile), b(ST_THREADS);
+        if (first && last)
+            hipLaunchKernelGGL((radix_pass_kernel<true, true>), g, b, 0, st, kin, vin, ko, vo, n,
+                               8 * p, db, s.status, s.ticket + p, err);
+        else if (first)
+            hipLaunchKernelGGL((radix_pass_kernel<true, false>), g, b, 0, st, kin, vin, ko, vo, n,
+                               8 * p, db, s.status, s.ticket + p, err);
+        else if (last)
+            hipLaunchKernelGGL((radix_pass_kernel<false, true>), g, b, 0, st, kin, vin, ko, vo, n,
+                               8 * p, db, s.status, s.ticket + p, err);
+        else
+            hipLaunchKernelGGL((radix_pass_kernel<false, false>), g, b, 0, st, kin, vin, ko, vo, n,
+                               8 * p, db, s.status, s.ticket + p, err);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+        kin = ko;
+        vin = vo;
+        pb ^= 1;
+    }
+    const uint32_t *perm = vin;
+    s.perm = perm;
+    const unsigned nb = (unsigned)((n + 255) / 256);
+    if (precision == PBPIC_PREC_FP64)
+        hipLaunchKernelGGL(gather_kernel<double>, dim3(nb), dim3(256), 0, st, perm, n, sp, dst);
+    else
+        hipLaunchKernelGGL(gather_kernel<float>, dim3(nb), dim3(256), 0, st, perm, n, sp, dst);
+    return hipGetLastError();
+}
+
+}  // namespace pbpic

@@ -1,0 +1,134 @@
+// pbpic_impl.h -- internal declarations shared by the HIP translation units of
+// libpbpic.so (not installed; the public interface is include/pbpic.h).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "../../include/pbpic.h"
+
+namespace pbpic {
+
+// Device-side error flags, OR-ed by kernels into one int and reported by
+// pbpic_sync() and every call that returns data.
+enum : int {
+    ERRF_CFL = 1,        // a particle moved more than one cell
+    ERRF_NONFINITE = 2,  // NaN/Inf momentum or Lorentz factor
+    ERRF_CELL = 4,       // cell index out of range on input
+    ERRF_SPIN = 8,       // sort look-back hit its spin limit
+};
+
+// FP64 device accumulators.
+enum : int {
+    ACC_KE_E = 0,      // sum (gamma-1)*ms over electrons in the last push
+    ACC_KE_I = 1,      // same over ions
+    ACC_JE_STEP = 2,   // sum_cells J_raw . E_avg of the last field solve
+    ACC_JE_INT = 3,    // time integral of the scaled J.E before the last solve
+    ACC_WE = 4,        // sum of squared node-averaged E
+    ACC_WB = 5,        // sum of squared node-averaged B
+    ACC_G_L2 = 6,      // Gauss residual sum of squares
+    ACC_G_LINF = 7,    // Gauss residual max (bit pattern of a double >= 0)
+    ACC_G_DL2 = 8,     // sum of squares of the residual change
+    ACC_G_DLINF = 9,   // max of the residual change
+    ACC_N = 16,
+};
+
+constexpr int MAX_SPECIES = 4;
+
+// Division by a run-time constant d >= 1 for 32-bit unsigned n
+// (Granlund & Montgomery, "Division by invariant integers using
+// multiplication", PLDI'94, round-up variant with a 33-bit magic).
+struct FastDiv {
+    uint32_t d, m, s;
+    void init(uint32_t dd)
+    {
+        d = dd;
+        s = 0;
+        while ((1ull << s) < dd) s++;
+        m = (uint32_t)((((1ull << 32) * ((1ull << s) - dd)) / dd) + 1);
+    }
+    __device__ __forceinline__ uint32_t div(uint32_t n) const
+    {
+        uint32_t hi = __umulhi(n, m);
+        return (uint32_t)(((uint64_t)hi + n) >> s);
+    }
+};
+
+// One species as the kernels see it.
+struct SpeciesDev {
+    int32_t *cell;             // 0-based compact id cx*ny + cy (= grid - 1)
+    void *ox, *oy, *ux, *uy, *uz;
+    int32_t *pid;              // may be null
+    int64_t n;
+    int32_t q;                 // +1 ion, -1 electron
+    int32_t tile0;             // first tile of this species in a push launch
+    double ms;                 // mass: mime for q > 0, 1 otherwise (:964-968)
+    double c0, c1;             // q*dt*mime/ms/2 and c0/cva (:969-970)
+    double qx, qy;             // q*dx/dt, q*dy/dt (:1237-1238)
+};
+
+struct PushParams {
+    int nx, ny;
+    FastDiv dny;
+    int compat, deposit_rho;
+    int maxw;                  // LDS window capacity in cells
+    int ntiles;
+    int nspecies;
+    double dt, cva, idx, idy, dtcva;   // dtcva = dt*cva (:1003)
+    const double *ex, *ey, *ez, *bx, *by, *bz;
+    void *jx, *jy, *jz, *rho;  // unnormalized current, double or float
+    double *acc;
+    int *err;
+    SpeciesDev sp[MAX_SPECIES];
+};
+
+struct FdtdParams {
+    int nx, ny;
+    double dth, dtc2, idx, idy, in0;   // dt/2, dt*cva^2, 1/dx, 1/dy, in0
+    double *ex, *ey, *ez, *bx, *by, *bz;
+    const void *jx, *jy, *jz;
+    double *acc;
+};
+
+// Launchers (one per translation unit family).
+hipError_t launch_push(const PushParams &p, int precision, hipStream_t st);
+hipError_t launch_deposit_batch(const PushParams &p, int precision, int64_t n,
+                                const int32_t *i1, const int32_t *i2,
+                                const int32_t *j1, const int32_t *j2,
+                                const double *dx1, const double *dx2,
+                                const double *dy1, const double *dy2,
+                                const int32_t *q, const double *vz,
+                                hipStream_t st);
+hipError_t launch_fdtd(const FdtdParams &p, int precision, hipStream_t st);
+hipError_t launch_energy(const FdtdParams &p, hipStream_t st);
+hipError_t launch_gauss(const PushParams &p, int precision, double *rho_scratch,
+                        double *gauss0, int first, double rho_bg,
+                        double cva2in0, hipStream_t st);
+hipError_t launch_fill_u32(uint32_t *p, int64_t n, uint32_t v, hipStream_t st);
+hipError_t launch_zero_current(void *jx, void *jy, void *jz, void *rho,
+                               int64_t ncell, int elem_bytes, double *acc,
+                               int reset_ke, hipStream_t st);
+hipError_t launch_acc_op(double *acc, int op, double a, hipStream_t st);
+hipError_t launch_init_maxwellian(const SpeciesDev &s, int precision, int nx,
+                                  int ny, int ppc, double vth, double vdrift,
+                                  int counter_stream, uint64_t seed,
+                                  uint32_t stream, hipStream_t st);
+hipError_t launch_check_cells(const int32_t *cell, int64_t n, int64_t ncell,
+                              int *err, hipStream_t st);
+
+// Sort (stable LSD radix over the cell id, 8-bit digits).
+struct SortScratch {
+    uint32_t *keys[2] = {nullptr, nullptr};
+    uint32_t *vals[2] = {nullptr, nullptr};
+    uint32_t *status = nullptr;     // ntiles * 256
+    uint32_t *dhist = nullptr;      // npass * 256 (counts, then offsets)
+    uint32_t *ticket = nullptr;     // one counter per pass
+    int32_t *counts = nullptr;      // ncell
+    const uint32_t *perm = nullptr; // permutation of the last sort (in vals)
+    int64_t cap = 0, status_cap = 0;
+};
+hipError_t sort_species(SortScratch &s, const SpeciesDev &sp, int precision,
+                        int64_t ncell, SpeciesDev &dst, int *err,
+                        int want_counts, hipStream_t st);
+
+}  // namespace pbpic

@@ -1,0 +1,77 @@
+"""Seeded test cases shared by the oracle and GPU parity tests.
+
+A case is a small periodic plasma in the reference's normalization
+(pic2d.f90:15-43): two species (ions q=+1 first, electrons q=-1), optional
+random fields.  The oracle holds both species in one interleaved array like
+the reference's `ptl`; the GPU holds one array per species.
+"""
+import numpy as np
+
+import oracle
+
+SPECIES_Q = (1, -1)
+
+
+def make_case(nx=24, ny=20, ppc=8, seed=7, vth=(0.05, 0.3), efield=0.3, bfield=1.5,
+              shuffle=False, sort=True, dx=0.1, dy=0.1, cva=15.0, mime=25.0):
+    rng = np.random.default_rng(seed)
+    dt = 0.2 / (cva * np.sqrt(mime))
+    in0 = 1.0 / ppc
+    sp = []
+    for k, q in enumerate(SPECIES_Q):
+        p = oracle.init_maxwellian(nx, ny, ppc, vth[k], seed=seed, stream=k)
+        if shuffle:
+            order = rng.permutation(len(p["grid"]))
+            p = {key: v[order] for key, v in p.items()}
+        p["pid"] = np.arange(1, len(p["grid"]) + 1, dtype=np.int32) + k * 10_000_000
+        sp.append(p)
+    fields = {}
+    for name, amp in (("ex", efield), ("ey", efield), ("ez", efield),
+                      ("bx", bfield), ("by", bfield), ("bz", bfield)):
+        fields[name] = amp * rng.standard_normal((ny, nx)) if amp else np.zeros((ny, nx))
+    return dict(nx=nx, ny=ny, dx=dx, dy=dy, dt=dt, cva=cva, mime=mime, in0=in0,
+                species=sp, fields=fields, sort=sort)
+
+
+def oracle_for(case, prec=8, compat=oracle.COMPAT_REFERENCE):
+    o = oracle.Oracle(case["nx"], case["ny"], case["dx"], case["dy"], case["dt"],
+                      case["cva"], case["mime"], case["in0"], prec=prec, compat=compat)
+    sp = case["species"]
+    cat = lambda k: np.concatenate([s[k] for s in sp])
+    q = np.concatenate([np.full(len(s["grid"]), qq, np.int32) for s, qq in zip(sp, SPECIES_Q)])
+    o.set_particles(cat("grid"), cat("dx"), cat("dy"), cat("ux"), cat("uy"), cat("uz"), q,
+                    cat("pid"))
+    for name, plane in case["fields"].items():
+        o.set_field(name, plane)
+    return o
+
+
+def oracle_species(o, case):
+    """Splits the oracle's interleaved particle array back into species."""
+    p = o.get_particles()
+    out = []
+    for q in SPECIES_Q:
+        m = p["q"] == q
+        out.append({k: v[m] for k, v in p.items()})
+    return out
+
+
+def gpu_for(case, precision=0, compat=oracle.COMPAT_REFERENCE, sort_every=0, deposit_rho=True):
+    import pbpic_amd
+    ctx = pbpic_amd.Context(case["nx"], case["ny"], case["dx"], case["dy"], case["dt"],
+                            case["cva"], case["mime"], case["in0"], precision=precision,
+                            compat=compat, sort_every=sort_every, deposit_rho=deposit_rho,
+                            track_pid=True)
+    for q, s in zip(SPECIES_Q, case["species"]):
+        k = ctx.add_species(q, len(s["grid"]))
+        ctx.set_particles(k, s["grid"], s["dx"], s["dy"], s["ux"], s["uy"], s["uz"], s["pid"])
+    for name, plane in case["fields"].items():
+        ctx.set_field(name, plane)
+    return ctx
+
+
+def rel_norm(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    den = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / (den if den > 0 else 1.0)

@@ -1,0 +1,256 @@
+"""GPU parity: the HIP path through the C ABI against the CPU restatement of
+pic2d.f90 (the oracle) on the same seeded inputs.
+
+Bars: bit-exact for the sort (counts and permutation); FP64 single step to
+1e-12 relative (particles element-wise, currents and fields norm-wise, since
+the current sums are accumulated in a different order); the split-precision
+form against the FP64 oracle within FP32 tolerances stated per test.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from cases import SPECIES_Q, gpu_for, make_case, oracle_for, oracle_species, rel_norm
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-12
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+# ------------------------------------------------------------------ sort
+@pytest.mark.parametrize("kind", ["random", "nearly_sorted", "one_cell", "reverse", "ragged"])
+@pytest.mark.parametrize("nx,ny", [(16, 12), (300, 257)])
+def test_countingsort_bitexact(gpu, kind, nx, ny):
+    """countingsort (pic2d.f90:1303-1329): counts == bincount, permutation ==
+    stable argsort, and both == the oracle's restatement."""
+    _say(f"sort kind={kind} nx={nx} ny={ny}")
+    rng = np.random.default_rng(11)
+    ncell = nx * ny
+    n = {"random": 20000, "nearly_sorted": 30000, "one_cell": 5000, "reverse": 9000,
+         "ragged": 4097}[kind]
+    if kind == "random":
+        grid = rng.integers(1, ncell + 1, n)
+    elif kind == "nearly_sorted":
+        grid = np.sort(rng.integers(1, ncell + 1, n))
+        grid = np.clip(grid + rng.integers(-3, 4, n), 1, ncell)
+    elif kind == "one_cell":
+        grid = np.full(n, ncell // 2 + 1)
+    elif kind == "reverse":
+        grid = np.sort(rng.integers(1, ncell + 1, n))[::-1].copy()
+    else:
+        grid = rng.integers(1, min(ncell, 40) + 1, n)
+    case = make_case(nx=nx, ny=ny, ppc=1, seed=3)
+    case["species"] = [dict(grid=grid.astype(np.int32), dx=rng.random(n) - 0.5,
+                            dy=rng.random(n) - 0.5, ux=rng.standard_normal(n),
+                            uy=rng.standard_normal(n), uz=rng.standard_normal(n),
+                            pid=np.arange(1, n + 1, dtype=np.int32))]
+    import pbpic_amd
+    ctx = pbpic_amd.Context(nx, ny, 0.1, 0.1, 0.01, 15.0, 25.0, 0.1, track_pid=True)
+    k = ctx.add_species(-1, n)
+    s = case["species"][0]
+    ctx.set_particles(k, s["grid"], s["dx"], s["dy"], s["ux"], s["uy"], s["uz"], s["pid"])
+    counts, perm = ctx.countingsort(k, want=True)
+    assert np.array_equal(counts, np.bincount(grid - 1, minlength=ncell))
+    assert np.array_equal(perm, np.argsort(grid, kind="stable"))
+    o = oracle.Oracle(nx, ny, 0.1, 0.1, 0.01, 15.0, 25.0, 0.1)
+    o.set_particles(s["grid"], s["dx"], s["dy"], s["ux"], s["uy"], s["uz"],
+                    np.full(n, -1, np.int32), s["pid"])
+    oc, op = o.countingsort(want=True)
+    assert np.array_equal(counts, oc) and np.array_equal(perm, op)
+    g = ctx.get_particles(k)
+    assert np.array_equal(g["grid"], grid[perm])
+    assert np.array_equal(g["pid"], s["pid"][perm])
+    assert np.array_equal(g["ux"], s["ux"][perm])
+
+
+def test_countingsort_empty(gpu):
+    import pbpic_amd
+    ctx = pbpic_amd.Context(8, 8, 0.1, 0.1, 0.01, 15.0, 25.0, 0.1)
+    k = ctx.add_species(1, 10)
+    counts, perm = ctx.countingsort(k, want=True)
+    assert counts.sum() == 0 and len(perm) == 0
+
+
+# ------------------------------------------------------------- deposit
+@pytest.mark.parametrize("compat", [7, 0])
+def test_currentdeposit_batch(gpu, compat):
+    """currentdeposit (pic2d.f90:1226-1296), every crossing pattern, incl.
+    boundary cells where the reference drops jy at column mx+2."""
+    _say(f"deposit compat={compat}")
+    nx, ny = 10, 9
+    rng = np.random.default_rng(5)
+    n = 4000
+    i1 = rng.integers(1, nx + 1, n)
+    j1 = rng.integers(1, ny + 1, n)
+    a = rng.integers(-1, 2, n)
+    b = rng.integers(-1, 2, n)
+    i1[:50] = nx
+    a[:50] = 1                      # right-boundary crossers (jy column mx+2)
+    dx1 = rng.random(n) - 0.5
+    dy1 = rng.random(n) - 0.5
+    dx2 = np.where(a == 0, np.clip(dx1 + 0.3 * (rng.random(n) - 0.5), -0.5, 0.5),
+                   -a * 0.5 + a * 0.4 * rng.random(n))
+    dy2 = np.where(b == 0, np.clip(dy1 + 0.3 * (rng.random(n) - 0.5), -0.5, 0.5),
+                   -b * 0.5 + b * 0.4 * rng.random(n))
+    q = rng.choice([-1, 1], n)
+    vz = rng.standard_normal(n)
+    args = (i1, i1 + a, j1, j1 + b, dx1, dx2, dy1, dy2, q, vz)
+    import pbpic_amd
+    ctx = pbpic_amd.Context(nx, ny, 0.1, 0.12, 0.003, 15.0, 25.0, 0.05, compat=compat)
+    ctx.currentdeposit(*args)
+    o = oracle.Oracle(nx, ny, 0.1, 0.12, 0.003, 15.0, 25.0, 0.05, compat=compat)
+    o.deposit_batch(*args)
+    o.finish_current()
+    for f in ("jx", "jy", "jz", "rho"):
+        assert rel_norm(ctx.get_field(f), o.get_field(f)) < 1e-13, f
+
+
+# ------------------------------------------------------------- fields
+def test_fieldsolver_fp64(gpu):
+    """fieldsolver (pic2d.f90:678-860) with random E, B, J: agrees to rounding."""
+    case = make_case(nx=70, ny=37, ppc=1, seed=9)
+    case["species"] = []
+    rng = np.random.default_rng(2)
+    jx, jy, jz = (rng.standard_normal((37, 70)) for _ in range(3))
+    ctx = gpu_for(case)
+    o = oracle_for(case)
+    for name, plane in (("jx", jx), ("jy", jy), ("jz", jz)):
+        ctx.set_field(name, plane)
+        o.set_field(name, plane)
+    ctx.fieldsolver()
+    o.fieldsolver()
+    for f in ("ex", "ey", "ez", "bx", "by", "bz"):
+        assert rel_norm(ctx.get_field(f), o.get_field(f)) < 1e-14, f
+    ge, gb = ctx.fieldsenergy()
+    oe, ob = o.fieldsenergy()
+    assert abs(ge - oe) <= 1e-12 * abs(oe) and abs(gb - ob) <= 1e-12 * abs(ob)
+    d, od = ctx.diag(), o.diag()
+    assert abs(d["je_step"] - od["je_step"]) <= 1e-12 * abs(od["je_step"])
+
+
+def test_emfgrid_matches(gpu):
+    case = make_case(nx=20, ny=16, ppc=1, seed=4)
+    case["species"] = []
+    ctx = gpu_for(case)
+    o = oracle_for(case)
+    for f in ("gex", "gey", "gez", "gbx", "gby", "gbz"):
+        assert np.array_equal(ctx.get_field(f), o.get_field(f)), f
+
+
+# ------------------------------------------------------------- one step
+@pytest.mark.parametrize("compat", [7, 0])
+@pytest.mark.parametrize("shuffle", [False, True])
+def test_one_step_fp64(gpu, compat, shuffle):
+    """zero J, pusher, fieldsolver (pic2d.f90:173-178) in FP64: particles to
+    1e-12 element-wise (cells exact), J/rho/E/B to 1e-12 norm-wise.  shuffle
+    feeds unsorted particles (the global-atomic fall-back path)."""
+    _say(f"one step fp64 compat={compat} shuffle={shuffle}")
+    case = make_case(nx=40, ny=33, ppc=12, seed=21, shuffle=shuffle)
+    ctx = gpu_for(case, compat=compat)
+    o = oracle_for(case, compat=compat)
+    ctx.zero_current(); ctx.pusher()
+    o.zero_current(); o.pusher()
+    for f in ("jx", "jy", "jz", "rho"):
+        assert rel_norm(ctx.get_field(f), o.get_field(f)) < FP64_TOL, f
+    ctx.fieldsolver(); o.fieldsolver()
+    for f in ("ex", "ey", "ez", "bx", "by", "bz"):
+        assert rel_norm(ctx.get_field(f), o.get_field(f)) < FP64_TOL, f
+    osp = oracle_species(o, case)
+    for k in range(2):
+        g = ctx.get_particles(k)
+        assert np.array_equal(g["pid"], osp[k]["pid"])
+        assert np.array_equal(g["grid"], osp[k]["grid"])
+        for f in ("dx", "dy", "ux", "uy", "uz"):
+            err = np.max(np.abs(g[f] - osp[k][f]) / np.maximum(np.abs(osp[k][f]), 1e-300))
+            assert err < FP64_TOL, (k, f, err)
+    d, od = ctx.diag(), o.diag()
+    for key in ("ke_e", "ke_i", "w_e", "w_b", "je_step"):
+        assert abs(d[key] - od[key]) <= 1e-12 * abs(od[key]) + 1e-300, key
+
+
+def test_multi_step_fp64_diagnostics(gpu):
+    """30 steps with a sort every 10: diagnostics track the oracle (trajectories
+    stay within rounding for a few dozen steps of a weakly driven plasma)."""
+    case = make_case(nx=32, ny=32, ppc=16, seed=5, efield=0.0, bfield=0.0)
+    ctx = gpu_for(case, sort_every=10)
+    o = oracle_for(case)
+    ctx.countingsort(); o.countingsort()
+    ctx.step(30)
+    o.step(30, sort_every=10)
+    d, od = ctx.diag(), o.diag()
+    for key in ("ke_e", "ke_i", "w_e", "w_b", "je_int"):
+        assert abs(d[key] - od[key]) <= 1e-9 * abs(od[key]), (key, d[key], od[key])
+    assert abs(d["gauss_l2"] - od["gauss_l2"]) <= 1e-9 * od["gauss_l2"]
+
+
+# ------------------------------------------------------- split precision
+def test_one_step_split32(gpu):
+    """Split precision (int cell + FP32 offset, FP32 push/deposit) against the
+    FP64 oracle: offsets within 2e-6 cells, momenta within 1e-5 relative,
+    cells exact except where an FP64 offset lies within 1e-5 of a cell face."""
+    case = make_case(nx=40, ny=33, ppc=12, seed=22)
+    ctx = gpu_for(case, precision=1, compat=7)
+    o = oracle_for(case, compat=7)
+    ctx.zero_current(); ctx.pusher(); ctx.fieldsolver()
+    o.zero_current(); o.pusher(); o.fieldsolver()
+    osp = oracle_species(o, case)
+    for k in range(2):
+        g = ctx.get_particles(k)
+        same = g["grid"] == osp[k]["grid"]
+        near_face = (np.abs(np.abs(osp[k]["dx"]) - 0.5) < 1e-5) | (np.abs(np.abs(osp[k]["dy"]) - 0.5) < 1e-5)
+        assert np.all(same | near_face)
+        for f in ("dx", "dy"):
+            assert np.max(np.abs(g[f] - osp[k][f])[same]) < 2e-6, f
+        for f in ("ux", "uy", "uz"):
+            assert np.max(np.abs(g[f] - osp[k][f])) < 1e-5 * np.max(np.abs(osp[k][f])), f
+    for f in ("jx", "jy", "jz"):
+        assert rel_norm(ctx.get_field(f), o.get_field(f)) < 1e-5, f
+    for f in ("ex", "ey", "ez", "bx", "by", "bz"):
+        assert rel_norm(ctx.get_field(f), o.get_field(f)) < 1e-6, f
+
+
+# ------------------------------------------------------------- loader
+def test_device_loader_matches_oracle(gpu):
+    """The device Philox loader produces the oracle loader's particles: cells
+    and offsets bit-identical, Box-Muller momenta within 1e-13."""
+    import pbpic_amd
+    nx, ny, ppc = 17, 9, 5
+    ctx = pbpic_amd.Context(nx, ny, 0.1, 0.1, 0.01, 15.0, 25.0, 0.2)
+    k = ctx.add_species(-1, nx * ny * ppc)
+    ctx.init_maxwellian(k, ppc, 0.1, 0.2, counter_stream=True, seed=99, stream=3)
+    g = ctx.get_particles(k)
+    r = oracle.init_maxwellian(nx, ny, ppc, 0.1, 0.2, counter_stream=True, seed=99, stream=3)
+    assert np.array_equal(g["grid"], r["grid"])
+    assert np.array_equal(g["dx"], r["dx"]) and np.array_equal(g["dy"], r["dy"])
+    for f in ("ux", "uy", "uz"):
+        assert np.max(np.abs(g[f] - r[f])) < 1e-13
+
+
+# ---------------------------------------------------------- error paths
+def test_cfl_violation_reported(gpu):
+    """A particle moving more than one cell (the reference mishandles it
+    silently, pic2d.f90:1008-1023) is reported as PBPIC_ERR_CFL."""
+    import pbpic_amd
+    ctx = pbpic_amd.Context(8, 8, 0.01, 0.01, 0.01, 15.0, 25.0, 0.1)
+    k = ctx.add_species(-1, 4)
+    ctx.set_particles(k, [1, 2, 3, 4], [0.0] * 4, [0.0] * 4, [50.0, 0, 0, 0], [0.0] * 4, [0.0] * 4)
+    ctx.zero_current(); ctx.pusher()
+    with pytest.raises(pbpic_amd.PbpicError) as e:
+        ctx.sync()
+    assert e.value.code == -4
+    ctx.sync()   # flags are cleared once reported
+
+
+def test_nonfinite_reported(gpu):
+    import pbpic_amd
+    ctx = pbpic_amd.Context(8, 8, 0.1, 0.1, 0.001, 15.0, 25.0, 0.1)
+    k = ctx.add_species(1, 2)
+    ctx.set_particles(k, [1, 2], [0.0, 0.0], [0.0, 0.0], [np.nan, 0.0], [0.0, 0.0], [0.0, 0.0])
+    ctx.zero_current(); ctx.pusher()
+    with pytest.raises(pbpic_amd.PbpicError) as e:
+        ctx.sync()
+    assert e.value.code == -5
