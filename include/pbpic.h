@@ -252,6 +252,34 @@ int pbpic_field_device_ptrs(pbpic_ctx *ctx, void **ptrs);
 /* The HIP stream (hipStream_t) the context orders its work on.              */
 void *pbpic_stream(pbpic_ctx *ctx);
 
+/* Device-to-device variants of the setters, for callers that hold the state
+ * on the GPU (the PyTorch binding).  cell is the 0-based compact id
+ * (grid - 1); the other arrays have the context's particle precision (double
+ * for PBPIC_PREC_FP64, float for PBPIC_PREC_SPLIT32).                        */
+int pbpic_set_particles_device(pbpic_ctx *ctx, int32_t species, int64_t n,
+                               const int32_t *cell, const void *dx,
+                               const void *dy, const void *ux, const void *uy,
+                               const void *uz);
+int pbpic_set_field_device(pbpic_ctx *ctx, int32_t which, const double *plane);
+
+/* ---------------------------------------------------------------- adjoint
+ * Reverse-mode gradient of the time-integrated J.E, the analysis the
+ * reference names but leaves empty (python/analysis_jdote.py:1-31); no
+ * entry point of the reference corresponds.  pbpic_adjoint_forward runs
+ * nsteps steps from the current state (as pbpic_step, sorting at its cadence),
+ * keeping particle+field checkpoints every ckpt_every steps, and returns
+ * loss = sum_n dt (mime/c^2) sum_cells J^{n+1/2}.(E^n+E^{n+1})/2.
+ * pbpic_adjoint_backward then writes dloss/d(initial state): field_grads holds
+ * 6 device pointers (nx*ny doubles: Ex, Ey, Ez, Bx, By, Bz), particle_grads
+ * 5 device pointers per species in species order (dx, dy, ux, uy, uz, in the
+ * particle precision, in the order the particles had before the forward).
+ * Segments between checkpoints are recomputed.  The context ends in the state
+ * after the forward.                                                         */
+int pbpic_adjoint_forward(pbpic_ctx *ctx, int32_t nsteps, int32_t ckpt_every,
+                          double *loss);
+int pbpic_adjoint_backward(pbpic_ctx *ctx, double *const *field_grads,
+                           void *const *particle_grads);
+
 /* Per-kernel timing with HIP events on the context's stream (measurement
  * support; the reference has no timers, pic2d.f90:172 only prints itime).
  * enable != 0 starts collecting (and clears earlier samples), 0 stops.
@@ -261,7 +289,8 @@ void *pbpic_stream(pbpic_ctx *ctx);
 #define PBPIC_PROF_FIELD  1   /* FDTD kernel                                  */
 #define PBPIC_PROF_SORT   2   /* counting sort incl. payload gather           */
 #define PBPIC_PROF_ZERO   3   /* current clear                                */
-#define PBPIC_NPROF       4
+#define PBPIC_PROF_ADJ    4   /* adjoint backward (replay + reverse sweep)    */
+#define PBPIC_NPROF       5
 int pbpic_profile(pbpic_ctx *ctx, int32_t enable);
 int pbpic_profile_read(pbpic_ctx *ctx, double ms[PBPIC_NPROF],
                        int64_t count[PBPIC_NPROF]);

@@ -97,6 +97,10 @@ def load_library(path=None):
         "pbpic_stream": (vp, [vp]),
         "pbpic_profile": (i32, [vp, i32]),
         "pbpic_profile_read": (i32, [vp, pdbl, pi64]),
+        "pbpic_set_particles_device": (i32, [vp, i32, i64, vp, vp, vp, vp, vp, vp]),
+        "pbpic_set_field_device": (i32, [vp, i32, vp]),
+        "pbpic_adjoint_forward": (i32, [vp, i32, i32, pdbl]),
+        "pbpic_adjoint_backward": (i32, [vp, P(vp), P(vp)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -294,8 +298,34 @@ class Context:
     def stream(self):
         return self._lib.pbpic_stream(self._h)
 
+    # ------------------------------------------- device-side state, adjoint
+    def set_field_device(self, which, ptr):
+        """Copies one field plane from device memory (nx*ny doubles)."""
+        w = FIELDS[which] if isinstance(which, str) else int(which)
+        self._chk(self._lib.pbpic_set_field_device(self._h, w, ptr))
+
+    def set_particles_device(self, species, n, cell, dx, dy, ux, uy, uz):
+        """Copies a species' state from device memory (cell = grid - 1)."""
+        self._chk(self._lib.pbpic_set_particles_device(self._h, species, int(n), cell, dx, dy,
+                                                       ux, uy, uz))
+
+    def adjoint_forward(self, nsteps, ckpt_every):
+        """Runs nsteps steps keeping checkpoints every ckpt_every steps and
+        returns the time-integrated J.E (the differentiable loss)."""
+        out = ctypes.c_double()
+        self._chk(self._lib.pbpic_adjoint_forward(self._h, int(nsteps), int(ckpt_every),
+                                                  ctypes.byref(out)))
+        return out.value
+
+    def adjoint_backward(self, field_grad_ptrs, particle_grad_ptrs):
+        """Writes d(loss)/d(initial state) into caller device buffers: 6 field
+        planes and 5 arrays per species (dx, dy, ux, uy, uz)."""
+        fg = (ctypes.c_void_p * 6)(*field_grad_ptrs)
+        pg = (ctypes.c_void_p * len(particle_grad_ptrs))(*particle_grad_ptrs)
+        self._chk(self._lib.pbpic_adjoint_backward(self._h, fg, pg))
+
     # ------------------------------------------------------------ timing
-    PROF_SLOTS = ("push", "field", "sort", "zero")
+    PROF_SLOTS = ("push", "field", "sort", "zero", "adjoint")
 
     def profile(self, enable=True):
         """Starts (clearing earlier samples) or stops HIP-event timing of each
@@ -303,7 +333,7 @@ class Context:
         self._chk(self._lib.pbpic_profile(self._h, int(bool(enable))))
 
     def profile_read(self):
-        ms = (ctypes.c_double * 4)()
-        cnt = (ctypes.c_int64 * 4)()
+        ms = (ctypes.c_double * 5)()
+        cnt = (ctypes.c_int64 * 5)()
         self._chk(self._lib.pbpic_profile_read(self._h, ms, cnt))
         return {k: (ms[i], cnt[i]) for i, k in enumerate(self.PROF_SLOTS)}

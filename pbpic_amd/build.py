@@ -11,7 +11,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libpbpic.so")
 
-SOURCES = ["capi.hip", "k_push.hip", "k_field.hip", "k_sort.hip"]
+SOURCES = ["capi.hip", "k_push.hip", "k_field.hip", "k_sort.hip", "k_adjoint.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",
@@ -33,14 +33,15 @@ def hipcc():
     raise RuntimeError("hipcc not found")
 
 
-def build_library(verbose=True):
+def build_library(verbose=True, out=None):
+    out = out or LIB
     cmd = [hipcc(), *HIPCC_FLAGS, "-I", os.path.join(ROOT, "include"),
-           *[os.path.join(CSRC, s) for s in SOURCES], "-o", LIB + ".tmp"]
+           *[os.path.join(CSRC, s) for s in SOURCES], "-o", out + ".tmp"]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
-    os.replace(LIB + ".tmp", LIB)
-    return LIB
+    os.replace(out + ".tmp", out)
+    return out
 
 
 def build_fortran(verbose=True):

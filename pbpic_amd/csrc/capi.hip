@@ -1,6 +1,7 @@
 // capi.hip -- the C ABI of include/pbpic.h: context, buffers, host I/O and
 // the step entry points that mirror pic2d.f90's subroutines.
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -46,6 +47,16 @@ struct pbpic_ctx {
     int maxw = 0;
     double idx = 0, idy = 0, dth = 0, dtc2 = 0, dtcva = 0;
     std::string last_error;
+    // adjoint: checkpoints of a differentiable forward run
+    struct {
+        bool valid = false;
+        int32_t nsteps = 0, ckpt = 0;
+        int64_t step0 = 0;
+        std::vector<char *> ckpts;     // particle+field state before step k*ckpt
+        char *final_state = nullptr;   // state after the forward
+        double acc_after[ACC_N] = {0};
+        double loss = 0;
+    } adj;
     // event timing
     bool prof = false;
     std::vector<hipEvent_t> ev_pool;
@@ -158,7 +169,7 @@ int valid_species(pbpic_ctx *c, int32_t s)
 
 hipError_t prof_mark(pbpic_ctx *c, int slot, bool begin);
 
-int sort_one(pbpic_ctx *c, int s, int32_t *counts, int32_t *perm)
+int sort_one(pbpic_ctx *c, int s, int32_t *counts, int32_t *perm, uint32_t *perm_dev = nullptr)
 {
     Species &S = c->sp[s];
     SpeciesDev src = species_dev(c, s, S.cur), dst = species_dev(c, s, S.cur ^ 1);
@@ -166,6 +177,9 @@ int sort_one(pbpic_ctx *c, int s, int32_t *counts, int32_t *perm)
     HIPCHK(c, sort_species(c->sort, src, c->p.precision, ncell(c), dst, c->err, counts != nullptr, c->st));
     HIPCHK(c, prof_mark(c, PBPIC_PROF_SORT, false));
     S.cur ^= 1;
+    if (perm_dev && S.n > 0)
+        HIPCHK(c, hipMemcpyAsync(perm_dev, c->sort.perm, sizeof(uint32_t) * S.n,
+                                 hipMemcpyDeviceToDevice, c->st));
     if (counts)
         HIPCHK(c, hipMemcpyAsync(counts, c->sort.counts, sizeof(int32_t) * ncell(c),
                                  hipMemcpyDeviceToHost, c->st));
@@ -206,6 +220,106 @@ int fieldsolve_impl(pbpic_ctx *c)
     HIPCHK(c, prof_mark(c, PBPIC_PROF_FIELD, false));
     c->steps++;
     return PBPIC_OK;
+}
+
+// One time-loop body (pic2d.f90:171-188).  Optionally copies J after the push
+// (jcopy: 3 device planes) and the sort permutations (perm_dev: one device
+// array per species) for the adjoint replay.
+int step_impl(pbpic_ctx *c, void *const *jcopy, uint32_t *const *perm_dev, bool *sorted)
+{
+    HIPCHK(c, prof_mark(c, PBPIC_PROF_ZERO, true));
+    HIPCHK(c, launch_zero_current(c->J[0], c->J[1], c->J[2], c->J[3], ncell(c), c->rsz, c->acc, 1, c->st));
+    HIPCHK(c, prof_mark(c, PBPIC_PROF_ZERO, false));
+    int rc = push_impl(c, false);
+    if (rc) return rc;
+    if (jcopy)
+        for (int k = 0; k < 3; k++)
+            HIPCHK(c, hipMemcpyAsync(jcopy[k], c->J[k], (size_t)c->rsz * ncell(c),
+                                     hipMemcpyDeviceToDevice, c->st));
+    if ((rc = fieldsolve_impl(c))) return rc;
+    if (sorted) *sorted = false;
+    if (c->p.sort_every > 0 && c->steps % c->p.sort_every == 0) {
+        for (int s = 0; s < (int)c->sp.size(); s++)
+            if ((rc = sort_one(c, s, nullptr, nullptr, perm_dev ? perm_dev[s] : nullptr))) return rc;
+        if (sorted) *sorted = true;
+    }
+    return PBPIC_OK;
+}
+
+// Flat device layout of one particle+field state (adjoint checkpoints/tape).
+struct StateLayout {
+    std::vector<size_t> sp_off;   // per species: start of [cell|ox|oy|ux|uy|uz|pid]
+    size_t field_off = 0, total = 0;
+};
+
+size_t al256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+StateLayout state_layout(pbpic_ctx *c)
+{
+    StateLayout L;
+    size_t o = 0;
+    for (auto &S : c->sp) {
+        L.sp_off.push_back(o);
+        const size_t n = (size_t)S.n;
+        o += al256(4 * n) + 5 * al256((size_t)c->rsz * n) + (S.pid[0] ? al256(4 * n) : 0);
+    }
+    L.field_off = o;
+    o += 6 * al256(8 * (size_t)ncell(c));
+    L.total = o;
+    return L;
+}
+
+// Pointers into a state buffer: part 0 = cell, 1..5 = dx..uz, 6 = pid;
+// fields f0..f5 = Ex..Bz.
+void *state_ptr(pbpic_ctx *c, const StateLayout &L, char *buf, int s, int part)
+{
+    const size_t n = (size_t)c->sp[s].n;
+    size_t o = L.sp_off[s];
+    if (part >= 1) o += al256(4 * n);
+    if (part >= 2) o += (size_t)(part - 1) * al256((size_t)c->rsz * n);
+    if (part == 6) o = L.sp_off[s] + al256(4 * n) + 5 * al256((size_t)c->rsz * n);
+    return buf + o;
+}
+double *state_field(pbpic_ctx *c, const StateLayout &L, char *buf, int f)
+{
+    return (double *)(buf + L.field_off + (size_t)f * al256(8 * (size_t)ncell(c)));
+}
+
+int state_copy(pbpic_ctx *c, const StateLayout &L, char *buf, bool save)
+{
+    const size_t nc = (size_t)ncell(c);
+    for (int s = 0; s < (int)c->sp.size(); s++) {
+        Species &S = c->sp[s];
+        const int b = S.cur;
+        void *dev[7] = {S.cell[b], S.ox[b], S.oy[b], S.ux[b], S.uy[b], S.uz[b], S.pid[b]};
+        for (int part = 0; part < 7; part++) {
+            if (!dev[part] || S.n == 0) continue;
+            const size_t bytes = (part == 0 || part == 6 ? 4 : (size_t)c->rsz) * (size_t)S.n;
+            void *st = state_ptr(c, L, buf, s, part);
+            HIPCHK(c, hipMemcpyAsync(save ? st : dev[part], save ? dev[part] : st, bytes,
+                                     hipMemcpyDeviceToDevice, c->st));
+        }
+    }
+    for (int f = 0; f < 6; f++) {
+        double *dev = f < 3 ? c->E[f] : c->B[f - 3];
+        double *st = state_field(c, L, buf, f);
+        HIPCHK(c, hipMemcpyAsync(save ? st : dev, save ? dev : st, nc * 8, hipMemcpyDeviceToDevice, c->st));
+    }
+    return PBPIC_OK;
+}
+
+double je_total(pbpic_ctx *c, const double *a)
+{
+    return a[ACC_JE_INT] + a[ACC_JE_STEP] * (c->p.mime / (c->p.cva * c->p.cva)) * c->p.dt;
+}
+
+void adj_free(pbpic_ctx *c)
+{
+    for (auto p : c->adj.ckpts) (void)hipFree(p);
+    c->adj.ckpts.clear();
+    if (c->adj.final_state) (void)hipFree(c->adj.final_state);
+    c->adj.final_state = nullptr;
+    c->adj.valid = false;
 }
 
 template <typename H>
@@ -382,6 +496,8 @@ int pbpic_ctx_destroy(pbpic_ctx *c)
         }
     for (int b = 0; b < 2; b++) { f(c->sort.keys[b]); f(c->sort.vals[b]); }
     f(c->sort.status); f(c->sort.dhist); f(c->sort.ticket); f(c->sort.counts);
+    for (auto p : c->adj.ckpts) f(p);
+    f(c->adj.final_state);
     for (auto e : c->ev_pool) (void)hipEventDestroy(e);
     if (c->st) (void)hipStreamDestroy(c->st);
     delete c;
@@ -656,14 +772,8 @@ int pbpic_step(pbpic_ctx *c, int32_t nsteps)
 {
     if (!c || nsteps < 0) return PBPIC_ERR_ARG;
     for (int32_t k = 0; k < nsteps; k++) {
-        HIPCHK(c, prof_mark(c, PBPIC_PROF_ZERO, true));
-        HIPCHK(c, launch_zero_current(c->J[0], c->J[1], c->J[2], c->J[3], ncell(c), c->rsz, c->acc, 1, c->st));
-        HIPCHK(c, prof_mark(c, PBPIC_PROF_ZERO, false));
-        int rc = push_impl(c, false);
+        int rc = step_impl(c, nullptr, nullptr, nullptr);
         if (rc) return rc;
-        if ((rc = fieldsolve_impl(c))) return rc;
-        if (c->p.sort_every > 0 && c->steps % c->p.sort_every == 0)
-            if ((rc = pbpic_countingsort(c))) return rc;
     }
     return PBPIC_OK;
 }
@@ -725,6 +835,190 @@ int pbpic_field_device_ptrs(pbpic_ctx *c, void **ptrs)
 }
 
 void *pbpic_stream(pbpic_ctx *c) { return c ? (void *)c->st : nullptr; }
+
+int pbpic_set_particles_device(pbpic_ctx *c, int32_t s, int64_t n, const int32_t *cell,
+                               const void *dx, const void *dy, const void *ux, const void *uy,
+                               const void *uz)
+{
+    int rc = valid_species(c, s);
+    if (rc) return rc;
+    Species &S = c->sp[s];
+    if (n < 0 || n > S.cap) return fail(c, PBPIC_ERR_CAPACITY, "n=%lld exceeds capacity", (long long)n);
+    const int b = S.cur;
+    HIPCHK(c, hipMemcpyAsync(S.cell[b], cell, 4 * n, hipMemcpyDeviceToDevice, c->st));
+    const void *src[5] = {dx, dy, ux, uy, uz};
+    void *dst[5] = {S.ox[b], S.oy[b], S.ux[b], S.uy[b], S.uz[b]};
+    for (int k = 0; k < 5; k++)
+        HIPCHK(c, hipMemcpyAsync(dst[k], src[k], (size_t)c->rsz * n, hipMemcpyDeviceToDevice, c->st));
+    HIPCHK(c, launch_check_cells(S.cell[b], n, ncell(c), c->err, c->st));
+    S.n = n;
+    return PBPIC_OK;
+}
+
+int pbpic_set_field_device(pbpic_ctx *c, int32_t which, const double *plane)
+{
+    if (!c || !plane) return PBPIC_ERR_ARG;
+    const size_t nc = (size_t)ncell(c);
+    double *dst;
+    if (which >= PBPIC_EX && which <= PBPIC_EZ) dst = c->E[which - PBPIC_EX];
+    else if (which >= PBPIC_BX && which <= PBPIC_BZ) dst = c->B[which - PBPIC_BX];
+    else return fail(c, PBPIC_ERR_ARG, "field %d cannot be set from the device", which);
+    HIPCHK(c, hipMemcpyAsync(dst, plane, nc * 8, hipMemcpyDeviceToDevice, c->st));
+    return PBPIC_OK;
+}
+
+int pbpic_adjoint_forward(pbpic_ctx *c, int32_t nsteps, int32_t ckpt, double *loss)
+{
+    if (!c || nsteps < 1 || ckpt < 1) return PBPIC_ERR_ARG;
+    int rc = check_flags(c);
+    if (rc) return rc;
+    adj_free(c);
+    const StateLayout L = state_layout(c);
+    const int nck = (nsteps + ckpt - 1) / ckpt;
+    for (int k = 0; k < nck; k++) {
+        char *p = nullptr;
+        hipError_t e = hipMalloc(&p, L.total);
+        if (e != hipSuccess) { adj_free(c); return fail(c, PBPIC_ERR_NOMEM, "checkpoint allocation: %s", hipGetErrorString(e)); }
+        c->adj.ckpts.push_back(p);
+    }
+    HIPCHK(c, hipMalloc(&c->adj.final_state, L.total));
+    double a0[ACC_N], a1[ACC_N];
+    HIPCHK(c, hipMemcpyAsync(a0, c->acc, sizeof a0, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    c->adj.step0 = c->steps;
+    for (int k = 0; k < nsteps; k++) {
+        if (k % ckpt == 0 && (rc = state_copy(c, L, c->adj.ckpts[k / ckpt], true))) return rc;
+        if ((rc = step_impl(c, nullptr, nullptr, nullptr))) return rc;
+    }
+    if ((rc = state_copy(c, L, c->adj.final_state, true))) return rc;
+    HIPCHK(c, hipMemcpyAsync(a1, c->acc, sizeof a1, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    if ((rc = check_flags(c))) return rc;
+    memcpy(c->adj.acc_after, a1, sizeof a1);
+    c->adj.nsteps = nsteps;
+    c->adj.ckpt = ckpt;
+    c->adj.loss = je_total(c, a1) - je_total(c, a0);
+    c->adj.valid = true;
+    if (loss) *loss = c->adj.loss;
+    return PBPIC_OK;
+}
+
+int pbpic_adjoint_backward(pbpic_ctx *c, double *const *fg, void *const *pg)
+{
+    if (!c || !fg || !pg) return PBPIC_ERR_ARG;
+    if (!c->adj.valid) return fail(c, PBPIC_ERR_STATE, "pbpic_adjoint_forward has not been run");
+    const StateLayout L = state_layout(c);
+    const int K = c->adj.ckpt, N = c->adj.nsteps, ns = (int)c->sp.size();
+    const size_t nc = (size_t)ncell(c), rz = (size_t)c->rsz;
+    int64_t ntot = 0, nmax = 0;
+    for (auto &S : c->sp) { ntot += S.n; nmax = std::max(nmax, S.n); }
+    // one allocation for the tape and the adjoint buffers
+    const size_t tape_states = (size_t)(K + 1) * L.total;
+    const size_t tape_j = (size_t)K * 3 * al256(rz * nc);
+    const size_t tape_perm = (size_t)K * al256(4 * (size_t)ntot + 256 * ns);
+    const size_t fadj = 18 * al256(8 * nc);                    // Ebar, Bbar, E', Jbar, node x6
+    const size_t padj = 5 * al256(rz * (size_t)ntot + 256 * ns) + 5 * al256(rz * (size_t)nmax);
+    char *base = nullptr;
+    hipError_t e = hipMalloc(&base, tape_states + tape_j + tape_perm + fadj + padj);
+    if (e != hipSuccess) return fail(c, PBPIC_ERR_NOMEM, "adjoint tape allocation: %s", hipGetErrorString(e));
+    int rc = PBPIC_OK;
+    char *p = base;
+    auto take = [&](size_t b) { char *q = p; p += al256(b); return q; };
+    std::vector<char *> tstate(K + 1);
+    for (int k = 0; k <= K; k++) tstate[k] = take(L.total);
+    std::vector<std::array<void *, 3>> tj(K);
+    for (int k = 0; k < K; k++) for (int q = 0; q < 3; q++) tj[k][q] = take(rz * nc);
+    std::vector<std::vector<uint32_t *>> tperm(K, std::vector<uint32_t *>(ns));
+    for (int k = 0; k < K; k++) for (int s = 0; s < ns; s++) tperm[k][s] = (uint32_t *)take(4 * (size_t)c->sp[s].n);
+    double *fa[18];
+    for (int q = 0; q < 18; q++) fa[q] = (double *)take(8 * nc);
+    std::vector<std::array<void *, 5>> pa(ns), tmp(1);
+    for (int s = 0; s < ns; s++) for (int q = 0; q < 5; q++) pa[s][q] = take(rz * (size_t)c->sp[s].n);
+    for (int q = 0; q < 5; q++) tmp[0][q] = take(rz * (size_t)nmax);
+    std::vector<bool> sorted(K);
+    auto run = [&]() -> int {
+        HIPCHK(c, hipMemsetAsync(fa[0], 0, (size_t)(p - (char *)fa[0]), c->st));
+        HIPCHK(c, prof_mark(c, PBPIC_PROF_ADJ, true));
+        const int nseg = (N + K - 1) / K;
+        for (int seg = nseg - 1; seg >= 0; seg--) {
+            const int s0 = seg * K, kk = std::min(N, s0 + K) - s0;
+            int r = state_copy(c, L, c->adj.ckpts[seg], false);
+            if (r) return r;
+            c->steps = c->adj.step0 + s0;
+            for (int k = 0; k < kk; k++) {
+                if ((r = state_copy(c, L, tstate[k], true))) return r;
+                bool so = false;
+                if ((r = step_impl(c, tj[k].data(), tperm[k].data(), &so))) return r;
+                sorted[k] = so;
+            }
+            if ((r = state_copy(c, L, tstate[kk], true))) return r;
+            for (int k = kk - 1; k >= 0; k--) {
+                if (sorted[k]) {
+                    for (int s = 0; s < ns; s++) {
+                        HIPCHK(c, launch_unpermute(tperm[k][s], c->sp[s].n, c->p.precision,
+                                                   pa[s].data(), tmp[0].data(), c->st));
+                        for (int q = 0; q < 5; q++)
+                            HIPCHK(c, hipMemcpyAsync(pa[s][q], tmp[0][q], rz * c->sp[s].n,
+                                                     hipMemcpyDeviceToDevice, c->st));
+                    }
+                }
+                AdjFieldParams F{};
+                F.nx = c->p.nx; F.ny = c->p.ny;
+                F.dth = c->dth; F.dtc2 = c->dtc2; F.idx = c->idx; F.idy = c->idy;
+                F.in0 = c->p.in0; F.w = c->p.dt * c->p.mime / (c->p.cva * c->p.cva);
+                F.exa = fa[0]; F.eya = fa[1]; F.eza = fa[2];
+                F.bxa = fa[3]; F.bya = fa[4]; F.bza = fa[5];
+                F.etx = fa[6]; F.ety = fa[7]; F.etz = fa[8];
+                F.jxa = fa[9]; F.jya = fa[10]; F.jza = fa[11];
+                F.e0x = state_field(c, L, tstate[k], 0); F.e0y = state_field(c, L, tstate[k], 1);
+                F.e0z = state_field(c, L, tstate[k], 2);
+                F.e1x = state_field(c, L, tstate[k + 1], 0); F.e1y = state_field(c, L, tstate[k + 1], 1);
+                F.e1z = state_field(c, L, tstate[k + 1], 2);
+                F.jx = tj[k][0]; F.jy = tj[k][1]; F.jz = tj[k][2];
+                HIPCHK(c, launch_fdtd_adj(F, c->p.precision, c->st));
+                AdjParams A{};
+                A.nx = c->p.nx; A.ny = c->p.ny; A.compat = c->p.compat;
+                A.maxw = c->p.precision == PBPIC_PREC_FP64 ? 640 : 1280;
+                A.nspecies = ns;
+                A.dtcva = c->dtcva; A.idx = c->idx; A.idy = c->idy; A.cva = c->p.cva;
+                A.ex = state_field(c, L, tstate[k], 0); A.ey = state_field(c, L, tstate[k], 1);
+                A.ez = state_field(c, L, tstate[k], 2); A.bx = state_field(c, L, tstate[k], 3);
+                A.by = state_field(c, L, tstate[k], 4); A.bz = state_field(c, L, tstate[k], 5);
+                A.jadj[0] = fa[9]; A.jadj[1] = fa[10]; A.jadj[2] = fa[11];
+                A.nadj = fa[12];
+                for (int s = 0; s < ns; s++) {
+                    Species &S = c->sp[s];
+                    AdjSpecies &T = A.sp[s];
+                    T.cell = (const int32_t *)state_ptr(c, L, tstate[k], s, 0);
+                    T.ox = state_ptr(c, L, tstate[k], s, 1); T.oy = state_ptr(c, L, tstate[k], s, 2);
+                    T.ux = state_ptr(c, L, tstate[k], s, 3); T.uy = state_ptr(c, L, tstate[k], s, 4);
+                    T.uz = state_ptr(c, L, tstate[k], s, 5);
+                    T.n = S.n; T.q = S.q; T.ms = S.ms; T.c0 = S.c0; T.c1 = S.c1; T.qx = S.qx; T.qy = S.qy;
+                    A.xa[s] = pa[s][0]; A.ya[s] = pa[s][1];
+                    A.uxa[s] = pa[s][2]; A.uya[s] = pa[s][3]; A.uza[s] = pa[s][4];
+                }
+                HIPCHK(c, launch_push_adj(A, c->p.precision, c->st));
+                HIPCHK(c, launch_node_transpose(F, fa[12], c->st));
+            }
+        }
+        HIPCHK(c, prof_mark(c, PBPIC_PROF_ADJ, false));
+        for (int q = 0; q < 6; q++)
+            HIPCHK(c, hipMemcpyAsync(fg[q], fa[q], 8 * nc, hipMemcpyDeviceToDevice, c->st));
+        for (int s = 0; s < ns; s++)
+            for (int q = 0; q < 5; q++)
+                HIPCHK(c, hipMemcpyAsync(pg[5 * s + q], pa[s][q], rz * c->sp[s].n, hipMemcpyDeviceToDevice, c->st));
+        int r = state_copy(c, L, c->adj.final_state, false);
+        if (r) return r;
+        c->steps = c->adj.step0 + N;
+        HIPCHK(c, hipMemcpyAsync(c->acc, c->adj.acc_after, sizeof c->adj.acc_after, hipMemcpyHostToDevice, c->st));
+        HIPCHK(c, hipStreamSynchronize(c->st));
+        return check_flags(c);
+    };
+    rc = run();
+    (void)hipStreamSynchronize(c->st);
+    (void)hipFree(base);
+    return rc;
+}
 
 int pbpic_profile(pbpic_ctx *c, int32_t enable)
 {

@@ -1,0 +1,576 @@
+// k_adjoint.hip -- hand-written discrete adjoint of the PIC step for gfx950.
+//
+// Loss: L = sum_n dt (mime/c^2) sum_cells J^{n+1/2} . (E^n + E^{n+1})/2, the
+// time-integrated J.E of the step (the same quantity as PBPIC_D_JE_INT).
+// For one step, in reverse:
+//   field solve   transposed curl stencils (fdtd_adj_* kernels);
+//   deposit       a gather of the current adjoint at the particle's deposit
+//                 targets (the transpose of the scatter), differentiated
+//                 through the zigzag relay point and the CIC weights of Jz;
+//   move          through u/gamma;
+//   Boris push    reverse mode through the rotation (both the reference's
+//                 pre-kick-u form, pic2d.f90:985-987, and the textbook form);
+//   gather        a CIC scatter of the force adjoint onto nodes (LDS window,
+//                 like the forward deposit) plus the weight derivative for
+//                 the position adjoint, then the transpose of the node
+//                 averaging onto the staggered fields;
+//   sort          the stored permutation applied in reverse.
+// The forward quantities of each particle are recomputed from the stored
+// state (checkpointing happens in the host code of capi.hip).
+#include "pbpic_impl.h"
+
+// the per-particle forward recompute follows the forward kernel's arithmetic
+// exactly (same expression order, no contraction)
+#pragma clang fp contract(off)
+
+namespace pbpic {
+
+namespace {
+
+constexpr int ADJ_THREADS = 256;
+constexpr int ADJ_ROUNDS = 8;
+constexpr int ADJ_GROUP = 8;
+constexpr int ADJ_RUN = ADJ_ROUNDS * ADJ_GROUP;         // 64
+constexpr int ADJ_TILE = ADJ_THREADS * ADJ_ROUNDS;      // 2048
+
+__device__ __forceinline__ int a_xcd(int b, int nb)
+{
+    const int q = nb >> 3, r = nb & 7, x = b & 7, k = b >> 3;
+    return x * q + (x < r ? x : r) + k;
+}
+
+__device__ __forceinline__ int a_wrap(int g, int n)
+{
+    while (g < 0) g += n;
+    while (g >= n) g -= n;
+    return g;
+}
+
+template <typename R>
+__device__ __forceinline__ void a_lds_add(R *p, R v)
+{
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+__device__ __forceinline__ void a_node_fields(const AdjParams &P, int gx, int gy, double f[6])
+{
+    const int nx = P.nx, ny = P.ny;
+    const int gxm = gx == 0 ? nx - 1 : gx - 1, gym = gy == 0 ? ny - 1 : gy - 1;
+    const size_t c = (size_t)gy * nx + gx, xm = (size_t)gy * nx + gxm;
+    const size_t ym = (size_t)gym * nx + gx, xym = (size_t)gym * nx + gxm;
+    f[0] = (P.ex[c] + P.ex[xm]) * 0.5;
+    f[1] = (P.ey[c] + P.ey[ym]) * 0.5;
+    f[2] = (P.ez[c] + P.ez[xm] + P.ez[ym] + P.ez[xym]) * 0.25;
+    f[3] = (P.bx[c] + P.bx[xm]) * 0.5;
+    f[4] = (P.by[c] + P.by[ym]) * 0.5;
+    f[5] = (P.bz[c] + P.bz[xm] + P.bz[ym] + P.bz[xym]) * 0.25;
+}
+
+template <typename R> __device__ __forceinline__ R a_sgn(R x) { return signbit(x) ? (R)-1 : (R)1; }
+
+__device__ __forceinline__ void cross3(const double a[3], const double b[3], double o[3])
+{
+    o[0] = a[1] * b[2] - a[2] * b[1];
+    o[1] = a[2] * b[0] - a[0] * b[2];
+    o[2] = a[0] * b[1] - a[1] * b[0];
+}
+__device__ __forceinline__ void cross3(const float a[3], const float b[3], float o[3])
+{
+    o[0] = a[1] * b[2] - a[2] * b[1];
+    o[1] = a[2] * b[0] - a[0] * b[2];
+    o[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+// Reverse mode through one zigzag segment pair along one axis.  Given the
+// forward inputs (a = cell displacement, o1, o2 = old/new offsets, qf = q*dx/dt)
+// and the adjoints of f1, f2, w1, w2, returns the adjoints of o1 and o2.
+template <typename R>
+__device__ __forceinline__ void zig_axis(int a, R o1, R o2, R qf, R &f1, R &f2, R &w1, R &w2,
+                                         R &g, bool fwd_only)
+{
+    const R H = (R)0.5;
+    const R imr = ((R)a + o1 + o2 + (R)1) * H;
+    const R lo = (R)(a > 0 ? a : 0), hi = (R)(a < 0 ? a : 0) + (R)1;
+    const R irr = imr < lo ? lo : (imr > hi ? hi : imr);
+    g = (imr > lo && imr < hi) ? H : (R)0;     // d irr / d o1 = d irr / d o2
+    f1 = qf * (irr - o1 - H);
+    f2 = qf * ((R)a + o2 + H - irr);
+    w1 = (irr + o1 + H) * H;
+    w2 = (irr - (R)a + o2 + H) * H;
+    (void)fwd_only;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- push
+// One step of the particle adjoint.  State n of the particle (cell, offsets,
+// momenta) is read from the tape; the adjoint of the state at n+1 is read
+// from, and the adjoint at n written to, the xa..ua arrays (in place).
+template <typename R, bool SPLIT>
+__global__ __launch_bounds__(ADJ_THREADS, 2) void push_adj_kernel(AdjParams P)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ int red[4][4];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int t = a_xcd(blockIdx.x, gridDim.x);
+    int s = 0;
+    while (s + 1 < P.nspecies && t >= P.sp[s + 1].tile0) s++;
+    const AdjSpecies S = P.sp[s];
+    const int64_t tstart = (int64_t)(t - S.tile0) * ADJ_TILE;
+    const int G = wave * 8 + (lane >> 3), gp = lane & 7;
+    const int64_t pbase = tstart + (int64_t)G * ADJ_RUN + gp;
+    const int nx = P.nx, ny = P.ny, nxh = nx >> 1, nyh = ny >> 1;
+
+    const int c00 = S.cell[tstart];
+    const bool c00ok = (unsigned)c00 < (unsigned)(nx * ny);
+    const int x0 = c00ok ? c00 / ny : 0, y0 = c00ok ? c00 - (c00 / ny) * ny : 0;
+    int cl[ADJ_ROUNDS];
+    int mnx = 1 << 30, mxx = -(1 << 30), mny = 1 << 30, mxy = -(1 << 30);
+#pragma unroll
+    for (int r = 0; r < ADJ_ROUNDS; r++) {
+        const int64_t i = pbase + r * ADJ_GROUP;
+        cl[r] = -1;
+        if (i < S.n) {
+            const int c = S.cell[i];
+            if ((unsigned)c >= (unsigned)(nx * ny)) continue;
+            cl[r] = c;
+            const int cx = c / ny, cy = c - cx * ny;
+            int rx = cx - x0, ry = cy - y0;
+            if (rx > nxh) rx -= nx; else if (rx < -nxh) rx += nx;
+            if (ry > nyh) ry -= ny; else if (ry < -nyh) ry += ny;
+            mnx = min(mnx, rx); mxx = max(mxx, rx);
+            mny = min(mny, ry); mxy = max(mxy, ry);
+        }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        mnx = min(mnx, __shfl_xor(mnx, o));
+        mxx = max(mxx, __shfl_xor(mxx, o));
+        mny = min(mny, __shfl_xor(mny, o));
+        mxy = max(mxy, __shfl_xor(mxy, o));
+    }
+    if (lane == 0) { red[wave][0] = mnx; red[wave][1] = mxx; red[wave][2] = mny; red[wave][3] = mxy; }
+    __syncthreads();
+    mnx = min(min(red[0][0], red[1][0]), min(red[2][0], red[3][0]));
+    mxx = max(max(red[0][1], red[1][1]), max(red[2][1], red[3][1]));
+    mny = min(min(red[0][2], red[1][2]), min(red[2][2], red[3][2]));
+    mxy = max(max(red[0][3], red[1][3]), max(red[2][3], red[3][3]));
+    // nodes used: the cell's 4 corners -> window = box + 1 node
+    const int wx0 = mnx, wy0 = mny;
+    const int W = mxx - mnx + 2, H = mxy - mny + 2;
+    const bool inlds = mnx <= mxx && (int64_t)W * H <= P.maxw;
+    const int WH = inlds ? W * H : 0;
+    R *nf = (R *)smem;                      // [WH][6] node fields
+    R *na = nf + (size_t)6 * P.maxw;        // [6][WH] node adjoint accumulators
+    if (inlds) {
+        for (int k = tid; k < WH; k += ADJ_THREADS) {
+            const int wj = k / W, wi = k - wj * W;
+            double f[6];
+            a_node_fields(P, a_wrap(x0 + wx0 + wi, nx), a_wrap(y0 + wy0 + wj, ny), f);
+#pragma unroll
+            for (int q = 0; q < 6; q++) nf[(size_t)k * 6 + q] = (R)f[q];
+        }
+        for (int k = tid; k < 6 * WH; k += ADJ_THREADS) na[k] = (R)0;
+    }
+    __syncthreads();
+
+    const R c0 = (R)S.c0, c1 = (R)S.c1, qx = (R)S.qx, qy = (R)S.qy, qr = (R)S.q;
+    const R dtcva = (R)P.dtcva, idx = (R)P.idx, idy = (R)P.idy, cva = (R)P.cva;
+    const bool oldu = (P.compat & PBPIC_COMPAT_BORIS_OLD_U) != 0;
+    const double *ja = P.jadj[0], *jb = P.jadj[1], *jc = P.jadj[2];
+    double *nadj = P.nadj;
+    const size_t NC = (size_t)nx * ny;
+#pragma unroll 1
+    for (int r = 0; r < ADJ_ROUNDS; r++) {
+        const int c = cl[r];
+        if (c < 0) continue;
+        const int64_t i = pbase + r * ADJ_GROUP;
+        const R ox = ((const R *)S.ox)[i], oy = ((const R *)S.oy)[i];
+        const R u[3] = {((const R *)S.ux)[i], ((const R *)S.uy)[i], ((const R *)S.uz)[i]};
+        const int cx = c / ny, cy = c - cx * ny;
+        int rx = cx - x0, ry = cy - y0;
+        if (rx > nxh) rx -= nx; else if (rx < -nxh) rx += nx;
+        if (ry > nyh) ry -= ny; else if (ry < -nyh) ry += ny;
+        const int lx = rx - wx0, ly = ry - wy0;
+        const int cx1 = cx + 1 == nx ? 0 : cx + 1, cy1 = cy + 1 == ny ? 0 : cy + 1;
+        R n00[6], n10[6], n01[6], n11[6];
+        if (inlds) {
+            const R *b = nf + (size_t)(ly * W + lx) * 6;
+#pragma unroll
+            for (int q = 0; q < 6; q++) {
+                n00[q] = b[q]; n10[q] = b[6 + q];
+                n01[q] = b[(size_t)W * 6 + q]; n11[q] = b[(size_t)W * 6 + 6 + q];
+            }
+        } else {
+            double f[6];
+            a_node_fields(P, cx, cy, f);  for (int q = 0; q < 6; q++) n00[q] = (R)f[q];
+            a_node_fields(P, cx1, cy, f); for (int q = 0; q < 6; q++) n10[q] = (R)f[q];
+            a_node_fields(P, cx, cy1, f); for (int q = 0; q < 6; q++) n01[q] = (R)f[q];
+            a_node_fields(P, cx1, cy1, f); for (int q = 0; q < 6; q++) n11[q] = (R)f[q];
+        }
+        // ---- forward recompute: gather (:948-963), same arithmetic as the push
+        const R Hh = (R)0.5, ONE = (R)1;
+        const R wxp = Hh + ox, wxm = Hh - ox, wyp = Hh + oy, wym = Hh - oy;
+        const R s11 = (Hh + ox) * (Hh + oy);
+        const R s10 = Hh + ox - s11;
+        const R s01 = Hh + oy - s11;
+        const R s00 = ONE - s11 - s10 - s01;
+        R F[6];
+#pragma unroll
+        for (int q = 0; q < 6; q++) F[q] = n00[q] * s00 + n10[q] * s10 + n01[q] * s01 + n11[q] * s11;
+        const R E[3] = {F[0], F[1], F[2]}, Bf[3] = {F[3], F[4], F[5]};
+        // Boris (:964-1000)
+        R um[3], t[3], sv[3], u0[3], up[3], un[3];
+        for (int k = 0; k < 3; k++) um[k] = u[k] + c1 * E[k];
+        const R g1 = sqrt(ONE + um[0] * um[0] + um[1] * um[1] + um[2] * um[2]);
+        const R c2 = c0 / g1;
+        for (int k = 0; k < 3; k++) t[k] = c2 * Bf[k];
+        const R t2 = (R)2 / (ONE + t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
+        for (int k = 0; k < 3; k++) sv[k] = t2 * t[k];
+        R uq[3];
+        for (int k = 0; k < 3; k++) uq[k] = oldu ? u[k] : um[k];
+        R cr[3];
+        cross3(uq, t, cr);
+        for (int k = 0; k < 3; k++) u0[k] = um[k] + cr[k];
+        cross3(u0, sv, cr);
+        for (int k = 0; k < 3; k++) up[k] = um[k] + cr[k];
+        for (int k = 0; k < 3; k++) un[k] = up[k] + c1 * E[k];
+        const R gn = sqrt(ONE + un[0] * un[0] + un[1] * un[1] + un[2] * un[2]);
+        const R ig = ONE / gn;
+        const R v[3] = {un[0] * ig, un[1] * ig, un[2] * ig};
+        // move (:1002-1026)
+        const R c3 = dtcva / gn;
+        const R dx1 = ox + c3 * un[0] * idx;
+        const R dy1 = oy + c3 * un[1] * idy;
+        const int a = fabs(dx1) > Hh ? (int)a_sgn(dx1) : 0;
+        const int b = fabs(dy1) > Hh ? (int)a_sgn(dy1) : 0;
+        const R ox2 = dx1 + (R)(-a), oy2 = dy1 + (R)(-b);
+        const R vz = un[2] * cva / gn;
+        // deposit terms (relative form of :1237-1282)
+        R fx1, fx2, wx1, wx2, gx, fy1, fy2, wy1, wy2, gy;
+        zig_axis<R>(a, ox, ox2, qx, fx1, fx2, wx1, wx2, gx, false);
+        zig_axis<R>(b, oy, oy2, qy, fy1, fy2, wy1, wy2, gy, false);
+        const R dimid = ((R)a + ox + ox2) * Hh, djmid = ((R)b + oy + oy2) * Hh;
+        const int di = (int)a_sgn(dimid), dj = (int)a_sgn(djmid);
+        const R A = fabs(dimid), Bq = fabs(djmid);
+        const R quz = qr * vz;
+        // ---- reverse: deposit = gather of the current adjoint at the targets
+        auto at = [&](int ddx, int ddy) -> size_t {
+            return (size_t)a_wrap(cy + ddy, ny) * nx + a_wrap(cx + ddx, nx);
+        };
+        const R l1 = (R)ja[at(0, 0)], l2 = (R)ja[at(0, 1)];
+        const R l3 = (R)jb[at(0, 0)], l4 = (R)jb[at(1, 0)];
+        const R l5 = (R)ja[at(a, b)], l6 = (R)ja[at(a, b + 1)];
+        const R l7 = (R)jb[at(a, b)];
+        const bool drop = (P.compat & PBPIC_COMPAT_JY_DROP) && cx == nx - 1 && a == 1;
+        const R l8 = drop ? (R)0 : (R)jb[at(a + 1, b)];
+        const R z0 = (R)jc[at(0, 0)], z1 = (R)jc[at(di, 0)], z2 = (R)jc[at(0, dj)],
+                z3 = (R)jc[at(di, dj)];
+        const R bfx1 = l1 * (ONE - wy1) + l2 * wy1, bwy1 = fx1 * (l2 - l1);
+        const R bfy1 = l3 * (ONE - wx1) + l4 * wx1, bwx1 = fy1 * (l4 - l3);
+        const R bfx2 = l5 * (ONE - wy2) + l6 * wy2, bwy2 = fx2 * (l6 - l5);
+        const R bfy2 = l7 * (ONE - wx2) + l8 * wx2, bwx2 = fy2 * (l8 - l7);
+        const R birx = qx * bfx1 - qx * bfx2 + Hh * bwx1 + Hh * bwx2;
+        const R biry = qy * bfy1 - qy * bfy2 + Hh * bwy1 + Hh * bwy2;
+        R bo1x = -qx * bfx1 + Hh * bwx1 + birx * gx;
+        R bo2x = qx * bfx2 + Hh * bwx2 + birx * gx;
+        R bo1y = -qy * bfy1 + Hh * bwy1 + biry * gy;
+        R bo2y = qy * bfy2 + Hh * bwy2 + biry * gy;
+        // Jz: s00=(1-A)(1-B) at (0,0), (di,0): A(1-B), (0,dj): (1-A)B, (di,dj): AB
+        const R bquz = z0 * (ONE - A) * (ONE - Bq) + z1 * A * (ONE - Bq) + z2 * (ONE - A) * Bq +
+                       z3 * A * Bq;
+        const R bA = quz * ((z1 - z0) * (ONE - Bq) + (z3 - z2) * Bq);
+        const R bB = quz * ((z2 - z0) * (ONE - A) + (z3 - z1) * A);
+        const R bdim = bA * (R)di, bdjm = bB * (R)dj;
+        bo1x += Hh * bdim; bo2x += Hh * bdim;
+        bo1y += Hh * bdjm; bo2y += Hh * bdjm;
+        const R bvz = qr * bquz;
+        // ---- move: new offset = old + dt c u_x/(gamma dx) - a
+        const int64_t ia = i;
+        R xa = ((R *)P.xa[s])[ia], ya = ((R *)P.ya[s])[ia];
+        R ua[3] = {((R *)P.uxa[s])[ia], ((R *)P.uya[s])[ia], ((R *)P.uza[s])[ia]};
+        const R xn = xa + bo2x, yn = ya + bo2y;         // adjoint of the new offsets
+        R xo = xn + bo1x, yo = yn + bo1y;               // adjoint of the old offsets
+        // Q_i = k_i un_i / gamma: dQ_i/dun_j = k_i (delta_ij - v_i v_j)/gamma
+        const R kk[3] = {xn * dtcva * idx, yn * dtcva * idy, bvz * cva};
+        const R vk = v[0] * kk[0] + v[1] * kk[1] + v[2] * kk[2];
+        R bun[3];
+        for (int k = 0; k < 3; k++) bun[k] = ua[k] + ig * (kk[k] - v[k] * vk);
+        // ---- Boris reverse
+        R bE[3], bB3[3], bu[3], bum[3], bu0[3], bs[3], bt[3], buq[3];
+        for (int k = 0; k < 3; k++) { bE[k] = c1 * bun[k]; bum[k] = bun[k]; }
+        // up = um + u0 x s
+        cross3(sv, bun, bu0);            // u0_bar = s x up_bar
+        cross3(bun, u0, bs);             // s_bar  = up_bar x u0
+        // u0 = um + uq x t
+        for (int k = 0; k < 3; k++) bum[k] += bu0[k];
+        cross3(t, bu0, buq);             // uq_bar = t x u0_bar
+        cross3(bu0, uq, bt);             // t_bar  = u0_bar x uq
+        // s = t2 t, t2 = 2/(1+t.t)
+        const R bt2 = bs[0] * t[0] + bs[1] * t[1] + bs[2] * t[2];
+        for (int k = 0; k < 3; k++) bt[k] += t2 * bs[k] - t2 * t2 * bt2 * t[k];
+        // t = c2 B, c2 = c0/g1, g1 = sqrt(1+um.um)
+        const R bc2 = bt[0] * Bf[0] + bt[1] * Bf[1] + bt[2] * Bf[2];
+        for (int k = 0; k < 3; k++) bB3[k] = c2 * bt[k];
+        const R bg1 = -c2 / g1 * bc2;
+        for (int k = 0; k < 3; k++) {
+            if (oldu) bu[k] = buq[k];
+            else { bu[k] = 0; bum[k] += buq[k]; }
+        }
+        for (int k = 0; k < 3; k++) bum[k] += bg1 * um[k] / g1;
+        for (int k = 0; k < 3; k++) { bu[k] += bum[k]; bE[k] += c1 * bum[k]; }
+        // ---- gather reverse: force adjoint -> nodes; weights -> position
+        const R bF[6] = {bE[0], bE[1], bE[2], bB3[0], bB3[1], bB3[2]};
+        R dFx = 0, dFy = 0;
+#pragma unroll
+        for (int q = 0; q < 6; q++) {
+            dFx += bF[q] * (wym * (n10[q] - n00[q]) + wyp * (n11[q] - n01[q]));
+            dFy += bF[q] * (wxm * (n01[q] - n00[q]) + wxp * (n11[q] - n10[q]));
+        }
+        xo += dFx;
+        yo += dFy;
+        if (inlds) {
+            const int base = ly * W + lx;
+#pragma unroll
+            for (int q = 0; q < 6; q++) {
+                R *pq = na + (size_t)q * WH;
+                a_lds_add(pq + base, s00 * bF[q]);
+                a_lds_add(pq + base + 1, s10 * bF[q]);
+                a_lds_add(pq + base + W, s01 * bF[q]);
+                a_lds_add(pq + base + W + 1, s11 * bF[q]);
+            }
+        } else {
+            const size_t g00 = (size_t)cy * nx + cx, g10 = (size_t)cy * nx + cx1;
+            const size_t g01 = (size_t)cy1 * nx + cx, g11 = (size_t)cy1 * nx + cx1;
+#pragma unroll
+            for (int q = 0; q < 6; q++) {
+                double *pq = nadj + q * NC;
+                unsafeAtomicAdd(pq + g00, (double)(s00 * bF[q]));
+                unsafeAtomicAdd(pq + g10, (double)(s10 * bF[q]));
+                unsafeAtomicAdd(pq + g01, (double)(s01 * bF[q]));
+                unsafeAtomicAdd(pq + g11, (double)(s11 * bF[q]));
+            }
+        }
+        ((R *)P.xa[s])[ia] = xo;
+        ((R *)P.ya[s])[ia] = yo;
+        ((R *)P.uxa[s])[ia] = bu[0];
+        ((R *)P.uya[s])[ia] = bu[1];
+        ((R *)P.uza[s])[ia] = bu[2];
+    }
+    __syncthreads();
+    if (inlds) {
+        for (int k = tid; k < WH; k += ADJ_THREADS) {
+            const int wj = k / W, wi = k - wj * W;
+            const size_t g = (size_t)a_wrap(y0 + wy0 + wj, ny) * nx + a_wrap(x0 + wx0 + wi, nx);
+#pragma unroll
+            for (int q = 0; q < 6; q++) {
+                const R vq = na[(size_t)q * WH + k];
+                if (vq != (R)0) unsafeAtomicAdd(nadj + q * NC + g, (double)vq);
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------- fields
+// Curl operators of the step, on compact periodic planes:
+//   CE(E): Cx = dEz/dy, Cy = -dEz/dx, Cz = -(dEx/dy - dEy/dx)    (B -= dth CE)
+//   CB(B): Dx = dBz/dy, Dy = -dBz/dx, Dz = dBy/dx - dBx/dy       (E += dtc2 CB)
+// with forward differences toward +1 for dEx/dy, dEy/dx, dBy/dx, dBx/dy and
+// backward differences for dEz/d*, dBz/d* (pic2d.f90:733-736, 790-793).
+// The transposes below swap the difference direction and the sign.
+__device__ __forceinline__ size_t fi(int x, int y, int nx, int ny)
+{
+    x = x < 0 ? x + nx : (x >= nx ? x - nx : x);
+    y = y < 0 ? y + ny : (y >= ny ? y - ny : y);
+    return (size_t)y * nx + x;
+}
+
+// CE^T(Bbar) -> (ex, ey, ez) contributions at (x, y)
+__device__ __forceinline__ void ceT(const double *bx, const double *by, const double *bz, int x,
+                                    int y, int nx, int ny, double idx, double idy, double o[3])
+{
+    const size_t c = fi(x, y, nx, ny);
+    o[0] = -idy * (bz[fi(x, y - 1, nx, ny)] - bz[c]);
+    o[1] = idx * (bz[fi(x - 1, y, nx, ny)] - bz[c]);
+    o[2] = idy * (bx[c] - bx[fi(x, y + 1, nx, ny)]) - idx * (by[c] - by[fi(x + 1, y, nx, ny)]);
+}
+
+// CB^T(Ebar) -> (bx, by, bz) contributions at (x, y)
+__device__ __forceinline__ void cbT(const double *ex, const double *ey, const double *ez, int x,
+                                    int y, int nx, int ny, double idx, double idy, double o[3])
+{
+    const size_t c = fi(x, y, nx, ny);
+    o[0] = -idy * (ez[fi(x, y - 1, nx, ny)] - ez[c]);
+    o[1] = idx * (ez[fi(x - 1, y, nx, ny)] - ez[c]);
+    o[2] = idy * (ex[c] - ex[fi(x, y + 1, nx, ny)]) - idx * (ey[c] - ey[fi(x + 1, y, nx, ny)]);
+}
+
+// Pass A: Ebar' = Ebar^{n+1} + (w/2) J - dth CE^T(Bbar^{n+1});
+//         Jbar  = -dtc2 in0 Ebar' + (w/2)(E^n + E^{n+1}).
+template <typename RJ>
+__global__ void fdtd_adj_a(AdjFieldParams P)
+{
+    const int64_t n = (int64_t)P.nx * P.ny;
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
+         k += (int64_t)gridDim.x * blockDim.x) {
+        const int y = (int)(k / P.nx), x = (int)(k - (int64_t)y * P.nx);
+        double o[3];
+        ceT(P.bxa, P.bya, P.bza, x, y, P.nx, P.ny, P.idx, P.idy, o);
+        const RJ *J[3] = {(const RJ *)P.jx, (const RJ *)P.jy, (const RJ *)P.jz};
+        double *Ea[3] = {P.exa, P.eya, P.eza};
+        const double *E0[3] = {P.e0x, P.e0y, P.e0z}, *E1[3] = {P.e1x, P.e1y, P.e1z};
+        double *Jb[3] = {P.jxa, P.jya, P.jza};
+        double *Ep[3] = {P.etx, P.ety, P.etz};
+        for (int q = 0; q < 3; q++) {
+            const double jr = (double)J[q][k];
+            const double e = Ea[q][k] + 0.5 * P.w * jr - P.dth * o[q];
+            Ep[q][k] = e;
+            Jb[q][k] = -P.dtc2 * P.in0 * e + 0.5 * P.w * (E0[q][k] + E1[q][k]);
+        }
+    }
+}
+
+// Pass B: Bbar_half = Bbar^{n+1} + dtc2 CB^T(Ebar') (in place on Bbar).
+__global__ void fdtd_adj_b(AdjFieldParams P)
+{
+    const int64_t n = (int64_t)P.nx * P.ny;
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
+         k += (int64_t)gridDim.x * blockDim.x) {
+        const int y = (int)(k / P.nx), x = (int)(k - (int64_t)y * P.nx);
+        double o[3];
+        cbT(P.etx, P.ety, P.etz, x, y, P.nx, P.ny, P.idx, P.idy, o);
+        P.bxa[k] += P.dtc2 * o[0];
+        P.bya[k] += P.dtc2 * o[1];
+        P.bza[k] += P.dtc2 * o[2];
+    }
+}
+
+// Pass C: Ebar^n = Ebar' - dth CE^T(Bbar_half) + (w/2) J.
+template <typename RJ>
+__global__ void fdtd_adj_c(AdjFieldParams P)
+{
+    const int64_t n = (int64_t)P.nx * P.ny;
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
+         k += (int64_t)gridDim.x * blockDim.x) {
+        const int y = (int)(k / P.nx), x = (int)(k - (int64_t)y * P.nx);
+        double o[3];
+        ceT(P.bxa, P.bya, P.bza, x, y, P.nx, P.ny, P.idx, P.idy, o);
+        const RJ *J[3] = {(const RJ *)P.jx, (const RJ *)P.jy, (const RJ *)P.jz};
+        double *Ea[3] = {P.exa, P.eya, P.eza};
+        const double *Ep[3] = {P.etx, P.ety, P.etz};
+        for (int q = 0; q < 3; q++) Ea[q][k] = Ep[q][k] - P.dth * o[q] + 0.5 * P.w * (double)J[q][k];
+    }
+}
+
+// Transpose of the node averaging (emfgrid, pic2d.f90:837-845): node adjoint
+// planes -> staggered field adjoints; clears the node planes afterwards.
+__global__ void node_transpose_kernel(AdjFieldParams P, double *nadj)
+{
+    const int nx = P.nx, ny = P.ny;
+    const int64_t n = (int64_t)nx * ny;
+    double *dst[6] = {P.exa, P.eya, P.eza, P.bxa, P.bya, P.bza};
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
+         k += (int64_t)gridDim.x * blockDim.x) {
+        const int y = (int)(k / nx), x = (int)(k - (int64_t)y * nx);
+        const size_t xp = fi(x + 1, y, nx, ny), yp = fi(x, y + 1, nx, ny), xyp = fi(x + 1, y + 1, nx, ny);
+        for (int c = 0; c < 2; c++) {
+            const double *a0 = nadj + (3 * c + 0) * n, *a1 = nadj + (3 * c + 1) * n,
+                         *a2 = nadj + (3 * c + 2) * n;
+            dst[3 * c + 0][k] += 0.5 * (a0[k] + a0[xp]);
+            dst[3 * c + 1][k] += 0.5 * (a1[k] + a1[yp]);
+            dst[3 * c + 2][k] += 0.25 * (a2[k] + a2[xp] + a2[yp] + a2[xyp]);
+        }
+    }
+}
+
+__global__ void zero_f64(double *p, int64_t n)
+{
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
+         k += (int64_t)gridDim.x * blockDim.x)
+        p[k] = 0.0;
+}
+
+// Reverse of a sort: pre[perm[k]] = post[k] for the five adjoint arrays.
+struct Ptr5 {
+    void *p[5];
+};
+
+template <typename R>
+__global__ void unpermute_kernel(const uint32_t *perm, int64_t n, Ptr5 post, Ptr5 pre)
+{
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const uint32_t p = perm[k];
+#pragma unroll
+    for (int q = 0; q < 5; q++) ((R *)pre.p[q])[p] = ((const R *)post.p[q])[k];
+}
+
+size_t adj_lds_bytes(int precision, int maxw)
+{
+    return (size_t)maxw * 12 * (precision == PBPIC_PREC_FP64 ? 8 : 4);
+}
+
+hipError_t launch_push_adj(const AdjParams &p0, int precision, hipStream_t st)
+{
+    AdjParams p = p0;
+    int t = 0;
+    for (int s = 0; s < p.nspecies; s++) {
+        p.sp[s].tile0 = t;
+        t += (int)((p.sp[s].n + ADJ_TILE - 1) / ADJ_TILE);
+    }
+    if (t == 0) return hipSuccess;
+    const size_t lds = adj_lds_bytes(precision, p.maxw);
+    if (precision == PBPIC_PREC_FP64) {
+        (void)hipFuncSetAttribute((const void *)push_adj_kernel<double, false>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        hipLaunchKernelGGL((push_adj_kernel<double, false>), dim3(t), dim3(ADJ_THREADS), lds, st, p);
+    } else {
+        (void)hipFuncSetAttribute((const void *)push_adj_kernel<float, true>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        hipLaunchKernelGGL((push_adj_kernel<float, true>), dim3(t), dim3(ADJ_THREADS), lds, st, p);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_fdtd_adj(const AdjFieldParams &p, int precision, hipStream_t st)
+{
+    const int64_t n = (int64_t)p.nx * p.ny;
+    int nb = (int)((n + 255) / 256);
+    if (nb > 8192) nb = 8192;
+    if (precision == PBPIC_PREC_FP64)
+        hipLaunchKernelGGL(fdtd_adj_a<double>, dim3(nb), dim3(256), 0, st, p);
+    else
+        hipLaunchKernelGGL(fdtd_adj_a<float>, dim3(nb), dim3(256), 0, st, p);
+    hipLaunchKernelGGL(fdtd_adj_b, dim3(nb), dim3(256), 0, st, p);
+    if (precision == PBPIC_PREC_FP64)
+        hipLaunchKernelGGL(fdtd_adj_c<double>, dim3(nb), dim3(256), 0, st, p);
+    else
+        hipLaunchKernelGGL(fdtd_adj_c<float>, dim3(nb), dim3(256), 0, st, p);
+    return hipGetLastError();
+}
+
+hipError_t launch_node_transpose(const AdjFieldParams &p, double *nadj, hipStream_t st)
+{
+    const int64_t n = (int64_t)p.nx * p.ny;
+    int nb = (int)((n + 255) / 256);
+    if (nb > 8192) nb = 8192;
+    hipLaunchKernelGGL(node_transpose_kernel, dim3(nb), dim3(256), 0, st, p, nadj);
+    hipLaunchKernelGGL(zero_f64, dim3(nb), dim3(256), 0, st, nadj, 6 * n);
+    return hipGetLastError();
+}
+
+hipError_t launch_unpermute(const uint32_t *perm, int64_t n, int precision, void *const *post,
+                            void *const *pre, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    const unsigned nb = (unsigned)((n + 255) / 256);
+    Ptr5 a, b;
+    for (int q = 0; q < 5; q++) { a.p[q] = post[q]; b.p[q] = pre[q]; }
+    if (precision == PBPIC_PREC_FP64)
+        hipLaunchKernelGGL(unpermute_kernel<double>, dim3(nb), dim3(256), 0, st, perm, n, a, b);
+    else
+        hipLaunchKernelGGL(unpermute_kernel<float>, dim3(nb), dim3(256), 0, st, perm, n, a, b);
+    return hipGetLastError();
+}
+
+}  // namespace pbpic

@@ -90,6 +90,38 @@ struct FdtdParams {
     double *acc;
 };
 
+// Adjoint of the push for one step (k_adjoint.hip).  The particle state of
+// step n comes from the tape (cell, ox..uz); xa..uza hold the adjoint of the
+// state at n+1 on entry and at n on exit.
+struct AdjSpecies {
+    const int32_t *cell;
+    const void *ox, *oy, *ux, *uy, *uz;
+    int64_t n;
+    int32_t q, tile0;
+    double ms, c0, c1, qx, qy;
+};
+
+struct AdjParams {
+    int nx, ny, compat, maxw, nspecies;
+    double dtcva, idx, idy, cva;
+    const double *ex, *ey, *ez, *bx, *by, *bz;   // fields at step n
+    const double *jadj[3];                       // current adjoint (raw J)
+    double *nadj;                                // 6 node-adjoint planes
+    void *xa[MAX_SPECIES], *ya[MAX_SPECIES], *uxa[MAX_SPECIES], *uya[MAX_SPECIES],
+        *uza[MAX_SPECIES];
+    AdjSpecies sp[MAX_SPECIES];
+};
+
+struct AdjFieldParams {
+    int nx, ny;
+    double dth, dtc2, idx, idy, in0, w;          // w = dt*mime/cva^2
+    double *exa, *eya, *eza, *bxa, *bya, *bza;   // field adjoints (in/out)
+    double *etx, *ety, *etz;                     // scratch: Ebar'
+    double *jxa, *jya, *jza;                     // current adjoint (out)
+    const double *e0x, *e0y, *e0z, *e1x, *e1y, *e1z;   // E^n, E^{n+1}
+    const void *jx, *jy, *jz;                    // J^{n+1/2} (raw)
+};
+
 // Launchers (one per translation unit family).
 hipError_t launch_push(const PushParams &p, int precision, hipStream_t st);
 hipError_t launch_deposit_batch(const PushParams &p, int precision, int64_t n,
@@ -127,6 +159,12 @@ struct SortScratch {
     const uint32_t *perm = nullptr; // permutation of the last sort (in vals)
     int64_t cap = 0, status_cap = 0;
 };
+hipError_t launch_push_adj(const AdjParams &p, int precision, hipStream_t st);
+hipError_t launch_fdtd_adj(const AdjFieldParams &p, int precision, hipStream_t st);
+hipError_t launch_node_transpose(const AdjFieldParams &p, double *nadj, hipStream_t st);
+hipError_t launch_unpermute(const uint32_t *perm, int64_t n, int precision,
+                            void *const *post, void *const *pre, hipStream_t st);
+
 hipError_t sort_species(SortScratch &s, const SpeciesDev &sp, int precision,
                         int64_t ncell, SpeciesDev &dst, int *err,
                         int want_counts, hipStream_t st);

@@ -1,0 +1,99 @@
+"""GPU adjoint: the hand-written adjoint kernels (k_adjoint.hip) through the C
+ABI and the PyTorch binding, against torch's reverse mode applied to the
+differentiable restatement oracle/torch_ref.py (itself pinned to the C
+oracle and to central finite differences in tests/test_torch_ref.py)."""
+import numpy as np
+import pytest
+import torch
+
+from cases import make_case, rel_norm
+from oracle import torch_ref as tr
+
+pytestmark = pytest.mark.gpu
+
+PART = ("dx", "dy", "ux", "uy", "uz")
+
+
+def _case():
+    return make_case(nx=24, ny=16, ppc=6, seed=4, efield=0.0, bfield=0.0, vth=(0.02, 0.1))
+
+
+def _ref_grads(case, A0, ud0, nsteps, ckpt_sort, compat):
+    nx, ny = case["nx"], case["ny"]
+    P = dict(nx=nx, ny=ny, dx=case["dx"], dy=case["dy"], dt=case["dt"], cva=case["cva"],
+             mime=case["mime"], in0=case["in0"])
+    x = torch.arange(nx, dtype=torch.float64)[None, :].expand(ny, nx)
+    F = torch.zeros(6, ny, nx, dtype=torch.float64)
+    F[1] = A0 * torch.sin(2 * np.pi * x / nx)
+    F.requires_grad_(True)
+    sps = []
+    for q, s in zip((1, -1), case["species"]):
+        d = dict(q=q, cell=torch.tensor(s["grid"] - 1, dtype=torch.long))
+        for k in PART:
+            v = torch.tensor(s[k])
+            if q < 0 and k == "ux":
+                v = v + ud0
+            d[k] = v.clone().requires_grad_(True)
+        sps.append(d)
+    loss, *_ = tr.run(P, F[:3], F[3:], sps, nsteps, compat=compat, sort_every=ckpt_sort)
+    loss.backward()
+    return loss.item(), F.grad.numpy(), [{k: sp[k].grad.numpy() for k in PART} for sp in sps]
+
+
+def _gpu_grads(case, A0, ud0, nsteps, ckpt, sort_every, compat, precision):
+    import pbpic_amd
+    from pbpic_amd.autograd import jdote
+    nx, ny = case["nx"], case["ny"]
+    sim = pbpic_amd.Context(nx, ny, case["dx"], case["dy"], case["dt"], case["cva"], case["mime"],
+                            case["in0"], precision=precision, compat=compat, sort_every=sort_every,
+                            deposit_rho=False)
+    dt = torch.float64 if precision == 0 else torch.float32
+    species = []
+    for q, s in zip((1, -1), case["species"]):
+        sim.add_species(q, len(s["grid"]))
+        d = {"cell": torch.tensor(s["grid"] - 1, dtype=torch.int32, device="cuda")}
+        for k in PART:
+            v = torch.tensor(s[k], dtype=torch.float64)
+            if q < 0 and k == "ux":
+                v = v + ud0
+            d[k] = v.to(dt).cuda().requires_grad_(True)
+        species.append(d)
+    x = torch.arange(nx, dtype=torch.float64, device="cuda")[None, :].expand(ny, nx)
+    F = torch.zeros(6, ny, nx, dtype=torch.float64, device="cuda")
+    F[1] = A0 * torch.sin(2 * np.pi * x / nx)
+    F.requires_grad_(True)
+    loss = jdote(sim, nsteps, ckpt, F, species)
+    loss.backward()
+    out = (loss.item(), F.grad.cpu().numpy(),
+           [{k: sp[k].grad.double().cpu().numpy() for k in PART} for sp in species])
+    sim.close()
+    return out
+
+
+@pytest.mark.parametrize("compat", [7, 0])
+def test_adjoint_fp64_matches_reverse_mode(gpu, compat):
+    """All gradients (6 initial field planes, 5 arrays per species) of a 10-step
+    run with a checkpoint every 4 steps and a sort every 5 steps."""
+    print(f"adjoint fp64 compat={compat}", flush=True)
+    case = _case()
+    L, gF, gP = _ref_grads(case, 0.01, 0.05, 10, 5, compat)
+    Lg, gFg, gPg = _gpu_grads(case, 0.01, 0.05, 10, 4, 5, compat, 0)
+    assert abs(Lg - L) <= 1e-10 * abs(L)
+    for k in range(6):
+        assert rel_norm(gFg[k], gF[k]) < 1e-8, k
+    for s in range(2):
+        for k in PART:
+            assert rel_norm(gPg[s][k], gP[s][k]) < 1e-8, (s, k)
+
+
+def test_adjoint_split32_near_fp64(gpu):
+    """The split-precision adjoint against the FP64 reference gradient."""
+    case = _case()
+    L, gF, gP = _ref_grads(case, 0.01, 0.05, 10, 5, 7)
+    Lg, gFg, gPg = _gpu_grads(case, 0.01, 0.05, 10, 4, 5, 7, 1)
+    assert abs(Lg - L) <= 1e-4 * abs(L)
+    for k in range(6):
+        assert rel_norm(gFg[k], gF[k]) < 1e-3, k
+    for s in range(2):
+        for k in PART:
+            assert rel_norm(gPg[s][k], gP[s][k]) < 1e-3, (s, k)
