@@ -132,6 +132,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=20260101)
+    ap.add_argument("--calib-copy-gib", type=float, default=0.0,
+                    help="first run one plain device copy of this many GiB (a known-size "
+                         "kernel that calibrates rocprof FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -143,6 +146,16 @@ def main():
     from pbpic_amd import configs
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     cfg = configs.two_stream_c2(args.nx, args.ny, args.ppc)
+    if args.calib_copy_gib > 0:
+        nb = int(args.calib_copy_gib * (1 << 30))
+        a = torch.ones(nb, dtype=torch.uint8, device="cuda")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        log(f"[bench] calibration copy of {nb} bytes done")
+        del a, b
+        torch.cuda.empty_cache()
     forms = [f.strip() for f in args.forms.split(",") if f.strip()]
     res = {}
     for f in forms:

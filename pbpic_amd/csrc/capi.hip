@@ -439,7 +439,7 @@ int pbpic_ctx_create(const pbpic_params *pp, pbpic_ctx **out)
     pbpic_ctx *c = new pbpic_ctx();
     c->p = p;
     c->rsz = p.precision == PBPIC_PREC_FP64 ? 8 : 4;
-    c->maxw = p.precision == PBPIC_PREC_FP64 ? 800 : 1280;
+    c->maxw = p.precision == PBPIC_PREC_FP64 ? 800 : 896;
     c->idx = 1.0 / p.dx;                   // pic2d.f90:16
     c->idy = 1.0 / p.dy;
     c->dth = p.dt / 2.0;                   // :43

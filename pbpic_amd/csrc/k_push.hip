@@ -247,8 +247,56 @@ __device__ __forceinline__ void push_math(const PushParams &P, R c0, R c1,
     gam = gama;
 }
 
-template <typename R>
-__device__ __forceinline__ R ldg_r(const void *p, int64_t i) { return ((const R *)p)[i]; }
+// LDS node record: 6 field components, padded to 8 floats in FP32 so that
+// every record starts 16-byte aligned (48-byte FP64 records already are).
+template <typename R> struct NodeStride { static constexpr int v = 6; };
+template <> struct NodeStride<float> { static constexpr int v = 8; };
+
+// Split-precision form of push_math: the same scheme in FP32 with one
+// reciprocal square root shared by every 1/gamma and contracted multiply-adds;
+// the kinetic energy is returned as u^2/(gamma+1), which equals gamma-1
+// without the FP32 cancellation of a small gamma-1.
+__device__ __forceinline__ void push_math_split(const PushParams &P, float c0, float c1,
+                                                float &ox, float &oy, float &ux, float &uy,
+                                                float &uz, const float *n00, const float *n10,
+                                                const float *n01, const float *n11, int &dcx,
+                                                int &dcy, float &vz, float &kin, int &err)
+{
+#pragma clang fp contract(fast)
+    const float wx = 0.5f + ox, wy = 0.5f + oy;
+    const float s11 = wx * wy, s10 = wx - s11, s01 = wy - s11, s00 = 1.0f - wx - s01;
+    float f[6];
+#pragma unroll
+    for (int k = 0; k < 6; k++) f[k] = n00[k] * s00 + n10[k] * s10 + n01[k] * s01 + n11[k] * s11;
+    const float umx = ux + c1 * f[0], umy = uy + c1 * f[1], umz = uz + c1 * f[2];
+    const float ig1 = rsqrtf(1.0f + umx * umx + umy * umy + umz * umz);
+    const float c2 = c0 * ig1;
+    const float tx = c2 * f[3], ty = c2 * f[4], tz = c2 * f[5];
+    const float t2 = 2.0f * __builtin_amdgcn_rcpf(1.0f + tx * tx + ty * ty + tz * tz);
+    const float sx = t2 * tx, sy = t2 * ty, sz = t2 * tz;
+    float qx = umx, qy = umy, qz = umz;
+    if (P.compat & PBPIC_COMPAT_BORIS_OLD_U) { qx = ux; qy = uy; qz = uz; }
+    const float u0x = umx + (qy * tz - qz * ty);
+    const float u0y = umy + (qz * tx - qx * tz);
+    const float u0z = umz + (qx * ty - qy * tx);
+    ux = umx + (u0y * sz - u0z * sy) + c1 * f[0];
+    uy = umy + (u0z * sx - u0x * sz) + c1 * f[1];
+    uz = umz + (u0x * sy - u0y * sx) + c1 * f[2];
+    const float u2 = ux * ux + uy * uy + uz * uz;
+    const float ig = rsqrtf(1.0f + u2);
+    const float c3 = (float)P.dtcva * ig;
+    const float dx1 = ox + c3 * ux * (float)P.idx;
+    const float dy1 = oy + c3 * uy * (float)P.idy;
+    dcx = dx1 > 0.5f ? 1 : (dx1 < -0.5f ? -1 : 0);
+    dcy = dy1 > 0.5f ? 1 : (dy1 < -0.5f ? -1 : 0);
+    ox = dx1 - (float)dcx;
+    oy = dy1 - (float)dcy;
+    if (fabsf(ox) > 0.5f || fabsf(oy) > 0.5f) err |= ERRF_CFL;
+    if (!isfinite(u2)) err |= ERRF_NONFINITE;
+    vz = uz * (float)P.cva * ig;
+    const float gam = __builtin_amdgcn_rcpf(ig);
+    kin = u2 * __builtin_amdgcn_rcpf(gam + 1.0f);
+}
 
 template <typename R, bool SPLIT>
 __global__ __launch_bounds__(PUSH_THREADS, 2) void push_kernel(PushParams P)
@@ -308,8 +356,11 @@ __global__ __launch_bounds__(PUSH_THREADS, 2) void push_kernel(PushParams P)
     const bool inlds = mnx <= mxx && (int64_t)W * H <= P.maxw;
     const int WH = inlds ? W * H : 0;
     const int njc = P.deposit_rho ? 4 : 3;
-    R *nf = (R *)smem;                       // [WH][6] node fields
-    R *jw = nf + (size_t)6 * P.maxw;         // [njc][WH] current window
+    constexpr int NS = NodeStride<R>::v;
+    R *nf = (R *)smem;                       // [WH][NS] node fields
+    // current window accumulated in FP64 in both forms: LDS f64 atomic adds
+    // sustain same-address traffic far better than f32 ones on gfx950
+    double *jw = (double *)(nf + (size_t)NS * P.maxw);   // [njc][WH]
 
     // ---- B: stage node fields, zero the current window
     if (inlds) {
@@ -319,9 +370,9 @@ __global__ __launch_bounds__(PUSH_THREADS, 2) void push_kernel(PushParams P)
             double f[6];
             node_fields(P, gx, gy, f);
 #pragma unroll
-            for (int q = 0; q < 6; q++) nf[(size_t)k * 6 + q] = (R)f[q];
+            for (int q = 0; q < 6; q++) nf[(size_t)k * NS + q] = (R)f[q];
         }
-        for (int k = tid; k < njc * WH; k += PUSH_THREADS) jw[k] = (R)0;
+        for (int k = tid; k < njc * WH; k += PUSH_THREADS) jw[k] = 0.0;
     }
     __syncthreads();
 
@@ -345,13 +396,13 @@ __global__ __launch_bounds__(PUSH_THREADS, 2) void push_kernel(PushParams P)
         const int lx = rx - wx0, ly = ry - wy0;
         R n00[6], n10[6], n01[6], n11[6];
         if (inlds) {
-            const R *b = nf + (size_t)(ly * W + lx) * 6;
+            const R *b = nf + (size_t)(ly * W + lx) * NS;
 #pragma unroll
             for (int q = 0; q < 6; q++) {
                 n00[q] = b[q];
-                n10[q] = b[6 + q];
-                n01[q] = b[(size_t)W * 6 + q];
-                n11[q] = b[(size_t)W * 6 + 6 + q];
+                n10[q] = b[NS + q];
+                n01[q] = b[(size_t)W * NS + q];
+                n11[q] = b[(size_t)W * NS + NS + q];
             }
         } else {
             const int cx1 = cx + 1 == nx ? 0 : cx + 1, cy1 = cy + 1 == ny ? 0 : cy + 1;
@@ -367,9 +418,15 @@ __global__ __launch_bounds__(PUSH_THREADS, 2) void push_kernel(PushParams P)
         }
         const R dx0 = ox, dy0 = oy;
         int dcx, dcy;
-        R vz, gam;
-        push_math<R>(P, c0, c1, ox, oy, ux, uy, uz, n00, n10, n01, n11, dcx, dcy,
-                     vz, gam, err);
+        R vz, gam, kin;
+        if constexpr (SPLIT) {
+            push_math_split(P, c0, c1, ox, oy, ux, uy, uz, n00, n10, n01, n11, dcx, dcy, vz,
+                            kin, err);
+        } else {
+            push_math<R>(P, c0, c1, ox, oy, ux, uy, uz, n00, n10, n01, n11, dcx, dcy, vz, gam,
+                         err);
+            kin = gam - (R)1;                                                      // :1062
+        }
         Dep<R> d;
         deposit_terms<R, SPLIT>(cx + 1, cy + 1, dcx, dcy, dx0, ox, dy0, oy, qx, qy,
                                 qr, vz, P.compat, want_rho, d);
@@ -380,28 +437,28 @@ __global__ __launch_bounds__(PUSH_THREADS, 2) void push_kernel(PushParams P)
         }
         if (inlds) {
             const int base = ly * W + lx;
-            R *wx = jw, *wy = jw + WH, *wz = jw + 2 * WH;
-            lds_add(wx + base, d.jxa0);
-            lds_add(wx + base + W, d.jxa1);
-            lds_add(wy + base, d.jya0);
-            lds_add(wy + base + 1, d.jya1);
+            double *wx = jw, *wy = jw + WH, *wz = jw + 2 * WH;
+            lds_add(wx + base, (double)d.jxa0);
+            lds_add(wx + base + W, (double)d.jxa1);
+            lds_add(wy + base, (double)d.jya0);
+            lds_add(wy + base + 1, (double)d.jya1);
             if (d.a != 0 || d.b != 0) {
                 const int bb = base + d.b * W + d.a;
-                lds_add(wx + bb, d.jxb0);
-                lds_add(wx + bb + W, d.jxb1);
-                lds_add(wy + bb, d.jyb0);
-                lds_add(wy + bb + 1, d.jyb1);
+                lds_add(wx + bb, (double)d.jxb0);
+                lds_add(wx + bb + W, (double)d.jxb1);
+                lds_add(wy + bb, (double)d.jyb0);
+                lds_add(wy + bb + 1, (double)d.jyb1);
             }
-            lds_add(wz + base, d.jz0);
-            lds_add(wz + base + d.di, d.jz1);
-            lds_add(wz + base + d.dj * W, d.jz2);
-            lds_add(wz + base + d.dj * W + d.di, d.jz3);
+            lds_add(wz + base, (double)d.jz0);
+            lds_add(wz + base + d.di, (double)d.jz1);
+            lds_add(wz + base + d.dj * W, (double)d.jz2);
+            lds_add(wz + base + d.dj * W + d.di, (double)d.jz3);
             if (want_rho) {
-                R *wr = jw + 3 * WH;
-                lds_add(wr + base, d.r0);
-                lds_add(wr + base + 1, d.r1);
-                lds_add(wr + base + W, d.r2);
-                lds_add(wr + base + W + 1, d.r3);
+                double *wr = jw + 3 * WH;
+                lds_add(wr + base, (double)d.r0);
+                lds_add(wr + base + 1, (double)d.r1);
+                lds_add(wr + base + W, (double)d.r2);
+                lds_add(wr + base + W + 1, (double)d.r3);
             }
         } else {
             const int xm = cx == 0 ? nx - 1 : cx - 1, xp = cx + 1 == nx ? 0 : cx + 1;
@@ -412,25 +469,25 @@ __global__ __launch_bounds__(PUSH_THREADS, 2) void push_kernel(PushParams P)
                 int gy = ddy == 2 ? (yp + 1 == ny ? 0 : yp + 1) : ys[ddy + 1];
                 return (size_t)gy * nx + gx;
             };
-            gadd(gjx + at(0, 0), d.jxa0);
-            gadd(gjx + at(0, 1), d.jxa1);
-            gadd(gjy + at(0, 0), d.jya0);
-            gadd(gjy + at(1, 0), d.jya1);
+            gadd(gjx + at(0, 0), (double)d.jxa0);
+            gadd(gjx + at(0, 1), (double)d.jxa1);
+            gadd(gjy + at(0, 0), (double)d.jya0);
+            gadd(gjy + at(1, 0), (double)d.jya1);
             if (d.a != 0 || d.b != 0) {
-                gadd(gjx + at(d.a, d.b), d.jxb0);
-                gadd(gjx + at(d.a, d.b + 1), d.jxb1);
-                gadd(gjy + at(d.a, d.b), d.jyb0);
-                gadd(gjy + at(d.a + 1, d.b), d.jyb1);
+                gadd(gjx + at(d.a, d.b), (double)d.jxb0);
+                gadd(gjx + at(d.a, d.b + 1), (double)d.jxb1);
+                gadd(gjy + at(d.a, d.b), (double)d.jyb0);
+                gadd(gjy + at(d.a + 1, d.b), (double)d.jyb1);
             }
-            gadd(gjz + at(0, 0), d.jz0);
-            gadd(gjz + at(d.di, 0), d.jz1);
-            gadd(gjz + at(0, d.dj), d.jz2);
-            gadd(gjz + at(d.di, d.dj), d.jz3);
+            gadd(gjz + at(0, 0), (double)d.jz0);
+            gadd(gjz + at(d.di, 0), (double)d.jz1);
+            gadd(gjz + at(0, d.dj), (double)d.jz2);
+            gadd(gjz + at(d.di, d.dj), (double)d.jz3);
             if (want_rho) {
-                gadd(grho + at(0, 0), d.r0);
-                gadd(grho + at(1, 0), d.r1);
-                gadd(grho + at(0, 1), d.r2);
-                gadd(grho + at(1, 1), d.r3);
+                gadd(grho + at(0, 0), (double)d.r0);
+                gadd(grho + at(1, 0), (double)d.r1);
+                gadd(grho + at(0, 1), (double)d.r2);
+                gadd(grho + at(1, 1), (double)d.r3);
             }
         }
         // periodic wrap (:1032-1052) and write-back
@@ -439,7 +496,7 @@ __global__ __launch_bounds__(PUSH_THREADS, 2) void push_kernel(PushParams P)
         ncy = ncy < 0 ? ncy + ny : (ncy >= ny ? ncy - ny : ncy);
         S.cell[i] = ncx * ny + ncy;
         ox_p[i] = ox; oy_p[i] = oy; ux_p[i] = ux; uy_p[i] = uy; uz_p[i] = uz;
-        ke += (double)((gam - (R)1) * (R)S.ms);                                // :1062
+        ke += (double)(kin * (R)S.ms);
     }
 
     // ---- D: flush the window, reduce the kinetic energy and error flags
@@ -449,11 +506,11 @@ __global__ __launch_bounds__(PUSH_THREADS, 2) void push_kernel(PushParams P)
             const int wj = k / W, wi = k - wj * W;
             const int gx = wrapi(x0 + wx0 + wi, nx), gy = wrapi(y0 + wy0 + wj, ny);
             const size_t g = (size_t)gy * nx + gx;
-            R v;
-            if ((v = jw[k]) != (R)0) gadd(gjx + g, v);
-            if ((v = jw[WH + k]) != (R)0) gadd(gjy + g, v);
-            if ((v = jw[2 * WH + k]) != (R)0) gadd(gjz + g, v);
-            if (want_rho && (v = jw[3 * WH + k]) != (R)0) gadd(grho + g, v);
+            double v;
+            if ((v = jw[k]) != 0.0) gadd(gjx + g, (R)v);
+            if ((v = jw[WH + k]) != 0.0) gadd(gjy + g, (R)v);
+            if ((v = jw[2 * WH + k]) != 0.0) gadd(gjz + g, (R)v);
+            if (want_rho && (v = jw[3 * WH + k]) != 0.0) gadd(grho + g, (R)v);
         }
     }
 #pragma unroll
@@ -498,23 +555,23 @@ __global__ void deposit_batch_kernel(PushParams P, int64_t n, const int32_t *i1,
         return (size_t)wrapi(cy + ddy, ny) * nx + wrapi(cx + ddx, nx);
     };
     R *gjx = (R *)P.jx, *gjy = (R *)P.jy, *gjz = (R *)P.jz, *grho = (R *)P.rho;
-    gadd(gjx + at(0, 0), d.jxa0);
-    gadd(gjx + at(0, 1), d.jxa1);
-    gadd(gjy + at(0, 0), d.jya0);
-    gadd(gjy + at(1, 0), d.jya1);
-    gadd(gjx + at(a, b), d.jxb0);
-    gadd(gjx + at(a, b + 1), d.jxb1);
-    gadd(gjy + at(a, b), d.jyb0);
-    gadd(gjy + at(a + 1, b), d.jyb1);
-    gadd(gjz + at(0, 0), d.jz0);
-    gadd(gjz + at(d.di, 0), d.jz1);
-    gadd(gjz + at(0, d.dj), d.jz2);
-    gadd(gjz + at(d.di, d.dj), d.jz3);
+    gadd(gjx + at(0, 0), (double)d.jxa0);
+    gadd(gjx + at(0, 1), (double)d.jxa1);
+    gadd(gjy + at(0, 0), (double)d.jya0);
+    gadd(gjy + at(1, 0), (double)d.jya1);
+    gadd(gjx + at(a, b), (double)d.jxb0);
+    gadd(gjx + at(a, b + 1), (double)d.jxb1);
+    gadd(gjy + at(a, b), (double)d.jyb0);
+    gadd(gjy + at(a + 1, b), (double)d.jyb1);
+    gadd(gjz + at(0, 0), (double)d.jz0);
+    gadd(gjz + at(d.di, 0), (double)d.jz1);
+    gadd(gjz + at(0, d.dj), (double)d.jz2);
+    gadd(gjz + at(d.di, d.dj), (double)d.jz3);
     if (P.deposit_rho) {
-        gadd(grho + at(0, 0), d.r0);
-        gadd(grho + at(1, 0), d.r1);
-        gadd(grho + at(0, 1), d.r2);
-        gadd(grho + at(1, 1), d.r3);
+        gadd(grho + at(0, 0), (double)d.r0);
+        gadd(grho + at(1, 0), (double)d.r1);
+        gadd(grho + at(0, 1), (double)d.r2);
+        gadd(grho + at(1, 1), (double)d.r3);
     }
 }
 
@@ -533,7 +590,8 @@ static int push_tiles(const PushParams &p, PushParams &q)
 size_t push_lds_bytes(int precision, int maxw, int deposit_rho)
 {
     const size_t r = precision == PBPIC_PREC_FP64 ? 8 : 4;
-    return (size_t)maxw * (6 + (deposit_rho ? 4 : 3)) * r;
+    const size_t ns = precision == PBPIC_PREC_FP64 ? 6 : 8;
+    return (size_t)maxw * (ns * r + (deposit_rho ? 4 : 3) * 8);
 }
 
 hipError_t launch_push(const PushParams &p, int precision, hipStream_t st)

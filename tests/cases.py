@@ -37,10 +37,13 @@ def oracle_for(case, prec=8, compat=oracle.COMPAT_REFERENCE):
     o = oracle.Oracle(case["nx"], case["ny"], case["dx"], case["dy"], case["dt"],
                       case["cva"], case["mime"], case["in0"], prec=prec, compat=compat)
     sp = case["species"]
-    cat = lambda k: np.concatenate([s[k] for s in sp])
-    q = np.concatenate([np.full(len(s["grid"]), qq, np.int32) for s, qq in zip(sp, SPECIES_Q)])
-    o.set_particles(cat("grid"), cat("dx"), cat("dy"), cat("ux"), cat("uy"), cat("uz"), q,
-                    cat("pid"))
+    if sp:
+        cat = lambda k: np.concatenate([s[k] for s in sp])
+        q = np.concatenate([np.full(len(s["grid"]), qq, np.int32) for s, qq in zip(sp, SPECIES_Q)])
+        o.set_particles(cat("grid"), cat("dx"), cat("dy"), cat("ux"), cat("uy"), cat("uz"), q,
+                        cat("pid"))
+    else:
+        o.set_particles([], [], [], [], [], [], [])
     for name, plane in case["fields"].items():
         o.set_field(name, plane)
     return o
