@@ -87,6 +87,65 @@ def step_roofline(precision, res, cfg, sort_every):
                 frac=gbs / HBM_PEAK_GBS)
 
 
+def adjoint_leg(precision, seed, nsteps=200, ckpt=20):
+    """The differentiable-PIC configuration: 256x256 periodic thermal plasma
+    with the CPU-case parameters, 32 particles/cell/species, Ey = A sin(2 pi
+    x/Lx) with A = 0.01, electron drift u_x = 0.05c, loss = integrated J.E
+    over 200 steps, checkpoints every 20 steps.  Times the plain forward (the
+    step loop) against the checkpointed forward and the adjoint sweep."""
+    import numpy as np
+    import torch
+    import pbpic_amd
+    from pbpic_amd import configs
+
+    cfg = configs.thermal_c0(256, 256, 32)
+    x = np.arange(cfg["nx"])[None, :] * np.ones((cfg["ny"], 1))
+    ey = 0.01 * np.sin(2 * np.pi * x / cfg["nx"])
+
+    def make():
+        ctx = pbpic_amd.Context(cfg["nx"], cfg["ny"], cfg["dx"], cfg["dy"], cfg["dt"], cfg["cva"],
+                                cfg["mime"], cfg["in0"], precision=precision, sort_every=50,
+                                deposit_rho=False)
+        for k, sp in enumerate(cfg["species"]):
+            ctx.add_species(sp["q"], cfg["nx"] * cfg["ny"] * sp["ppc"])
+            ctx.init_maxwellian(k, sp["ppc"], sp["vth"], 0.05 if sp["q"] < 0 else 0.0,
+                                seed=seed, stream=k)
+        ctx.set_field("ey", ey)
+        ctx.sync()
+        return ctx
+
+    ctx = make()
+    ctx.step(5)                        # warm the kernels up on a throw-away copy
+    ctx.close()
+    ctx = make()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ctx.step(nsteps)
+    ctx.sync()
+    t_fwd = time.perf_counter() - t0
+    ctx.close()
+
+    ctx = make()
+    npart = [ctx.num_particles(k) for k in range(len(cfg["species"]))]
+    pdt = torch.float64 if precision == 0 else torch.float32
+    fg = torch.zeros(6, cfg["ny"], cfg["nx"], dtype=torch.float64, device="cuda")
+    pg = [torch.zeros(n, dtype=pdt, device="cuda") for n in npart for _ in range(5)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    loss = ctx.adjoint_forward(nsteps, ckpt)
+    ctx.sync()
+    t1 = time.perf_counter()
+    ctx.adjoint_backward([fg[k].data_ptr() for k in range(6)], [t.data_ptr() for t in pg])
+    ctx.sync()
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    ctx.close()
+    gA = float((fg[1].cpu().numpy() * (ey / 0.01)).sum())     # dL/dA through Ey = A sin
+    return dict(forward_s=t_fwd, ckpt_forward_s=t1 - t0, adjoint_s=t2 - t1,
+                ratio=(t2 - t1) / t_fwd, ratio_total=(t2 - t0) / t_fwd, loss=loss, dloss_dA=gA,
+                particles=sum(npart), steps=nsteps, ckpt_every=ckpt)
+
+
 def cpu_baseline(budget_s, seed):
     """The oracle (CPU restatement of pic2d.f90, AoS `particle` records in the
     reference's default FP32) on a bounded sample of the same workload: the
@@ -131,6 +190,7 @@ def main():
     ap.add_argument("--forms", default="split,fp64")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--seed", type=int, default=20260101)
     ap.add_argument("--calib-copy-gib", type=float, default=0.0,
                     help="first run one plain device copy of this many GiB (a known-size "
@@ -197,6 +257,17 @@ def main():
                        "kernel_ms": {k: (v[0] / max(v[1], 1), v[1]) for k, v in r["prof"].items()}}
         out["ratio_split_vs_fp64"] = res[head]["ps"] / r["ps"]
     out["adjoint_forward_ratio"] = None
+    if not args.no_adjoint:
+        adj = {}
+        for f in forms:
+            log(f"[bench] adjoint leg form={f}")
+            adj[f] = adjoint_leg(1 if f == "split" else 0, args.seed)
+            log(f"[bench] adjoint form={f}: {adj[f]}")
+        out["adjoint"] = dict(workload="differentiable PIC: 256x256 thermal plasma, 32 ppc/species, "
+                                       "200 steps, checkpoint every 20, loss = integrated J.E",
+                              **adj)
+        # the larger of the forms' adjoint-sweep / plain-forward time ratios
+        out["adjoint_forward_ratio"] = max(a["ratio"] for a in adj.values())
     out["diag"] = {f: {k: res[f]["diag1"][k] for k in ("ke_e", "w_e", "w_b", "je_int", "gauss_dlinf")}
                    for f in res}
     if not args.no_cpu:

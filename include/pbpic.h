@@ -75,6 +75,15 @@ extern "C" {
                                         (pic2d.f90:1283-1295)               */
 #define PBPIC_FLAG_TRACK_PID    0x2  /* keep a per-particle id array that the
                                         sort permutes (ptl%pid, :47-50)     */
+#define PBPIC_FLAG_PUSH_ORDER   0x4  /* sorts lay every full 4096-particle
+                                        tile out in the push kernel's lane
+                                        order: the tile holds the same
+                                        particles as the reference's cell
+                                        order (:1322-1327), permuted inside
+                                        the tile; the returned permutation
+                                        and counts describe that order.
+                                        Clear it for the reference's exact
+                                        order.  Set by pbpic_params_default */
 
 /* ------------------------------------------------------------ field names */
 #define PBPIC_EX   0
@@ -286,7 +295,7 @@ int pbpic_adjoint_backward(pbpic_ctx *ctx, double *const *field_grads,
  * pbpic_profile_read waits for the stream and returns, per PBPIC_PROF_* slot,
  * the summed milliseconds and the number of launches timed.                 */
 #define PBPIC_PROF_PUSH   0   /* push kernel (gather+Boris+move+deposit)      */
-#define PBPIC_PROF_FIELD  1   /* FDTD kernel                                  */
+#define PBPIC_PROF_FIELD  1   /* FDTD kernel and the push's node-field build  */
 #define PBPIC_PROF_SORT   2   /* counting sort incl. payload gather           */
 #define PBPIC_PROF_ZERO   3   /* current clear                                */
 #define PBPIC_PROF_ADJ    4   /* adjoint backward (replay + reverse sweep)    */

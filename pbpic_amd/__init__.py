@@ -23,7 +23,7 @@ LIB_PATH = os.path.join(_HERE, "libpbpic.so")
 PREC_FP64, PREC_SPLIT32 = 0, 1
 COMPAT_BORIS_OLD_U, COMPAT_RHO_MIRROR, COMPAT_JY_DROP = 1, 2, 4
 COMPAT_REFERENCE = 7
-FLAG_DEPOSIT_RHO, FLAG_TRACK_PID = 1, 2
+FLAG_DEPOSIT_RHO, FLAG_TRACK_PID, FLAG_PUSH_ORDER = 1, 2, 4
 FIELDS = {"ex": 0, "ey": 1, "ez": 2, "bx": 3, "by": 4, "bz": 5, "jx": 6,
           "jy": 7, "jz": 8, "rho": 9, "gex": 10, "gey": 11, "gez": 12,
           "gbx": 13, "gby": 14, "gbz": 15}
@@ -129,14 +129,15 @@ class Context:
 
     def __init__(self, nx, ny, dx, dy, dt, cva, mime, in0, *, precision=PREC_FP64,
                  compat=COMPAT_REFERENCE, sort_every=50, deposit_rho=True,
-                 track_pid=False, rho_background=0.0):
+                 track_pid=False, push_order=True, rho_background=0.0):
         self._lib = load_library()
         p = Params()
         self._lib.pbpic_params_default(ctypes.byref(p))
         p.nx, p.ny, p.dx, p.dy, p.dt = nx, ny, dx, dy, dt
         p.cva, p.mime, p.in0 = cva, mime, in0
         p.precision, p.compat, p.sort_every = precision, compat, sort_every
-        p.flags = (FLAG_DEPOSIT_RHO if deposit_rho else 0) | (FLAG_TRACK_PID if track_pid else 0)
+        p.flags = ((FLAG_DEPOSIT_RHO if deposit_rho else 0) | (FLAG_TRACK_PID if track_pid else 0) |
+                   (FLAG_PUSH_ORDER if push_order else 0))
         p.rho_background = rho_background
         self.params = p
         self.nx, self.ny = nx, ny

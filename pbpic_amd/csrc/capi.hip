@@ -37,6 +37,7 @@ struct pbpic_ctx {
     double *E[3] = {nullptr, nullptr, nullptr};
     double *B[3] = {nullptr, nullptr, nullptr};
     void *J[4] = {nullptr, nullptr, nullptr, nullptr};
+    void *NF = nullptr;        // node fields for the push (rebuilt before each push)
     double *acc = nullptr;
     int *err = nullptr;
     double *rho_scratch = nullptr, *gauss0 = nullptr;
@@ -95,6 +96,14 @@ hipError_t dalloc(T **p, size_t bytes)
     return e;
 }
 
+// 2^(29 - e) for M = m 2^e, 1 <= m < 2: the largest power of two s with
+// M s < 2^30.
+static float fx_scale(double M)
+{
+    if (!(M > 0) || !std::isfinite(M)) return 1.0f;
+    return (float)std::ldexp(1.0, 29 - std::ilogb(M));
+}
+
 SpeciesDev species_dev(pbpic_ctx *c, int s, int buf)
 {
     Species &S = c->sp[s];
@@ -106,6 +115,13 @@ SpeciesDev species_dev(pbpic_ctx *c, int s, int buf)
     d.n = S.n;
     d.q = S.q;
     d.ms = S.ms; d.c0 = S.c0; d.c1 = S.c1; d.qx = S.qx; d.qy = S.qy;
+    // bounds of one deposit contribution: |jx| <= |q dx/dt|, |jy| <= |q dy/dt|
+    // (segment fractions and weights lie in [-1, 1] and [0, 1], :1244-1257),
+    // |jz| <= |q| cva (|v_z| < c = cva, :1277), |rho| <= |q| (:1283-1295)
+    d.jsx = fx_scale(fabs(S.qx));
+    d.jsy = fx_scale(fabs(S.qy));
+    d.jsz = fx_scale(fabs((double)S.q) * c->p.cva);
+    d.jsr = fx_scale(fabs((double)S.q));
     return d;
 }
 
@@ -126,6 +142,7 @@ PushParams push_params(pbpic_ctx *c)
     P.ex = c->E[0]; P.ey = c->E[1]; P.ez = c->E[2];
     P.bx = c->B[0]; P.by = c->B[1]; P.bz = c->B[2];
     P.jx = c->J[0]; P.jy = c->J[1]; P.jz = c->J[2]; P.rho = c->J[3];
+    P.nf = c->NF;
     P.acc = c->acc;
     P.err = c->err;
     P.nspecies = (int)c->sp.size();
@@ -174,7 +191,8 @@ int sort_one(pbpic_ctx *c, int s, int32_t *counts, int32_t *perm, uint32_t *perm
     Species &S = c->sp[s];
     SpeciesDev src = species_dev(c, s, S.cur), dst = species_dev(c, s, S.cur ^ 1);
     HIPCHK(c, prof_mark(c, PBPIC_PROF_SORT, true));
-    HIPCHK(c, sort_species(c->sort, src, c->p.precision, ncell(c), dst, c->err, counts != nullptr, c->st));
+    HIPCHK(c, sort_species(c->sort, src, c->p.precision, (c->p.flags & PBPIC_FLAG_PUSH_ORDER) != 0,
+                           ncell(c), dst, c->err, counts != nullptr, c->st));
     HIPCHK(c, prof_mark(c, PBPIC_PROF_SORT, false));
     S.cur ^= 1;
     if (perm_dev && S.n > 0)
@@ -206,8 +224,13 @@ hipError_t prof_mark(pbpic_ctx *c, int slot, bool begin)
 int push_impl(pbpic_ctx *c, bool reset_ke)
 {
     if (reset_ke) HIPCHK(c, launch_acc_op(c->acc, 0, 0.0, c->st));
+    const PushParams P = push_params(c);
+    // node fields (emfgrid, pic2d.f90:837-845) are field work: timed as such
+    HIPCHK(c, prof_mark(c, PBPIC_PROF_FIELD, true));
+    HIPCHK(c, launch_nodefield(P, c->p.precision, c->st));
+    HIPCHK(c, prof_mark(c, PBPIC_PROF_FIELD, false));
     HIPCHK(c, prof_mark(c, PBPIC_PROF_PUSH, true));
-    HIPCHK(c, launch_push(push_params(c), c->p.precision, c->st));
+    HIPCHK(c, launch_push(P, c->p.precision, c->st));
     HIPCHK(c, prof_mark(c, PBPIC_PROF_PUSH, false));
     return PBPIC_OK;
 }
@@ -421,7 +444,7 @@ void pbpic_params_default(pbpic_params *p)
     p->in0 = 1.0 / 291.0;        // :31
     p->dt = 0.045 / 25.0;        // :42
     p->sort_every = 50;          // :180
-    p->flags = PBPIC_FLAG_DEPOSIT_RHO;
+    p->flags = PBPIC_FLAG_DEPOSIT_RHO | PBPIC_FLAG_PUSH_ORDER;
 }
 
 int pbpic_ctx_create(const pbpic_params *pp, pbpic_ctx **out)
@@ -457,6 +480,7 @@ int pbpic_ctx_create(const pbpic_params *pp, pbpic_ctx **out)
         chk(dalloc(&c->B[k], nc * 8), "alloc B");
     }
     for (int k = 0; k < 4; k++) chk(dalloc(&c->J[k], nc * c->rsz), "alloc J");
+    chk(dalloc(&c->NF, nodefield_bytes(c->p.precision, nc)), "alloc node fields");
     chk(dalloc(&c->acc, ACC_N * sizeof(double)), "alloc acc");
     chk(dalloc(&c->err, sizeof(int) * 4), "alloc err");
     chk(dalloc(&c->sort.counts, nc * sizeof(int32_t)), "alloc counts");
@@ -489,6 +513,7 @@ int pbpic_ctx_destroy(pbpic_ctx *c)
     auto f = [](void *p) { if (p) (void)hipFree(p); };
     for (int k = 0; k < 3; k++) { f(c->E[k]); f(c->B[k]); }
     for (int k = 0; k < 4; k++) f(c->J[k]);
+    f(c->NF);
     f(c->acc); f(c->err); f(c->rho_scratch); f(c->gauss0);
     for (auto &S : c->sp)
         for (int b = 0; b < 2; b++) {

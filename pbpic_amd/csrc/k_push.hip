@@ -34,12 +34,6 @@
 
 namespace pbpic {
 
-constexpr int PUSH_THREADS = 256;
-constexpr int PUSH_ROUNDS = 16;
-constexpr int PUSH_GROUP = 8;
-constexpr int PUSH_RUN = PUSH_ROUNDS * PUSH_GROUP;        // 128
-constexpr int PUSH_TILE = PUSH_THREADS * PUSH_ROUNDS;     // 4096
-
 __device__ __forceinline__ int xcd_remap(int b, int nb)
 {
     // Workgroups are dispatched round-robin over the 8 XCDs; give each XCD a
@@ -79,6 +73,52 @@ __device__ __forceinline__ void node_fields(const PushParams &P, int gx, int gy,
     f[3] = (P.bx[c] + P.bx[xm]) * 0.5;
     f[4] = (P.by[c] + P.by[ym]) * 0.5;
     f[5] = (P.bz[c] + P.bz[xm] + P.bz[ym] + P.bz[xym]) * 0.25;
+}
+
+// Node-field array (emfgrid, pic2d.f90:837-845): the six node-averaged
+// components of every node, rounded to the push precision R, stored y fastest
+// ((gx*ny + gy) records of NS values) so that a push tile -- a run of
+// consecutive cells cx*ny + cy -- stages its window from contiguous column
+// runs.  One 32 x 32 node tile per workgroup: the six staggered planes (x
+// fastest) are staged with their -x/-y halo in LDS, then written transposed.
+constexpr int NF_T = 32;
+template <typename R, int NS>
+__global__ __launch_bounds__(256) void nodefield_kernel(PushParams P, R *out)
+{
+    __shared__ double t[6][NF_T + 1][NF_T + 1];     // [plane][gy - Y0 + 1][gx - X0 + 1]
+    const int nx = P.nx, ny = P.ny;
+    const int X0 = blockIdx.x * NF_T, Y0 = blockIdx.y * NF_T;
+    const double *pl[6] = {P.ex, P.ey, P.ez, P.bx, P.by, P.bz};
+    for (int k = threadIdx.x; k < (NF_T + 1) * (NF_T + 1); k += 256) {
+        const int r = k / (NF_T + 1), cc = k - r * (NF_T + 1);
+        const size_t g = (size_t)wrapi(Y0 - 1 + r, ny) * nx + wrapi(X0 - 1 + cc, nx);
+#pragma unroll
+        for (int q = 0; q < 6; q++) t[q][r][cc] = pl[q][g];
+    }
+    __syncthreads();
+    const int j = threadIdx.x & (NF_T - 1);
+    for (int i = threadIdx.x / NF_T; i < NF_T; i += 256 / NF_T) {
+        const int gx = X0 + i, gy = Y0 + j;
+        if (gx >= nx || gy >= ny) continue;
+        // the same sums, in the same order, as node_fields()
+        R f[NS];
+        f[0] = (R)((t[0][j + 1][i + 1] + t[0][j + 1][i]) * 0.5);
+        f[1] = (R)((t[1][j + 1][i + 1] + t[1][j][i + 1]) * 0.5);
+        f[2] = (R)((t[2][j + 1][i + 1] + t[2][j + 1][i] + t[2][j][i + 1] + t[2][j][i]) * 0.25);
+        f[3] = (R)((t[3][j + 1][i + 1] + t[3][j + 1][i]) * 0.5);
+        f[4] = (R)((t[4][j + 1][i + 1] + t[4][j][i + 1]) * 0.5);
+        f[5] = (R)((t[5][j + 1][i + 1] + t[5][j + 1][i] + t[5][j][i + 1] + t[5][j][i]) * 0.25);
+#pragma unroll
+        for (int q = 6; q < NS; q++) f[q] = (R)0;
+        R *o = out + ((size_t)gx * ny + gy) * NS;
+#pragma unroll
+        for (int q = 0; q < NS; q += 16 / (int)sizeof(R)) {
+            if constexpr (sizeof(R) == 8)
+                *reinterpret_cast<double2 *>(o + q) = make_double2(f[q], f[q + 1]);
+            else
+                *reinterpret_cast<float4 *>(o + q) = make_float4(f[q], f[q + 1], f[q + 2], f[q + 3]);
+        }
+    }
 }
 
 // Deposit contributions of one particle, relative to its old cell (0,0).
@@ -298,11 +338,171 @@ __device__ __forceinline__ void push_math_split(const PushParams &P, float c0, f
     kin = u2 * __builtin_amdgcn_rcpf(gam + 1.0f);
 }
 
+// Copies one NS-value node record with 16-byte moves (records are 16-byte
+// aligned in both the node-field array and the LDS window).
+template <typename R>
+__device__ __forceinline__ void copy_node(R *dst, const R *src)
+{
+    constexpr int NS = NodeStride<R>::v;
+#pragma unroll
+    for (int q = 0; q < NS; q += 16 / (int)sizeof(R))
+        *reinterpret_cast<float4 *>(dst + q) = *reinterpret_cast<const float4 *>(src + q);
+}
+
+// Vector width of the particle streams: each lane moves 16 bytes per array
+// per round (2 doubles or 4 floats); a wave-wide load covers 1 KiB.
+template <typename R> struct Vw;
+template <> struct Vw<double> { static constexpr int V = push_vec(PBPIC_PREC_FP64); };
+template <> struct Vw<float> { static constexpr int V = push_vec(PBPIC_PREC_SPLIT32); };
+
+__device__ __forceinline__ void vld(const double *p, int64_t i0, int64_t n, double (&o)[2])
+{
+    if (i0 + 2 <= n) { const double2 t = *reinterpret_cast<const double2 *>(p + i0); o[0] = t.x; o[1] = t.y; }
+    else { o[0] = i0 < n ? p[i0] : 0.0; o[1] = 0.0; }
+}
+__device__ __forceinline__ void vld(const float *p, int64_t i0, int64_t n, float (&o)[4])
+{
+    if (i0 + 4 <= n) {
+        const float4 t = *reinterpret_cast<const float4 *>(p + i0);
+        o[0] = t.x; o[1] = t.y; o[2] = t.z; o[3] = t.w;
+    } else {
+#pragma unroll
+        for (int j = 0; j < 4; j++) o[j] = i0 + j < n ? p[i0 + j] : 0.0f;
+    }
+}
+__device__ __forceinline__ void vld(const int32_t *p, int64_t i0, int64_t n, int (&o)[2])
+{
+    if (i0 + 2 <= n) { const int2 t = *reinterpret_cast<const int2 *>(p + i0); o[0] = t.x; o[1] = t.y; }
+    else { o[0] = i0 < n ? p[i0] : -1; o[1] = -1; }
+}
+__device__ __forceinline__ void vld(const int32_t *p, int64_t i0, int64_t n, int (&o)[4])
+{
+    if (i0 + 4 <= n) {
+        const int4 t = *reinterpret_cast<const int4 *>(p + i0);
+        o[0] = t.x; o[1] = t.y; o[2] = t.z; o[3] = t.w;
+    } else {
+#pragma unroll
+        for (int j = 0; j < 4; j++) o[j] = i0 + j < n ? p[i0 + j] : -1;
+    }
+}
+__device__ __forceinline__ void vst(double *p, int64_t i0, int64_t n, const double (&v)[2])
+{
+    if (i0 + 2 <= n) *reinterpret_cast<double2 *>(p + i0) = make_double2(v[0], v[1]);
+    else if (i0 < n) p[i0] = v[0];
+}
+__device__ __forceinline__ void vst(float *p, int64_t i0, int64_t n, const float (&v)[4])
+{
+    if (i0 + 4 <= n) *reinterpret_cast<float4 *>(p + i0) = make_float4(v[0], v[1], v[2], v[3]);
+    else {
+#pragma unroll
+        for (int j = 0; j < 4; j++) if (i0 + j < n) p[i0 + j] = v[j];
+    }
+}
+__device__ __forceinline__ void vst(int32_t *p, int64_t i0, int64_t n, const int (&v)[2])
+{
+    if (i0 + 2 <= n) *reinterpret_cast<int2 *>(p + i0) = make_int2(v[0], v[1]);
+    else if (i0 < n) p[i0] = v[0];
+}
+__device__ __forceinline__ void vst(int32_t *p, int64_t i0, int64_t n, const int (&v)[4])
+{
+    if (i0 + 4 <= n) *reinterpret_cast<int4 *>(p + i0) = make_int4(v[0], v[1], v[2], v[3]);
+    else {
+#pragma unroll
+        for (int j = 0; j < 4; j++) if (i0 + j < n) p[i0 + j] = v[j];
+    }
+}
+
+// Unchecked 16-byte loads and stores for full tiles (16-byte aligned).
+__device__ __forceinline__ void ldu(const double *p, double (&o)[2])
+{
+    const double2 t = *reinterpret_cast<const double2 *>(p);
+    o[0] = t.x; o[1] = t.y;
+}
+__device__ __forceinline__ void ldu(const float *p, float (&o)[4])
+{
+    const float4 t = *reinterpret_cast<const float4 *>(p);
+    o[0] = t.x; o[1] = t.y; o[2] = t.z; o[3] = t.w;
+}
+__device__ __forceinline__ void ldu(const int32_t *p, int (&o)[2])
+{
+    const int2 t = *reinterpret_cast<const int2 *>(p);
+    o[0] = t.x; o[1] = t.y;
+}
+__device__ __forceinline__ void ldu(const int32_t *p, int (&o)[4])
+{
+    const int4 t = *reinterpret_cast<const int4 *>(p);
+    o[0] = t.x; o[1] = t.y; o[2] = t.z; o[3] = t.w;
+}
+__device__ __forceinline__ void stu(double *p, const double (&v)[2])
+{
+    *reinterpret_cast<double2 *>(p) = make_double2(v[0], v[1]);
+}
+__device__ __forceinline__ void stu(float *p, const float (&v)[4])
+{
+    *reinterpret_cast<float4 *>(p) = make_float4(v[0], v[1], v[2], v[3]);
+}
+__device__ __forceinline__ void stu(int32_t *p, const int (&v)[2])
+{
+    *reinterpret_cast<int2 *>(p) = make_int2(v[0], v[1]);
+}
+__device__ __forceinline__ void stu(int32_t *p, const int (&v)[4])
+{
+    *reinterpret_cast<int4 *>(p) = make_int4(v[0], v[1], v[2], v[3]);
+}
+
+// One contribution to the LDS current window.  FP64 form: FP64 atomic add.
+// Split form: the contribution arrives pre-multiplied by its plane's scale
+// (a power of two, so the product is exact) and below 2^30 in magnitude; it
+// is rounded to the nearest integer and added as a 64-bit integer, which
+// gfx950's LDS sustains at a higher rate than floating-point atomics and
+// which makes the window sums exact and independent of the lane order.
+__device__ __forceinline__ void jadd(double *p, double v) { lds_add(p, v); }
+__device__ __forceinline__ void jadd(double *p, float v)
+{
+    const long long q = (long long)__float2int_rn(v);
+    __hip_atomic_fetch_add(reinterpret_cast<unsigned long long *>(p), (unsigned long long)q,
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// Adds one particle's deposit to the LDS current window (planes jx, jy, jz
+// and optionally rho, WH nodes each, x fastest, row length W).
+template <typename R>
+__device__ __forceinline__ void deposit_lds(double *jw, int WH, int W, int base,
+                                            const Dep<R> &d, bool want_rho)
+{
+    double *wx = jw, *wy = jw + WH, *wz = jw + 2 * WH;
+    jadd(wx + base, d.jxa0);
+    jadd(wx + base + W, d.jxa1);
+    jadd(wy + base, d.jya0);
+    jadd(wy + base + 1, d.jya1);
+    if (d.a != 0 || d.b != 0) {
+        const int bb = base + d.b * W + d.a;
+        jadd(wx + bb, d.jxb0);
+        jadd(wx + bb + W, d.jxb1);
+        jadd(wy + bb, d.jyb0);
+        jadd(wy + bb + 1, d.jyb1);
+    }
+    jadd(wz + base, d.jz0);
+    jadd(wz + base + d.di, d.jz1);
+    jadd(wz + base + d.dj * W, d.jz2);
+    jadd(wz + base + d.dj * W + d.di, d.jz3);
+    if (want_rho) {
+        double *wr = jw + 3 * WH;
+        jadd(wr + base, d.r0);
+        jadd(wr + base + 1, d.r1);
+        jadd(wr + base + W, d.r2);
+        jadd(wr + base + W + 1, d.r3);
+    }
+}
+
 template <typename R, bool SPLIT>
 __global__ __launch_bounds__(PUSH_THREADS, 2) void push_kernel(PushParams P)
 {
+    constexpr int V = Vw<R>::V;
+    constexpr int CH = 64 * V;                                   // slots per wave-wide load
+    constexpr int ROUNDS = PUSH_TILE / (CH * (PUSH_THREADS / 64));   // 8 (FP64) or 4 (FP32)
     extern __shared__ __align__(16) unsigned char smem[];
-    __shared__ int red[4][4];
+    __shared__ int red[4][5];
     __shared__ double kred[4];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int t = xcd_remap(blockIdx.x, gridDim.x);
@@ -310,26 +510,27 @@ __global__ __launch_bounds__(PUSH_THREADS, 2) void push_kernel(PushParams P)
     while (s + 1 < P.nspecies && t >= P.sp[s + 1].tile0) s++;
     const SpeciesDev S = P.sp[s];
     const int64_t tstart = (int64_t)(t - S.tile0) * PUSH_TILE;
-    const int G = wave * 8 + (lane >> 3), gp = lane & 7;
-    const int64_t pbase = tstart + (int64_t)G * PUSH_RUN + gp;
+    // wave w owns rounds w*ROUNDS .. w*ROUNDS+ROUNDS-1 of the tile (tile_rank)
+    const int64_t pbase = tstart + (int64_t)wave * ROUNDS * CH + lane * V;
     const int nx = P.nx, ny = P.ny, nxh = P.nx >> 1, nyh = P.ny >> 1;
+    const int64_t n = S.n;
 
     // ---- A: bounding box of the tile's cells, unwrapped around its first
     const int c00 = S.cell[tstart];
     const bool c00ok = (unsigned)c00 < (unsigned)(nx * ny);
     const int x0 = c00ok ? (int)P.dny.div((uint32_t)c00) : 0;
     const int y0 = c00ok ? c00 - x0 * ny : 0;
-    int cl[PUSH_ROUNDS];
     int mnx = 1 << 30, mxx = -(1 << 30), mny = 1 << 30, mxy = -(1 << 30);
     int err = 0;
 #pragma unroll
-    for (int r = 0; r < PUSH_ROUNDS; r++) {
-        const int64_t i = pbase + r * PUSH_GROUP;
-        cl[r] = -1;
-        if (i < S.n) {
-            int c = S.cell[i];
+    for (int r = 0; r < ROUNDS; r++) {
+        int cv[V];
+        vld(S.cell, pbase + r * CH, n, cv);
+#pragma unroll
+        for (int j = 0; j < V; j++) {
+            const int c = cv[j];
+            if (pbase + r * CH + j >= n) continue;
             if ((unsigned)c >= (unsigned)(nx * ny)) { err |= ERRF_CELL; continue; }
-            cl[r] = c;
             const int cx = (int)P.dny.div((uint32_t)c), cy = c - cx * ny;
             int rx = cx - x0, ry = cy - y0;
             if (rx > nxh) rx -= nx; else if (rx < -nxh) rx += nx;
@@ -338,15 +539,21 @@ __global__ __launch_bounds__(PUSH_THREADS, 2) void push_kernel(PushParams P)
             mny = min(mny, ry); mxy = max(mxy, ry);
         }
     }
+    int aerr = err;
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
         mnx = min(mnx, __shfl_xor(mnx, o));
         mxx = max(mxx, __shfl_xor(mxx, o));
         mny = min(mny, __shfl_xor(mny, o));
         mxy = max(mxy, __shfl_xor(mxy, o));
+        aerr |= __shfl_xor(aerr, o);
     }
-    if (lane == 0) { red[wave][0] = mnx; red[wave][1] = mxx; red[wave][2] = mny; red[wave][3] = mxy; }
+    if (lane == 0) {
+        red[wave][0] = mnx; red[wave][1] = mxx; red[wave][2] = mny; red[wave][3] = mxy;
+        red[wave][4] = aerr;
+    }
     __syncthreads();
+    aerr = (red[0][4] | red[1][4]) | (red[2][4] | red[3][4]);
     mnx = min(min(red[0][0], red[1][0]), min(red[2][0], red[3][0]));
     mxx = max(max(red[0][1], red[1][1]), max(red[2][1], red[3][1]));
     mny = min(min(red[0][2], red[1][2]), min(red[2][2], red[3][2]));
@@ -355,40 +562,46 @@ __global__ __launch_bounds__(PUSH_THREADS, 2) void push_kernel(PushParams P)
     const int W = mxx - mnx + 4, H = mxy - mny + 4;
     const bool inlds = mnx <= mxx && (int64_t)W * H <= P.maxw;
     const int WH = inlds ? W * H : 0;
+    // fast path: a full tile of valid cells, at least one cell away from every
+    // domain edge (no periodic unwrap or wrap for any particle or move)
+    const bool fast = inlds && aerr == 0 && tstart + PUSH_TILE <= n && x0 + mnx >= 1 &&
+                      x0 + mxx <= nx - 2 && y0 + mny >= 1 && y0 + mxy <= ny - 2;
     const int njc = P.deposit_rho ? 4 : 3;
     constexpr int NS = NodeStride<R>::v;
-    R *nf = (R *)smem;                       // [WH][NS] node fields
-    // current window accumulated in FP64 in both forms: LDS f64 atomic adds
-    // sustain same-address traffic far better than f32 ones on gfx950
-    double *jw = (double *)(nf + (size_t)NS * P.maxw);   // [njc][WH]
+    R *nf = (R *)smem;                       // [W][H][NS] node fields, y fastest
+    // current window [njc][H][W], x fastest: FP64 sums (FP64 form) or 64-bit
+    // fixed-point sums (split form)
+    double *jw = (double *)(nf + (size_t)NS * P.maxw);
 
-    // ---- B: stage node fields, zero the current window
+    // ---- B: stage node fields (contiguous column runs of the y-fastest
+    // node-field array), zero the current window
     if (inlds) {
+        const R *gnf = (const R *)P.nf;
         for (int k = tid; k < WH; k += PUSH_THREADS) {
-            const int wj = k / W, wi = k - wj * W;
+            const int wi = k / H, wj = k - wi * H;
             const int gx = wrapi(x0 + wx0 + wi, nx), gy = wrapi(y0 + wy0 + wj, ny);
-            double f[6];
-            node_fields(P, gx, gy, f);
-#pragma unroll
-            for (int q = 0; q < 6; q++) nf[(size_t)k * NS + q] = (R)f[q];
+            copy_node<R>(nf + (size_t)k * NS, gnf + ((size_t)gx * ny + gy) * NS);
         }
         for (int k = tid; k < njc * WH; k += PUSH_THREADS) jw[k] = 0.0;
     }
     __syncthreads();
 
-    // ---- C: push
+    // ---- C: push, V particles per lane per round, next round prefetched
     const R c0 = (R)S.c0, c1 = (R)S.c1, qx = (R)S.qx, qy = (R)S.qy, qr = (R)S.q;
     R *ox_p = (R *)S.ox, *oy_p = (R *)S.oy, *ux_p = (R *)S.ux, *uy_p = (R *)S.uy,
       *uz_p = (R *)S.uz;
     double ke = 0;
     R *gjx = (R *)P.jx, *gjy = (R *)P.jy, *gjz = (R *)P.jz, *grho = (R *)P.rho;
     const bool want_rho = P.deposit_rho != 0;
-#pragma unroll 1
-    for (int r = 0; r < PUSH_ROUNDS; r++) {
-        const int c = cl[r];
-        if (c < 0) continue;
-        const int64_t i = pbase + r * PUSH_GROUP;
-        R ox = ox_p[i], oy = oy_p[i], ux = ux_p[i], uy = uy_p[i], uz = uz_p[i];
+    // split form with an LDS window: deposit contributions come out of
+    // deposit_terms already multiplied by their plane's fixed-point scale
+    // (q*dx/dt, q*dy/dt and q pre-scaled by powers of two: exact)
+    const bool fxw = SPLIT && inlds;
+    const R dqx = fxw ? qx * (R)S.jsx : qx, dqy = fxw ? qy * (R)S.jsy : qy,
+            dqz = fxw ? qr * (R)S.jsz : qr, rsc = fxw ? (R)S.jsr / (R)S.jsz : (R)1;
+
+    // one particle: gather, push, move, deposit; returns the new cell
+    auto one = [&](int c, R &ox, R &oy, R &ux, R &uy, R &uz) -> int {
         const int cx = (int)P.dny.div((uint32_t)c), cy = c - cx * ny;
         int rx = cx - x0, ry = cy - y0;
         if (rx > nxh) rx -= nx; else if (rx < -nxh) rx += nx;
@@ -396,13 +609,13 @@ __global__ __launch_bounds__(PUSH_THREADS, 2) void push_kernel(PushParams P)
         const int lx = rx - wx0, ly = ry - wy0;
         R n00[6], n10[6], n01[6], n11[6];
         if (inlds) {
-            const R *b = nf + (size_t)(ly * W + lx) * NS;
+            const R *b = nf + (size_t)(lx * H + ly) * NS;
 #pragma unroll
             for (int q = 0; q < 6; q++) {
                 n00[q] = b[q];
-                n10[q] = b[NS + q];
-                n01[q] = b[(size_t)W * NS + q];
-                n11[q] = b[(size_t)W * NS + NS + q];
+                n10[q] = b[(size_t)H * NS + q];
+                n01[q] = b[NS + q];
+                n11[q] = b[(size_t)H * NS + NS + q];
             }
         } else {
             const int cx1 = cx + 1 == nx ? 0 : cx + 1, cy1 = cy + 1 == ny ? 0 : cy + 1;
@@ -428,38 +641,16 @@ __global__ __launch_bounds__(PUSH_THREADS, 2) void push_kernel(PushParams P)
             kin = gam - (R)1;                                                      // :1062
         }
         Dep<R> d;
-        deposit_terms<R, SPLIT>(cx + 1, cy + 1, dcx, dcy, dx0, ox, dy0, oy, qx, qy,
-                                qr, vz, P.compat, want_rho, d);
+        deposit_terms<R, SPLIT>(cx + 1, cy + 1, dcx, dcy, dx0, ox, dy0, oy, dqx, dqy,
+                                dqz, vz, P.compat, want_rho, d);
+        if (fxw && want_rho) { d.r0 *= rsc; d.r1 *= rsc; d.r2 *= rsc; d.r3 *= rsc; }
         // jy at column mx+2 is lost by the reference's fold (:1156, :1159)
         if ((P.compat & PBPIC_COMPAT_JY_DROP) && cx == nx - 1 && dcx == 1) d.jyb1 = (R)0;
         if (d.a == 0 && d.b == 0) {
             d.jxa0 += d.jxb0; d.jxa1 += d.jxb1; d.jya0 += d.jyb0; d.jya1 += d.jyb1;
         }
         if (inlds) {
-            const int base = ly * W + lx;
-            double *wx = jw, *wy = jw + WH, *wz = jw + 2 * WH;
-            lds_add(wx + base, (double)d.jxa0);
-            lds_add(wx + base + W, (double)d.jxa1);
-            lds_add(wy + base, (double)d.jya0);
-            lds_add(wy + base + 1, (double)d.jya1);
-            if (d.a != 0 || d.b != 0) {
-                const int bb = base + d.b * W + d.a;
-                lds_add(wx + bb, (double)d.jxb0);
-                lds_add(wx + bb + W, (double)d.jxb1);
-                lds_add(wy + bb, (double)d.jyb0);
-                lds_add(wy + bb + 1, (double)d.jyb1);
-            }
-            lds_add(wz + base, (double)d.jz0);
-            lds_add(wz + base + d.di, (double)d.jz1);
-            lds_add(wz + base + d.dj * W, (double)d.jz2);
-            lds_add(wz + base + d.dj * W + d.di, (double)d.jz3);
-            if (want_rho) {
-                double *wr = jw + 3 * WH;
-                lds_add(wr + base, (double)d.r0);
-                lds_add(wr + base + 1, (double)d.r1);
-                lds_add(wr + base + W, (double)d.r2);
-                lds_add(wr + base + W + 1, (double)d.r3);
-            }
+            deposit_lds(jw, WH, W, ly * W + lx, d, want_rho);
         } else {
             const int xm = cx == 0 ? nx - 1 : cx - 1, xp = cx + 1 == nx ? 0 : cx + 1;
             const int ym = cy == 0 ? ny - 1 : cy - 1, yp = cy + 1 == ny ? 0 : cy + 1;
@@ -469,48 +660,166 @@ __global__ __launch_bounds__(PUSH_THREADS, 2) void push_kernel(PushParams P)
                 int gy = ddy == 2 ? (yp + 1 == ny ? 0 : yp + 1) : ys[ddy + 1];
                 return (size_t)gy * nx + gx;
             };
-            gadd(gjx + at(0, 0), (double)d.jxa0);
-            gadd(gjx + at(0, 1), (double)d.jxa1);
-            gadd(gjy + at(0, 0), (double)d.jya0);
-            gadd(gjy + at(1, 0), (double)d.jya1);
+            gadd(gjx + at(0, 0), d.jxa0);
+            gadd(gjx + at(0, 1), d.jxa1);
+            gadd(gjy + at(0, 0), d.jya0);
+            gadd(gjy + at(1, 0), d.jya1);
             if (d.a != 0 || d.b != 0) {
-                gadd(gjx + at(d.a, d.b), (double)d.jxb0);
-                gadd(gjx + at(d.a, d.b + 1), (double)d.jxb1);
-                gadd(gjy + at(d.a, d.b), (double)d.jyb0);
-                gadd(gjy + at(d.a + 1, d.b), (double)d.jyb1);
+                gadd(gjx + at(d.a, d.b), d.jxb0);
+                gadd(gjx + at(d.a, d.b + 1), d.jxb1);
+                gadd(gjy + at(d.a, d.b), d.jyb0);
+                gadd(gjy + at(d.a + 1, d.b), d.jyb1);
             }
-            gadd(gjz + at(0, 0), (double)d.jz0);
-            gadd(gjz + at(d.di, 0), (double)d.jz1);
-            gadd(gjz + at(0, d.dj), (double)d.jz2);
-            gadd(gjz + at(d.di, d.dj), (double)d.jz3);
+            gadd(gjz + at(0, 0), d.jz0);
+            gadd(gjz + at(d.di, 0), d.jz1);
+            gadd(gjz + at(0, d.dj), d.jz2);
+            gadd(gjz + at(d.di, d.dj), d.jz3);
             if (want_rho) {
-                gadd(grho + at(0, 0), (double)d.r0);
-                gadd(grho + at(1, 0), (double)d.r1);
-                gadd(grho + at(0, 1), (double)d.r2);
-                gadd(grho + at(1, 1), (double)d.r3);
+                gadd(grho + at(0, 0), d.r0);
+                gadd(grho + at(1, 0), d.r1);
+                gadd(grho + at(0, 1), d.r2);
+                gadd(grho + at(1, 1), d.r3);
             }
         }
-        // periodic wrap (:1032-1052) and write-back
+        ke += (double)(kin * (R)S.ms);
+        // periodic wrap (:1032-1052)
         int ncx = cx + dcx, ncy = cy + dcy;
         ncx = ncx < 0 ? ncx + nx : (ncx >= nx ? ncx - nx : ncx);
         ncy = ncy < 0 ? ncy + ny : (ncy >= ny ? ncy - ny : ncy);
-        S.cell[i] = ncx * ny + ncy;
-        ox_p[i] = ox; oy_p[i] = oy; ux_p[i] = ux; uy_p[i] = uy; uz_p[i] = uz;
-        ke += (double)(kin * (R)S.ms);
+        return ncx * ny + ncy;
+    };
+
+    if (fast) {
+        const int XW = x0 + wx0, YW = y0 + wy0;        // window origin, absolute
+        int32_t *cp = S.cell + tstart;
+        R *op = ox_p + tstart, *pp = oy_p + tstart, *up = ux_p + tstart, *vp = uy_p + tstart,
+          *wp = uz_p + tstart;
+        const int lo = wave * ROUNDS * CH + lane * V;
+        R kacc = 0;
+        auto onef = [&](int c, R &ox, R &oy, R &ux, R &uy, R &uz) -> int {
+            const int cx = (int)P.dny.div((uint32_t)c), cy = c - cx * ny;
+            const R *b = nf + (size_t)((cx - XW) * H + (cy - YW)) * NS;
+            R n00[6], n10[6], n01[6], n11[6];
+#pragma unroll
+            for (int q = 0; q < 6; q++) {
+                n00[q] = b[q];
+                n10[q] = b[(size_t)H * NS + q];
+                n01[q] = b[NS + q];
+                n11[q] = b[(size_t)H * NS + NS + q];
+            }
+            const R dx0 = ox, dy0 = oy;
+            int dcx, dcy;
+            R vz, gam, kin;
+            if constexpr (SPLIT) {
+                push_math_split(P, c0, c1, ox, oy, ux, uy, uz, n00, n10, n01, n11, dcx, dcy,
+                                vz, kin, err);
+            } else {
+                push_math<R>(P, c0, c1, ox, oy, ux, uy, uz, n00, n10, n01, n11, dcx, dcy, vz,
+                             gam, err);
+                kin = gam - (R)1;                                                  // :1062
+            }
+            Dep<R> d;
+            deposit_terms<R, SPLIT>(cx + 1, cy + 1, dcx, dcy, dx0, ox, dy0, oy, dqx, dqy, dqz,
+                                    vz, P.compat, want_rho, d);
+            if (fxw && want_rho) { d.r0 *= rsc; d.r1 *= rsc; d.r2 *= rsc; d.r3 *= rsc; }
+            if (d.a == 0 && d.b == 0) {
+                d.jxa0 += d.jxb0; d.jxa1 += d.jxb1; d.jya0 += d.jyb0; d.jya1 += d.jyb1;
+            }
+            deposit_lds(jw, WH, W, (cy - YW) * W + (cx - XW), d, want_rho);
+            kacc += kin;
+            return c + dcx * ny + dcy;          // interior tile: no wrap (:1032-1052)
+        };
+        R cur[5][V], nxt[5][V];
+        int ccur[V], cnxt[V];
+        ldu(cp + lo, ccur);
+        ldu(op + lo, cur[0]); ldu(pp + lo, cur[1]); ldu(up + lo, cur[2]);
+        ldu(vp + lo, cur[3]); ldu(wp + lo, cur[4]);
+#pragma unroll 1
+        for (int r = 0; r < ROUNDS; r++) {
+            const int o = lo + r * CH;
+            if (r + 1 < ROUNDS) {
+                ldu(cp + o + CH, cnxt);
+                ldu(op + o + CH, nxt[0]); ldu(pp + o + CH, nxt[1]); ldu(up + o + CH, nxt[2]);
+                ldu(vp + o + CH, nxt[3]); ldu(wp + o + CH, nxt[4]);
+            }
+            int nc[V];
+#pragma unroll
+            for (int j = 0; j < V; j++)
+                nc[j] = onef(ccur[j], cur[0][j], cur[1][j], cur[2][j], cur[3][j], cur[4][j]);
+            stu(cp + o, nc);
+            stu(op + o, cur[0]); stu(pp + o, cur[1]); stu(up + o, cur[2]);
+            stu(vp + o, cur[3]); stu(wp + o, cur[4]);
+#pragma unroll
+            for (int j = 0; j < V; j++) {
+                ccur[j] = cnxt[j];
+#pragma unroll
+                for (int q = 0; q < 5; q++) cur[q][j] = nxt[q][j];
+            }
+        }
+        ke = (double)kacc * S.ms;
+    } else {
+    // the tile's cells were read in phase A and are re-read here from L2
+    R cur[5][V], nxt[5][V];
+    int ccur[V], cnxt[V];
+    {
+        const int64_t i0 = pbase;
+        vld(S.cell, i0, n, ccur);
+        vld(ox_p, i0, n, cur[0]); vld(oy_p, i0, n, cur[1]); vld(ux_p, i0, n, cur[2]);
+        vld(uy_p, i0, n, cur[3]); vld(uz_p, i0, n, cur[4]);
+    }
+#pragma unroll 1
+    for (int r = 0; r < ROUNDS; r++) {
+        const int64_t i0 = pbase + r * CH;
+        if (r + 1 < ROUNDS) {
+            const int64_t i1 = i0 + CH;
+            vld(S.cell, i1, n, cnxt);
+            vld(ox_p, i1, n, nxt[0]); vld(oy_p, i1, n, nxt[1]); vld(ux_p, i1, n, nxt[2]);
+            vld(uy_p, i1, n, nxt[3]); vld(uz_p, i1, n, nxt[4]);
+        }
+        if (i0 < n) {
+            int nc[V];
+#pragma unroll
+            for (int j = 0; j < V; j++) {
+                const int c = ccur[j];
+                nc[j] = c;
+                if (i0 + j < n && (unsigned)c < (unsigned)(nx * ny))
+                    nc[j] = one(c, cur[0][j], cur[1][j], cur[2][j], cur[3][j], cur[4][j]);
+            }
+            vst(S.cell, i0, n, nc);
+            vst(ox_p, i0, n, cur[0]); vst(oy_p, i0, n, cur[1]); vst(ux_p, i0, n, cur[2]);
+            vst(uy_p, i0, n, cur[3]); vst(uz_p, i0, n, cur[4]);
+        }
+#pragma unroll
+        for (int j = 0; j < V; j++) {
+            ccur[j] = cnxt[j];
+#pragma unroll
+            for (int q = 0; q < 5; q++) cur[q][j] = nxt[q][j];
+        }
+    }
     }
 
     // ---- D: flush the window, reduce the kinetic energy and error flags
     __syncthreads();
     if (inlds) {
+        const double isx = 1.0 / S.jsx, isy = 1.0 / S.jsy, isz = 1.0 / S.jsz, isr = 1.0 / S.jsr;
         for (int k = tid; k < WH; k += PUSH_THREADS) {
             const int wj = k / W, wi = k - wj * W;
             const int gx = wrapi(x0 + wx0 + wi, nx), gy = wrapi(y0 + wy0 + wj, ny);
             const size_t g = (size_t)gy * nx + gx;
-            double v;
-            if ((v = jw[k]) != 0.0) gadd(gjx + g, (R)v);
-            if ((v = jw[WH + k]) != 0.0) gadd(gjy + g, (R)v);
-            if ((v = jw[2 * WH + k]) != 0.0) gadd(gjz + g, (R)v);
-            if (want_rho && (v = jw[3 * WH + k]) != 0.0) gadd(grho + g, (R)v);
+            if constexpr (SPLIT) {
+                const long long *jb = reinterpret_cast<const long long *>(jw);
+                long long v;
+                if ((v = jb[k]) != 0) gadd(gjx + g, (R)((double)v * isx));
+                if ((v = jb[WH + k]) != 0) gadd(gjy + g, (R)((double)v * isy));
+                if ((v = jb[2 * WH + k]) != 0) gadd(gjz + g, (R)((double)v * isz));
+                if (want_rho && (v = jb[3 * WH + k]) != 0) gadd(grho + g, (R)((double)v * isr));
+            } else {
+                double v;
+                if ((v = jw[k]) != 0.0) gadd(gjx + g, (R)v);
+                if ((v = jw[WH + k]) != 0.0) gadd(gjy + g, (R)v);
+                if ((v = jw[2 * WH + k]) != 0.0) gadd(gjz + g, (R)v);
+                if (want_rho && (v = jw[3 * WH + k]) != 0.0) gadd(grho + g, (R)v);
+            }
         }
     }
 #pragma unroll
@@ -585,6 +894,21 @@ static int push_tiles(const PushParams &p, PushParams &q)
     }
     q.ntiles = t;
     return t;
+}
+
+size_t nodefield_bytes(int precision, int64_t ncell)
+{
+    return (size_t)ncell * (precision == PBPIC_PREC_FP64 ? 6 * 8 : 8 * 4);
+}
+
+hipError_t launch_nodefield(const PushParams &p, int precision, hipStream_t st)
+{
+    const dim3 g((unsigned)((p.nx + NF_T - 1) / NF_T), (unsigned)((p.ny + NF_T - 1) / NF_T));
+    if (precision == PBPIC_PREC_FP64)
+        hipLaunchKernelGGL((nodefield_kernel<double, 6>), g, dim3(256), 0, st, p, (double *)p.nf);
+    else
+        hipLaunchKernelGGL((nodefield_kernel<float, 8>), g, dim3(256), 0, st, p, (float *)p.nf);
+    return hipGetLastError();
 }
 
 size_t push_lds_bytes(int precision, int maxw, int deposit_rho)

@@ -169,12 +169,22 @@ __global__ __launch_bounds__(ST_THREADS) void radix_pass_kernel(
     }
 }
 
+// dst[q] = src[perm[rank(q)]]; rank(q) is q in cell order, or the push order
+// (tile_rank) inside full tiles when tv = V > 0.  perm_out[q] receives the
+// source index of slot q.
 template <typename R>
-__global__ void gather_kernel(const uint32_t *perm, int64_t n, SpeciesDev src, SpeciesDev dst)
+__global__ void gather_kernel(const uint32_t *perm, int64_t n, int tv, uint32_t *perm_out,
+                              SpeciesDev src, SpeciesDev dst)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const uint32_t p = perm[i];
+    int64_t k = i;
+    if (tv) {
+        const int64_t t0 = i & ~(int64_t)(PUSH_TILE - 1);
+        if (t0 + PUSH_TILE <= n) k = t0 + tile_rank((int)(i - t0), tv);
+    }
+    const uint32_t p = perm[k];
+    if (perm_out) perm_out[i] = p;
     dst.cell[i] = src.cell[p];
     ((R *)dst.ox)[i] = ((const R *)src.ox)[p];
     ((R *)dst.oy)[i] = ((const R *)src.oy)[p];
@@ -185,7 +195,8 @@ __global__ void gather_kernel(const uint32_t *perm, int64_t n, SpeciesDev src, S
 }
 
 // Sorts species sp into dst (a second buffer set of the same capacity).
-hipError_t sort_species(SortScratch &s, const SpeciesDev &sp, int precision, int64_t ncell,
+hipError_t sort_species(SortScratch &s, const SpeciesDev &sp, int precision, int push_order,
+                        int64_t ncell,
                         SpeciesDev &dst, int *err, int want_counts, hipStream_t st)
 {
     (void)want_counts;
@@ -238,11 +249,16 @@ hipError_t sort_species(SortScratch &s, const SpeciesDev &sp, int precision, int
     }
     const uint32_t *perm = vin;
     s.perm = perm;
+    // the push order needs the slot->source map in a buffer of its own: the
+    // value buffer the last pass did not write is free
+    const int tv = push_order && n >= PUSH_TILE ? push_vec(precision) : 0;
+    uint32_t *pout = nullptr;
+    if (tv) s.perm = pout = s.vals[pb];
     const unsigned nb = (unsigned)((n + 255) / 256);
     if (precision == PBPIC_PREC_FP64)
-        hipLaunchKernelGGL(gather_kernel<double>, dim3(nb), dim3(256), 0, st, perm, n, sp, dst);
+        hipLaunchKernelGGL(gather_kernel<double>, dim3(nb), dim3(256), 0, st, perm, n, tv, pout, sp, dst);
     else
-        hipLaunchKernelGGL(gather_kernel<float>, dim3(nb), dim3(256), 0, st, perm, n, sp, dst);
+        hipLaunchKernelGGL(gather_kernel<float>, dim3(nb), dim3(256), 0, st, perm, n, tv, pout, sp, dst);
     return hipGetLastError();
 }
 

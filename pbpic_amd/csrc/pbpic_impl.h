@@ -9,6 +9,25 @@
 
 namespace pbpic {
 
+// Push tiling.  A workgroup of PUSH_THREADS lanes pushes one tile of
+// PUSH_TILE = 64 x 64 consecutive particles of a species; each lane moves
+// push_vec(precision) consecutive particles (16 bytes per array) per round.
+constexpr int PUSH_THREADS = 256;
+constexpr int PUSH_TILE = 4096;
+__host__ __device__ constexpr int push_vec(int precision) { return precision == 0 ? 2 : 4; }
+
+// Push order.  Inside every full tile the sort can lay the particles out so
+// that the q-th slot of the tile holds the particle of cell-order rank
+// tile_rank(q, V): slot q = (c, l, j) with q = c*64V + l*V + j (round c of
+// the tile, lane l, element j) takes rank l*64 + c*V + j.  The tile keeps the
+// same particles as in cell order; only their order inside it changes, so
+// that the 64 lanes of one wave-wide load hold particles 64 ranks apart (in
+// different cells) and their deposits hit different LDS addresses.
+__host__ __device__ constexpr int tile_rank(int q, int V)
+{
+    return ((q % (64 * V)) / V) * 64 + (q / (64 * V)) * V + q % V;
+}
+
 // Device-side error flags, OR-ed by kernels into one int and reported by
 // pbpic_sync() and every call that returns data.
 enum : int {
@@ -65,6 +84,10 @@ struct SpeciesDev {
     double ms;                 // mass: mime for q > 0, 1 otherwise (:964-968)
     double c0, c1;             // q*dt*mime/ms/2 and c0/cva (:969-970)
     double qx, qy;             // q*dx/dt, q*dy/dt (:1237-1238)
+    // Split form: the LDS current window holds 64-bit fixed-point sums of the
+    // FP32 contributions times these powers of two (jx, jy, jz, rho), each
+    // chosen so that one contribution stays below 2^30 in magnitude.
+    float jsx, jsy, jsz, jsr;
 };
 
 struct PushParams {
@@ -77,6 +100,7 @@ struct PushParams {
     double dt, cva, idx, idy, dtcva;   // dtcva = dt*cva (:1003)
     const double *ex, *ey, *ez, *bx, *by, *bz;
     void *jx, *jy, *jz, *rho;  // unnormalized current, double or float
+    void *nf;                  // node-field array (nodefield_bytes), push precision
     double *acc;
     int *err;
     SpeciesDev sp[MAX_SPECIES];
@@ -124,6 +148,9 @@ struct AdjFieldParams {
 
 // Launchers (one per translation unit family).
 hipError_t launch_push(const PushParams &p, int precision, hipStream_t st);
+size_t nodefield_bytes(int precision, int64_t ncell);
+// Fills p.nf from the E and B planes; launch_push reads p.nf.
+hipError_t launch_nodefield(const PushParams &p, int precision, hipStream_t st);
 hipError_t launch_deposit_batch(const PushParams &p, int precision, int64_t n,
                                 const int32_t *i1, const int32_t *i2,
                                 const int32_t *j1, const int32_t *j2,
@@ -165,7 +192,8 @@ hipError_t launch_node_transpose(const AdjFieldParams &p, double *nadj, hipStrea
 hipError_t launch_unpermute(const uint32_t *perm, int64_t n, int precision,
                             void *const *post, void *const *pre, hipStream_t st);
 
-hipError_t sort_species(SortScratch &s, const SpeciesDev &sp, int precision,
+// push_order != 0: full tiles come out in the push order (tile_rank).
+hipError_t sort_species(SortScratch &s, const SpeciesDev &sp, int precision, int push_order,
                         int64_t ncell, SpeciesDev &dst, int *err,
                         int want_counts, hipStream_t st);
 

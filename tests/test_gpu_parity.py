@@ -49,7 +49,8 @@ def test_countingsort_bitexact(gpu, kind, nx, ny):
                             uy=rng.standard_normal(n), uz=rng.standard_normal(n),
                             pid=np.arange(1, n + 1, dtype=np.int32))]
     import pbpic_amd
-    ctx = pbpic_amd.Context(nx, ny, 0.1, 0.1, 0.01, 15.0, 25.0, 0.1, track_pid=True)
+    ctx = pbpic_amd.Context(nx, ny, 0.1, 0.1, 0.01, 15.0, 25.0, 0.1, track_pid=True,
+                            push_order=False)
     k = ctx.add_species(-1, n)
     s = case["species"][0]
     ctx.set_particles(k, s["grid"], s["dx"], s["dy"], s["ux"], s["uy"], s["uz"], s["pid"])
@@ -65,6 +66,50 @@ def test_countingsort_bitexact(gpu, kind, nx, ny):
     assert np.array_equal(g["grid"], grid[perm])
     assert np.array_equal(g["pid"], s["pid"][perm])
     assert np.array_equal(g["ux"], s["ux"][perm])
+
+
+def _push_order(ref_perm, V):
+    """The push order of a cell-order permutation: inside each full tile of
+    4096 slots, slot q = c*64V + l*V + j (round c, lane l, element j) holds
+    cell-order rank l*64 + c*V + j; a ragged last tile keeps cell order."""
+    out = ref_perm.copy()
+    q = np.arange(4096)
+    rank = ((q % (64 * V)) // V) * 64 + (q // (64 * V)) * V + q % V
+    assert np.array_equal(np.sort(rank), q)                 # a bijection
+    for t0 in range(0, len(ref_perm) - 4095, 4096):
+        out[t0:t0 + 4096] = ref_perm[t0 + rank]
+    return out
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("n", [4095, 4096, 20000, 3 * 4096 + 17])
+def test_countingsort_push_order(gpu, precision, n):
+    """With the push order (the default) every full 4096-particle tile holds
+    exactly the particles the reference's cell order puts there, the 64 lanes
+    of a wave-wide load hold particles 64 cell-order ranks apart, and the
+    counts are the reference's."""
+    _say(f"push-order sort precision={precision} n={n}")
+    nx, ny = 40, 37
+    rng = np.random.default_rng(5 + n)
+    grid = rng.integers(1, nx * ny + 1, n).astype(np.int32)
+    ux = rng.standard_normal(n)
+    pid = np.arange(1, n + 1, dtype=np.int32)
+    import pbpic_amd
+    ctx = pbpic_amd.Context(nx, ny, 0.1, 0.1, 0.01, 15.0, 25.0, 0.1, precision=precision,
+                            track_pid=True)
+    k = ctx.add_species(-1, n)
+    z = np.zeros(n)
+    ctx.set_particles(k, grid, z, z, ux, z, z, pid)
+    counts, perm = ctx.countingsort(k, want=True)
+    ref = np.argsort(grid, kind="stable")
+    assert np.array_equal(counts, np.bincount(grid - 1, minlength=nx * ny))
+    for t0 in range(0, n, 4096):
+        assert np.array_equal(np.sort(perm[t0:t0 + 4096]), np.sort(ref[t0:t0 + 4096]))
+    assert np.array_equal(perm, _push_order(ref, 2 if precision == 0 else 4))
+    g = ctx.get_particles(k)
+    assert np.array_equal(g["grid"], grid[perm]) and np.array_equal(g["pid"], pid[perm])
+    want_ux = ux[perm] if precision == 0 else ux.astype(np.float32)[perm]
+    assert np.array_equal(g["ux"], want_ux)
 
 
 def test_countingsort_empty(gpu):
@@ -170,6 +215,51 @@ def test_one_step_fp64(gpu, compat, shuffle):
     d, od = ctx.diag(), o.diag()
     for key in ("ke_e", "ke_i", "w_e", "w_b", "je_step"):
         assert abs(d[key] - od[key]) <= 1e-12 * abs(od[key]) + 1e-300, key
+
+
+def _by_pid(sp):
+    order = np.argsort(sp["pid"])
+    return {k: v[order] for k, v in sp.items()}
+
+
+@pytest.mark.parametrize("precision,compat", [(0, 7), (0, 0), (1, 7)])
+def test_one_step_full_tiles(gpu, precision, compat):
+    """Sorted species of 48 full 4096-particle tiles (12 x 256 cells, 64 per
+    cell), most of them interior: the push of whole tiles in the push order
+    against the oracle's push of the same particles in cell order, matched
+    by particle id.  FP64 to 1e-12; split precision to the bars of
+    test_one_step_split32."""
+    _say(f"one step full tiles precision={precision} compat={compat}")
+    case = make_case(nx=12, ny=256, ppc=64, seed=31, efield=0.05, bfield=0.5)
+    ctx = gpu_for(case, precision=precision, compat=compat)
+    o = oracle_for(case, compat=compat)
+    ctx.countingsort(); o.countingsort()
+    ctx.zero_current(); ctx.pusher()
+    o.zero_current(); o.pusher()
+    tol_j = FP64_TOL if precision == 0 else 1e-5
+    for f in ("jx", "jy", "jz", "rho"):
+        assert rel_norm(ctx.get_field(f), o.get_field(f)) < tol_j, f
+    osp = oracle_species(o, case)
+    for k in range(2):
+        g, r = _by_pid(ctx.get_particles(k)), _by_pid(osp[k])
+        assert np.array_equal(g["pid"], r["pid"])
+        if precision == 0:
+            assert np.array_equal(g["grid"], r["grid"])
+            for f in ("dx", "dy", "ux", "uy", "uz"):
+                err = np.max(np.abs(g[f] - r[f]) / np.maximum(np.abs(r[f]), 1e-300))
+                assert err < FP64_TOL, (k, f, err)
+        else:
+            same = g["grid"] == r["grid"]
+            near_face = (np.abs(np.abs(r["dx"]) - 0.5) < 1e-5) | (np.abs(np.abs(r["dy"]) - 0.5) < 1e-5)
+            assert np.all(same | near_face)
+            for f in ("dx", "dy"):
+                assert np.max(np.abs(g[f] - r[f])[same]) < 2e-6, f
+            for f in ("ux", "uy", "uz"):
+                assert np.max(np.abs(g[f] - r[f])) < 1e-5 * np.max(np.abs(r[f])), f
+    d, od = ctx.diag(), o.diag()
+    tol_d = 1e-12 if precision == 0 else 1e-5
+    for key in ("ke_e", "ke_i", "je_step"):
+        assert abs(d[key] - od[key]) <= tol_d * abs(od[key]), key
 
 
 def test_multi_step_fp64_diagnostics(gpu):
