@@ -29,14 +29,14 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def run_form(precision, cfg, steps, warmup, sort_every, seed):
+def run_form(precision, cfg, steps, warmup, sort_every, seed, push_order=True):
     import torch
     import pbpic_amd
 
     sp = cfg["species"][0]
     ctx = pbpic_amd.Context(cfg["nx"], cfg["ny"], cfg["dx"], cfg["dy"], cfg["dt"], cfg["cva"],
                             cfg["mime"], cfg["in0"], precision=precision,
-                            sort_every=sort_every, deposit_rho=False,
+                            sort_every=sort_every, deposit_rho=False, push_order=push_order,
                             rho_background=cfg["rho_background"])
     npart = cfg["nx"] * cfg["ny"] * sp["ppc"]
     k = ctx.add_species(sp["q"], npart)
@@ -187,6 +187,9 @@ def main():
     ap.add_argument("--ny", type=int, default=2048)
     ap.add_argument("--ppc", type=int, default=100)
     ap.add_argument("--sort-every", type=int, default=50)
+    ap.add_argument("--order", choices=("push", "cell"), default="push",
+                    help="particle order the sort writes: the push kernel's lane order inside "
+                         "each 4096-particle tile (default) or the reference's cell order")
     ap.add_argument("--forms", default="split,fp64")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -222,7 +225,8 @@ def main():
         prec = 1 if f == "split" else 0
         log(f"[bench] form={f} nx={args.nx} ny={args.ny} ppc={args.ppc} steps={args.steps} "
             f"warmup={args.warmup}")
-        res[f] = run_form(prec, cfg, args.steps, args.warmup, args.sort_every, args.seed)
+        res[f] = run_form(prec, cfg, args.steps, args.warmup, args.sort_every, args.seed,
+                          push_order=args.order == "push")
         r = res[f]
         log(f"[bench] form={f}: {r['ps']:.4e} particle-steps/s, {r['ms_per_step']:.3f} ms/step, "
             f"kernels(ms,count)={r['prof']}")
@@ -245,7 +249,8 @@ def main():
                                f"({res[head]['npart']} particles), K_sort={args.sort_every}, "
                                f"form={head}",
                    "nx": args.nx, "ny": args.ny, "ppc": args.ppc,
-                   "sort_every": args.sort_every, "parallelism": "single GPU"},
+                   "sort_every": args.sort_every, "particle_order": args.order,
+                   "parallelism": "single GPU"},
         "roofline": roofline(hp, res[head], cfg),
         "step_roofline": step_roofline(hp, res[head], cfg, args.sort_every),
         "kernel_ms": {k: (v[0] / max(v[1], 1), v[1]) for k, v in res[head]["prof"].items()},

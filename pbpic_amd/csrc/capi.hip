@@ -25,6 +25,8 @@ struct Species {
     void *uz[2] = {nullptr, nullptr};
     int32_t *pid[2] = {nullptr, nullptr};
     int cur = 0;
+    int32_t *tbox = nullptr;   // per push tile bounding boxes (SpeciesDev::tbox)
+    bool tbox_ok = false;      // cleared by anything that moves or rewrites particles
 };
 
 }  // namespace
@@ -122,6 +124,8 @@ SpeciesDev species_dev(pbpic_ctx *c, int s, int buf)
     d.jsy = fx_scale(fabs(S.qy));
     d.jsz = fx_scale(fabs((double)S.q) * c->p.cva);
     d.jsr = fx_scale(fabs((double)S.q));
+    d.tbox = S.tbox;
+    d.tbox_ok = S.tbox_ok ? 1 : 0;
     return d;
 }
 
@@ -174,6 +178,10 @@ int check_flags(pbpic_ctx *c)
     if (h & ERRF_CFL) return fail(c, PBPIC_ERR_CFL, "a particle moved more than one cell in one step");
     if (h & ERRF_NONFINITE) return fail(c, PBPIC_ERR_NONFINITE, "non-finite particle momentum");
     if (h & ERRF_SPIN) return fail(c, PBPIC_ERR_SPIN, "sort look-back exceeded its spin limit");
+    if (h & ERRF_STATE)
+        return fail(c, PBPIC_ERR_STATE, "particle cells changed outside the library since the last "
+                                        "push (call pbpic_species_device_ptrs or "
+                                        "pbpic_set_particles_device after writing them)");
     return fail(c, PBPIC_ERR_STATE, "unknown device error flag %d", h);
 }
 
@@ -195,6 +203,7 @@ int sort_one(pbpic_ctx *c, int s, int32_t *counts, int32_t *perm, uint32_t *perm
                            ncell(c), dst, c->err, counts != nullptr, c->st));
     HIPCHK(c, prof_mark(c, PBPIC_PROF_SORT, false));
     S.cur ^= 1;
+    S.tbox_ok = false;
     if (perm_dev && S.n > 0)
         HIPCHK(c, hipMemcpyAsync(perm_dev, c->sort.perm, sizeof(uint32_t) * S.n,
                                  hipMemcpyDeviceToDevice, c->st));
@@ -231,6 +240,7 @@ int push_impl(pbpic_ctx *c, bool reset_ke)
     HIPCHK(c, prof_mark(c, PBPIC_PROF_FIELD, false));
     HIPCHK(c, prof_mark(c, PBPIC_PROF_PUSH, true));
     HIPCHK(c, launch_push(P, c->p.precision, c->st));
+    for (auto &S : c->sp) S.tbox_ok = true;
     HIPCHK(c, prof_mark(c, PBPIC_PROF_PUSH, false));
     return PBPIC_OK;
 }
@@ -314,6 +324,7 @@ int state_copy(pbpic_ctx *c, const StateLayout &L, char *buf, bool save)
     for (int s = 0; s < (int)c->sp.size(); s++) {
         Species &S = c->sp[s];
         const int b = S.cur;
+        if (!save) S.tbox_ok = false;
         void *dev[7] = {S.cell[b], S.ox[b], S.oy[b], S.ux[b], S.uy[b], S.uz[b], S.pid[b]};
         for (int part = 0; part < 7; part++) {
             if (!dev[part] || S.n == 0) continue;
@@ -361,6 +372,7 @@ int set_particles_t(pbpic_ctx *c, int32_t s, int64_t n, const int32_t *grid, con
         cell[(size_t)i] = grid[i] - 1;
     }
     const int b = S.cur;
+    S.tbox_ok = false;
     HIPCHK(c, hipMemcpyAsync(S.cell[b], cell.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, c->st));
     const H *src[5] = {dx, dy, ux, uy, uz};
     void *dst[5] = {S.ox[b], S.oy[b], S.ux[b], S.uy[b], S.uz[b]};
@@ -519,6 +531,7 @@ int pbpic_ctx_destroy(pbpic_ctx *c)
         for (int b = 0; b < 2; b++) {
             f(S.cell[b]); f(S.ox[b]); f(S.oy[b]); f(S.ux[b]); f(S.uy[b]); f(S.uz[b]); f(S.pid[b]);
         }
+    for (auto &S : c->sp) f(S.tbox);
     for (int b = 0; b < 2; b++) { f(c->sort.keys[b]); f(c->sort.vals[b]); }
     f(c->sort.status); f(c->sort.dhist); f(c->sort.ticket); f(c->sort.counts);
     for (auto p : c->adj.ckpts) f(p);
@@ -562,6 +575,7 @@ int pbpic_add_species(pbpic_ctx *c, int32_t q, int64_t capacity, int32_t *specie
         HIPCHK(c, dalloc(&S.uz[b], n * c->rsz));
         if (p.flags & PBPIC_FLAG_TRACK_PID) HIPCHK(c, dalloc(&S.pid[b], n * 4));
     }
+    HIPCHK(c, dalloc(&S.tbox, (n / PUSH_TILE + 1) * 4 * sizeof(int32_t)));
     // sort scratch sized for the largest species
     if (capacity > c->sort.cap) {
         for (int b = 0; b < 2; b++) {
@@ -626,6 +640,7 @@ int pbpic_init_maxwellian(pbpic_ctx *c, int32_t s, int32_t ppc, double vth, doub
     const int64_t n = ncell(c) * (int64_t)ppc;
     if (n > S.cap) return fail(c, PBPIC_ERR_CAPACITY, "ppc*cells=%lld exceeds capacity %lld", (long long)n, (long long)S.cap);
     S.n = n;
+    S.tbox_ok = false;
     SpeciesDev d = species_dev(c, s, S.cur);
     HIPCHK(c, launch_init_maxwellian(d, c->p.precision, c->p.nx, c->p.ny, ppc, vth, vdrift_x,
                                      counter_stream, seed, stream, c->st));
@@ -846,6 +861,7 @@ int pbpic_species_device_ptrs(pbpic_ctx *c, int32_t s, void **ptrs)
     if (!ptrs) return PBPIC_ERR_ARG;
     Species &S = c->sp[s];
     const int b = S.cur;
+    S.tbox_ok = false;          // the caller may write through the pointers
     ptrs[0] = S.cell[b]; ptrs[1] = S.ox[b]; ptrs[2] = S.oy[b];
     ptrs[3] = S.ux[b]; ptrs[4] = S.uy[b]; ptrs[5] = S.uz[b]; ptrs[6] = S.pid[b];
     return PBPIC_OK;
@@ -870,6 +886,7 @@ int pbpic_set_particles_device(pbpic_ctx *c, int32_t s, int64_t n, const int32_t
     Species &S = c->sp[s];
     if (n < 0 || n > S.cap) return fail(c, PBPIC_ERR_CAPACITY, "n=%lld exceeds capacity", (long long)n);
     const int b = S.cur;
+    S.tbox_ok = false;
     HIPCHK(c, hipMemcpyAsync(S.cell[b], cell, 4 * n, hipMemcpyDeviceToDevice, c->st));
     const void *src[5] = {dx, dy, ux, uy, uz};
     void *dst[5] = {S.ox[b], S.oy[b], S.ux[b], S.uy[b], S.uz[b]};

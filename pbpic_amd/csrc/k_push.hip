@@ -459,7 +459,11 @@ __device__ __forceinline__ void stu(int32_t *p, const int (&v)[4])
 __device__ __forceinline__ void jadd(double *p, double v) { lds_add(p, v); }
 __device__ __forceinline__ void jadd(double *p, float v)
 {
-    const long long q = (long long)__float2int_rn(v);
+    // v_cvt_rpi_i32_f32: floor(v + 0.5) in one instruction (the compiler
+    // emits round-to-nearest-even plus convert for rintf)
+    int qi;
+    asm("v_cvt_rpi_i32_f32 %0, %1" : "=v"(qi) : "v"(v));
+    const long long q = (long long)qi;
     __hip_atomic_fetch_add(reinterpret_cast<unsigned long long *>(p), (unsigned long long)q,
                            __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
@@ -515,13 +519,24 @@ __global__ __launch_bounds__(PUSH_THREADS, 2) void push_kernel(PushParams P)
     const int nx = P.nx, ny = P.ny, nxh = P.nx >> 1, nyh = P.ny >> 1;
     const int64_t n = S.n;
 
-    // ---- A: bounding box of the tile's cells, unwrapped around its first
+    // ---- A: bounding box of the tile's cells, unwrapped around (x0, y0):
+    // the box the previous push recorded, or else a pass over the cells
+    int x0 = 0, y0 = 0, mnx = 1 << 30, mxx = -(1 << 30), mny = 1 << 30, mxy = -(1 << 30);
+    int err = 0, aerr = 0;
+    bool boxed = false;
+    if (S.tbox_ok) {
+        const int4 b = *reinterpret_cast<const int4 *>(S.tbox + 4 * (t - S.tile0));
+        if (b.y >= 0 && b.w >= 0) {
+            x0 = b.x; y0 = b.z; mnx = 0; mxx = b.y; mny = 0; mxy = b.w;
+            boxed = true;
+        }
+    }
+    if (!boxed) {
     const int c00 = S.cell[tstart];
-    const bool c00ok = (unsigned)c00 < (unsigned)(nx * ny);
-    const int x0 = c00ok ? (int)P.dny.div((uint32_t)c00) : 0;
-    const int y0 = c00ok ? c00 - x0 * ny : 0;
-    int mnx = 1 << 30, mxx = -(1 << 30), mny = 1 << 30, mxy = -(1 << 30);
-    int err = 0;
+    if ((unsigned)c00 < (unsigned)(nx * ny)) {
+        x0 = (int)P.dny.div((uint32_t)c00);
+        y0 = c00 - x0 * ny;
+    }
 #pragma unroll
     for (int r = 0; r < ROUNDS; r++) {
         int cv[V];
@@ -539,7 +554,7 @@ __global__ __launch_bounds__(PUSH_THREADS, 2) void push_kernel(PushParams P)
             mny = min(mny, ry); mxy = max(mxy, ry);
         }
     }
-    int aerr = err;
+    aerr = err;
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
         mnx = min(mnx, __shfl_xor(mnx, o));
@@ -558,6 +573,7 @@ __global__ __launch_bounds__(PUSH_THREADS, 2) void push_kernel(PushParams P)
     mxx = max(max(red[0][1], red[1][1]), max(red[2][1], red[3][1]));
     mny = min(min(red[0][2], red[1][2]), min(red[2][2], red[3][2]));
     mxy = max(max(red[0][3], red[1][3]), max(red[2][3], red[3][3]));
+    }
     const int wx0 = mnx - 1, wy0 = mny - 1;
     const int W = mxx - mnx + 4, H = mxy - mny + 4;
     const bool inlds = mnx <= mxx && (int64_t)W * H <= P.maxw;
@@ -599,6 +615,16 @@ __global__ __launch_bounds__(PUSH_THREADS, 2) void push_kernel(PushParams P)
     const bool fxw = SPLIT && inlds;
     const R dqx = fxw ? qx * (R)S.jsx : qx, dqy = fxw ? qy * (R)S.jsy : qy,
             dqz = fxw ? qr * (R)S.jsz : qr, rsc = fxw ? (R)S.jsr / (R)S.jsz : (R)1;
+    // window coordinates of the cells this push reads (o*) and writes (n*):
+    // the first check the recorded box, the second become the next one
+    int omnx = 1 << 30, omxx = -(1 << 30), omny = 1 << 30, omxy = -(1 << 30);
+    int nmnx = 1 << 30, nmxx = -(1 << 30), nmny = 1 << 30, nmxy = -(1 << 30);
+    auto track = [&](int lx, int ly, int dcx, int dcy) {
+        omnx = min(omnx, lx); omxx = max(omxx, lx);
+        omny = min(omny, ly); omxy = max(omxy, ly);
+        nmnx = min(nmnx, lx + dcx); nmxx = max(nmxx, lx + dcx);
+        nmny = min(nmny, ly + dcy); nmxy = max(nmxy, ly + dcy);
+    };
 
     // one particle: gather, push, move, deposit; returns the new cell
     auto one = [&](int c, R &ox, R &oy, R &ux, R &uy, R &uz) -> int {
@@ -682,6 +708,7 @@ __global__ __launch_bounds__(PUSH_THREADS, 2) void push_kernel(PushParams P)
             }
         }
         ke += (double)(kin * (R)S.ms);
+        track(lx, ly, dcx, dcy);
         // periodic wrap (:1032-1052)
         int ncx = cx + dcx, ncy = cy + dcy;
         ncx = ncx < 0 ? ncx + nx : (ncx >= nx ? ncx - nx : ncx);
@@ -727,6 +754,7 @@ __global__ __launch_bounds__(PUSH_THREADS, 2) void push_kernel(PushParams P)
             }
             deposit_lds(jw, WH, W, (cy - YW) * W + (cx - XW), d, want_rho);
             kacc += kin;
+            track(cx - XW, cy - YW, dcx, dcy);
             return c + dcx * ny + dcy;          // interior tile: no wrap (:1032-1052)
         };
         R cur[5][V], nxt[5][V];
@@ -835,6 +863,38 @@ __global__ __launch_bounds__(PUSH_THREADS, 2) void push_kernel(PushParams P)
     if (tid == 0)
         unsafeAtomicAdd(P.acc + (S.q > 0 ? ACC_KE_I : ACC_KE_E),
                         (kred[0] + kred[1]) + (kred[2] + kred[3]));
+
+    // ---- E: record the box of the new cells for the next push; check that
+    // every cell read lay inside the box this push assumed
+    int v[8] = {omnx, -omxx, omny, -omxy, nmnx, -nmxx, nmny, -nmxy};
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+#pragma unroll
+        for (int q = 0; q < 8; q++) v[q] = min(v[q], __shfl_xor(v[q], o));
+    __shared__ int bred[4][8];
+    if (lane == 0)
+#pragma unroll
+        for (int q = 0; q < 8; q++) bred[wave][q] = v[q];
+    __syncthreads();
+    if (tid == 0) {
+#pragma unroll
+        for (int q = 0; q < 8; q++) v[q] = min(min(bred[0][q], bred[1][q]), min(bred[2][q], bred[3][q]));
+        const bool any = v[4] <= -v[5];            // at least one valid particle
+        // cells read must lie inside the window's interior [1, W-3] x [1, H-3]
+        if (any && (v[0] < 1 || -v[1] > W - 3 || v[2] < 1 || -v[3] > H - 3))
+            atomicOr(P.err, ERRF_STATE);
+        // a box wider than half the domain would be ambiguous under the
+        // periodic unwrap: leave it unknown (the next push scans the cells)
+        int4 nb = make_int4(0, -1, 0, -1);
+        const int ex = -v[5] - v[4], ey = -v[7] - v[6];
+        if (any && aerr == 0 && ex <= nxh && ey <= nyh) {
+            nb.x = wrapi(x0 + wx0 + v[4], nx);
+            nb.y = ex;
+            nb.z = wrapi(y0 + wy0 + v[6], ny);
+            nb.w = ey;
+        }
+        *reinterpret_cast<int4 *>(S.tbox + 4 * (t - S.tile0)) = nb;
+    }
 }
 
 // Batched currentdeposit with global atomics (unit-test entry point).

@@ -35,6 +35,7 @@ enum : int {
     ERRF_NONFINITE = 2,  // NaN/Inf momentum or Lorentz factor
     ERRF_CELL = 4,       // cell index out of range on input
     ERRF_SPIN = 8,       // sort look-back hit its spin limit
+    ERRF_STATE = 16,     // a particle lay outside its tile's recorded bounding box
 };
 
 // FP64 device accumulators.
@@ -88,6 +89,11 @@ struct SpeciesDev {
     // FP32 contributions times these powers of two (jx, jy, jz, rho), each
     // chosen so that one contribution stays below 2^30 in magnitude.
     float jsx, jsy, jsz, jsr;
+    // Per push tile: bounding box of the tile's cells as the last push left
+    // them, {x0, x extent, y0, y extent} with x0 in [0, nx), y0 in [0, ny)
+    // (extent -1: unknown).  tbox_ok: the boxes describe the current state.
+    int32_t *tbox;
+    int32_t tbox_ok;
 };
 
 struct PushParams {
