@@ -273,7 +273,7 @@ def main():
                               **adj)
         # the larger of the forms' adjoint-sweep / plain-forward time ratios
         out["adjoint_forward_ratio"] = max(a["ratio"] for a in adj.values())
-    out["diag"] = {f: {k: res[f]["diag1"][k] for k in ("ke_e", "w_e", "w_b", "je_int", "gauss_dlinf")}
+    out["diag"] = {f: {k: res[f]["diag1"][k] for k in ("ke_e", "w_e", "w_b", "je_int", "gauss_dlinf", "tiles_spilled")}
                    for f in res}
     if not args.no_cpu:
         log(f"[bench] cpu baseline ({args.cpu_seconds:.0f} s budget)")

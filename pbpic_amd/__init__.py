@@ -28,7 +28,7 @@ FIELDS = {"ex": 0, "ey": 1, "ez": 2, "bx": 3, "by": 4, "bz": 5, "jx": 6,
           "jy": 7, "jz": 8, "rho": 9, "gex": 10, "gey": 11, "gez": 12,
           "gbx": 13, "gby": 14, "gbz": 15}
 DIAG_NAMES = ["ke_e", "ke_i", "w_e", "w_b", "je_step", "je_int", "gauss_l2",
-              "gauss_linf", "gauss_dl2", "gauss_dlinf", "steps"]
+              "gauss_linf", "gauss_dl2", "gauss_dlinf", "steps", "tiles_spilled"]
 _ERR = {-1: "invalid argument", -2: "HIP runtime error", -3: "capacity",
         -4: "CFL violation", -5: "non-finite value", -6: "cell out of range",
         -7: "bad state", -8: "out of device memory", -9: "sort spin limit"}

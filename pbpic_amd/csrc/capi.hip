@@ -199,20 +199,29 @@ int sort_one(pbpic_ctx *c, int s, int32_t *counts, int32_t *perm, uint32_t *perm
     Species &S = c->sp[s];
     SpeciesDev src = species_dev(c, s, S.cur), dst = species_dev(c, s, S.cur ^ 1);
     HIPCHK(c, prof_mark(c, PBPIC_PROF_SORT, true));
-    HIPCHK(c, sort_species(c->sort, src, c->p.precision, (c->p.flags & PBPIC_FLAG_PUSH_ORDER) != 0,
-                           ncell(c), dst, c->err, counts != nullptr, c->st));
+    const bool po = (c->p.flags & PBPIC_FLAG_PUSH_ORDER) != 0;
+    CellKey key;
+    key.init(c->p.nx, c->p.ny, po ? 1 : 0);
+    HIPCHK(c, sort_species(c->sort, src, c->p.precision, po ? 1 : 0, key, ncell(c), dst, c->err,
+                           counts != nullptr, c->st));
     HIPCHK(c, prof_mark(c, PBPIC_PROF_SORT, false));
     S.cur ^= 1;
     S.tbox_ok = false;
     if (perm_dev && S.n > 0)
         HIPCHK(c, hipMemcpyAsync(perm_dev, c->sort.perm, sizeof(uint32_t) * S.n,
                                  hipMemcpyDeviceToDevice, c->st));
-    if (counts)
-        HIPCHK(c, hipMemcpyAsync(counts, c->sort.counts, sizeof(int32_t) * ncell(c),
+    std::vector<int32_t> kcounts;
+    if (counts) {
+        // the sort counts per key; the caller gets counts per cell (grid - 1)
+        kcounts.resize((size_t)key.nkeys);
+        HIPCHK(c, hipMemcpyAsync(kcounts.data(), c->sort.counts, sizeof(int32_t) * key.nkeys,
                                  hipMemcpyDeviceToHost, c->st));
+    }
     if (perm && S.n > 0)
         HIPCHK(c, hipMemcpyAsync(perm, c->sort.perm, sizeof(int32_t) * S.n, hipMemcpyDeviceToHost, c->st));
     if (counts || perm) HIPCHK(c, hipStreamSynchronize(c->st));
+    if (counts)
+        for (int64_t k = 0; k < ncell(c); k++) counts[k] = kcounts[key((uint32_t)k)];
     return PBPIC_OK;
 }
 
@@ -474,7 +483,9 @@ int pbpic_ctx_create(const pbpic_params *pp, pbpic_ctx **out)
     pbpic_ctx *c = new pbpic_ctx();
     c->p = p;
     c->rsz = p.precision == PBPIC_PREC_FP64 ? 8 : 4;
-    c->maxw = p.precision == PBPIC_PREC_FP64 ? 800 : 896;
+    // window capacity in nodes, sized so that three push workgroups share a
+    // CU's 160 KiB of LDS (push_lds_bytes without the rho plane)
+    c->maxw = p.precision == PBPIC_PREC_FP64 ? 752 : 768;
     c->idx = 1.0 / p.dx;                   // pic2d.f90:16
     c->idy = 1.0 / p.dy;
     c->dth = p.dt / 2.0;                   // :43
@@ -495,7 +506,12 @@ int pbpic_ctx_create(const pbpic_params *pp, pbpic_ctx **out)
     chk(dalloc(&c->NF, nodefield_bytes(c->p.precision, nc)), "alloc node fields");
     chk(dalloc(&c->acc, ACC_N * sizeof(double)), "alloc acc");
     chk(dalloc(&c->err, sizeof(int) * 4), "alloc err");
-    chk(dalloc(&c->sort.counts, nc * sizeof(int32_t)), "alloc counts");
+    {
+        CellKey key;                       // counts are indexed by sort key
+        key.init(p.nx, p.ny, 1);
+        chk(dalloc(&c->sort.counts, (size_t)std::max<int64_t>(key.nkeys, nc) * sizeof(int32_t)),
+            "alloc counts");
+    }
     chk(dalloc(&c->sort.dhist, 4 * 256 * sizeof(uint32_t)), "alloc dhist");
     chk(dalloc(&c->sort.ticket, 8 * sizeof(uint32_t)), "alloc ticket");
     if (rc == PBPIC_OK) {
@@ -851,6 +867,7 @@ int pbpic_diag(pbpic_ctx *c, double out[PBPIC_NDIAG])
     out[PBPIC_D_GAUSS_DL2] = sqrt(a[ACC_G_DL2]);
     out[PBPIC_D_GAUSS_DLINF] = a[ACC_G_DLINF];
     out[PBPIC_D_STEPS] = (double)c->steps;
+    out[PBPIC_D_TILES_SPILLED] = a[ACC_SPILL];
     return PBPIC_OK;
 }
 

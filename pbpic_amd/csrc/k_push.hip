@@ -165,18 +165,20 @@ __device__ __forceinline__ void deposit_terms(int i1, int j1, int a, int b,
         dimid = im - (R)i1 - H;                                              // :1263
         djmid = jm - (R)j1 - H;
     } else {
-        const R imr = ((R)a + dx1 + dx2 + ONE) * H;      // im - i1
-        const R jmr = ((R)b + dy1 + dy2 + ONE) * H;
-        irr = rmin((R)(a < 0 ? a : 0) + ONE, rmax((R)(a > 0 ? a : 0), imr));
-        jrr = rmin((R)(b < 0 ? b : 0) + ONE, rmax((R)(b > 0 ? b : 0), jmr));
+        const R af = (R)a, bf = (R)b;
+        const R imr = (af + dx1 + dx2 + ONE) * H;        // im - i1
+        const R jmr = (bf + dy1 + dy2 + ONE) * H;
+        // v_min/v_max: the same values for every finite input
+        irr = fmin(fmin(af, (R)0) + ONE, fmax(fmax(af, (R)0), imr));
+        jrr = fmin(fmin(bf, (R)0) + ONE, fmax(fmax(bf, (R)0), jmr));
         fx1 = qx * (irr - dx1 - H);
         fy1 = qy * (jrr - dy1 - H);
-        fx2 = qx * ((R)a + dx2 + H - irr);
-        fy2 = qy * ((R)b + dy2 + H - jrr);
+        fx2 = qx * (af + dx2 + H - irr);
+        fy2 = qy * (bf + dy2 + H - jrr);
         wx1 = (irr + dx1 + H) * H;
         wy1 = (jrr + dy1 + H) * H;
-        wx2 = (irr - (R)a + dx2 + H) * H;
-        wy2 = (jrr - (R)b + dy2 + H) * H;
+        wx2 = (irr - af + dx2 + H) * H;
+        wy2 = (jrr - bf + dy2 + H) * H;
         dimid = imr - H;
         djmid = jmr - H;
     }
@@ -309,7 +311,8 @@ __device__ __forceinline__ void push_math_split(const PushParams &P, float c0, f
 #pragma unroll
     for (int k = 0; k < 6; k++) f[k] = n00[k] * s00 + n10[k] * s10 + n01[k] * s01 + n11[k] * s11;
     const float umx = ux + c1 * f[0], umy = uy + c1 * f[1], umz = uz + c1 * f[2];
-    const float ig1 = rsqrtf(1.0f + umx * umx + umy * umy + umz * umz);
+    // 1 + u^2 >= 1: the raw reciprocal square root needs no denormal fix-up
+    const float ig1 = __builtin_amdgcn_rsqf(1.0f + umx * umx + umy * umy + umz * umz);
     const float c2 = c0 * ig1;
     const float tx = c2 * f[3], ty = c2 * f[4], tz = c2 * f[5];
     const float t2 = 2.0f * __builtin_amdgcn_rcpf(1.0f + tx * tx + ty * ty + tz * tz);
@@ -323,7 +326,7 @@ __device__ __forceinline__ void push_math_split(const PushParams &P, float c0, f
     uy = umy + (u0z * sx - u0x * sz) + c1 * f[1];
     uz = umz + (u0x * sy - u0y * sx) + c1 * f[2];
     const float u2 = ux * ux + uy * uy + uz * uz;
-    const float ig = rsqrtf(1.0f + u2);
+    const float ig = __builtin_amdgcn_rsqf(1.0f + u2);
     const float c3 = (float)P.dtcva * ig;
     const float dx1 = ox + c3 * ux * (float)P.idx;
     const float dy1 = oy + c3 * uy * (float)P.idy;
@@ -499,8 +502,132 @@ __device__ __forceinline__ void deposit_lds(double *jw, int WH, int W, int base,
     }
 }
 
+// Split form, interior tiles: the jx/jy deposit of particles that cross a
+// cell face is deferred to a per-wave LDS queue and done 64 at a time, so that
+// the two-segment zigzag (pic2d.f90:1239-1261) runs on full waves; a particle
+// that stays in its cell deposits the closed form of its two segments.
+constexpr int XQ_N = 128;                    // entries per wave (a power of two)
+constexpr size_t XQ_BYTES = (size_t)XQ_N * (2 * sizeof(float2) + sizeof(int));
+struct CrossQueue {
+    float2 *qx, *qy;                         // (old, new) offsets in x and in y
+    int *qc;                                 // window node | (a + 1) << 16 | (b + 1) << 18
+    int head, tail, lane;
+    __device__ void init(unsigned char *base, int wave, int ln)
+    {
+        unsigned char *w = base + (size_t)wave * XQ_BYTES;
+        qx = (float2 *)w;
+        qy = qx + XQ_N;
+        qc = (int *)(qy + XQ_N);
+        head = tail = 0;
+        lane = ln;
+    }
+    __device__ void push(bool cross, int node, int a, int b, float x1, float x2, float y1,
+                         float y2)
+    {
+        const uint64_t m = __ballot(cross);
+        if (cross) {
+            const int rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                                       __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
+            const int slot = (tail + rank) & (XQ_N - 1);
+            qx[slot] = make_float2(x1, x2);
+            qy[slot] = make_float2(y1, y2);
+            qc[slot] = node | ((a + 1) << 16) | ((b + 1) << 18);
+        }
+        tail += __popcll(m);
+    }
+    // the k oldest entries (k <= 64), one per lane
+    __device__ void drain(double *jw, int WH, int W, float qxs, float qys, int compat, int k)
+    {
+        if (k <= 0) return;
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        if (lane < k) {
+            const int slot = (head + lane) & (XQ_N - 1);
+            const float2 x = qx[slot], y = qy[slot];
+            const int pk = qc[slot];
+            const int node = pk & 0xffff, a = ((pk >> 16) & 3) - 1, b = ((pk >> 18) & 3) - 1;
+            Dep<float> d;
+            deposit_terms<float, true>(0, 0, a, b, x.x, x.y, y.x, y.y, qxs, qys, 0.0f, 0.0f,
+                                       compat, false, d);
+            double *wx = jw, *wy = jw + WH;
+            jadd(wx + node, d.jxa0);
+            jadd(wx + node + W, d.jxa1);
+            jadd(wy + node, d.jya0);
+            jadd(wy + node + 1, d.jya1);
+            const int bb = node + b * W + a;
+            jadd(wx + bb, d.jxb0);
+            jadd(wx + bb + W, d.jxb1);
+            jadd(wy + bb, d.jyb0);
+            jadd(wy + bb + 1, d.jyb1);
+        }
+        head += k;
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    }
+};
+
+// LDS layout of a push workgroup: node-field window [maxw][NS], current
+// window [3 or 4 planes][maxw] of 8-byte sums, then (split form) the four
+// waves' crossing queues.  Split-form offset of the queues:
+__host__ __device__ constexpr size_t push_queue_offset(int maxw, int deposit_rho)
+{
+    return (size_t)maxw * (8 * 4 + (deposit_rho ? 4 : 3) * 8);
+}
+
+// One particle's deposit in the split form on an interior tile (contributions
+// pre-scaled to the fixed-point window).  jz: the CIC weights of the move's
+// midpoint about the old cell centre (:1263-1281), for every particle.  jx,
+// jy: a move inside one cell has its zigzag relay point at the midpoint, and
+// the sum of its two segments is q dx/dt * Dx * (1/2 -+ ym) on jx(0,0) and
+// jx(0,1) and q dy/dt * Dy * (1/2 -+ xm) on jy(0,0) and jy(1,0) (xm, ym the
+// midpoint offsets); a move across a face goes to the queue.
+__device__ __forceinline__ void deposit_split_fast(double *jw, int WH, int W, int base, int a,
+                                                   int b, float dx1, float dx2, float dy1,
+                                                   float dy2, float qxs, float qys, float qzs,
+                                                   float rsc, float vz, int compat,
+                                                   bool want_rho, CrossQueue &Q)
+{
+    double *wx = jw, *wy = jw + WH, *wz = jw + 2 * WH;
+    const float dimid = ((float)a + dx1 + dx2 + 1.0f) * 0.5f - 0.5f;
+    const float djmid = ((float)b + dy1 + dy2 + 1.0f) * 0.5f - 0.5f;
+    const int di = signbit(dimid) ? -1 : 1, dj = signbit(djmid) ? -1 : 1;
+    const float ai = fabsf(dimid), aj = fabsf(djmid);
+    const float s1 = ai * aj, s2 = ai - s1, s3 = aj - s1, s4 = 1.0f - s1 - s2 - s3;
+    const float quz = qzs * vz;
+    jadd(wz + base, quz * s4);
+    jadd(wz + base + di, quz * s2);
+    jadd(wz + base + dj * W, quz * s3);
+    jadd(wz + base + dj * W + di, quz * s1);
+    const bool cross = (a | b) != 0;
+    if (!cross) {
+        const float xm = (dx1 + dx2) * 0.5f, ym = (dy1 + dy2) * 0.5f;
+        const float fx = qxs * (dx2 - dx1), fy = qys * (dy2 - dy1);
+        jadd(wx + base, fx * (0.5f - ym));
+        jadd(wx + base + W, fx * (0.5f + ym));
+        jadd(wy + base, fy * (0.5f - xm));
+        jadd(wy + base + 1, fy * (0.5f + xm));
+    }
+    if (want_rho) {                          // rho at time n (:1283-1295)
+        float r1, r2, r3;
+        if (compat & PBPIC_COMPAT_RHO_MIRROR) {
+            r1 = (0.5f - dx1) * (0.5f - dy1);
+            r2 = 0.5f - dx1 - r1;
+            r3 = 0.5f - dy1 - r1;
+        } else {
+            r1 = (0.5f + dx1) * (0.5f + dy1);
+            r2 = 0.5f + dx1 - r1;
+            r3 = 0.5f + dy1 - r1;
+        }
+        const float r4 = 1.0f - r1 - r2 - r3, qr = qzs * rsc;
+        double *wr = jw + 3 * WH;
+        jadd(wr + base, r4 * qr);
+        jadd(wr + base + 1, r2 * qr);
+        jadd(wr + base + W, r3 * qr);
+        jadd(wr + base + W + 1, r1 * qr);
+    }
+    Q.push(cross, base, a, b, dx1, dx2, dy1, dy2);
+}
+
 template <typename R, bool SPLIT>
-__global__ __launch_bounds__(PUSH_THREADS, 2) void push_kernel(PushParams P)
+__global__ __launch_bounds__(PUSH_THREADS, 3) void push_kernel(PushParams P)
 {
     constexpr int V = Vw<R>::V;
     constexpr int CH = 64 * V;                                   // slots per wave-wide load
@@ -534,7 +661,7 @@ __global__ __launch_bounds__(PUSH_THREADS, 2) void push_kernel(PushParams P)
     if (!boxed) {
     const int c00 = S.cell[tstart];
     if ((unsigned)c00 < (unsigned)(nx * ny)) {
-        x0 = (int)P.dny.div((uint32_t)c00);
+        x0 = (int)P.dny.div31((uint32_t)c00);
         y0 = c00 - x0 * ny;
     }
 #pragma unroll
@@ -546,7 +673,7 @@ __global__ __launch_bounds__(PUSH_THREADS, 2) void push_kernel(PushParams P)
             const int c = cv[j];
             if (pbase + r * CH + j >= n) continue;
             if ((unsigned)c >= (unsigned)(nx * ny)) { err |= ERRF_CELL; continue; }
-            const int cx = (int)P.dny.div((uint32_t)c), cy = c - cx * ny;
+            const int cx = (int)P.dny.div31((uint32_t)c), cy = c - cx * ny;
             int rx = cx - x0, ry = cy - y0;
             if (rx > nxh) rx -= nx; else if (rx < -nxh) rx += nx;
             if (ry > nyh) ry -= ny; else if (ry < -nyh) ry += ny;
@@ -577,6 +704,7 @@ __global__ __launch_bounds__(PUSH_THREADS, 2) void push_kernel(PushParams P)
     const int wx0 = mnx - 1, wy0 = mny - 1;
     const int W = mxx - mnx + 4, H = mxy - mny + 4;
     const bool inlds = mnx <= mxx && (int64_t)W * H <= P.maxw;
+    if (!inlds && mnx <= mxx && tid == 0) unsafeAtomicAdd(P.acc + ACC_SPILL, 1.0);
     const int WH = inlds ? W * H : 0;
     // fast path: a full tile of valid cells, at least one cell away from every
     // domain edge (no periodic unwrap or wrap for any particle or move)
@@ -628,7 +756,7 @@ __global__ __launch_bounds__(PUSH_THREADS, 2) void push_kernel(PushParams P)
 
     // one particle: gather, push, move, deposit; returns the new cell
     auto one = [&](int c, R &ox, R &oy, R &ux, R &uy, R &uz) -> int {
-        const int cx = (int)P.dny.div((uint32_t)c), cy = c - cx * ny;
+        const int cx = (int)P.dny.div31((uint32_t)c), cy = c - cx * ny;
         int rx = cx - x0, ry = cy - y0;
         if (rx > nxh) rx -= nx; else if (rx < -nxh) rx += nx;
         if (ry > nyh) ry -= ny; else if (ry < -nyh) ry += ny;
@@ -723,16 +851,18 @@ __global__ __launch_bounds__(PUSH_THREADS, 2) void push_kernel(PushParams P)
           *wp = uz_p + tstart;
         const int lo = wave * ROUNDS * CH + lane * V;
         R kacc = 0;
+        CrossQueue Q;
+        Q.init(smem + push_queue_offset(P.maxw, P.deposit_rho), wave, lane);
         auto onef = [&](int c, R &ox, R &oy, R &ux, R &uy, R &uz) -> int {
-            const int cx = (int)P.dny.div((uint32_t)c), cy = c - cx * ny;
-            const R *b = nf + (size_t)((cx - XW) * H + (cy - YW)) * NS;
+            const int cx = (int)P.dny.div31((uint32_t)c), cy = c - cx * ny;
+            const R *b = nf + ((cx - XW) * H + (cy - YW)) * NS;
             R n00[6], n10[6], n01[6], n11[6];
 #pragma unroll
             for (int q = 0; q < 6; q++) {
                 n00[q] = b[q];
-                n10[q] = b[(size_t)H * NS + q];
+                n10[q] = b[H * NS + q];
                 n01[q] = b[NS + q];
-                n11[q] = b[(size_t)H * NS + NS + q];
+                n11[q] = b[H * NS + NS + q];
             }
             const R dx0 = ox, dy0 = oy;
             int dcx, dcy;
@@ -745,14 +875,20 @@ __global__ __launch_bounds__(PUSH_THREADS, 2) void push_kernel(PushParams P)
                              gam, err);
                 kin = gam - (R)1;                                                  // :1062
             }
-            Dep<R> d;
-            deposit_terms<R, SPLIT>(cx + 1, cy + 1, dcx, dcy, dx0, ox, dy0, oy, dqx, dqy, dqz,
-                                    vz, P.compat, want_rho, d);
-            if (fxw && want_rho) { d.r0 *= rsc; d.r1 *= rsc; d.r2 *= rsc; d.r3 *= rsc; }
-            if (d.a == 0 && d.b == 0) {
-                d.jxa0 += d.jxb0; d.jxa1 += d.jxb1; d.jya0 += d.jyb0; d.jya1 += d.jyb1;
+            const int base = (cy - YW) * W + (cx - XW);
+            if constexpr (SPLIT) {
+                deposit_split_fast(jw, WH, W, base, dcx, dcy, dx0, ox, dy0, oy, dqx, dqy, dqz,
+                                   rsc, vz, P.compat, want_rho, Q);
+                if (Q.tail - Q.head >= 64) Q.drain(jw, WH, W, dqx, dqy, P.compat, 64);
+            } else {
+                Dep<R> d;
+                deposit_terms<R, SPLIT>(cx + 1, cy + 1, dcx, dcy, dx0, ox, dy0, oy, dqx, dqy,
+                                        dqz, vz, P.compat, want_rho, d);
+                if (d.a == 0 && d.b == 0) {
+                    d.jxa0 += d.jxb0; d.jxa1 += d.jxb1; d.jya0 += d.jyb0; d.jya1 += d.jyb1;
+                }
+                deposit_lds(jw, WH, W, base, d, want_rho);
             }
-            deposit_lds(jw, WH, W, (cy - YW) * W + (cx - XW), d, want_rho);
             kacc += kin;
             track(cx - XW, cy - YW, dcx, dcy);
             return c + dcx * ny + dcy;          // interior tile: no wrap (:1032-1052)
@@ -784,6 +920,7 @@ __global__ __launch_bounds__(PUSH_THREADS, 2) void push_kernel(PushParams P)
                 for (int q = 0; q < 5; q++) cur[q][j] = nxt[q][j];
             }
         }
+        if constexpr (SPLIT) Q.drain(jw, WH, W, dqx, dqy, P.compat, Q.tail - Q.head);
         ke = (double)kacc * S.ms;
     } else {
     // the tile's cells were read in phase A and are re-read here from L2
@@ -975,7 +1112,9 @@ size_t push_lds_bytes(int precision, int maxw, int deposit_rho)
 {
     const size_t r = precision == PBPIC_PREC_FP64 ? 8 : 4;
     const size_t ns = precision == PBPIC_PREC_FP64 ? 6 : 8;
-    return (size_t)maxw * (ns * r + (deposit_rho ? 4 : 3) * 8);
+    const size_t windows = (size_t)maxw * (ns * r + (deposit_rho ? 4 : 3) * 8);
+    return precision == PBPIC_PREC_FP64 ? windows
+                                        : windows + (PUSH_THREADS / 64) * XQ_BYTES;
 }
 
 hipError_t launch_push(const PushParams &p, int precision, hipStream_t st)

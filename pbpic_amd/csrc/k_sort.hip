@@ -22,7 +22,8 @@ constexpr int SPIN_LIMIT = 1 << 24;
 // Per-cell counts with run-length aggregation inside a wave: consecutive
 // equal cell ids (the common case for an almost sorted array) cost one atomic.
 __global__ __launch_bounds__(256) void cell_hist_kernel(const int32_t *cell, int64_t n,
-                                                        int64_t ncell, int32_t *counts, int *err)
+                                                        int64_t ncell, CellKey key,
+                                                        int32_t *counts, int *err)
 {
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     const int lane = threadIdx.x & 63;
@@ -31,7 +32,7 @@ __global__ __launch_bounds__(256) void cell_hist_kernel(const int32_t *cell, int
         atomicOr(err, ERRF_CELL);
         valid = false;
     }
-    const int k = valid ? cell[i] : -1;
+    const int k = valid ? (int)key((uint32_t)cell[i]) : -1;
     const int prev = __shfl_up(k, 1);
     // a run ends at the next lane whose key differs (invalid lanes carry -1)
     const bool boundary = lane == 0 || k != prev;
@@ -89,7 +90,7 @@ __global__ void digit_scan_kernel(uint32_t *dhist, int npass)
 template <bool FIRST, bool LAST>
 __global__ __launch_bounds__(ST_THREADS) void radix_pass_kernel(
     const uint32_t *kin, const uint32_t *vin, uint32_t *kout, uint32_t *vout, int64_t n,
-    int shift, const uint32_t *dbase, uint32_t *status, uint32_t *ticket, int *err)
+    int shift, const uint32_t *dbase, uint32_t *status, uint32_t *ticket, int *err, CellKey ck)
 {
     __shared__ uint32_t whist[4][256];
     __shared__ uint32_t woff[4][256];
@@ -107,7 +108,8 @@ __global__ __launch_bounds__(ST_THREADS) void radix_pass_kernel(
     for (int r = 0; r < ST_ROUNDS; r++) {
         const int64_t i = wbase + r * 64 + lane;
         const bool valid = i < n;
-        const uint32_t k = valid ? kin[i] : 0u;
+        // the first pass reads cell ids and sorts by their key
+        const uint32_t k = valid ? (FIRST ? ck(kin[i]) : kin[i]) : 0u;
         const uint32_t v = FIRST ? (uint32_t)i : (valid ? vin[i] : 0u);
         const uint32_t d = (k >> shift) & 255u;
         uint64_t m = __ballot(valid);
@@ -196,26 +198,27 @@ __global__ void gather_kernel(const uint32_t *perm, int64_t n, int tv, uint32_t 
 
 // Sorts species sp into dst (a second buffer set of the same capacity).
 hipError_t sort_species(SortScratch &s, const SpeciesDev &sp, int precision, int push_order,
-                        int64_t ncell,
+                        const CellKey &key, int64_t ncell,
                         SpeciesDev &dst, int *err, int want_counts, hipStream_t st)
 {
     (void)want_counts;
     const int64_t n = sp.n;
     hipError_t e;
     dst.n = n;
-    if (n == 0) return hipSuccess;
+    const int64_t nkeys = key.nkeys;
+    if (n == 0) return launch_fill_u32((uint32_t *)s.counts, nkeys, 0u, st);
     int bits = 0;
-    while ((1ll << bits) < ncell) bits++;
+    while ((1ll << bits) < nkeys) bits++;
     int npass = (bits + 7) / 8;
     if (npass < 1) npass = 1;
     const int64_t ntile = (n + ST_TILE - 1) / ST_TILE;
 
-    if ((e = launch_fill_u32((uint32_t *)s.counts, ncell, 0u, st)) != hipSuccess) return e;
+    if ((e = launch_fill_u32((uint32_t *)s.counts, nkeys, 0u, st)) != hipSuccess) return e;
     if ((e = launch_fill_u32(s.dhist, 4 * 256, 0u, st)) != hipSuccess) return e;
     hipLaunchKernelGGL(cell_hist_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
-                       sp.cell, n, ncell, s.counts, err);
-    const int64_t nhb = (ncell + 256 * 64 - 1) / (256 * 64);
-    hipLaunchKernelGGL(digit_hist_kernel, dim3((unsigned)nhb), dim3(256), 0, st, s.counts, ncell,
+                       sp.cell, n, ncell, key, s.counts, err);
+    const int64_t nhb = (nkeys + 256 * 64 - 1) / (256 * 64);
+    hipLaunchKernelGGL(digit_hist_kernel, dim3((unsigned)nhb), dim3(256), 0, st, s.counts, nkeys,
                        npass, s.dhist);
     hipLaunchKernelGGL(digit_scan_kernel, dim3(1), dim3(64), 0, st, s.dhist, npass);
     if ((e = hipGetLastError()) != hipSuccess) return e;
@@ -232,16 +235,16 @@ hipError_t sort_species(SortScratch &s, const SpeciesDev &sp, int precision, int
         dim3 g((unsigned)ntile), b(ST_THREADS);
         if (first && last)
             hipLaunchKernelGGL((radix_pass_kernel<true, true>), g, b, 0, st, kin, vin, ko, vo, n,
-                               8 * p, db, s.status, s.ticket + p, err);
+                               8 * p, db, s.status, s.ticket + p, err, key);
         else if (first)
             hipLaunchKernelGGL((radix_pass_kernel<true, false>), g, b, 0, st, kin, vin, ko, vo, n,
-                               8 * p, db, s.status, s.ticket + p, err);
+                               8 * p, db, s.status, s.ticket + p, err, key);
         else if (last)
             hipLaunchKernelGGL((radix_pass_kernel<false, true>), g, b, 0, st, kin, vin, ko, vo, n,
-                               8 * p, db, s.status, s.ticket + p, err);
+                               8 * p, db, s.status, s.ticket + p, err, key);
         else
             hipLaunchKernelGGL((radix_pass_kernel<false, false>), g, b, 0, st, kin, vin, ko, vo, n,
-                               8 * p, db, s.status, s.ticket + p, err);
+                               8 * p, db, s.status, s.ticket + p, err, key);
         if ((e = hipGetLastError()) != hipSuccess) return e;
         kin = ko;
         vin = vo;

@@ -50,6 +50,7 @@ enum : int {
     ACC_G_LINF = 7,    // Gauss residual max (bit pattern of a double >= 0)
     ACC_G_DL2 = 8,     // sum of squares of the residual change
     ACC_G_DLINF = 9,   // max of the residual change
+    ACC_SPILL = 10,    // push tiles whose window did not fit in LDS (global atomics)
     ACC_N = 16,
 };
 
@@ -71,6 +72,12 @@ struct FastDiv {
     {
         uint32_t hi = __umulhi(n, m);
         return (uint32_t)(((uint64_t)hi + n) >> s);
+    }
+    // n < 2^31 (every cell id: contexts hold fewer than 2^30 cells): hi <= n,
+    // so hi + n does not overflow 32 bits
+    __host__ __device__ __forceinline__ uint32_t div31(uint32_t n) const
+    {
+        return ((uint32_t)(((uint64_t)n * m) >> 32) + n) >> s;
     }
 };
 
@@ -198,9 +205,36 @@ hipError_t launch_node_transpose(const AdjFieldParams &p, double *nadj, hipStrea
 hipError_t launch_unpermute(const uint32_t *perm, int64_t n, int precision,
                             void *const *post, void *const *pre, hipStream_t st);
 
-// push_order != 0: full tiles come out in the push order (tile_rank).
+// Sort key of a cell.  Reference order: the compact cell id cx*ny + cy.
+// Push order: cells grouped in 8 x 8 supertiles, the supertiles x-major
+// (sx*nby + sy) and the cells x-major inside each, so that a run of 4096
+// particles (~41 cells at 100 per cell) covers a compact ~6 x 8 block
+// instead of a 41-cell column strip.
+struct CellKey {
+    FastDiv dny;
+    uint32_t ny, nby;
+    int tiled;
+    int64_t nkeys;            // keys lie in [0, nkeys)
+    void init(int nx, int ny_, int tiled_)
+    {
+        dny.init((uint32_t)ny_);
+        ny = (uint32_t)ny_;
+        nby = (uint32_t)((ny_ + 7) / 8);
+        tiled = tiled_;
+        nkeys = tiled ? (int64_t)((nx + 7) / 8) * nby * 64 : (int64_t)nx * ny_;
+    }
+    __host__ __device__ uint32_t operator()(uint32_t c) const
+    {
+        if (!tiled) return c;
+        const uint32_t cx = dny.div31(c), cy = c - cx * ny;
+        return (((cx >> 3) * nby + (cy >> 3)) << 6) | ((cx & 7u) << 3) | (cy & 7u);
+    }
+};
+
+// push_order != 0: cells in CellKey's supertile order and full tiles in the
+// push order (tile_rank).  counts are indexed by key.(s.counts holds nkeys).
 hipError_t sort_species(SortScratch &s, const SpeciesDev &sp, int precision, int push_order,
-                        int64_t ncell, SpeciesDev &dst, int *err,
+                        const CellKey &key, int64_t ncell, SpeciesDev &dst, int *err,
                         int want_counts, hipStream_t st);
 
 }  // namespace pbpic

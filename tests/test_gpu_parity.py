@@ -68,6 +68,14 @@ def test_countingsort_bitexact(gpu, kind, nx, ny):
     assert np.array_equal(g["ux"], s["ux"][perm])
 
 
+def _supertile_key(cell, nx, ny):
+    """Push-order key of a 0-based cell id: 8 x 8 supertiles x-major, cells
+    x-major inside a supertile."""
+    cx, cy = cell // ny, cell % ny
+    nby = (ny + 7) // 8
+    return ((cx >> 3) * nby + (cy >> 3)) * 64 + (cx & 7) * 8 + (cy & 7)
+
+
 def _push_order(ref_perm, V):
     """The push order of a cell-order permutation: inside each full tile of
     4096 slots, slot q = c*64V + l*V + j (round c, lane l, element j) holds
@@ -84,10 +92,10 @@ def _push_order(ref_perm, V):
 @pytest.mark.parametrize("precision", [0, 1])
 @pytest.mark.parametrize("n", [4095, 4096, 20000, 3 * 4096 + 17])
 def test_countingsort_push_order(gpu, precision, n):
-    """With the push order (the default) every full 4096-particle tile holds
-    exactly the particles the reference's cell order puts there, the 64 lanes
-    of a wave-wide load hold particles 64 cell-order ranks apart, and the
-    counts are the reference's."""
+    """With the push order (the default) the particles are sorted stably by
+    supertile key (8 x 8 cell blocks), every full 4096-particle tile is laid
+    out so that the 64 lanes of a wave-wide load hold particles 64 ranks
+    apart, and the per-cell counts are the reference's."""
     _say(f"push-order sort precision={precision} n={n}")
     nx, ny = 40, 37
     rng = np.random.default_rng(5 + n)
@@ -101,7 +109,7 @@ def test_countingsort_push_order(gpu, precision, n):
     z = np.zeros(n)
     ctx.set_particles(k, grid, z, z, ux, z, z, pid)
     counts, perm = ctx.countingsort(k, want=True)
-    ref = np.argsort(grid, kind="stable")
+    ref = np.argsort(_supertile_key(grid - 1, nx, ny), kind="stable")
     assert np.array_equal(counts, np.bincount(grid - 1, minlength=nx * ny))
     for t0 in range(0, n, 4096):
         assert np.array_equal(np.sort(perm[t0:t0 + 4096]), np.sort(ref[t0:t0 + 4096]))
@@ -224,13 +232,13 @@ def _by_pid(sp):
 
 @pytest.mark.parametrize("precision,compat", [(0, 7), (0, 0), (1, 7)])
 def test_one_step_full_tiles(gpu, precision, compat):
-    """Sorted species of 48 full 4096-particle tiles (12 x 256 cells, 64 per
-    cell), most of them interior: the push of whole tiles in the push order
-    against the oracle's push of the same particles in cell order, matched
-    by particle id.  FP64 to 1e-12; split precision to the bars of
-    test_one_step_split32."""
+    """Sorted species of 32 full 4096-particle tiles (32 x 64 cells, 64 per
+    cell: one 8 x 8 supertile per tile), 12 of them interior: the push of
+    whole tiles in the push order against the oracle's push of the same
+    particles in cell order, matched by particle id.  FP64 to 1e-12; split
+    precision to the bars of test_one_step_split32."""
     _say(f"one step full tiles precision={precision} compat={compat}")
-    case = make_case(nx=12, ny=256, ppc=64, seed=31, efield=0.05, bfield=0.5)
+    case = make_case(nx=32, ny=64, ppc=64, seed=31, efield=0.05, bfield=0.5)
     ctx = gpu_for(case, precision=precision, compat=compat)
     o = oracle_for(case, compat=compat)
     ctx.countingsort(); o.countingsort()
