@@ -44,6 +44,56 @@ __global__ __launch_bounds__(256) void cell_hist_kernel(const int32_t *cell, int
     }
 }
 
+// Per-key counts, one push tile (PUSH_TILE particles) per workgroup, using
+// the tile's bounding box that the last push recorded (SpeciesDev::tbox):
+// counts go to an LDS histogram over the box and are flushed with one global
+// atomic per occupied cell.  A tile with no recorded box, a box larger than
+// the LDS histogram, or a cell outside its box counts with global atomics.
+constexpr int HB_BINS = 2048;
+__global__ __launch_bounds__(256) void cell_hist_tiles_kernel(const int32_t *cell, int64_t n,
+                                                              int nx, int64_t ncell,
+                                                              CellKey key, const int32_t *tbox,
+                                                              int32_t *counts, int *err)
+{
+    __shared__ uint32_t h[HB_BINS];
+    const int64_t t0 = (int64_t)blockIdx.x * PUSH_TILE;
+    const int4 b = *reinterpret_cast<const int4 *>(tbox + 4 * (size_t)blockIdx.x);
+    const int W = b.y + 1, H = b.w + 1;
+    const bool boxed = b.y >= 0 && b.w >= 0 && (int64_t)W * H <= HB_BINS;
+    const int ny = (int)key.ny;
+    if (boxed)
+        for (int k = threadIdx.x; k < W * H; k += 256) h[k] = 0;
+    __syncthreads();
+    for (int q = threadIdx.x; q < PUSH_TILE; q += 256) {
+        const int64_t i = t0 + q;
+        if (i >= n) break;
+        const int c = cell[i];
+        if (c < 0 || c >= ncell) { atomicOr(err, ERRF_CELL); continue; }
+        if (boxed) {
+            const int cx = (int)key.dny.div31((uint32_t)c), cy = c - cx * ny;
+            int rx = cx - b.x, ry = cy - b.z;
+            if (rx < 0) rx += nx;
+            if (ry < 0) ry += ny;
+            if (rx < W && ry < H) {
+                atomicAdd(&h[ry * W + rx], 1u);
+                continue;
+            }
+        }
+        atomicAdd(counts + key((uint32_t)c), 1);
+    }
+    __syncthreads();
+    if (boxed)
+        for (int k = threadIdx.x; k < W * H; k += 256) {
+            const uint32_t v = h[k];
+            if (!v) continue;
+            const int ky = k / W, kx = k - ky * W;
+            int gx = b.x + kx, gy = b.z + ky;
+            if (gx >= nx) gx -= nx;
+            if (gy >= ny) gy -= ny;
+            atomicAdd(counts + key((uint32_t)(gx * ny + gy)), (int)v);
+        }
+}
+
 // Digit histograms of all passes from the per-cell counts.
 __global__ __launch_bounds__(256) void digit_hist_kernel(const int32_t *counts, int64_t ncell,
                                                          int npass, uint32_t *dhist)
@@ -173,27 +223,79 @@ __global__ __launch_bounds__(ST_THREADS) void radix_pass_kernel(
 
 // dst[q] = src[perm[rank(q)]]; rank(q) is q in cell order, or the push order
 // (tile_rank) inside full tiles when tv = V > 0.  perm_out[q] receives the
-// source index of slot q.
-template <typename R>
-__global__ void gather_kernel(const uint32_t *perm, int64_t n, int tv, uint32_t *perm_out,
-                              SpeciesDev src, SpeciesDev dst)
+// source index of slot q.  One lane moves G = 16 bytes / sizeof(R) consecutive
+// slots, whose ranks are consecutive in both orders (G equals the push
+// kernel's vector width); when their sources are G consecutive, aligned
+// slots -- particles that kept their order since the last sort -- each array
+// moves with one vector load, otherwise element by element.
+template <typename T, int G> struct Vec;
+template <> struct Vec<float, 4> { using t = float4; };
+template <> struct Vec<double, 2> { using t = double2; };
+template <> struct Vec<int32_t, 4> { using t = int4; };
+template <> struct Vec<int32_t, 2> { using t = int2; };
+template <> struct Vec<uint32_t, 4> { using t = uint4; };
+template <> struct Vec<uint32_t, 2> { using t = uint2; };
+
+template <int G, typename T>
+__device__ __forceinline__ void gmove(T *dst, const T *src, const uint32_t (&p)[G], bool contig)
 {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    int64_t k = i;
-    if (tv) {
-        const int64_t t0 = i & ~(int64_t)(PUSH_TILE - 1);
-        if (t0 + PUSH_TILE <= n) k = t0 + tile_rank((int)(i - t0), tv);
+    using VT = typename Vec<T, G>::t;
+    if (contig) {
+        *reinterpret_cast<VT *>(dst) = *reinterpret_cast<const VT *>(src + p[0]);
+    } else {
+        T v[G];
+#pragma unroll
+        for (int j = 0; j < G; j++) v[j] = src[p[j]];
+        *reinterpret_cast<VT *>(dst) = *reinterpret_cast<const VT *>(v);
     }
-    const uint32_t p = perm[k];
-    if (perm_out) perm_out[i] = p;
-    dst.cell[i] = src.cell[p];
-    ((R *)dst.ox)[i] = ((const R *)src.ox)[p];
-    ((R *)dst.oy)[i] = ((const R *)src.oy)[p];
-    ((R *)dst.ux)[i] = ((const R *)src.ux)[p];
-    ((R *)dst.uy)[i] = ((const R *)src.uy)[p];
-    ((R *)dst.uz)[i] = ((const R *)src.uz)[p];
-    if (src.pid) dst.pid[i] = src.pid[p];
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void gather_kernel(const uint32_t *perm, int64_t n, int tv,
+                                                     uint32_t *perm_out, SpeciesDev src,
+                                                     SpeciesDev dst)
+{
+    constexpr int G = 16 / (int)sizeof(R);
+    const int64_t i0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * G;
+    if (i0 >= n) return;
+    if (i0 + G > n) {                        // ragged end: element by element
+        for (int64_t i = i0; i < n; i++) {
+            const uint32_t p = perm[i];
+            if (perm_out) perm_out[i] = p;
+            dst.cell[i] = src.cell[p];
+            ((R *)dst.ox)[i] = ((const R *)src.ox)[p];
+            ((R *)dst.oy)[i] = ((const R *)src.oy)[p];
+            ((R *)dst.ux)[i] = ((const R *)src.ux)[p];
+            ((R *)dst.uy)[i] = ((const R *)src.uy)[p];
+            ((R *)dst.uz)[i] = ((const R *)src.uz)[p];
+            if (src.pid) dst.pid[i] = src.pid[p];
+        }
+        return;
+    }
+    int64_t k0 = i0;
+    if (tv) {
+        const int64_t t0 = i0 & ~(int64_t)(PUSH_TILE - 1);
+        if (t0 + PUSH_TILE <= n) k0 = t0 + tile_rank((int)(i0 - t0), tv);
+    }
+    uint32_t p[G];
+    {
+        using VU = typename Vec<uint32_t, G>::t;
+        *reinterpret_cast<VU *>(p) = *reinterpret_cast<const VU *>(perm + k0);
+    }
+    bool contig = p[0] % G == 0;
+#pragma unroll
+    for (int j = 1; j < G; j++) contig = contig && p[j] == p[0] + j;
+    if (perm_out) {
+        using VU = typename Vec<uint32_t, G>::t;
+        *reinterpret_cast<VU *>(perm_out + i0) = *reinterpret_cast<const VU *>(p);
+    }
+    gmove<G>(dst.cell + i0, src.cell, p, contig);
+    gmove<G>((R *)dst.ox + i0, (const R *)src.ox, p, contig);
+    gmove<G>((R *)dst.oy + i0, (const R *)src.oy, p, contig);
+    gmove<G>((R *)dst.ux + i0, (const R *)src.ux, p, contig);
+    gmove<G>((R *)dst.uy + i0, (const R *)src.uy, p, contig);
+    gmove<G>((R *)dst.uz + i0, (const R *)src.uz, p, contig);
+    if (src.pid) gmove<G>(dst.pid + i0, src.pid, p, contig);
 }
 
 // Sorts species sp into dst (a second buffer set of the same capacity).
@@ -215,8 +317,13 @@ hipError_t sort_species(SortScratch &s, const SpeciesDev &sp, int precision, int
 
     if ((e = launch_fill_u32((uint32_t *)s.counts, nkeys, 0u, st)) != hipSuccess) return e;
     if ((e = launch_fill_u32(s.dhist, 4 * 256, 0u, st)) != hipSuccess) return e;
-    hipLaunchKernelGGL(cell_hist_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
-                       sp.cell, n, ncell, key, s.counts, err);
+    if (sp.tbox_ok)
+        hipLaunchKernelGGL(cell_hist_tiles_kernel, dim3((unsigned)((n + PUSH_TILE - 1) / PUSH_TILE)),
+                           dim3(256), 0, st, sp.cell, n, (int)(ncell / key.ny), ncell, key,
+                           (const int32_t *)sp.tbox, s.counts, err);
+    else
+        hipLaunchKernelGGL(cell_hist_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
+                           sp.cell, n, ncell, key, s.counts, err);
     const int64_t nhb = (nkeys + 256 * 64 - 1) / (256 * 64);
     hipLaunchKernelGGL(digit_hist_kernel, dim3((unsigned)nhb), dim3(256), 0, st, s.counts, nkeys,
                        npass, s.dhist);
@@ -257,7 +364,8 @@ hipError_t sort_species(SortScratch &s, const SpeciesDev &sp, int precision, int
     const int tv = push_order && n >= PUSH_TILE ? push_vec(precision) : 0;
     uint32_t *pout = nullptr;
     if (tv) s.perm = pout = s.vals[pb];
-    const unsigned nb = (unsigned)((n + 255) / 256);
+    const int G = precision == PBPIC_PREC_FP64 ? 2 : 4;        // 16 bytes of R per lane
+    const unsigned nb = (unsigned)(((n + G - 1) / G + 255) / 256);
     if (precision == PBPIC_PREC_FP64)
         hipLaunchKernelGGL(gather_kernel<double>, dim3(nb), dim3(256), 0, st, perm, n, tv, pout, sp, dst);
     else
