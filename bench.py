@@ -61,6 +61,13 @@ def run_form(precision, cfg, steps, warmup, sort_every, seed, push_order=True):
                 prof=prof, diag0=d0, diag1=d1)
 
 
+# HBM bytes per push launch at 2048x2048 x 100 ppc, steady state: rocprofv3
+# FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (factors from a 4 GiB copy), from the
+# commands in profiles/push_kernel_traffic_two_stream_2048x2048_ppc100.md.
+TRAFFIC_2048_PPC100 = {1: 10.301e9 + 10.339e9, 0: 19.225e9 + 19.228e9}
+TRAFFIC_SOURCE = "profiles/push_kernel_traffic_two_stream_2048x2048_ppc100.md"
+
+
 def roofline(precision, res, cfg):
     """Push kernel: algorithmic bytes per launch = particles x (read + write of
     one particle record) + cells x (6 staggered field planes read, FP64, + 3
@@ -72,8 +79,11 @@ def roofline(precision, res, cfg):
     ms, cnt = res["prof"]["push"]
     avg_s = ms / cnt / 1e3
     achieved = bytes_launch / avg_s / 1e9
+    at_scale = (cfg["nx"], cfg["ny"], cfg["species"][0]["ppc"]) == (2048, 2048, 100)
+    traffic = TRAFFIC_2048_PPC100[precision] if at_scale else None
     return dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s",
-                frac=achieved / HBM_PEAK_GBS, traffic=None, kernel="push_kernel",
+                frac=achieved / HBM_PEAK_GBS, traffic=traffic,
+                traffic_source=TRAFFIC_SOURCE if at_scale else None, kernel="push_kernel",
                 bytes_per_launch=bytes_launch, avg_ms=avg_s * 1e3)
 
 
