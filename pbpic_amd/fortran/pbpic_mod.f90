@@ -14,7 +14,8 @@ module pbpic_gpu
 
     integer(c_int32_t), parameter :: PBPIC_PREC_FP64 = 0, PBPIC_PREC_SPLIT32 = 1
     integer(c_int32_t), parameter :: PBPIC_COMPAT_REFERENCE = 7
-    integer(c_int32_t), parameter :: PBPIC_FLAG_DEPOSIT_RHO = 1
+    integer(c_int32_t), parameter :: PBPIC_FLAG_DEPOSIT_RHO = 1, PBPIC_FLAG_TRACK_PID = 2
+    integer(c_int32_t), parameter :: PBPIC_FLAG_PUSH_ORDER = 4
     integer, parameter :: PBPIC_NDIAG = 16
 
     type, bind(c) :: pbpic_params
@@ -125,14 +126,20 @@ contains
     end subroutine
 
     ! Context creation: the allocation part of init (pic2d.f90:197-308).
-    subroutine pbpic_init(nx, ny, dx, dy, dt, cva, mime, in0, precision, sort_every)
+    ! push_order = .false.: sorts give the reference's exact permutation
+    ! (pic2d.f90:1303-1329) instead of the push kernel's tile order
+    subroutine pbpic_init(nx, ny, dx, dy, dt, cva, mime, in0, precision, sort_every, push_order)
         integer, intent(in) :: nx, ny, precision, sort_every
         real(c_double), intent(in) :: dx, dy, dt, cva, mime, in0
+        logical, intent(in), optional :: push_order
         type(pbpic_params) :: p
         call c_params_default(p)
         p%nx = nx; p%ny = ny; p%dx = dx; p%dy = dy; p%dt = dt
         p%cva = cva; p%mime = mime; p%in0 = in0
         p%precision = precision; p%sort_every = sort_every
+        if (present(push_order)) then
+            if (.not. push_order) p%flags = iand(p%flags, not(PBPIC_FLAG_PUSH_ORDER))
+        end if
         call pbpic_check(c_ctx_create(p, ctx), 'pbpic_ctx_create')
     end subroutine
 
