@@ -104,8 +104,9 @@ extern "C" {
 #define PBPIC_GBZ  15
 
 /* ----------------------------------------------------- diagnostics slots */
-#define PBPIC_D_KE_E        0  /* electron kinetic energy of the last push,
-                                  sum (gamma-1)*ms  (totene(1,:), :1061)    */
+#define PBPIC_D_KE_E        0  /* electron kinetic energy of the current
+                                  momenta, sum (gamma-1)*ms (totene(1,:),
+                                  :1061; after a push, that push's tally)   */
 #define PBPIC_D_KE_I        1  /* ion kinetic energy (totene(2,:))          */
 #define PBPIC_D_WE          2  /* electric energy (totene(3,:), :653-673)    */
 #define PBPIC_D_WB          3  /* magnetic energy (totene(4,:))              */
@@ -211,8 +212,10 @@ int pbpic_zero_current(pbpic_ctx *ctx);
  * node-averaged fields, relativistic Boris push, move (at most one cell),
  * Umeda zigzag deposit of Jx, Jy, CIC Jz (and rho if enabled) through the
  * same arithmetic as `currentdeposit` (:1226-1296), periodic wrap in x and y
- * (the MPI exchange of :1085-1216 with the rank as its own neighbour), and
- * the per-species kinetic-energy tally (:1061-1062).  Accumulates into J.   */
+ * (the MPI exchange of :1085-1216 with the rank as its own neighbour).
+ * Accumulates into J.  The kinetic-energy tally the reference keeps in the
+ * pusher (:1061-1062) is computed on demand from the pushed momenta
+ * (pbpic_kineticenergy, pbpic_diag): the same values, no per-step cost.      */
 int pbpic_push(pbpic_ctx *ctx);
 
 /* `currentdeposit` (pic2d.f90:1226-1296), batched over n host-array entries.
@@ -232,6 +235,11 @@ int pbpic_fieldsolve(pbpic_ctx *ctx);
 /* `fieldsenergy` (pic2d.f90:653-673): out[0] = electric, out[1] = magnetic
  * energy of the node-averaged fields, FP64 reduction.                        */
 int pbpic_fieldsenergy(pbpic_ctx *ctx, double out[2]);
+
+/* totene(1:2) of `pusher` (pic2d.f90:1061-1062): out[0] = electron, out[1] =
+ * ion kinetic energy, sum (gamma-1)*ms over the current momenta (gamma at
+ * n+1/2 after a push), FP64 reduction.                                       */
+int pbpic_kineticenergy(pbpic_ctx *ctx, double out[2]);
 
 /* `countingsort` (pic2d.f90:1303-1329) of every species: stable sort of the
  * particles by cell id.                                                      */

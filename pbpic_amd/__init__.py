@@ -88,6 +88,7 @@ def load_library(path=None):
         "pbpic_deposit": (i32, [vp, i64, pi32, pi32, pi32, pi32, pdbl, pdbl, pdbl, pdbl, pi32, pdbl]),
         "pbpic_fieldsolve": (i32, [vp]),
         "pbpic_fieldsenergy": (i32, [vp, pdbl]),
+        "pbpic_kineticenergy": (i32, [vp, pdbl]),
         "pbpic_countingsort": (i32, [vp]),
         "pbpic_countingsort_species": (i32, [vp, i32, pi32, pi32]),
         "pbpic_step": (i32, [vp, i32]),
@@ -258,6 +259,13 @@ class Context:
         """`fieldsenergy` (pic2d.f90:653-673): (electric, magnetic) energy."""
         out = (ctypes.c_double * 2)()
         self._chk(self._lib.pbpic_fieldsenergy(self._h, out))
+        return out[0], out[1]
+
+    def kineticenergy(self):
+        """totene(1:2) of `pusher` (pic2d.f90:1061-1062): (electron, ion)
+        kinetic energy of the current momenta."""
+        out = (ctypes.c_double * 2)()
+        self._chk(self._lib.pbpic_kineticenergy(self._h, out))
         return out[0], out[1]
 
     def countingsort(self, species=None, want=False):

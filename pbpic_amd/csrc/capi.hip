@@ -239,9 +239,17 @@ hipError_t prof_mark(pbpic_ctx *c, int slot, bool begin)
     return hipEventRecord(c->ev_pool[c->ev_used++], c->st);
 }
 
-int push_impl(pbpic_ctx *c, bool reset_ke)
+// Kinetic energy of every species from the current momenta into acc.
+int kinetic_impl(pbpic_ctx *c)
 {
-    if (reset_ke) HIPCHK(c, launch_acc_op(c->acc, 0, 0.0, c->st));
+    HIPCHK(c, launch_acc_op(c->acc, 0, 0.0, c->st));
+    for (int s = 0; s < (int)c->sp.size(); s++)
+        HIPCHK(c, launch_kinetic(species_dev(c, s, c->sp[s].cur), c->p.precision, c->acc, c->st));
+    return PBPIC_OK;
+}
+
+int push_impl(pbpic_ctx *c)
+{
     const PushParams P = push_params(c);
     // node fields (emfgrid, pic2d.f90:837-845) are field work: timed as such
     HIPCHK(c, prof_mark(c, PBPIC_PROF_FIELD, true));
@@ -272,7 +280,7 @@ int step_impl(pbpic_ctx *c, void *const *jcopy, uint32_t *const *perm_dev, bool 
     HIPCHK(c, prof_mark(c, PBPIC_PROF_ZERO, true));
     HIPCHK(c, launch_zero_current(c->J[0], c->J[1], c->J[2], c->J[3], ncell(c), c->rsz, c->acc, 1, c->st));
     HIPCHK(c, prof_mark(c, PBPIC_PROF_ZERO, false));
-    int rc = push_impl(c, false);
+    int rc = push_impl(c);
     if (rc) return rc;
     if (jcopy)
         for (int k = 0; k < 3; k++)
@@ -743,7 +751,7 @@ int pbpic_zero_current(pbpic_ctx *c)
 int pbpic_push(pbpic_ctx *c)
 {
     if (!c) return PBPIC_ERR_ARG;
-    return push_impl(c, true);
+    return push_impl(c);
 }
 
 int pbpic_deposit(pbpic_ctx *c, int64_t n, const int32_t *i1, const int32_t *i2, const int32_t *j1,
@@ -807,6 +815,19 @@ int pbpic_fieldsenergy(pbpic_ctx *c, double out[2])
     return PBPIC_OK;
 }
 
+int pbpic_kineticenergy(pbpic_ctx *c, double out[2])
+{
+    if (!c || !out) return PBPIC_ERR_ARG;
+    int rc = kinetic_impl(c);
+    if (rc) return rc;
+    double a[ACC_N];
+    HIPCHK(c, hipMemcpyAsync(a, c->acc, sizeof a, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    out[0] = a[ACC_KE_E];
+    out[1] = a[ACC_KE_I];
+    return PBPIC_OK;
+}
+
 int pbpic_countingsort_species(pbpic_ctx *c, int32_t s, int32_t *counts, int32_t *perm)
 {
     int rc = valid_species(c, s);
@@ -848,6 +869,7 @@ int pbpic_diag(pbpic_ctx *c, double out[PBPIC_NDIAG])
     }
     HIPCHK(c, launch_acc_op(c->acc, 2, 0.0, c->st));
     HIPCHK(c, launch_energy(fdtd_params(c), c->st));
+    if ((rc = kinetic_impl(c))) return rc;
     const double cva2 = c->p.cva * c->p.cva;
     HIPCHK(c, launch_gauss(push_params(c), c->p.precision, c->rho_scratch, c->gauss0,
                            c->gauss_first ? 1 : 0, c->p.rho_background, cva2 * c->p.in0, c->st));

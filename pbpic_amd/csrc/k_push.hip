@@ -295,14 +295,12 @@ template <typename R> struct NodeStride { static constexpr int v = 6; };
 template <> struct NodeStride<float> { static constexpr int v = 8; };
 
 // Split-precision form of push_math: the same scheme in FP32 with one
-// reciprocal square root shared by every 1/gamma and contracted multiply-adds;
-// the kinetic energy is returned as u^2/(gamma+1), which equals gamma-1
-// without the FP32 cancellation of a small gamma-1.
+// reciprocal square root shared by every 1/gamma and contracted multiply-adds.
 __device__ __forceinline__ void push_math_split(const PushParams &P, float c0, float c1,
                                                 float &ox, float &oy, float &ux, float &uy,
                                                 float &uz, const float *n00, const float *n10,
                                                 const float *n01, const float *n11, int &dcx,
-                                                int &dcy, float &vz, float &kin, int &err)
+                                                int &dcy, float &vz, int &err)
 {
 #pragma clang fp contract(fast)
     const float wx = 0.5f + ox, wy = 0.5f + oy;
@@ -337,8 +335,21 @@ __device__ __forceinline__ void push_math_split(const PushParams &P, float c0, f
     if (fabsf(ox) > 0.5f || fabsf(oy) > 0.5f) err |= ERRF_CFL;
     if (!isfinite(u2)) err |= ERRF_NONFINITE;
     vz = uz * (float)P.cva * ig;
-    const float gam = __builtin_amdgcn_rcpf(ig);
-    kin = u2 * __builtin_amdgcn_rcpf(gam + 1.0f);
+}
+
+// Kinetic energy (gamma - 1) of one particle from its momentum, in the push
+// precision: FP64 as the reference (:1062), FP32 as u^2/(gamma+1), which
+// equals gamma-1 without the FP32 cancellation of a small gamma-1.
+__device__ __forceinline__ double kin_energy(double ux, double uy, double uz)
+{
+    return sqrt(1.0 + ux * ux + uy * uy + uz * uz) - 1.0;
+}
+__device__ __forceinline__ float kin_energy(float ux, float uy, float uz)
+{
+#pragma clang fp contract(fast)
+    const float u2 = ux * ux + uy * uy + uz * uz;
+    const float gam = __builtin_amdgcn_rcpf(__builtin_amdgcn_rsqf(1.0f + u2));
+    return u2 * __builtin_amdgcn_rcpf(gam + 1.0f);
 }
 
 // Copies one NS-value node record with 16-byte moves (records are 16-byte
@@ -634,7 +645,6 @@ __global__ __launch_bounds__(PUSH_THREADS, 3) void push_kernel(PushParams P)
     constexpr int ROUNDS = PUSH_TILE / (CH * (PUSH_THREADS / 64));   // 8 (FP64) or 4 (FP32)
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ int red[4][5];
-    __shared__ double kred[4];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int t = xcd_remap(blockIdx.x, gridDim.x);
     int s = 0;
@@ -734,7 +744,6 @@ __global__ __launch_bounds__(PUSH_THREADS, 3) void push_kernel(PushParams P)
     const R c0 = (R)S.c0, c1 = (R)S.c1, qx = (R)S.qx, qy = (R)S.qy, qr = (R)S.q;
     R *ox_p = (R *)S.ox, *oy_p = (R *)S.oy, *ux_p = (R *)S.ux, *uy_p = (R *)S.uy,
       *uz_p = (R *)S.uz;
-    double ke = 0;
     R *gjx = (R *)P.jx, *gjy = (R *)P.jy, *gjz = (R *)P.jz, *grho = (R *)P.rho;
     const bool want_rho = P.deposit_rho != 0;
     // split form with an LDS window: deposit contributions come out of
@@ -785,14 +794,13 @@ __global__ __launch_bounds__(PUSH_THREADS, 3) void push_kernel(PushParams P)
         }
         const R dx0 = ox, dy0 = oy;
         int dcx, dcy;
-        R vz, gam, kin;
+        R vz, gam;
         if constexpr (SPLIT) {
             push_math_split(P, c0, c1, ox, oy, ux, uy, uz, n00, n10, n01, n11, dcx, dcy, vz,
-                            kin, err);
+                            err);
         } else {
             push_math<R>(P, c0, c1, ox, oy, ux, uy, uz, n00, n10, n01, n11, dcx, dcy, vz, gam,
                          err);
-            kin = gam - (R)1;                                                      // :1062
         }
         Dep<R> d;
         deposit_terms<R, SPLIT>(cx + 1, cy + 1, dcx, dcy, dx0, ox, dy0, oy, dqx, dqy,
@@ -835,7 +843,6 @@ __global__ __launch_bounds__(PUSH_THREADS, 3) void push_kernel(PushParams P)
                 gadd(grho + at(1, 1), d.r3);
             }
         }
-        ke += (double)(kin * (R)S.ms);
         track(lx, ly, dcx, dcy);
         // periodic wrap (:1032-1052)
         int ncx = cx + dcx, ncy = cy + dcy;
@@ -850,7 +857,6 @@ __global__ __launch_bounds__(PUSH_THREADS, 3) void push_kernel(PushParams P)
         R *op = ox_p + tstart, *pp = oy_p + tstart, *up = ux_p + tstart, *vp = uy_p + tstart,
           *wp = uz_p + tstart;
         const int lo = wave * ROUNDS * CH + lane * V;
-        R kacc = 0;
         CrossQueue Q;
         Q.init(smem + push_queue_offset(P.maxw, P.deposit_rho), wave, lane);
         auto onef = [&](int c, R &ox, R &oy, R &ux, R &uy, R &uz) -> int {
@@ -866,14 +872,13 @@ __global__ __launch_bounds__(PUSH_THREADS, 3) void push_kernel(PushParams P)
             }
             const R dx0 = ox, dy0 = oy;
             int dcx, dcy;
-            R vz, gam, kin;
+            R vz, gam;
             if constexpr (SPLIT) {
                 push_math_split(P, c0, c1, ox, oy, ux, uy, uz, n00, n10, n01, n11, dcx, dcy,
-                                vz, kin, err);
+                                vz, err);
             } else {
                 push_math<R>(P, c0, c1, ox, oy, ux, uy, uz, n00, n10, n01, n11, dcx, dcy, vz,
                              gam, err);
-                kin = gam - (R)1;                                                  // :1062
             }
             const int base = (cy - YW) * W + (cx - XW);
             if constexpr (SPLIT) {
@@ -889,7 +894,6 @@ __global__ __launch_bounds__(PUSH_THREADS, 3) void push_kernel(PushParams P)
                 }
                 deposit_lds(jw, WH, W, base, d, want_rho);
             }
-            kacc += kin;
             track(cx - XW, cy - YW, dcx, dcy);
             return c + dcx * ny + dcy;          // interior tile: no wrap (:1032-1052)
         };
@@ -921,7 +925,6 @@ __global__ __launch_bounds__(PUSH_THREADS, 3) void push_kernel(PushParams P)
             }
         }
         if constexpr (SPLIT) Q.drain(jw, WH, W, dqx, dqy, P.compat, Q.tail - Q.head);
-        ke = (double)kacc * S.ms;
     } else {
     // the tile's cells were read in phase A and are re-read here from L2
     R cur[5][V], nxt[5][V];
@@ -988,18 +991,8 @@ __global__ __launch_bounds__(PUSH_THREADS, 3) void push_kernel(PushParams P)
         }
     }
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        ke += __shfl_xor(ke, o);
-        err |= __shfl_xor(err, o);
-    }
-    if (lane == 0) {
-        kred[wave] = ke;
-        if (err) atomicOr(P.err, err);
-    }
-    __syncthreads();
-    if (tid == 0)
-        unsafeAtomicAdd(P.acc + (S.q > 0 ? ACC_KE_I : ACC_KE_E),
-                        (kred[0] + kred[1]) + (kred[2] + kred[3]));
+    for (int o = 32; o > 0; o >>= 1) err |= __shfl_xor(err, o);
+    if (lane == 0 && err) atomicOr(P.err, err);
 
     // ---- E: record the box of the new cells for the next push; check that
     // every cell read lay inside the box this push assumed
@@ -1091,6 +1084,38 @@ static int push_tiles(const PushParams &p, PushParams &q)
     }
     q.ntiles = t;
     return t;
+}
+
+// Kinetic energy tally (pic2d.f90:1061-1062): sum over a species of
+// (gamma - 1) * ms from the momenta as the last push left them, FP64 sums.
+template <typename R>
+__global__ __launch_bounds__(256) void kinetic_kernel(SpeciesDev S, double *acc)
+{
+    __shared__ double red[4];
+    double k = 0;
+    const R *ux = (const R *)S.ux, *uy = (const R *)S.uy, *uz = (const R *)S.uz;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < S.n;
+         i += (int64_t)gridDim.x * 256)
+        k += (double)kin_energy(ux[i], uy[i], uz[i]);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) k += __shfl_xor(k, o);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = k;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        unsafeAtomicAdd(acc + (S.q > 0 ? ACC_KE_I : ACC_KE_E),
+                        ((red[0] + red[1]) + (red[2] + red[3])) * S.ms);
+}
+
+hipError_t launch_kinetic(const SpeciesDev &S, int precision, double *acc, hipStream_t st)
+{
+    if (S.n == 0) return hipSuccess;
+    int64_t nb = (S.n + 255) / 256;
+    if (nb > 4096) nb = 4096;
+    if (precision == PBPIC_PREC_FP64)
+        hipLaunchKernelGGL(kinetic_kernel<double>, dim3((unsigned)nb), dim3(256), 0, st, S, acc);
+    else
+        hipLaunchKernelGGL(kinetic_kernel<float>, dim3((unsigned)nb), dim3(256), 0, st, S, acc);
+    return hipGetLastError();
 }
 
 size_t nodefield_bytes(int precision, int64_t ncell)

@@ -164,6 +164,8 @@ hipError_t launch_push(const PushParams &p, int precision, hipStream_t st);
 size_t nodefield_bytes(int precision, int64_t ncell);
 // Fills p.nf from the E and B planes; launch_push reads p.nf.
 hipError_t launch_nodefield(const PushParams &p, int precision, hipStream_t st);
+// Adds species S's kinetic energy to acc[ACC_KE_E] or acc[ACC_KE_I].
+hipError_t launch_kinetic(const SpeciesDev &S, int precision, double *acc, hipStream_t st);
 hipError_t launch_deposit_batch(const PushParams &p, int precision, int64_t n,
                                 const int32_t *i1, const int32_t *i2,
                                 const int32_t *j1, const int32_t *j2,

@@ -83,6 +83,11 @@ module pbpic_gpu
             type(c_ptr), value :: ctx
             real(c_double), intent(out) :: out(2)
         end function
+        integer(c_int32_t) function c_kineticenergy(ctx, out) bind(c, name="pbpic_kineticenergy")
+            import :: c_ptr, c_int32_t, c_double
+            type(c_ptr), value :: ctx
+            real(c_double), intent(out) :: out(2)
+        end function
         integer(c_int32_t) function c_countingsort(ctx) bind(c, name="pbpic_countingsort")
             import :: c_ptr, c_int32_t
             type(c_ptr), value :: ctx
@@ -155,12 +160,11 @@ contains
     end subroutine
 
     ! pusher (pic2d.f90:912-1217): gather, Boris push, move, deposit, wrap, KE
+    ! The kinetic-energy tally totene(1:2) (:1061-1062) is taken with the field
+    ! energies in fieldsenergy, from the pushed momenta: the pusher itself never
+    ! waits for the device.
     subroutine pusher
-        real(c_double) :: d(16)
         call pbpic_check(c_push(ctx), 'pusher')
-        call pbpic_check(c_diag(ctx, d), 'pusher')
-        totene(1) = d(1)
-        totene(2) = d(2)
     end subroutine
 
     ! currentdeposit (pic2d.f90:1226-1296) for one particle; uz is the velocity
@@ -178,10 +182,14 @@ contains
         call pbpic_check(c_fieldsolve(ctx), 'fieldsolver')
     end subroutine
 
-    ! fieldsenergy (pic2d.f90:653-673)
+    ! fieldsenergy (pic2d.f90:653-673), with the kinetic energies of the last
+    ! push (:1061-1062): all of totene(1:4)
     subroutine fieldsenergy
-        real(c_double) :: e(2)
+        real(c_double) :: e(2), k(2)
         call pbpic_check(c_fieldsenergy(ctx, e), 'fieldsenergy')
+        call pbpic_check(c_kineticenergy(ctx, k), 'fieldsenergy')
+        totene(1) = k(1)
+        totene(2) = k(2)
         totene(3) = e(1)
         totene(4) = e(2)
     end subroutine
