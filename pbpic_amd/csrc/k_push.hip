@@ -225,7 +225,7 @@ __device__ __forceinline__ void deposit_terms(int i1, int j1, int a, int b,
 
 // Gather + Boris + move for one particle (pic2d.f90:948-1028).  n00..n11 are
 // the 6-component node fields at (ix,iy), (ix+1,iy), (ix,iy+1), (ix+1,iy+1).
-template <typename R>
+template <typename R, bool OLDU>
 __device__ __forceinline__ void push_math(const PushParams &P, R c0, R c1,
                                           R &ox, R &oy, R &ux, R &uy, R &uz,
                                           const R *n00, const R *n10,
@@ -250,7 +250,7 @@ __device__ __forceinline__ void push_math(const PushParams &P, R c0, R c1,
     const R t2 = (R)2 / (ONE + tx * tx + ty * ty + tz * tz);
     const R sx = t2 * tx, sy = t2 * ty, sz = t2 * tz;
     R u0x, u0y, u0z;
-    if (P.compat & PBPIC_COMPAT_BORIS_OLD_U) {                              // :985
+    if (OLDU) {                                                              // :985
         u0x = umx + (uy * tz - uz * ty);
         u0y = umy + (uz * tx - ux * tz);
         u0z = umz + (ux * ty - uy * tx);
@@ -296,6 +296,7 @@ template <> struct NodeStride<float> { static constexpr int v = 8; };
 
 // Split-precision form of push_math: the same scheme in FP32 with one
 // reciprocal square root shared by every 1/gamma and contracted multiply-adds.
+template <bool OLDU>
 __device__ __forceinline__ void push_math_split(const PushParams &P, float c0, float c1,
                                                 float &ox, float &oy, float &ux, float &uy,
                                                 float &uz, const float *n00, const float *n10,
@@ -316,7 +317,7 @@ __device__ __forceinline__ void push_math_split(const PushParams &P, float c0, f
     const float t2 = 2.0f * __builtin_amdgcn_rcpf(1.0f + tx * tx + ty * ty + tz * tz);
     const float sx = t2 * tx, sy = t2 * ty, sz = t2 * tz;
     float qx = umx, qy = umy, qz = umz;
-    if (P.compat & PBPIC_COMPAT_BORIS_OLD_U) { qx = ux; qy = uy; qz = uz; }
+    if (OLDU) { qx = ux; qy = uy; qz = uz; }                                // :985
     const float u0x = umx + (qy * tz - qz * ty);
     const float u0y = umy + (qz * tx - qx * tz);
     const float u0z = umz + (qx * ty - qy * tx);
@@ -637,7 +638,9 @@ __device__ __forceinline__ void deposit_split_fast(double *jw, int WH, int W, in
     Q.push(cross, base, a, b, dx1, dx2, dy1, dy2);
 }
 
-template <typename R, bool SPLIT>
+// OLDU: the reference's Boris rotation with the pre-kick momentum
+// (PBPIC_COMPAT_BORIS_OLD_U); RHO: deposit rho as well (PBPIC_FLAG_DEPOSIT_RHO).
+template <typename R, bool SPLIT, bool OLDU, bool RHO>
 __global__ __launch_bounds__(PUSH_THREADS, 3) void push_kernel(PushParams P)
 {
     constexpr int V = Vw<R>::V;
@@ -720,7 +723,7 @@ __global__ __launch_bounds__(PUSH_THREADS, 3) void push_kernel(PushParams P)
     // domain edge (no periodic unwrap or wrap for any particle or move)
     const bool fast = inlds && aerr == 0 && tstart + PUSH_TILE <= n && x0 + mnx >= 1 &&
                       x0 + mxx <= nx - 2 && y0 + mny >= 1 && y0 + mxy <= ny - 2;
-    const int njc = P.deposit_rho ? 4 : 3;
+    const int njc = RHO ? 4 : 3;
     constexpr int NS = NodeStride<R>::v;
     R *nf = (R *)smem;                       // [W][H][NS] node fields, y fastest
     // current window [njc][H][W], x fastest: FP64 sums (FP64 form) or 64-bit
@@ -745,7 +748,7 @@ __global__ __launch_bounds__(PUSH_THREADS, 3) void push_kernel(PushParams P)
     R *ox_p = (R *)S.ox, *oy_p = (R *)S.oy, *ux_p = (R *)S.ux, *uy_p = (R *)S.uy,
       *uz_p = (R *)S.uz;
     R *gjx = (R *)P.jx, *gjy = (R *)P.jy, *gjz = (R *)P.jz, *grho = (R *)P.rho;
-    const bool want_rho = P.deposit_rho != 0;
+    constexpr bool want_rho = RHO;
     // split form with an LDS window: deposit contributions come out of
     // deposit_terms already multiplied by their plane's fixed-point scale
     // (q*dx/dt, q*dy/dt and q pre-scaled by powers of two: exact)
@@ -796,10 +799,10 @@ __global__ __launch_bounds__(PUSH_THREADS, 3) void push_kernel(PushParams P)
         int dcx, dcy;
         R vz, gam;
         if constexpr (SPLIT) {
-            push_math_split(P, c0, c1, ox, oy, ux, uy, uz, n00, n10, n01, n11, dcx, dcy, vz,
+            push_math_split<OLDU>(P, c0, c1, ox, oy, ux, uy, uz, n00, n10, n01, n11, dcx, dcy, vz,
                             err);
         } else {
-            push_math<R>(P, c0, c1, ox, oy, ux, uy, uz, n00, n10, n01, n11, dcx, dcy, vz, gam,
+            push_math<R, OLDU>(P, c0, c1, ox, oy, ux, uy, uz, n00, n10, n01, n11, dcx, dcy, vz, gam,
                          err);
         }
         Dep<R> d;
@@ -858,7 +861,7 @@ __global__ __launch_bounds__(PUSH_THREADS, 3) void push_kernel(PushParams P)
           *wp = uz_p + tstart;
         const int lo = wave * ROUNDS * CH + lane * V;
         CrossQueue Q;
-        Q.init(smem + push_queue_offset(P.maxw, P.deposit_rho), wave, lane);
+        Q.init(smem + push_queue_offset(P.maxw, RHO ? 1 : 0), wave, lane);
         auto onef = [&](int c, R &ox, R &oy, R &ux, R &uy, R &uz) -> int {
             const int cx = (int)P.dny.div31((uint32_t)c), cy = c - cx * ny;
             const R *b = nf + ((cx - XW) * H + (cy - YW)) * NS;
@@ -874,10 +877,10 @@ __global__ __launch_bounds__(PUSH_THREADS, 3) void push_kernel(PushParams P)
             int dcx, dcy;
             R vz, gam;
             if constexpr (SPLIT) {
-                push_math_split(P, c0, c1, ox, oy, ux, uy, uz, n00, n10, n01, n11, dcx, dcy,
+                push_math_split<OLDU>(P, c0, c1, ox, oy, ux, uy, uz, n00, n10, n01, n11, dcx, dcy,
                                 vz, err);
             } else {
-                push_math<R>(P, c0, c1, ox, oy, ux, uy, uz, n00, n10, n01, n11, dcx, dcy, vz,
+                push_math<R, OLDU>(P, c0, c1, ox, oy, ux, uy, uz, n00, n10, n01, n11, dcx, dcy, vz,
                              gam, err);
             }
             const int base = (cy - YW) * W + (cx - XW);
@@ -1142,21 +1145,33 @@ size_t push_lds_bytes(int precision, int maxw, int deposit_rho)
                                         : windows + (PUSH_THREADS / 64) * XQ_BYTES;
 }
 
+template <typename R, bool SPLIT, bool OLDU, bool RHO>
+static void launch_push_t(const PushParams &q, int nt, size_t lds, hipStream_t st)
+{
+    (void)hipFuncSetAttribute((const void *)push_kernel<R, SPLIT, OLDU, RHO>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL((push_kernel<R, SPLIT, OLDU, RHO>), dim3(nt), dim3(PUSH_THREADS), lds, st,
+                       q);
+}
+
+template <typename R, bool SPLIT>
+static void launch_push_f(const PushParams &q, int nt, size_t lds, hipStream_t st)
+{
+    const bool oldu = (q.compat & PBPIC_COMPAT_BORIS_OLD_U) != 0, rho = q.deposit_rho != 0;
+    if (oldu && rho) launch_push_t<R, SPLIT, true, true>(q, nt, lds, st);
+    else if (oldu) launch_push_t<R, SPLIT, true, false>(q, nt, lds, st);
+    else if (rho) launch_push_t<R, SPLIT, false, true>(q, nt, lds, st);
+    else launch_push_t<R, SPLIT, false, false>(q, nt, lds, st);
+}
+
 hipError_t launch_push(const PushParams &p, int precision, hipStream_t st)
 {
     PushParams q;
     const int nt = push_tiles(p, q);
     if (nt == 0) return hipSuccess;
     const size_t lds = push_lds_bytes(precision, p.maxw, p.deposit_rho);
-    if (precision == PBPIC_PREC_FP64) {
-        (void)hipFuncSetAttribute((const void *)push_kernel<double, false>,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        hipLaunchKernelGGL((push_kernel<double, false>), dim3(nt), dim3(PUSH_THREADS), lds, st, q);
-    } else {
-        (void)hipFuncSetAttribute((const void *)push_kernel<float, true>,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        hipLaunchKernelGGL((push_kernel<float, true>), dim3(nt), dim3(PUSH_THREADS), lds, st, q);
-    }
+    if (precision == PBPIC_PREC_FP64) launch_push_f<double, false>(q, nt, lds, st);
+    else launch_push_f<float, true>(q, nt, lds, st);
     return hipGetLastError();
 }
 
