@@ -206,23 +206,99 @@ hipError_t launch_energy(const FdtdParams &p, hipStream_t st)
 
 // Gauss residual: rho by CIC with the gather's weights (:948-951) from the
 // current positions, then divE - c^2 in0 (rho + rho_bg) at every node.
+// The charge is deposited one push tile (PUSH_TILE particles) per workgroup:
+// over the tile's bounding box (the one the last push recorded in
+// SpeciesDev::tbox, else a pass over the tile's cells unwrapped around its
+// first cell) the charge goes to an LDS node histogram, flushed with one
+// global atomic per node; particles outside the box, or all of a tile whose
+// box exceeds the histogram, add to rho directly.
+constexpr int RB_NODES = 2048;
 template <typename R>
-__global__ void rho_cic_kernel(PushParams P, SpeciesDev S, double *rho)
+__global__ __launch_bounds__(256) void rho_cic_tiles_kernel(PushParams P, SpeciesDev S,
+                                                            double *rho)
 {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= S.n) return;
+    __shared__ double h[RB_NODES];
+    __shared__ int bb[4];
     const int nx = P.nx, ny = P.ny;
-    const int c = S.cell[i];
-    if ((unsigned)c >= (unsigned)(nx * ny)) return;
-    const int cx = c / ny, cy = c - cx * ny;
-    const int cx1 = cx + 1 == nx ? 0 : cx + 1, cy1 = cy + 1 == ny ? 0 : cy + 1;
-    const double wx = 0.5 + (double)((const R *)S.ox)[i];
-    const double wy = 0.5 + (double)((const R *)S.oy)[i];
+    const int64_t t0 = (int64_t)blockIdx.x * PUSH_TILE;
+    int4 b = make_int4(0, -1, 0, -1);
+    if (S.tbox_ok) {
+        b = *reinterpret_cast<const int4 *>(S.tbox + 4 * (size_t)blockIdx.x);
+    } else {
+        const int c0 = S.cell[t0];
+        const int x0 = (unsigned)c0 < (unsigned)(nx * ny) ? c0 / ny : 0;
+        const int y0 = (unsigned)c0 < (unsigned)(nx * ny) ? c0 - x0 * ny : 0;
+        const int nxh = nx >> 1, nyh = ny >> 1;
+        int mnx = 1 << 30, mxx = -(1 << 30), mny = 1 << 30, mxy = -(1 << 30);
+        if (threadIdx.x == 0) { bb[0] = mnx; bb[1] = mxx; bb[2] = mny; bb[3] = mxy; }
+        for (int u = threadIdx.x; u < PUSH_TILE; u += 256) {
+            const int64_t i = t0 + u;
+            if (i >= S.n) break;
+            const int c = S.cell[i];
+            if ((unsigned)c >= (unsigned)(nx * ny)) continue;
+            const int cx = c / ny, cy = c - cx * ny;
+            int rx = cx - x0, ry = cy - y0;
+            if (rx > nxh) rx -= nx; else if (rx < -nxh) rx += nx;
+            if (ry > nyh) ry -= ny; else if (ry < -nyh) ry += ny;
+            mnx = min(mnx, rx); mxx = max(mxx, rx); mny = min(mny, ry); mxy = max(mxy, ry);
+        }
+        __syncthreads();
+        atomicMin(&bb[0], mnx); atomicMax(&bb[1], mxx);
+        atomicMin(&bb[2], mny); atomicMax(&bb[3], mxy);
+        __syncthreads();
+        if (bb[0] <= bb[1] && bb[1] - bb[0] < nx && bb[3] - bb[2] < ny) {
+            int bx = x0 + bb[0], by = y0 + bb[2];
+            if (bx < 0) bx += nx; else if (bx >= nx) bx -= nx;
+            if (by < 0) by += ny; else if (by >= ny) by -= ny;
+            b = make_int4(bx, bb[1] - bb[0], by, bb[3] - bb[2]);
+        }
+    }
+    const int W = b.y + 2, H = b.w + 2;                 // nodes of the box's cells
+    const bool boxed = b.y >= 0 && b.w >= 0 && (int64_t)W * H <= RB_NODES;
+    if (boxed)
+        for (int k = threadIdx.x; k < W * H; k += 256) h[k] = 0.0;
+    __syncthreads();
     const double q = (double)S.q;
-    unsafeAtomicAdd(rho + (size_t)cy * nx + cx, q * (1 - wx) * (1 - wy));
-    unsafeAtomicAdd(rho + (size_t)cy * nx + cx1, q * wx * (1 - wy));
-    unsafeAtomicAdd(rho + (size_t)cy1 * nx + cx, q * (1 - wx) * wy);
-    unsafeAtomicAdd(rho + (size_t)cy1 * nx + cx1, q * wx * wy);
+    for (int u = threadIdx.x; u < PUSH_TILE; u += 256) {
+        const int64_t i = t0 + u;
+        if (i >= S.n) break;
+        const int c = S.cell[i];
+        if ((unsigned)c >= (unsigned)(nx * ny)) continue;
+        const int cx = c / ny, cy = c - cx * ny;
+        const double wx = 0.5 + (double)((const R *)S.ox)[i];
+        const double wy = 0.5 + (double)((const R *)S.oy)[i];
+        const double w00 = q * (1 - wx) * (1 - wy), w10 = q * wx * (1 - wy);
+        const double w01 = q * (1 - wx) * wy, w11 = q * wx * wy;
+        if (boxed) {
+            int rx = cx - b.x, ry = cy - b.z;
+            if (rx < 0) rx += nx;
+            if (ry < 0) ry += ny;
+            if (rx <= b.y && ry <= b.w) {
+                double *n0 = h + ry * W + rx;
+                atomicAdd(n0, w00);
+                atomicAdd(n0 + 1, w10);
+                atomicAdd(n0 + W, w01);
+                atomicAdd(n0 + W + 1, w11);
+                continue;
+            }
+        }
+        const int cx1 = cx + 1 == nx ? 0 : cx + 1, cy1 = cy + 1 == ny ? 0 : cy + 1;
+        unsafeAtomicAdd(rho + (size_t)cy * nx + cx, w00);
+        unsafeAtomicAdd(rho + (size_t)cy * nx + cx1, w10);
+        unsafeAtomicAdd(rho + (size_t)cy1 * nx + cx, w01);
+        unsafeAtomicAdd(rho + (size_t)cy1 * nx + cx1, w11);
+    }
+    __syncthreads();
+    if (boxed)
+        for (int k = threadIdx.x; k < W * H; k += 256) {
+            const double v = h[k];
+            if (v == 0.0) continue;
+            const int ky = k / W, kx = k - ky * W;
+            int gx = b.x + kx, gy = b.z + ky;
+            while (gx >= nx) gx -= nx;
+            while (gy >= ny) gy -= ny;
+            unsafeAtomicAdd(rho + (size_t)gy * nx + gx, v);
+        }
 }
 
 __global__ __launch_bounds__(256) void gauss_kernel(PushParams P, const double *rho,
@@ -287,11 +363,13 @@ hipError_t launch_gauss(const PushParams &p, int precision, double *rho_scratch,
     hipLaunchKernelGGL(zero_f64_kernel, dim3(1024), dim3(256), 0, st, rho_scratch, n);
     for (int s = 0; s < p.nspecies; s++) {
         if (p.sp[s].n == 0) continue;
-        const int nb = (int)((p.sp[s].n + 255) / 256);
+        const int nt = (int)((p.sp[s].n + PUSH_TILE - 1) / PUSH_TILE);
         if (precision == PBPIC_PREC_FP64)
-            hipLaunchKernelGGL(rho_cic_kernel<double>, dim3(nb), dim3(256), 0, st, p, p.sp[s], rho_scratch);
+            hipLaunchKernelGGL(rho_cic_tiles_kernel<double>, dim3(nt), dim3(256), 0, st, p,
+                               p.sp[s], rho_scratch);
         else
-            hipLaunchKernelGGL(rho_cic_kernel<float>, dim3(nb), dim3(256), 0, st, p, p.sp[s], rho_scratch);
+            hipLaunchKernelGGL(rho_cic_tiles_kernel<float>, dim3(nt), dim3(256), 0, st, p,
+                               p.sp[s], rho_scratch);
     }
     int nb = (int)((n + 255) / 256);
     if (nb > 4096) nb = 4096;

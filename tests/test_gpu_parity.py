@@ -270,6 +270,28 @@ def test_one_step_full_tiles(gpu, precision, compat):
         assert abs(d[key] - od[key]) <= tol_d * abs(od[key]), key
 
 
+@pytest.mark.parametrize("precision", [0, 1])
+def test_gauss_residual_tiled_charge(gpu, precision):
+    """The Gauss residual's charge deposit, per push tile: after a sort (no
+    recorded tile boxes: per-particle atomics) and after a push (tile boxes
+    recorded: LDS node histogram per tile), L2 and Linf norms of
+    divE - c^2 in0 rho against the oracle's.  FP64 to 1e-9 (atomic sum order),
+    split precision to 1e-5."""
+    _say(f"gauss tiled charge precision={precision}")
+    case = make_case(nx=32, ny=64, ppc=64, seed=41, efield=0.05, bfield=0.5)
+    ctx = gpu_for(case, precision=precision, compat=0)
+    o = oracle_for(case, compat=0)
+    ctx.countingsort(); o.countingsort()
+    tol = 1e-9 if precision == 0 else 1e-5
+    for stage in ("sorted", "pushed"):
+        if stage == "pushed":
+            ctx.zero_current(); ctx.pusher(); ctx.fieldsolver()
+            o.zero_current(); o.pusher(); o.fieldsolver()
+        d, od = ctx.diag(), o.diag()
+        for key in ("gauss_l2", "gauss_linf"):
+            assert abs(d[key] - od[key]) <= tol * od[key], (stage, key, d[key], od[key])
+
+
 def test_multi_step_fp64_diagnostics(gpu):
     """30 steps with a sort every 10: diagnostics track the oracle (trajectories
     stay within rounding for a few dozen steps of a weakly driven plasma)."""
