@@ -25,6 +25,9 @@
  *     `grid = my*(ix-1) + iy` with 1-based ix, iy (pic2d.f90:422, 938-939):
  *     y runs fastest.  The same 1-based `grid` is used at this boundary.
  *   - In-cell offsets dx, dy lie in (-1/2, 1/2] (pic2d.f90:420-421, 1008-1023).
+ *   - With params.ybound = PBPIC_YBOUND_CONDUCTING the last two rows of the
+ *     nx x ny lattice are guard rows (see PBPIC_YBOUND_CONDUCTING); the
+ *     domain is rows 1..ny-2 and particles must lie there.
  *   - Field planes at this boundary are nx*ny arrays, x fastest, i.e. the
  *     owned region emf(1:mx,1:my) of the reference's Fortran arrays
  *     (pic2d.f90:593, 221).  Staggering: Ex,Bx,Jx at (i-1/2, j-1);
@@ -85,6 +88,23 @@ extern "C" {
                                         Clear it for the reference's exact
                                         order.  Set by pbpic_params_default */
 
+/* ------------------------------------------------------- y boundary kinds
+ * The reference is periodic in y (the MPI ring of pic2d.f90:295-298 closes on
+ * itself); its perfectly conducting alternative is left commented out at
+ * :850-859.  PBPIC_YBOUND_CONDUCTING adds that boundary as new physics:
+ * perfectly conducting walls at y = 0 and y = (ny-2) dy with specular
+ * particle reflection.  The lattice keeps its periodic storage and the last
+ * two rows become guard rows (0-based rows ny-2 and ny-1 = row -1), which
+ * hold the mirror images of the fields across the nearer wall: Ey and Bz
+ * even, Ez and By odd across half rows; Ex is zero on the walls (rows 0 and
+ * ny-2) and Ex, Bx of row ny-1 mirror row 1 (odd, even).  Particles reaching
+ * a wall are reflected (offset and u_y change sign) before their deposit;
+ * Jz deposited in a guard row is folded back with its image sign.  Field
+ * energies weight the two wall rows by 1/2; the Gauss residual covers the
+ * rows 1..ny-3 between the walls.  Not available with the adjoint.          */
+#define PBPIC_YBOUND_PERIODIC    0
+#define PBPIC_YBOUND_CONDUCTING  1
+
 /* ------------------------------------------------------------ field names */
 #define PBPIC_EX   0
 #define PBPIC_EY   1
@@ -143,7 +163,8 @@ typedef struct pbpic_params {
     int32_t sort_every;      /* countingsort cadence in pbpic_step (:180);
                                 0 = never                                    */
     int32_t flags;           /* PBPIC_FLAG_* bits                            */
-    int32_t reserved[8];     /* must be zero                                 */
+    int32_t ybound;          /* PBPIC_YBOUND_*; 0 = periodic as the reference */
+    int32_t reserved[7];     /* must be zero                                 */
 } pbpic_params;
 
 /* Fill *p with the reference's own constants (pic2d.f90:12-42): 256x256,
@@ -197,9 +218,56 @@ int pbpic_init_maxwellian(pbpic_ctx *ctx, int32_t species, int32_t ppc,
                           double vth, double vdrift_x, int32_t counter_stream,
                           uint64_t seed, uint32_t stream);
 
+/* Seeded device-side Harris current-sheet loader (initparticles and
+ * initfields, pic2d.f90:357-648): fills an ion and an electron species with
+ * ppc*nx*(domain rows) particles each and sets E = 0 and the sheet's B.
+ * Sheet particles have density n0 sech^2(s/L) across the sheet (s: distance
+ * from its centre, drawn by the inverse CDF s = L/2 ln(u/(1-u)), :395-402) and
+ * the rest is a uniform background; ion i and electron i share their cell and
+ * their offset across the sheet (:418-428); thermal momenta as in
+ * pbpic_init_maxwellian, drifts along z.
+ *   geometry PBPIC_HARRIS_DOUBLE_X: the reference's two sheets across x at
+ *     x = Lx/4 (ions +vdi_z, electrons +vde_z) and x = 3Lx/4 (signs
+ *     reversed), By = B0 (tanh((x-Lx/4)/L) - tanh((x-3Lx/4)/L) - 1)
+ *     (:606-614); periodic y required.
+ *   geometry PBPIC_HARRIS_SINGLE_Y: one sheet across y at y = Ly/2,
+ *     Bx = B0 tanh((y-Ly/2)/L), plus the flux perturbation
+ *     psi = psi0 cos(2 pi x/Lx) cos(pi (y-Ly/2)/Ly) applied as the discrete
+ *     curl of psi at cell centres (so div B = 0 to rounding); requires
+ *     PBPIC_YBOUND_CONDUCTING.  Its current is -B0/L sech^2 along z, so an
+ *     equilibrium has vdi_z < 0 < vde_z.
+ * The sheet fraction of the particles is 2 L nsheets / (2 L nsheets + nb L_n)
+ * (L_n: domain length across the sheets), rounded up to an even count as at
+ * :375-376; lengths in the units of dx, dy.  Philox4x32-10 keyed by seed,
+ * counters (particle, stream, block).                                        */
+#define PBPIC_HARRIS_DOUBLE_X  0
+#define PBPIC_HARRIS_SINGLE_Y  1
+typedef struct pbpic_harris {
+    int32_t geometry;        /* PBPIC_HARRIS_*                               */
+    int32_t ppc;             /* mean particles per cell and species (ppg :30) */
+    double  half_width;      /* L (dcs, :24)                                  */
+    double  nb;              /* background / peak density (nbgcs, :29)        */
+    double  b0;              /* field amplitude (1 in the reference)          */
+    double  psi0;            /* perturbation amplitude (SINGLE_Y only)        */
+    double  vthe_cs, vthi_cs;/* sheet thermal speeds / c (:377-379)           */
+    double  vthe_bg, vthi_bg;/* background thermal speeds / c (:378-380)      */
+    double  vde_z, vdi_z;    /* sheet drift speeds / c along z, first sheet
+                                (the reference: -vdrift, +vdrift, :440-447)  */
+    int32_t thermal_z;       /* 0: sheet u_z is the drift alone and background
+                                u_z = 0, as the reference (:441-447, :523-529);
+                                1: Maxwellian u_z about the drift            */
+    int32_t reserved0;
+    uint64_t seed;
+    uint32_t stream;
+    int32_t reserved[7];     /* must be zero                                 */
+} pbpic_harris;
+int pbpic_init_harris(pbpic_ctx *ctx, int32_t ion_species,
+                      int32_t electron_species, const pbpic_harris *h);
+
 /* Host field I/O, nx*ny planes, x fastest, FP64 host values.  Setting J or
  * RHO replaces the accumulated current (normalized values, as after the
- * in0 scaling at pic2d.f90:1066-1069).                                       */
+ * in0 scaling at pic2d.f90:1066-1069).  With conducting y walls, setting a
+ * field plane also sets its guard rows to the wall images.                   */
 int pbpic_set_field(pbpic_ctx *ctx, int32_t which, const double *plane);
 int pbpic_get_field(pbpic_ctx *ctx, int32_t which, double *plane);
 

@@ -59,6 +59,7 @@ def lib():
                 "step": (None, [vp, i32, i32]),
                 "diag": (None, [vp, dbl, pdbl]),
                 "set_threads": (None, [vp, i32]),
+                "set_ybound": (i32, [vp, i32]),
             }
             for name, (res, args) in sig.items():
                 f = getattr(L, f"{pre}_{name}")
@@ -67,6 +68,11 @@ def lib():
         L.orc_init_maxwellian.restype = None
         L.orc_init_maxwellian.argtypes = [i32, i32, i32, dbl, dbl, i32, ctypes.c_uint64,
                                           ctypes.c_uint32, pi32, pdbl, pdbl, pdbl, pdbl, pdbl]
+        L.orc_harris_count.restype = i64
+        L.orc_harris_count.argtypes = [i32, i32, i32, dbl, dbl, i32, i32, pdbl, P(ctypes.c_int64)]
+        L.orc_init_harris.restype = None
+        L.orc_init_harris.argtypes = [i32, i32, i32, dbl, dbl, i32, i32, pdbl, i32, ctypes.c_uint64,
+                                      ctypes.c_uint32, pi32, pdbl, pi32, pdbl, pdbl]
         L.orc_philox4x32_10.restype = None
         L.orc_philox4x32_10.argtypes = [P(ctypes.c_uint32), P(ctypes.c_uint32), P(ctypes.c_uint32)]
         _lib = L
@@ -84,13 +90,16 @@ class Oracle:
     charge q (`type particle`, pic2d.f90:47-50)."""
 
     def __init__(self, nx, ny, dx, dy, dt, cva, mime, in0, prec=8, compat=COMPAT_REFERENCE,
-                 threads=1):
+                 threads=1, ybound=0):
         self.L = lib()
         self.pre = "orc8" if prec == 8 else "orc4"
         self.nx, self.ny = nx, ny
         self.h = self._f("create")(nx, ny, dx, dy, dt, cva, mime, in0, compat)
         if threads > 1:
             self._f("set_threads")(self.h, threads)
+        if ybound:
+            if self._f("set_ybound")(self.h, int(ybound)):
+                raise ValueError("conducting y walls need ny >= 4")
 
     def _f(self, name):
         return getattr(self.L, f"{self.pre}_{name}")
@@ -196,6 +205,42 @@ def init_maxwellian(nx, ny, ppc, vth, vdrift_x=0.0, counter_stream=False, seed=2
     lib().orc_init_maxwellian(nx, ny, ppc, vth, vdrift_x, int(bool(counter_stream)), seed, stream,
                               _p(g, ctypes.c_int32), *[_p(x, ctypes.c_double) for x in a])
     return dict(grid=g, dx=a[0], dy=a[1], ux=a[2], uy=a[3], uz=a[4])
+
+
+def wall_rows(grid, nyp, ny):
+    """Re-index 1-based cell ids of an nx x nyp lattice onto the rows 0..nyp-1
+    of an nx x ny lattice (the domain of conducting y walls, ny = nyp + 2)."""
+    g = np.asarray(grid, np.int64) - 1
+    return ((g // nyp) * ny + g % nyp + 1).astype(np.int32)
+
+
+HARRIS_KEYS = ("half_width", "nb", "b0", "psi0", "vthe_cs", "vthi_cs", "vthe_bg", "vthi_bg",
+               "vde_z", "vdi_z")
+
+
+def init_harris(nx, ny, ywall, dx, dy, h):
+    """Seeded Harris-sheet loader (pic2d.f90:357-648): returns (ions,
+    electrons, fields) with particle dicts (grid 1-based) and the six field
+    planes (ny, nx) keyed ex..bz.  h: dict with geometry, ppc, HARRIS_KEYS,
+    thermal_z, seed, stream."""
+    L = lib()
+    prm = np.array([float(h[k]) for k in HARRIS_KEYS], np.float64)
+    ncs = ctypes.c_int64()
+    n = L.orc_harris_count(nx, ny, int(ywall), dx, dy, int(h["geometry"]), int(h["ppc"]),
+                           _p(prm, ctypes.c_double), ctypes.byref(ncs))
+    gi, ge = np.empty(n, np.int32), np.empty(n, np.int32)
+    pi, pe = np.empty(5 * n, np.float64), np.empty(5 * n, np.float64)
+    f = np.empty(6 * nx * ny, np.float64)
+    L.orc_init_harris(nx, ny, int(ywall), dx, dy, int(h["geometry"]), int(h["ppc"]),
+                      _p(prm, ctypes.c_double), int(h.get("thermal_z", 0)), int(h.get("seed", 1)),
+                      int(h.get("stream", 0)), _p(gi, ctypes.c_int32), _p(pi, ctypes.c_double),
+                      _p(ge, ctypes.c_int32), _p(pe, ctypes.c_double), _p(f, ctypes.c_double))
+    def unpack(g, a):
+        return dict(grid=g, dx=a[:n], dy=a[n:2 * n], ux=a[2 * n:3 * n], uy=a[3 * n:4 * n],
+                    uz=a[4 * n:])
+    fields = {k: f[i * nx * ny:(i + 1) * nx * ny].reshape(ny, nx)
+              for i, k in enumerate(("ex", "ey", "ez", "bx", "by", "bz"))}
+    return unpack(gi, pi), unpack(ge, pe), fields, ncs.value
 
 
 def philox4x32_10(ctr, key):

@@ -55,7 +55,8 @@ void P##_fieldsenergy(orc_ctx *c, double out[2]);                              \
 void P##_countingsort(orc_ctx *c, int32_t *counts, int32_t *perm);             \
 void P##_step(orc_ctx *c, int nsteps, int sort_every);                         \
 void P##_diag(orc_ctx *c, double rho_bg, double out[16]);                      \
-void P##_set_threads(orc_ctx *c, int nthreads);
+void P##_set_threads(orc_ctx *c, int nthreads);                                \
+int  P##_set_ybound(orc_ctx *c, int kind);
 
 ORC_DECL(orc8)
 ORC_DECL(orc4)
@@ -66,6 +67,14 @@ void orc_init_maxwellian(int nx, int ny, int ppc, double vth, double vdrift_x,
                          int counter_stream, uint64_t seed, uint32_t stream,
                          int32_t *grid, double *dx, double *dy, double *ux,
                          double *uy, double *uz);
+/* Seeded Harris current-sheet loader (pic2d.f90:357-648); see pic_oracle.c.
+ * orc_harris_count returns the particles per species and the sheet count.   */
+int64_t orc_harris_count(int nx, int ny, int ywall, double dx, double dy,
+                         int geometry, int ppc, const double *prm, int64_t *ncs);
+void orc_init_harris(int nx, int ny, int ywall, double dx, double dy,
+                     int geometry, int ppc, const double *prm, int thermal_z,
+                     uint64_t seed, uint32_t stream, int32_t *grid_i,
+                     double *pi, int32_t *grid_e, double *pe, double *fields);
 /* Raw Philox4x32-10 block for counter ctr and key (for known-answer tests).  */
 void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2],
                        uint32_t out[4]);

@@ -19,6 +19,7 @@ typedef struct {
     /* module parameters (pic2d.f90:15-43) */
     REAL dx, dy, idx, idy, dt, dth, cva, cva2, mime, in0;
     int compat;
+    int ywall;                 /* conducting walls in y (rows my-1, my: guards) */
     int nthreads;
     /* emf(0:mx+1,0:my+1) components (type fields, :52-57) */
     REAL *ex, *dex_dy, *ey, *dey_dx, *ez, *dez_dx, *dez_dy, *dive;
@@ -91,6 +92,17 @@ void PFX(destroy)(orc_ctx *oc)
     for (size_t k = 0; k < sizeof(fp) / sizeof(fp[0]); k++) free(fp[k]);
     free(c->ptl); free(c->ptlsort); free(c->ptlalloc); free(c->gauss0);
     free(c);
+}
+
+/* y boundary: 0 periodic (the reference), 1 conducting walls (see
+ * wall_image).  Call before loading fields or particles. */
+int PFX(set_ybound)(orc_ctx *oc, int kind)
+{
+    CT *c = (CT *)oc;
+    if (kind != 0 && kind != 1) return -1;
+    if (kind == 1 && c->my < 4) return -1;
+    c->ywall = kind;
+    return 0;
 }
 
 void PFX(set_threads)(orc_ctx *oc, int nthreads)
@@ -202,6 +214,61 @@ static void PFX(ghost_d)(CT *c, REAL *a)
     }
 }
 
+/* Conducting walls in y.  pic2d.f90 is periodic in y; its perfectly
+ * conducting boundary is only sketched, commented out, at :850-859.  Here the
+ * domain is rows j = 1..my-2, the walls lie on node rows j = 1 (y = 0) and
+ * j = my-1, and the rows j = my-1 (cell row above the top wall) and j = my
+ * (= row 0, below the bottom wall) are guard rows holding the mirror images
+ * of the fields: for the components on half rows (Ey, By, Ez, Bz) row my-1
+ * mirrors row my-2 and row my mirrors row 1, even for the normal E and
+ * tangential B components (Ey, Bz), odd for Ez and By; of the components on
+ * node rows, Ex vanishes on both walls and Ex, Bx of row my (y = -dy) mirror
+ * row 2 (y = +dy), odd and even.  The periodic ghost rows follow. */
+#ifndef ORC_WALL_KINDS
+#define ORC_WALL_KINDS
+enum { WI_HALF_EVEN, WI_HALF_ODD, WI_NODE_E, WI_NODE_B };
+#endif
+static void PFX(wall_image)(CT *c, REAL *a, int kind)
+{
+    const int mx = c->mx, my = c->my;
+    for (int i = 0; i <= mx + 1; i++) {
+        switch (kind) {
+        case WI_HALF_EVEN:
+            F(a, i, my - 1) = F(a, i, my - 2);
+            F(a, i, my) = F(a, i, 1);
+            break;
+        case WI_HALF_ODD:
+            F(a, i, my - 1) = -F(a, i, my - 2);
+            F(a, i, my) = -F(a, i, 1);
+            break;
+        case WI_NODE_E:
+            F(a, i, 1) = 0;
+            F(a, i, my - 1) = 0;
+            F(a, i, my) = -F(a, i, 2);
+            break;
+        default:
+            F(a, i, my) = F(a, i, 2);
+            break;
+        }
+        F(a, i, 0) = F(a, i, my);
+        F(a, i, my + 1) = F(a, i, 1);
+    }
+}
+
+static void PFX(wall_images_e)(CT *c)
+{
+    PFX(wall_image)(c, c->ex, WI_NODE_E);
+    PFX(wall_image)(c, c->ey, WI_HALF_EVEN);
+    PFX(wall_image)(c, c->ez, WI_HALF_ODD);
+}
+
+static void PFX(wall_images_b)(CT *c)
+{
+    PFX(wall_image)(c, c->bx, WI_NODE_B);
+    PFX(wall_image)(c, c->by, WI_HALF_ODD);
+    PFX(wall_image)(c, c->bz, WI_HALF_EVEN);
+}
+
 /* Curl terms and emfgrid from E and B, as the restart path rebuilds them
  * (pic2d.f90:1906-1926). */
 static void PFX(rebuild)(CT *c)
@@ -262,7 +329,13 @@ void PFX(set_field)(orc_ctx *oc, int which, const double *plane)
             if (is_d) D(a, i, j) = v; else F(a, i, j) = v;
         }
     if (is_d) PFX(ghost_d)(c, a);
-    else { PFX(ghost_f)(c, a); PFX(rebuild)(c); }
+    else {
+        static const int kinds[6] = {WI_NODE_E, WI_HALF_EVEN, WI_HALF_ODD,
+                                     WI_NODE_B, WI_HALF_ODD, WI_HALF_EVEN};
+        PFX(ghost_f)(c, a);
+        if (c->ywall) PFX(wall_image)(c, a, kinds[which]);
+        PFX(rebuild)(c);
+    }
 }
 
 void PFX(get_field)(orc_ctx *oc, int which, double *plane)
@@ -423,6 +496,18 @@ void PFX(finish_current)(orc_ctx *oc)
         D(c->rho, i, my + 1) = D(c->rho, i, my + 1) + sb[4 * mx1 + i];
     }
     free(st); free(sb);
+    if (c->ywall) {
+        /* Jz deposited beyond a wall (guard rows my-1 and my) belongs to the
+         * mirror image of its particle: fold it back with the image's sign
+         * (the image of charge q at -y carries -q v_z). */
+        for (int i = 0; i <= mx + 2; i++) {
+            D(c->jz, i, 1) = D(c->jz, i, 1) - D(c->jz, i, my);
+            D(c->jz, i, my - 2) = D(c->jz, i, my - 2) - D(c->jz, i, my - 1);
+            D(c->jz, i, my - 1) = 0;
+            D(c->jz, i, my) = 0;
+            D(c->jz, i, 0) = 0;
+        }
+    }
 }
 
 /* One particle of pusher's loop body (pic2d.f90:934-1062). */
@@ -490,6 +575,14 @@ static int PFX(push_one)(const CT *c, PFX(ptl_t) *p, REAL *jx, REAL *jy,
     } else {
         dcelly = 0;
         p->dy = dy1;
+    }
+    if (c->ywall && ((iy == 1 && dcelly < 0) || (iy == my - 2 && dcelly > 0))) {
+        /* specular reflection at a conducting wall: the position mirrored
+         * into the cell it left from, u_y reversed, then the deposit */
+        dcelly = 0;
+        p->dy = -p->dy;
+        uy = -uy;
+        p->uy = uy;
     }
     int ix1 = ix + dcellx, iy1 = iy + dcelly;                   /* :1026 */
     REAL vz = uz * c->cva / gama;                               /* :1028 */
@@ -599,6 +692,7 @@ void PFX(fieldsolver)(orc_ctx *oc)
         F(c->bz, 0, j) = F(c->bz, mx, j); F(c->bz, 1, j) = F(c->bz, mx + 1, j);
     }
     PFX(halo2)(c, c->by, c->bz);                                /* :700-730 */
+    if (c->ywall) PFX(wall_images_b)(c);
     for (int j = 1; j <= my; j++)                               /* :733 */
         for (int i = 0; i <= mx + 1; i++)
             F(c->dbx_dy, i, j) = (F(c->bx, i, j + 1) - F(c->bx, i, j)) * idy;
@@ -622,6 +716,7 @@ void PFX(fieldsolver)(orc_ctx *oc)
             F(c->ey, i, j) = eyo - dtc2 * (F(c->dbz_dx, i, j) + D(c->jy, i, j));
             F(c->ez, i, j) = ezo + dtc2 * (F(c->dby_dx, i, j) -
                                            F(c->dbx_dy, i, j) - D(c->jz, i, j));
+            if (c->ywall && (j == 1 || j == my - 1)) F(c->ex, i, j) = 0;   /* wall */
             if (i >= 1 && i <= mx && j >= 1 && j <= my)
                 je += (double)D(c->jx, i, j) * 0.5 * ((double)exo + (double)F(c->ex, i, j)) +
                       (double)D(c->jy, i, j) * 0.5 * ((double)eyo + (double)F(c->ey, i, j)) +
@@ -635,6 +730,7 @@ void PFX(fieldsolver)(orc_ctx *oc)
         F(c->ez, 0, j) = F(c->ez, mx, j); F(c->ez, 1, j) = F(c->ez, mx + 1, j);
     }
     PFX(halo2)(c, c->ey, c->ez);                                /* :757-787 */
+    if (c->ywall) PFX(wall_images_e)(c);
     for (int j = 1; j <= my; j++)                               /* :790 */
         for (int i = 0; i <= mx + 1; i++)
             F(c->dex_dy, i, j) = (F(c->ex, i, j + 1) - F(c->ex, i, j)) * idy;
@@ -661,6 +757,7 @@ void PFX(fieldsolver)(orc_ctx *oc)
         c->bz[k] = c->bz[k] + dth * (c->dex_dy[k] - c->dey_dx[k]);
     }
     PFX(halo2)(c, c->by, c->bz);                                /* :805-835 */
+    if (c->ywall) PFX(wall_images_b)(c);
     PFX(emfgrid)(c);                                            /* :837-845 */
     double e2[2];
     PFX(fieldsenergy)(oc, e2);                                  /* :848 */
@@ -674,23 +771,31 @@ void PFX(fieldsenergy)(orc_ctx *oc, double out[2])
     CT *c = (CT *)oc;
     REAL t3 = 0, t4 = 0;
     double d3 = 0, d4 = 0;
-    for (int j = 1; j <= c->my; j++)
+    /* with walls: the node rows from wall to wall, the walls weighted 1/2 */
+    const int jhi = c->ywall ? c->my - 1 : c->my;
+    for (int j = 1; j <= jhi; j++) {
+        const int wall = c->ywall && (j == 1 || j == jhi);
         for (int i = 1; i <= c->mx; i++) {
             REAL ene = F(c->gex, i, j) * F(c->gex, i, j) +
                        F(c->gey, i, j) * F(c->gey, i, j) +
                        F(c->gez, i, j) * F(c->gez, i, j);
+            double e64 = (double)F(c->gex, i, j) * F(c->gex, i, j) +
+                         (double)F(c->gey, i, j) * F(c->gey, i, j) +
+                         (double)F(c->gez, i, j) * F(c->gez, i, j);
+            if (wall) { ene = ene * (REAL)0.5; e64 *= 0.5; }
             t3 = t3 + ene;
-            d3 += (double)F(c->gex, i, j) * F(c->gex, i, j) +
-                  (double)F(c->gey, i, j) * F(c->gey, i, j) +
-                  (double)F(c->gez, i, j) * F(c->gez, i, j);
+            d3 += e64;
             ene = F(c->gbx, i, j) * F(c->gbx, i, j) +
                   F(c->gby, i, j) * F(c->gby, i, j) +
                   F(c->gbz, i, j) * F(c->gbz, i, j);
-            t4 = t4 + ene;
-            d4 += (double)F(c->gbx, i, j) * F(c->gbx, i, j) +
+            e64 = (double)F(c->gbx, i, j) * F(c->gbx, i, j) +
                   (double)F(c->gby, i, j) * F(c->gby, i, j) +
                   (double)F(c->gbz, i, j) * F(c->gbz, i, j);
+            if (wall) { ene = ene * (REAL)0.5; e64 *= 0.5; }
+            t4 = t4 + ene;
+            d4 += e64;
         }
+    }
     c->totene[3] = (REAL)0.5 * t3 * c->mime / (c->cva2 * c->cva2 * c->in0);
     c->totene[4] = (REAL)0.5 * t4 * c->mime / (c->cva2 * c->in0);
     out[0] = 0.5 * d3 * (double)c->mime /
@@ -769,7 +874,9 @@ void PFX(diag)(orc_ctx *oc, double rho_bg, double out[16])
     double l2 = 0, li = 0, dl2 = 0, dli = 0;
     int first = c->gauss0 == NULL;
     if (first) c->gauss0 = (double *)calloc(nn, sizeof(double));
-    for (int j = 1; j <= my; j++)
+    /* with walls: the node rows strictly between the walls */
+    const int jlo = c->ywall ? 2 : 1, jhi = c->ywall ? my - 2 : my;
+    for (int j = jlo; j <= jhi; j++)
         for (int i = 1; i <= mx; i++) {
             double div = ((double)F(c->ex, i, j) - (double)F(c->ex, i - 1, j)) * (double)c->idx +
                          ((double)F(c->ey, i, j) - (double)F(c->ey, i, j - 1)) * (double)c->idy;

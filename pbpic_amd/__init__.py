@@ -24,6 +24,8 @@ PREC_FP64, PREC_SPLIT32 = 0, 1
 COMPAT_BORIS_OLD_U, COMPAT_RHO_MIRROR, COMPAT_JY_DROP = 1, 2, 4
 COMPAT_REFERENCE = 7
 FLAG_DEPOSIT_RHO, FLAG_TRACK_PID, FLAG_PUSH_ORDER = 1, 2, 4
+YBOUND_PERIODIC, YBOUND_CONDUCTING = 0, 1
+HARRIS_DOUBLE_X, HARRIS_SINGLE_Y = 0, 1
 FIELDS = {"ex": 0, "ey": 1, "ez": 2, "bx": 3, "by": 4, "bz": 5, "jx": 6,
           "jy": 7, "jz": 8, "rho": 9, "gex": 10, "gey": 11, "gez": 12,
           "gbx": 13, "gby": 14, "gbz": 15}
@@ -50,7 +52,21 @@ class Params(ctypes.Structure):
                 ("mime", ctypes.c_double), ("in0", ctypes.c_double),
                 ("rho_background", ctypes.c_double),
                 ("sort_every", ctypes.c_int32), ("flags", ctypes.c_int32),
-                ("reserved", ctypes.c_int32 * 8)]
+                ("ybound", ctypes.c_int32), ("reserved", ctypes.c_int32 * 7)]
+
+
+class Harris(ctypes.Structure):
+    """pbpic_harris (include/pbpic.h): the Harris-sheet loader's parameters,
+    the run-time form of the constants initparticles uses (pic2d.f90:22-30)."""
+    _fields_ = [("geometry", ctypes.c_int32), ("ppc", ctypes.c_int32),
+                ("half_width", ctypes.c_double), ("nb", ctypes.c_double),
+                ("b0", ctypes.c_double), ("psi0", ctypes.c_double),
+                ("vthe_cs", ctypes.c_double), ("vthi_cs", ctypes.c_double),
+                ("vthe_bg", ctypes.c_double), ("vthi_bg", ctypes.c_double),
+                ("vde_z", ctypes.c_double), ("vdi_z", ctypes.c_double),
+                ("thermal_z", ctypes.c_int32), ("reserved0", ctypes.c_int32),
+                ("seed", ctypes.c_uint64), ("stream", ctypes.c_uint32),
+                ("reserved", ctypes.c_int32 * 7)]
 
 
 _lib = None
@@ -81,6 +97,7 @@ def load_library(path=None):
         "pbpic_set_particles_f32": (i32, [vp, i32, i64, pi32, pflt, pflt, pflt, pflt, pflt, pi32]),
         "pbpic_get_particles_f32": (i32, [vp, i32, i64, pi64, pi32, pflt, pflt, pflt, pflt, pflt, pi32]),
         "pbpic_init_maxwellian": (i32, [vp, i32, i32, dbl, dbl, i32, ctypes.c_uint64, ctypes.c_uint32]),
+        "pbpic_init_harris": (i32, [vp, i32, i32, P(Harris)]),
         "pbpic_set_field": (i32, [vp, i32, pdbl]),
         "pbpic_get_field": (i32, [vp, i32, pdbl]),
         "pbpic_zero_current": (i32, [vp]),
@@ -130,7 +147,7 @@ class Context:
 
     def __init__(self, nx, ny, dx, dy, dt, cva, mime, in0, *, precision=PREC_FP64,
                  compat=COMPAT_REFERENCE, sort_every=50, deposit_rho=True,
-                 track_pid=False, push_order=True, rho_background=0.0):
+                 track_pid=False, push_order=True, rho_background=0.0, ybound=YBOUND_PERIODIC):
         self._lib = load_library()
         p = Params()
         self._lib.pbpic_params_default(ctypes.byref(p))
@@ -140,6 +157,7 @@ class Context:
         p.flags = ((FLAG_DEPOSIT_RHO if deposit_rho else 0) | (FLAG_TRACK_PID if track_pid else 0) |
                    (FLAG_PUSH_ORDER if push_order else 0))
         p.rho_background = rho_background
+        p.ybound = {"periodic": YBOUND_PERIODIC, "conducting": YBOUND_CONDUCTING}.get(ybound, ybound)
         self.params = p
         self.nx, self.ny = nx, ny
         self.precision = precision
@@ -216,6 +234,20 @@ class Context:
         self._chk(self._lib.pbpic_init_maxwellian(self._h, species, int(ppc), float(vth),
                                                   float(vdrift_x), int(bool(counter_stream)),
                                                   int(seed), int(stream)))
+
+    def init_harris(self, ion, electron, **kw):
+        """Seeded Harris current-sheet loader (initparticles / initfields,
+        pic2d.f90:357-648; pbpic_init_harris).  Keywords are the fields of
+        :class:`Harris`; ``geometry`` may be "double_x" or "single_y"."""
+        h = Harris()
+        geo = kw.pop("geometry", HARRIS_DOUBLE_X)
+        h.geometry = {"double_x": HARRIS_DOUBLE_X, "single_y": HARRIS_SINGLE_Y}.get(geo, geo)
+        h.b0 = 1.0
+        for k, v in kw.items():
+            if k not in dict(Harris._fields_) or k.startswith("reserved"):
+                raise TypeError(f"unknown Harris parameter {k}")
+            setattr(h, k, v)
+        self._chk(self._lib.pbpic_init_harris(self._h, ion, electron, ctypes.byref(h)))
 
     # ------------------------------------------------------------ fields
     def set_field(self, name, plane):

@@ -88,6 +88,9 @@ int fail(pbpic_ctx *c, int code, const char *fmt, ...)
     } while (0)
 
 int64_t ncell(const pbpic_ctx *c) { return (int64_t)c->p.nx * c->p.ny; }
+// Rows of the domain: ny, or ny - 2 with conducting y walls (two guard rows).
+int nyp_of(const pbpic_ctx *c) { return c->p.ybound == PBPIC_YBOUND_CONDUCTING ? c->p.ny - 2 : c->p.ny; }
+bool walls(const pbpic_ctx *c) { return c->p.ybound == PBPIC_YBOUND_CONDUCTING; }
 
 template <typename T>
 hipError_t dalloc(T **p, size_t bytes)
@@ -134,6 +137,7 @@ PushParams push_params(pbpic_ctx *c)
     PushParams P{};
     P.nx = c->p.nx;
     P.ny = c->p.ny;
+    P.nyp = walls(c) ? nyp_of(c) : 0;
     P.dny.init((uint32_t)c->p.ny);
     P.compat = c->p.compat;
     P.deposit_rho = (c->p.flags & PBPIC_FLAG_DEPOSIT_RHO) ? 1 : 0;
@@ -158,6 +162,7 @@ FdtdParams fdtd_params(pbpic_ctx *c)
 {
     FdtdParams F{};
     F.nx = c->p.nx; F.ny = c->p.ny;
+    F.nyp = walls(c) ? nyp_of(c) : 0;
     F.dth = c->dth; F.dtc2 = c->dtc2; F.idx = c->idx; F.idy = c->idy; F.in0 = c->p.in0;
     F.ex = c->E[0]; F.ey = c->E[1]; F.ez = c->E[2];
     F.bx = c->B[0]; F.by = c->B[1]; F.bz = c->B[2];
@@ -257,6 +262,7 @@ int push_impl(pbpic_ctx *c)
     HIPCHK(c, prof_mark(c, PBPIC_PROF_FIELD, false));
     HIPCHK(c, prof_mark(c, PBPIC_PROF_PUSH, true));
     HIPCHK(c, launch_push(P, c->p.precision, c->st));
+    if (walls(c)) HIPCHK(c, launch_wall_fold(c->J[2], c->p.nx, c->p.ny, P.nyp, c->rsz, c->st));
     for (auto &S : c->sp) S.tbox_ok = true;
     HIPCHK(c, prof_mark(c, PBPIC_PROF_PUSH, false));
     return PBPIC_OK;
@@ -267,6 +273,7 @@ int fieldsolve_impl(pbpic_ctx *c)
     HIPCHK(c, launch_acc_op(c->acc, 1, c->p.dt * c->p.mime / (c->p.cva * c->p.cva), c->st));
     HIPCHK(c, prof_mark(c, PBPIC_PROF_FIELD, true));
     HIPCHK(c, launch_fdtd(fdtd_params(c), c->p.precision, c->st));
+    if (walls(c)) HIPCHK(c, launch_wall_images(fdtd_params(c), 0x3f, c->st));
     HIPCHK(c, prof_mark(c, PBPIC_PROF_FIELD, false));
     c->steps++;
     return PBPIC_OK;
@@ -384,8 +391,12 @@ int set_particles_t(pbpic_ctx *c, int32_t s, int64_t n, const int32_t *grid, con
     if (n > 0 && (!grid || !dx || !dy || !ux || !uy || !uz)) return fail(c, PBPIC_ERR_ARG, "null particle array");
     const int64_t nc = ncell(c);
     std::vector<int32_t> cell((size_t)n);
+    const int ny = c->p.ny, nyp = nyp_of(c);
     for (int64_t i = 0; i < n; i++) {
         if (grid[i] < 1 || grid[i] > nc) return fail(c, PBPIC_ERR_CELL, "grid[%lld]=%d out of 1..%lld", (long long)i, grid[i], (long long)nc);
+        if ((grid[i] - 1) % ny >= nyp)
+            return fail(c, PBPIC_ERR_CELL, "grid[%lld]=%d lies in a guard row of the conducting walls",
+                        (long long)i, grid[i]);
         cell[(size_t)i] = grid[i] - 1;
     }
     const int b = S.cur;
@@ -486,8 +497,10 @@ int pbpic_ctx_create(const pbpic_params *pp, pbpic_ctx **out)
         return PBPIC_ERR_ARG;
     if (p.precision != PBPIC_PREC_FP64 && p.precision != PBPIC_PREC_SPLIT32) return PBPIC_ERR_ARG;
     if (p.compat & ~PBPIC_COMPAT_REFERENCE) return PBPIC_ERR_ARG;
-    for (int k = 0; k < 8; k++)
+    for (int k = 0; k < 7; k++)
         if (p.reserved[k]) return PBPIC_ERR_ARG;
+    if (p.ybound != PBPIC_YBOUND_PERIODIC && p.ybound != PBPIC_YBOUND_CONDUCTING) return PBPIC_ERR_ARG;
+    if (p.ybound == PBPIC_YBOUND_CONDUCTING && p.ny < 4) return PBPIC_ERR_ARG;
     pbpic_ctx *c = new pbpic_ctx();
     c->p = p;
     c->rsz = p.precision == PBPIC_PREC_FP64 ? 8 : 4;
@@ -661,13 +674,114 @@ int pbpic_init_maxwellian(pbpic_ctx *c, int32_t s, int32_t ppc, double vth, doub
     if (ppc < 1 || !(vth >= 0 && vth < 1) || !(fabs(vdrift_x) < 1))
         return fail(c, PBPIC_ERR_ARG, "bad loader arguments");
     Species &S = c->sp[s];
-    const int64_t n = ncell(c) * (int64_t)ppc;
+    const int64_t n = (int64_t)c->p.nx * nyp_of(c) * (int64_t)ppc;
     if (n > S.cap) return fail(c, PBPIC_ERR_CAPACITY, "ppc*cells=%lld exceeds capacity %lld", (long long)n, (long long)S.cap);
     S.n = n;
     S.tbox_ok = false;
     SpeciesDev d = species_dev(c, s, S.cur);
     HIPCHK(c, launch_init_maxwellian(d, c->p.precision, c->p.nx, c->p.ny, ppc, vth, vdrift_x,
-                                     counter_stream, seed, stream, c->st));
+                                     counter_stream, seed, stream, nyp_of(c), c->st));
+    return PBPIC_OK;
+}
+
+int pbpic_init_harris(pbpic_ctx *c, int32_t si, int32_t se, const pbpic_harris *h)
+{
+    int rc = valid_species(c, si);
+    if (rc) return rc;
+    if ((rc = valid_species(c, se))) return rc;
+    if (!h || si == se) return fail(c, PBPIC_ERR_ARG, "bad Harris arguments");
+    if (c->sp[si].q != 1 || c->sp[se].q != -1)
+        return fail(c, PBPIC_ERR_ARG, "the ion species must have q = +1, the electron species q = -1");
+    for (int k = 0; k < 7; k++)
+        if (h->reserved[k] || h->reserved0) return fail(c, PBPIC_ERR_ARG, "reserved fields must be zero");
+    if (h->geometry == PBPIC_HARRIS_DOUBLE_X) {
+        if (walls(c)) return fail(c, PBPIC_ERR_ARG, "the double sheet across x needs periodic y");
+    } else if (h->geometry == PBPIC_HARRIS_SINGLE_Y) {
+        if (!walls(c)) return fail(c, PBPIC_ERR_ARG, "the sheet across y needs conducting y walls");
+    } else {
+        return fail(c, PBPIC_ERR_ARG, "unknown Harris geometry %d", h->geometry);
+    }
+    const double sp[6] = {h->vthe_cs, h->vthi_cs, h->vthe_bg, h->vthi_bg, h->vde_z, h->vdi_z};
+    for (double v : sp)
+        if (!(fabs(v) < 1)) return fail(c, PBPIC_ERR_ARG, "speeds must lie below c");
+    if (h->ppc < 1 || !(h->half_width > 0) || !(h->nb >= 0) || !std::isfinite(h->b0) ||
+        !std::isfinite(h->psi0))
+        return fail(c, PBPIC_ERR_ARG, "bad Harris parameters");
+    const int nx = c->p.nx, ny = c->p.ny, nyp = nyp_of(c);
+    const bool dbl = h->geometry == PBPIC_HARRIS_DOUBLE_X;
+    const int64_t n = (int64_t)h->ppc * nx * nyp;
+    for (int32_t s : {si, se})
+        if (n > c->sp[s].cap)
+            return fail(c, PBPIC_ERR_CAPACITY, "ppc*cells=%lld exceeds capacity %lld", (long long)n,
+                        (long long)c->sp[s].cap);
+    // sheet particles: 2 L nsheets / (2 L nsheets + nb L_n) of the total,
+    // rounded up and made even (pic2d.f90:375-376)
+    const int nsh = dbl ? 2 : 1;
+    const double ln = dbl ? nx * c->p.dx : nyp * c->p.dy;
+    const double f = 2.0 * h->half_width * nsh / (2.0 * h->half_width * nsh + h->nb * ln);
+    int64_t ncs = (int64_t)ceil((double)n * f);
+    ncs = (ncs / 2) * 2;
+    if (ncs > n) ncs = n - (n & 1);
+    auto uc = [](double v) { return v / sqrt(1.0 - v * v); };
+    HarrisDev d{};
+    d.geometry = h->geometry;
+    d.ppc = h->ppc;
+    d.thermal_z = h->thermal_z ? 1 : 0;
+    d.nn = dbl ? nx : nyp;
+    d.nt = dbl ? nyp : nx;
+    d.ncs = ncs;
+    d.L = h->half_width;
+    d.dn = dbl ? c->p.dx : c->p.dy;
+    d.xc0 = dbl ? 0.25 * (nx * c->p.dx) : 0.5 * (nyp * c->p.dy);
+    d.xc1 = dbl ? 0.75 * (nx * c->p.dx) : d.xc0;
+    d.uec = uc(h->vthe_cs); d.uic = uc(h->vthi_cs);
+    d.ueb = uc(h->vthe_bg); d.uib = uc(h->vthi_bg);
+    d.ude = uc(h->vde_z); d.udi = uc(h->vdi_z);
+    d.k0 = (uint32_t)h->seed; d.k1 = (uint32_t)(h->seed >> 32);
+    d.stream = h->stream;
+    Species &I = c->sp[si], &E = c->sp[se];
+    I.n = E.n = n;
+    I.tbox_ok = E.tbox_ok = false;
+    HIPCHK(c, launch_init_harris(species_dev(c, si, I.cur), species_dev(c, se, E.cur),
+                                 c->p.precision, ny, d, c->st));
+    // fields (initfields, :583-648): E = 0 and the sheets' B
+    const size_t nc = (size_t)ncell(c);
+    std::vector<double> bx(nc, 0.0), by(nc, 0.0), zero(nc, 0.0);
+    const double L = h->half_width, b0 = h->b0, psi0 = h->psi0, dx = c->p.dx, dy = c->p.dy;
+    const double twopi = 6.283185307179586;
+    if (dbl) {
+        const double lx = nx * dx, xl = 0.25 * lx, xr = 0.75 * lx;
+        for (int gx = 0; gx < nx; gx++) {
+            const double x = gx * dx;                                       // :608
+            const double v = b0 * (tanh((x - xl) / L) - tanh((x - xr) / L) - 1.0);
+            for (int gy = 0; gy < ny; gy++) by[(size_t)gy * nx + gx] = v;
+        }
+    } else {
+        // Bx on node rows (walls included), By on half rows, both from the
+        // discrete curl of psi taken at cell centres
+        const double lx = nx * dx, ly = nyp * dy, yc = 0.5 * ly;
+        auto psi = [&](double x, double y) {
+            return psi0 * cos(twopi * x / lx) * cos(0.5 * twopi * (y - yc) / ly);
+        };
+        for (int gy = 0; gy <= nyp; gy++)
+            for (int gx = 0; gx < nx; gx++) {
+                const double x = (gx + 0.5) * dx, y = gy * dy;
+                bx[(size_t)gy * nx + gx] =
+                    b0 * tanh((y - yc) / L) - (psi(x, y + 0.5 * dy) - psi(x, y - 0.5 * dy)) / dy;
+            }
+        for (int gy = 0; gy < nyp; gy++)
+            for (int gx = 0; gx < nx; gx++) {
+                const double x = gx * dx, y = (gy + 0.5) * dy;
+                by[(size_t)gy * nx + gx] = (psi(x + 0.5 * dx, y) - psi(x - 0.5 * dx, y)) / dx;
+            }
+    }
+    for (int k = 0; k < 3; k++)
+        HIPCHK(c, hipMemcpyAsync(c->E[k], zero.data(), nc * 8, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(c->B[0], bx.data(), nc * 8, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(c->B[1], by.data(), nc * 8, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(c->B[2], zero.data(), nc * 8, hipMemcpyHostToDevice, c->st));
+    if (walls(c)) HIPCHK(c, launch_wall_images(fdtd_params(c), 0x3f, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
     return PBPIC_OK;
 }
 
@@ -677,8 +791,10 @@ int pbpic_set_field(pbpic_ctx *c, int32_t which, const double *plane)
     const size_t nc = (size_t)ncell(c);
     if (which >= PBPIC_EX && which <= PBPIC_EZ) {
         HIPCHK(c, hipMemcpy(c->E[which - PBPIC_EX], plane, nc * 8, hipMemcpyHostToDevice));
+        if (walls(c)) HIPCHK(c, launch_wall_images(fdtd_params(c), 1 << which, c->st));
     } else if (which >= PBPIC_BX && which <= PBPIC_BZ) {
         HIPCHK(c, hipMemcpy(c->B[which - PBPIC_BX], plane, nc * 8, hipMemcpyHostToDevice));
+        if (walls(c)) HIPCHK(c, launch_wall_images(fdtd_params(c), 1 << which, c->st));
     } else if (which >= PBPIC_JX && which <= PBPIC_RHO) {
         // the device holds the unnormalized sum; the boundary value is x in0
         std::vector<double> vd(nc);
@@ -931,7 +1047,7 @@ int pbpic_set_particles_device(pbpic_ctx *c, int32_t s, int64_t n, const int32_t
     void *dst[5] = {S.ox[b], S.oy[b], S.ux[b], S.uy[b], S.uz[b]};
     for (int k = 0; k < 5; k++)
         HIPCHK(c, hipMemcpyAsync(dst[k], src[k], (size_t)c->rsz * n, hipMemcpyDeviceToDevice, c->st));
-    HIPCHK(c, launch_check_cells(S.cell[b], n, ncell(c), c->err, c->st));
+    HIPCHK(c, launch_check_cells(S.cell[b], n, ncell(c), c->p.ny, nyp_of(c), c->err, c->st));
     S.n = n;
     return PBPIC_OK;
 }
@@ -945,12 +1061,14 @@ int pbpic_set_field_device(pbpic_ctx *c, int32_t which, const double *plane)
     else if (which >= PBPIC_BX && which <= PBPIC_BZ) dst = c->B[which - PBPIC_BX];
     else return fail(c, PBPIC_ERR_ARG, "field %d cannot be set from the device", which);
     HIPCHK(c, hipMemcpyAsync(dst, plane, nc * 8, hipMemcpyDeviceToDevice, c->st));
+    if (walls(c)) HIPCHK(c, launch_wall_images(fdtd_params(c), 1 << which, c->st));
     return PBPIC_OK;
 }
 
 int pbpic_adjoint_forward(pbpic_ctx *c, int32_t nsteps, int32_t ckpt, double *loss)
 {
     if (!c || nsteps < 1 || ckpt < 1) return PBPIC_ERR_ARG;
+    if (walls(c)) return fail(c, PBPIC_ERR_STATE, "the adjoint does not support conducting y walls");
     int rc = check_flags(c);
     if (rc) return rc;
     adj_free(c);

@@ -10,6 +10,15 @@
 // curl terms the reference stores between steps are recomputed from E and B,
 // which yields the same values (they are formed from the same fields by the
 // same expressions, :733-736, :790-793).
+//
+// Conducting y walls (P.nyp > 0; the reference leaves this boundary commented
+// out at :850-859): the last two rows are guards holding mirror images of the
+// fields.  Inside the kernel only two of them matter to the domain rows: Ex
+// is zero on the wall rows 0 and nyp (set as E^{n+1} is formed, before J.E)
+// and B^{n+1} on the wall rows reads Ez^{n+1} in the guard rows, imaged in
+// LDS (odd) before that stage.  B^{n+1/2} in the guard rows feeds only Ex on
+// the walls, which is overwritten.  wall_images_kernel then rewrites every
+// guard row of every plane after the launch.
 #include "pbpic_impl.h"
 
 #pragma clang fp contract(off)
@@ -107,6 +116,7 @@ __global__ __launch_bounds__(FT_THREADS) void fdtd_kernel(FdtdParams P)
             eyn = eyo + (double)(-(dtc2f * ((bzx * idxf) + (float)jyr * in0f)));
             ezn = ezo + (double)(dtc2f * (((byx * idxf) - (bxy * idyf)) - (float)jzr * in0f));
         }
+        if (P.nyp && (gy == 0 || gy == P.nyp)) exn = 0.0;   // tangential E on a wall
         se[0][c] = exn; se[1][c] = eyn; se[2][c] = ezn;
         const bool own = u >= FT_H && u < FT_H + FT_X && v >= FT_H && v < FT_H + FT_Y &&
                          x0 + u < nx && y0 + v < ny;
@@ -114,6 +124,28 @@ __global__ __launch_bounds__(FT_THREADS) void fdtd_kernel(FdtdParams P)
             je += jxr * 0.5 * (exo + exn) + jyr * 0.5 * (eyo + eyn) + jzr * 0.5 * (ezo + ezn);
     }
     __syncthreads();
+
+    if (P.nyp) {
+        // Ez^{n+1} in the guard rows of the E stage's range [2, FS_Y-2): the
+        // odd image of the adjacent domain row (row nyp mirrors nyp-1, row
+        // nyp+1 mirrors row 0).  A guard at the edge of the range whose
+        // mirror lies outside it feeds only guard rows of B^{n+1}.
+        bool any = false;
+        for (int v = 2; v < FS_Y - 2; v++) {
+            const int gy = fwrap(y0 + v, ny);
+            any |= gy >= P.nyp;
+        }
+        if (any) {
+            for (int k = tid; k < (FS_X - 4) * (FS_Y - 4); k += FT_THREADS) {
+                const int v = k / (FS_X - 4) + 2, u = k - (v - 2) * (FS_X - 4) + 2;
+                const int gy = fwrap(y0 + v, ny);
+                const int c = v * FS_X + u;
+                if (gy == P.nyp && v - 1 >= 2) se[2][c] = -se[2][c - FS_X];
+                else if (gy == P.nyp + 1 && v + 1 < FS_Y - 2) se[2][c] = -se[2][c + FS_X];
+            }
+            __syncthreads();
+        }
+    }
 
     // B^{n+1} and write-out on the owned tile
     for (int k = tid; k < FT_X * FT_Y; k += FT_THREADS) {
@@ -159,7 +191,8 @@ hipError_t launch_fdtd(const FdtdParams &p, int precision, hipStream_t st)
 }
 
 // fieldsenergy (pic2d.f90:653-673): sums of squared node-averaged E and B
-// (emfgrid, :837-845) over all nodes, FP64.
+// (emfgrid, :837-845) over all nodes, FP64.  With conducting y walls: over
+// the node rows 0..nyp, the two wall rows weighted 1/2.
 __global__ __launch_bounds__(256) void energy_kernel(FdtdParams P)
 {
     __shared__ double red[2][4];
@@ -176,8 +209,13 @@ __global__ __launch_bounds__(256) void energy_kernel(FdtdParams P)
         const double ez = (P.ez[c] + P.ez[xm] + P.ez[ym] + P.ez[xym]) * 0.25;
         const double bx = (P.bx[c] + P.bx[xm]) * 0.5, by = (P.by[c] + P.by[ym]) * 0.5;
         const double bz = (P.bz[c] + P.bz[xm] + P.bz[ym] + P.bz[xym]) * 0.25;
-        se += ex * ex + ey * ey + ez * ez;
-        sbv += bx * bx + by * by + bz * bz;
+        double e2 = ex * ex + ey * ey + ez * ez, b2 = bx * bx + by * by + bz * bz;
+        if (P.nyp) {
+            if (gy > P.nyp) continue;
+            if (gy == 0 || gy == P.nyp) { e2 *= 0.5; b2 *= 0.5; }
+        }
+        se += e2;
+        sbv += b2;
     }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
@@ -313,6 +351,8 @@ __global__ __launch_bounds__(256) void gauss_kernel(PushParams P, const double *
     for (int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x; k < n;
          k += (int64_t)gridDim.x * 256) {
         const int gy = (int)(k / nx), gx = (int)(k - (int64_t)gy * nx);
+        // conducting y walls: the node rows strictly between the walls
+        if (P.nyp && (gy == 0 || gy >= P.nyp)) continue;
         const int gxm = gx == 0 ? nx - 1 : gx - 1, gym = gy == 0 ? ny - 1 : gy - 1;
         const double div = (P.ex[k] - P.ex[(size_t)gy * nx + gxm]) * P.idx +
                            (P.ey[k] - P.ey[(size_t)gym * nx + gx]) * P.idy;
@@ -375,6 +415,66 @@ hipError_t launch_gauss(const PushParams &p, int precision, double *rho_scratch,
     if (nb > 4096) nb = 4096;
     hipLaunchKernelGGL(gauss_kernel, dim3(nb), dim3(256), 0, st, p, rho_scratch, gauss0,
                        first, rho_bg, cva2in0);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------ conducting y walls
+template <typename R>
+__global__ void wall_fold_kernel(R *jz, int nx, int nyp)
+{
+    const int gx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gx >= nx) return;
+    R *top = jz + (size_t)nyp * nx, *bot = jz + (size_t)(nyp + 1) * nx;
+    // the image of charge q at -y carries -q v_z: Jz beyond a wall folds back
+    // with sign -1 into the domain row it mirrors
+    jz[gx] -= bot[gx];
+    jz[(size_t)(nyp - 1) * nx + gx] -= top[gx];
+    top[gx] = (R)0;
+    bot[gx] = (R)0;
+}
+
+hipError_t launch_wall_fold(void *jz, int nx, int ny, int nyp, int elem_bytes, hipStream_t st)
+{
+    (void)ny;
+    const int nb = (nx + 255) / 256;
+    if (elem_bytes == 8)
+        hipLaunchKernelGGL(wall_fold_kernel<double>, dim3(nb), dim3(256), 0, st, (double *)jz, nx, nyp);
+    else
+        hipLaunchKernelGGL(wall_fold_kernel<float>, dim3(nb), dim3(256), 0, st, (float *)jz, nx, nyp);
+    return hipGetLastError();
+}
+
+// Guard rows of the field planes: half-row components (Ey, Bz even; Ez, By
+// odd) of row nyp mirror row nyp-1 and of row nyp+1 (= row -1) mirror row 0;
+// Ex is zero on the wall rows 0 and nyp; Ex, Bx of row nyp+1 (y = -dy) mirror
+// row 1 (odd, even).
+__global__ void wall_images_kernel(FdtdParams P, int mask)
+{
+    const int gx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gx >= P.nx) return;
+    const int nx = P.nx, nyp = P.nyp;
+    const size_t r0 = gx, r1 = (size_t)nx + gx, rt = (size_t)(nyp - 1) * nx + gx,
+                 gt = (size_t)nyp * nx + gx, gb = (size_t)(nyp + 1) * nx + gx;
+    double *half[4] = {P.ey, P.ez, P.by, P.bz};
+    const int hbit[4] = {1, 2, 4, 5};
+    const double hsgn[4] = {1.0, -1.0, -1.0, 1.0};
+    for (int k = 0; k < 4; k++) {
+        if (!(mask >> hbit[k] & 1)) continue;
+        double *a = half[k];
+        a[gt] = hsgn[k] * a[rt];
+        a[gb] = hsgn[k] * a[r0];
+    }
+    if (mask & 1) {
+        P.ex[r0] = 0.0;
+        P.ex[gt] = 0.0;
+        P.ex[gb] = -P.ex[r1];
+    }
+    if (mask >> 3 & 1) P.bx[gb] = P.bx[r1];
+}
+
+hipError_t launch_wall_images(const FdtdParams &p, int mask, hipStream_t st)
+{
+    hipLaunchKernelGGL(wall_images_kernel, dim3((p.nx + 255) / 256), dim3(256), 0, st, p, mask);
     return hipGetLastError();
 }
 
@@ -466,7 +566,7 @@ __device__ __forceinline__ double u01(uint32_t u) { return ((double)u + 0.5) * 2
 template <typename R>
 __global__ void init_maxwellian_kernel(SpeciesDev S, int ppc, double uth, double ud,
                                        int counter_stream, uint32_t k0, uint32_t k1,
-                                       uint32_t stream)
+                                       uint32_t stream, int nyp, int ny)
 {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= S.n) return;
@@ -477,7 +577,9 @@ __global__ void init_maxwellian_kernel(SpeciesDev S, int ppc, double uth, double
     const double r1 = sqrt(-2.0 * log(u01(a[2]))), t1 = twopi * u01(a[3]);
     const double r2 = sqrt(-2.0 * log(u01(b[0]))), t2 = twopi * u01(b[1]);
     const double drift = (counter_stream && ((p % ppc) & 1)) ? -ud : ud;
-    S.cell[p] = (int32_t)(p / ppc);
+    // cells of the domain rows [0, nyp) in order (nyp < ny: conducting walls)
+    const int64_t c = p / ppc;
+    S.cell[p] = (int32_t)(nyp == ny ? c : (c / nyp) * ny + c % nyp);
     ((R *)S.ox)[p] = (R)(u01(a[0]) - 0.5);
     ((R *)S.oy)[p] = (R)(u01(a[1]) - 0.5);
     ((R *)S.ux)[p] = (R)(uth * (r1 * cos(t1)) + drift);
@@ -488,9 +590,9 @@ __global__ void init_maxwellian_kernel(SpeciesDev S, int ppc, double uth, double
 
 hipError_t launch_init_maxwellian(const SpeciesDev &s, int precision, int nx, int ny, int ppc,
                                   double vth, double vdrift, int counter_stream, uint64_t seed,
-                                  uint32_t stream, hipStream_t st)
+                                  uint32_t stream, int nyp, hipStream_t st)
 {
-    (void)nx; (void)ny;
+    (void)nx;
     if (s.n <= 0) return hipSuccess;
     const double uth = vth / sqrt(1.0 - vth * vth);
     const double ud = vdrift / sqrt(1.0 - vdrift * vdrift);
@@ -498,26 +600,121 @@ hipError_t launch_init_maxwellian(const SpeciesDev &s, int precision, int nx, in
     if (precision == PBPIC_PREC_FP64)
         hipLaunchKernelGGL(init_maxwellian_kernel<double>, dim3((unsigned)nb), dim3(256), 0, st,
                            s, ppc, uth, ud, counter_stream, (uint32_t)seed,
-                           (uint32_t)(seed >> 32), stream);
+                           (uint32_t)(seed >> 32), stream, nyp, ny);
     else
         hipLaunchKernelGGL(init_maxwellian_kernel<float>, dim3((unsigned)nb), dim3(256), 0, st,
                            s, ppc, uth, ud, counter_stream, (uint32_t)seed,
-                           (uint32_t)(seed >> 32), stream);
+                           (uint32_t)(seed >> 32), stream, nyp, ny);
     return hipGetLastError();
 }
 
-__global__ void check_cells_kernel(const int32_t *cell, int64_t n, int64_t ncell, int *err)
+// Harris current-sheet loader (initparticles, pic2d.f90:357-578): ion p and
+// electron p of species `ion` / `ele` share their cell and their offset across
+// the sheet.  Words: counter (p, stream, block) under key (k0, k1); block 0 =
+// (ion position along the sheet, electron offset along it, ion Box-Muller pair
+// for u_x, u_y), block 1 = (electron pair, pair for the thermal u_z of ion and
+// electron), blocks 2..9 = four tries each of the position across the sheet,
+// the inverse CDF of sech^2 (:395-402; a background particle takes the first
+// word as a uniform position).
+__device__ __forceinline__ void harris_cell(double X, int n, int &c, double &off)
 {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n && (cell[i] < 0 || cell[i] >= ncell)) atomicOr(err, ERRF_CELL);
+    c = (int)ceil(X) - 1;                           // ix = ceiling(x/dx) (:403)
+    c = c < 0 ? 0 : (c > n - 1 ? n - 1 : c);
+    off = X - ((double)c + 0.5);                    // :420
 }
 
-hipError_t launch_check_cells(const int32_t *cell, int64_t n, int64_t ncell, int *err,
-                              hipStream_t st)
+template <typename R>
+__global__ void init_harris_kernel(SpeciesDev I, SpeciesDev E, int ny, HarrisDev h)
+{
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= I.n) return;
+    const uint32_t plo = (uint32_t)p, phi = (uint32_t)((uint64_t)p >> 32);
+    uint32_t w0[4], w1[4];
+    philox(plo, phi, h.stream, 0u, h.k0, h.k1, w0);
+    philox(plo, phi, h.stream, 1u, h.k0, h.k1, w1);
+    const bool sheet = p < h.ncs;
+    const bool second = sheet && h.geometry == PBPIC_HARRIS_DOUBLE_X && p >= h.ncs / 2;
+    double Xn;
+    if (sheet) {
+        const double xc = second ? h.xc1 : h.xc0;
+        Xn = -1.0;
+        for (int b = 0; b < 8 && Xn < 0; b++) {
+            uint32_t w[4];
+            philox(plo, phi, h.stream, 2u + b, h.k0, h.k1, w);
+            for (int t = 0; t < 4; t++) {
+                const double u = u01(w[t]);
+                const double s = h.L * log(u / (1.0 - u)) / 2.0;         // :401
+                const double X = (s + xc) / h.dn;
+                if (ceil(X) - 1 >= 0 && ceil(X) - 1 < h.nn) { Xn = X; break; }
+            }
+        }
+        if (Xn < 0) Xn = xc / h.dn;
+    } else {
+        uint32_t w[4];
+        philox(plo, phi, h.stream, 2u, h.k0, h.k1, w);
+        Xn = u01(w[0]) * h.nn;
+    }
+    int cn, ct;
+    double on, ot;
+    harris_cell(Xn, h.nn, cn, on);
+    harris_cell(u01(w0[0]) * h.nt, h.nt, ct, ot);
+    const double ote = u01(w0[1]) - 0.5;                                 // :425
+    const double twopi = 6.283185307179586;
+    const double r1 = sqrt(-2.0 * log(u01(w0[2]))), t1 = twopi * u01(w0[3]);
+    const double r2 = sqrt(-2.0 * log(u01(w1[0]))), t2 = twopi * u01(w1[1]);
+    const double r3 = sqrt(-2.0 * log(u01(w1[2]))), t3 = twopi * u01(w1[3]);
+    const double ui = sheet ? h.uic : h.uib, ue = sheet ? h.uec : h.ueb;
+    double dzi = sheet ? h.udi : 0.0, dze = sheet ? h.ude : 0.0;
+    if (second) { dzi = -dzi; dze = -dze; }                             // :491-497
+    int cx, cy;
+    double oxi, oyi, oxe, oye;
+    if (h.geometry == PBPIC_HARRIS_DOUBLE_X) { cx = cn; cy = ct; oxi = oxe = on; oyi = ot; oye = ote; }
+    else { cx = ct; cy = cn; oyi = oye = on; oxi = ot; oxe = ote; }
+    const int32_t cell = (int32_t)((int64_t)cx * ny + cy);
+    I.cell[p] = cell;
+    E.cell[p] = cell;
+    ((R *)I.ox)[p] = (R)oxi; ((R *)I.oy)[p] = (R)oyi;
+    ((R *)E.ox)[p] = (R)oxe; ((R *)E.oy)[p] = (R)oye;
+    ((R *)I.ux)[p] = (R)(ui * (r1 * cos(t1)));
+    ((R *)I.uy)[p] = (R)(ui * (r1 * sin(t1)));
+    ((R *)E.ux)[p] = (R)(ue * (r2 * cos(t2)));
+    ((R *)E.uy)[p] = (R)(ue * (r2 * sin(t2)));
+    ((R *)I.uz)[p] = (R)(dzi + (h.thermal_z ? ui * (r3 * cos(t3)) : 0.0));
+    ((R *)E.uz)[p] = (R)(dze + (h.thermal_z ? ue * (r3 * sin(t3)) : 0.0));
+    if (I.pid) I.pid[p] = (int32_t)(p + 1);
+    if (E.pid) E.pid[p] = (int32_t)(p + 1);
+}
+
+hipError_t launch_init_harris(const SpeciesDev &ion, const SpeciesDev &ele, int precision,
+                              int ny, const HarrisDev &h, hipStream_t st)
+{
+    if (ion.n <= 0) return hipSuccess;
+    const int64_t nb = (ion.n + 255) / 256;
+    if (precision == PBPIC_PREC_FP64)
+        hipLaunchKernelGGL(init_harris_kernel<double>, dim3((unsigned)nb), dim3(256), 0, st, ion,
+                           ele, ny, h);
+    else
+        hipLaunchKernelGGL(init_harris_kernel<float>, dim3((unsigned)nb), dim3(256), 0, st, ion,
+                           ele, ny, h);
+    return hipGetLastError();
+}
+
+// Cell ids in range, and (conducting y walls, nyp < ny) outside the guard rows.
+__global__ void check_cells_kernel(const int32_t *cell, int64_t n, int64_t ncell, int ny,
+                                   int nyp, int *err)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int c = cell[i];
+    if (c < 0 || c >= ncell || (nyp < ny && c % ny >= nyp)) atomicOr(err, ERRF_CELL);
+}
+
+hipError_t launch_check_cells(const int32_t *cell, int64_t n, int64_t ncell, int ny, int nyp,
+                              int *err, hipStream_t st)
 {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(check_cells_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
-                       cell, n, ncell, err);
+                       cell, n, ncell, ny, nyp, err);
     return hipGetLastError();
 }
 

@@ -289,6 +289,19 @@ __device__ __forceinline__ void push_math(const PushParams &P, R c0, R c1,
     gam = gama;
 }
 
+// Conducting y walls (nyp > 0): a particle that would leave the domain rows
+// [0, nyp) is reflected specularly before its deposit -- the offset mirrored
+// into the cell it left from, u_y reversed.  Periodic runs pass nyp = 0.
+template <typename R>
+__device__ __forceinline__ void wall_reflect(int nyp, int cy, int &dcy, R &oy, R &uy)
+{
+    if (nyp && ((cy == 0 && dcy < 0) || (cy == nyp - 1 && dcy > 0))) {
+        dcy = 0;
+        oy = -oy;
+        uy = -uy;
+    }
+}
+
 // LDS node record: 6 field components, padded to 8 floats in FP32 so that
 // every record starts 16-byte aligned (48-byte FP64 records already are).
 template <typename R> struct NodeStride { static constexpr int v = 6; };
@@ -805,6 +818,7 @@ __global__ __launch_bounds__(PUSH_THREADS, 3) void push_kernel(PushParams P)
             push_math<R, OLDU>(P, c0, c1, ox, oy, ux, uy, uz, n00, n10, n01, n11, dcx, dcy, vz, gam,
                          err);
         }
+        wall_reflect(P.nyp, cy, dcy, oy, uy);
         Dep<R> d;
         deposit_terms<R, SPLIT>(cx + 1, cy + 1, dcx, dcy, dx0, ox, dy0, oy, dqx, dqy,
                                 dqz, vz, P.compat, want_rho, d);
@@ -883,6 +897,7 @@ __global__ __launch_bounds__(PUSH_THREADS, 3) void push_kernel(PushParams P)
                 push_math<R, OLDU>(P, c0, c1, ox, oy, ux, uy, uz, n00, n10, n01, n11, dcx, dcy, vz,
                              gam, err);
             }
+            wall_reflect(P.nyp, cy, dcy, oy, uy);
             const int base = (cy - YW) * W + (cx - XW);
             if constexpr (SPLIT) {
                 deposit_split_fast(jw, WH, W, base, dcx, dcy, dx0, ox, dy0, oy, dqx, dqy, dqz,

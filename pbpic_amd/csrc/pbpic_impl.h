@@ -105,6 +105,7 @@ struct SpeciesDev {
 
 struct PushParams {
     int nx, ny;
+    int nyp;                   // conducting y walls: domain rows [0, nyp); 0 = periodic
     FastDiv dny;
     int compat, deposit_rho;
     int maxw;                  // LDS window capacity in cells
@@ -121,6 +122,7 @@ struct PushParams {
 
 struct FdtdParams {
     int nx, ny;
+    int nyp;                   // conducting y walls: domain rows [0, nyp); 0 = periodic
     double dth, dtc2, idx, idy, in0;   // dt/2, dt*cva^2, 1/dx, 1/dy, in0
     double *ex, *ey, *ez, *bx, *by, *bz;
     const void *jx, *jy, *jz;
@@ -179,6 +181,21 @@ hipError_t launch_gauss(const PushParams &p, int precision, double *rho_scratch,
                         double *gauss0, int first, double rho_bg,
                         double cva2in0, hipStream_t st);
 hipError_t launch_fill_u32(uint32_t *p, int64_t n, uint32_t v, hipStream_t st);
+// Conducting y walls: folds the Jz deposited in the guard rows back into the
+// domain (image sign -1) and clears them; sets the guard rows of the field
+// planes selected by mask (bit k: Ex, Ey, Ez, Bx, By, Bz) to their images.
+hipError_t launch_wall_fold(void *jz, int nx, int ny, int nyp, int elem_bytes, hipStream_t st);
+hipError_t launch_wall_images(const FdtdParams &p, int mask, hipStream_t st);
+// Harris-sheet loader (k_field.hip); see pbpic_init_harris.
+struct HarrisDev {
+    int geometry, ppc, thermal_z, nn, nt;   // nn, nt: cells across / along the sheets
+    int64_t ncs;                            // sheet particles per species
+    double L, dn, xc0, xc1;                 // half width, cell size across, centres
+    double uic, uec, uib, ueb, udi, ude;    // momenta per unit (u = v/sqrt(1-v^2))
+    uint32_t k0, k1, stream;
+};
+hipError_t launch_init_harris(const SpeciesDev &ion, const SpeciesDev &ele, int precision,
+                              int ny, const HarrisDev &h, hipStream_t st);
 hipError_t launch_zero_current(void *jx, void *jy, void *jz, void *rho,
                                int64_t ncell, int elem_bytes, double *acc,
                                int reset_ke, hipStream_t st);
@@ -186,9 +203,9 @@ hipError_t launch_acc_op(double *acc, int op, double a, hipStream_t st);
 hipError_t launch_init_maxwellian(const SpeciesDev &s, int precision, int nx,
                                   int ny, int ppc, double vth, double vdrift,
                                   int counter_stream, uint64_t seed,
-                                  uint32_t stream, hipStream_t st);
-hipError_t launch_check_cells(const int32_t *cell, int64_t n, int64_t ncell,
-                              int *err, hipStream_t st);
+                                  uint32_t stream, int nyp, hipStream_t st);
+hipError_t launch_check_cells(const int32_t *cell, int64_t n, int64_t ncell, int ny,
+                              int nyp, int *err, hipStream_t st);
 
 // Sort (stable LSD radix over the cell id, 8-bit digits).
 struct SortScratch {

@@ -16,13 +16,30 @@ module pbpic_gpu
     integer(c_int32_t), parameter :: PBPIC_COMPAT_REFERENCE = 7
     integer(c_int32_t), parameter :: PBPIC_FLAG_DEPOSIT_RHO = 1, PBPIC_FLAG_TRACK_PID = 2
     integer(c_int32_t), parameter :: PBPIC_FLAG_PUSH_ORDER = 4
+    integer(c_int32_t), parameter :: PBPIC_YBOUND_PERIODIC = 0, PBPIC_YBOUND_CONDUCTING = 1
+    integer(c_int32_t), parameter :: PBPIC_HARRIS_DOUBLE_X = 0, PBPIC_HARRIS_SINGLE_Y = 1
     integer, parameter :: PBPIC_NDIAG = 16
 
     type, bind(c) :: pbpic_params
         integer(c_int32_t) :: nx, ny, precision, compat
         real(c_double) :: dx, dy, dt, cva, mime, in0, rho_background
         integer(c_int32_t) :: sort_every, flags
-        integer(c_int32_t) :: reserved(8)
+        integer(c_int32_t) :: ybound
+        integer(c_int32_t) :: reserved(7)
+    end type
+
+    ! pbpic_harris: parameters of the Harris-sheet loader that replaces
+    ! initparticles/initfields (pic2d.f90:357-648; constants :22-30)
+    type, bind(c) :: pbpic_harris
+        integer(c_int32_t) :: geometry = PBPIC_HARRIS_DOUBLE_X, ppc = 64
+        real(c_double) :: half_width = 0.5d0, nb = 0.2d0, b0 = 1.0d0, psi0 = 0.0d0
+        real(c_double) :: vthe_cs = 0.092d0, vthi_cs = 0.0411d0
+        real(c_double) :: vthe_bg = 0.0291d0, vthi_bg = 0.0058d0
+        real(c_double) :: vde_z = -0.0667d0, vdi_z = 0.0667d0
+        integer(c_int32_t) :: thermal_z = 0, reserved0 = 0
+        integer(c_int64_t) :: seed = 1
+        integer(c_int32_t) :: stream = 0
+        integer(c_int32_t) :: reserved(7) = 0
     end type
 
     interface
@@ -57,6 +74,12 @@ module pbpic_gpu
             integer(c_int32_t), value :: sp, ppc, cs, stream
             real(c_double), value :: vth, vd
             integer(c_int64_t), value :: seed
+        end function
+        integer(c_int32_t) function c_init_harris(ctx, si, se, h) bind(c, name="pbpic_init_harris")
+            import :: c_ptr, c_int32_t, pbpic_harris
+            type(c_ptr), value :: ctx
+            integer(c_int32_t), value :: si, se
+            type(pbpic_harris), intent(in) :: h
         end function
         integer(c_int32_t) function c_zero_current(ctx) bind(c, name="pbpic_zero_current")
             import :: c_ptr, c_int32_t
@@ -133,10 +156,14 @@ contains
     ! Context creation: the allocation part of init (pic2d.f90:197-308).
     ! push_order = .false.: sorts give the reference's exact permutation
     ! (pic2d.f90:1303-1329) instead of the push kernel's tile order
-    subroutine pbpic_init(nx, ny, dx, dy, dt, cva, mime, in0, precision, sort_every, push_order)
+    ! ybound = PBPIC_YBOUND_CONDUCTING: conducting walls in y; ny then counts
+    ! two guard rows above the domain (see include/pbpic.h)
+    subroutine pbpic_init(nx, ny, dx, dy, dt, cva, mime, in0, precision, sort_every, push_order, &
+                          ybound)
         integer, intent(in) :: nx, ny, precision, sort_every
         real(c_double), intent(in) :: dx, dy, dt, cva, mime, in0
         logical, intent(in), optional :: push_order
+        integer, intent(in), optional :: ybound
         type(pbpic_params) :: p
         call c_params_default(p)
         p%nx = nx; p%ny = ny; p%dx = dx; p%dy = dy; p%dt = dt
@@ -145,7 +172,15 @@ contains
         if (present(push_order)) then
             if (.not. push_order) p%flags = iand(p%flags, not(PBPIC_FLAG_PUSH_ORDER))
         end if
+        if (present(ybound)) p%ybound = ybound
         call pbpic_check(c_ctx_create(p, ctx), 'pbpic_ctx_create')
+    end subroutine
+
+    ! initparticles + initfields (pic2d.f90:357-648) as the seeded device
+    ! loader: fills the module's ion and electron species and sets E, B
+    subroutine initharris(h)
+        type(pbpic_harris), intent(in) :: h
+        call pbpic_check(c_init_harris(ctx, sp_ion, sp_ele, h), 'initharris')
     end subroutine
 
     ! releasememory (pic2d.f90:2557-2582)

@@ -13,13 +13,18 @@ SPECIES_Q = (1, -1)
 
 
 def make_case(nx=24, ny=20, ppc=8, seed=7, vth=(0.05, 0.3), efield=0.3, bfield=1.5,
-              shuffle=False, sort=True, dx=0.1, dy=0.1, cva=15.0, mime=25.0):
+              shuffle=False, sort=True, dx=0.1, dy=0.1, cva=15.0, mime=25.0, walls=False):
+    """walls: conducting y walls; ny counts the two guard rows, the plasma
+    fills rows 0..ny-3."""
     rng = np.random.default_rng(seed)
     dt = 0.2 / (cva * np.sqrt(mime))
     in0 = 1.0 / ppc
     sp = []
+    nyp = ny - 2 if walls else ny
     for k, q in enumerate(SPECIES_Q):
-        p = oracle.init_maxwellian(nx, ny, ppc, vth[k], seed=seed, stream=k)
+        p = oracle.init_maxwellian(nx, nyp, ppc, vth[k], seed=seed, stream=k)
+        if walls:
+            p["grid"] = oracle.wall_rows(p["grid"], nyp, ny)
         if shuffle:
             order = rng.permutation(len(p["grid"]))
             p = {key: v[order] for key, v in p.items()}
@@ -30,12 +35,13 @@ def make_case(nx=24, ny=20, ppc=8, seed=7, vth=(0.05, 0.3), efield=0.3, bfield=1
                       ("bx", bfield), ("by", bfield), ("bz", bfield)):
         fields[name] = amp * rng.standard_normal((ny, nx)) if amp else np.zeros((ny, nx))
     return dict(nx=nx, ny=ny, dx=dx, dy=dy, dt=dt, cva=cva, mime=mime, in0=in0,
-                species=sp, fields=fields, sort=sort)
+                species=sp, fields=fields, sort=sort, ybound=1 if walls else 0)
 
 
 def oracle_for(case, prec=8, compat=oracle.COMPAT_REFERENCE):
     o = oracle.Oracle(case["nx"], case["ny"], case["dx"], case["dy"], case["dt"],
-                      case["cva"], case["mime"], case["in0"], prec=prec, compat=compat)
+                      case["cva"], case["mime"], case["in0"], prec=prec, compat=compat,
+                      ybound=case.get("ybound", 0))
     sp = case["species"]
     if sp:
         cat = lambda k: np.concatenate([s[k] for s in sp])
@@ -64,7 +70,7 @@ def gpu_for(case, precision=0, compat=oracle.COMPAT_REFERENCE, sort_every=0, dep
     ctx = pbpic_amd.Context(case["nx"], case["ny"], case["dx"], case["dy"], case["dt"],
                             case["cva"], case["mime"], case["in0"], precision=precision,
                             compat=compat, sort_every=sort_every, deposit_rho=deposit_rho,
-                            track_pid=True)
+                            track_pid=True, ybound=case.get("ybound", 0))
     for q, s in zip(SPECIES_Q, case["species"]):
         k = ctx.add_species(q, len(s["grid"]))
         ctx.set_particles(k, s["grid"], s["dx"], s["dy"], s["ux"], s["uy"], s["uz"], s["pid"])
