@@ -328,6 +328,27 @@ int pbpic_step(pbpic_ctx *ctx, int32_t nsteps);
 /* FP64 diagnostics, PBPIC_NDIAG slots (see PBPIC_D_*).                       */
 int pbpic_diag(pbpic_ctx *ctx, double out[PBPIC_NDIAG]);
 
+/* `energyspectra` (pic2d.f90:1957-1990) of one species: counts[k] (nbins
+ * entries, nbins <= 4096) = the particles whose kinetic energy
+ * ene = (gamma-1) ms, computed in FP64 as at :1966-1983, has
+ * ceil(ene/de) = k+1 and ene < emax.  The reference uses de = 0.005,
+ * emax = 5 for electrons and de = 0.025, emax = 25 for ions (:136-140).
+ * ene = 0 lands in bin 0 (the reference indexes its array at 0 there) and a
+ * quotient rounded past the last bin in the last.                            */
+int pbpic_energy_spectrum(pbpic_ctx *ctx, int32_t species, int32_t nbins,
+                          double de, double emax, int32_t *counts);
+
+/* `numdensity` (pic2d.f90:1343-1354): particles of all species per cell,
+ * nx*ny counts indexed by grid-1.                                            */
+int pbpic_number_density(pbpic_ctx *ctx, int32_t *counts);
+
+/* Run counters for checkpoint/restart: the step count (itime; it sets the
+ * sort cadence) and the J.E accumulators je[0] = the last field solve's raw
+ * sum, je[1] = the time integral before it (PBPIC_D_JE_INT is
+ * je[1] + je[0] mime/c^2 dt).  pbpic_set_counters restores them.            */
+int pbpic_get_counters(pbpic_ctx *ctx, int64_t *steps, double je[2]);
+int pbpic_set_counters(pbpic_ctx *ctx, int64_t steps, const double je[2]);
+
 /* Device buffers of a species (for the PyTorch binding).  ptrs receives 7
  * device pointers: cell (int32, 0-based compact id), dx, dy, ux, uy, uz
  * (double or float by precision), pid (int32 or NULL).  Valid until the next

@@ -98,6 +98,10 @@ def load_library(path=None):
         "pbpic_get_particles_f32": (i32, [vp, i32, i64, pi64, pi32, pflt, pflt, pflt, pflt, pflt, pi32]),
         "pbpic_init_maxwellian": (i32, [vp, i32, i32, dbl, dbl, i32, ctypes.c_uint64, ctypes.c_uint32]),
         "pbpic_init_harris": (i32, [vp, i32, i32, P(Harris)]),
+        "pbpic_energy_spectrum": (i32, [vp, i32, i32, dbl, dbl, pi32]),
+        "pbpic_number_density": (i32, [vp, pi32]),
+        "pbpic_get_counters": (i32, [vp, pi64, pdbl]),
+        "pbpic_set_counters": (i32, [vp, i64, pdbl]),
         "pbpic_set_field": (i32, [vp, i32, pdbl]),
         "pbpic_get_field": (i32, [vp, i32, pdbl]),
         "pbpic_zero_current": (i32, [vp]),
@@ -209,24 +213,46 @@ class Context:
         self._chk(self._lib.pbpic_num_particles(self._h, species, ctypes.byref(n)))
         return n.value
 
+    @property
+    def tracks_pid(self):
+        return bool(self.params.flags & FLAG_TRACK_PID)
+
     def set_particles(self, species, grid, dx, dy, ux, uy, uz, pid=None):
+        """Host particle state (grid 1-based).  float32 arrays go through the
+        FP32 entry point, which is exact for the split-precision form."""
         n = len(grid)
         g = _c(grid, np.int32)
-        arrs = [_c(a, np.float64) for a in (dx, dy, ux, uy, uz)]
         pd = _c(pid, np.int32) if pid is not None else None
+        f32 = all(np.asarray(a).dtype == np.float32 for a in (dx, dy, ux, uy, uz))
+        if f32:
+            arrs = [_c(a, np.float32) for a in (dx, dy, ux, uy, uz)]
+            self._chk(self._lib.pbpic_set_particles_f32(
+                self._h, species, n, _ptr(g, ctypes.c_int32),
+                *[_ptr(a, ctypes.c_float) for a in arrs], _ptr(pd, ctypes.c_int32)))
+            return
+        arrs = [_c(a, np.float64) for a in (dx, dy, ux, uy, uz)]
         self._chk(self._lib.pbpic_set_particles(
             self._h, species, n, _ptr(g, ctypes.c_int32), *[_ptr(a, ctypes.c_double) for a in arrs],
             _ptr(pd, ctypes.c_int32)))
 
-    def get_particles(self, species):
+    def get_particles(self, species, native=False):
+        """Host copy of a species (grid 1-based; pid zero unless tracked).
+        native=True returns the offsets and momenta at the device precision
+        (float32 for the split form) instead of float64."""
         n = self.num_particles(species)
         g = np.empty(n, np.int32)
-        a = [np.empty(n, np.float64) for _ in range(5)]
         pid = np.empty(n, np.int32)
         nn = ctypes.c_int64()
-        self._chk(self._lib.pbpic_get_particles(
-            self._h, species, n, ctypes.byref(nn), _ptr(g, ctypes.c_int32),
-            *[_ptr(x, ctypes.c_double) for x in a], _ptr(pid, ctypes.c_int32)))
+        if native and self.precision == PREC_SPLIT32:
+            a = [np.empty(n, np.float32) for _ in range(5)]
+            self._chk(self._lib.pbpic_get_particles_f32(
+                self._h, species, n, ctypes.byref(nn), _ptr(g, ctypes.c_int32),
+                *[_ptr(x, ctypes.c_float) for x in a], _ptr(pid, ctypes.c_int32)))
+        else:
+            a = [np.empty(n, np.float64) for _ in range(5)]
+            self._chk(self._lib.pbpic_get_particles(
+                self._h, species, n, ctypes.byref(nn), _ptr(g, ctypes.c_int32),
+                *[_ptr(x, ctypes.c_double) for x in a], _ptr(pid, ctypes.c_int32)))
         return dict(grid=g, dx=a[0], dy=a[1], ux=a[2], uy=a[3], uz=a[4], pid=pid)
 
     def init_maxwellian(self, species, ppc, vth, vdrift_x=0.0, counter_stream=False,
@@ -319,6 +345,37 @@ class Context:
     def step(self, nsteps=1):
         """The time-loop body (pic2d.f90:171-188), nsteps times."""
         self._chk(self._lib.pbpic_step(self._h, int(nsteps)))
+
+    def energy_spectrum(self, species, nbins=1000, de=None, emax=None):
+        """`energyspectra` (pic2d.f90:1957-1990) of one species: counts per
+        energy bin k of ceil(ene/de) = k+1, ene = (gamma-1) ms < emax.  The
+        reference's bins (:136-140) are the defaults: de = 0.005, emax = 5 for
+        electrons, de = 0.025, emax = 25 for ions."""
+        ion = self.species_q[species] > 0
+        de = (0.025 if ion else 0.005) if de is None else de
+        emax = (25.0 if ion else 5.0) if emax is None else emax
+        out = np.empty(nbins, np.int32)
+        self._chk(self._lib.pbpic_energy_spectrum(self._h, species, nbins, float(de), float(emax),
+                                                  _ptr(out, ctypes.c_int32)))
+        return out
+
+    def number_density(self):
+        """`numdensity` (pic2d.f90:1343-1354): particles per cell, all
+        species, as a (ny, nx) array indexed [iy-1, ix-1]."""
+        out = np.empty(self.nx * self.ny, np.int32)
+        self._chk(self._lib.pbpic_number_density(self._h, _ptr(out, ctypes.c_int32)))
+        return out.reshape(self.nx, self.ny).T.copy()
+
+    def counters(self):
+        """(steps, (je_step_raw, je_int)) for checkpoint/restart."""
+        steps = ctypes.c_int64()
+        je = (ctypes.c_double * 2)()
+        self._chk(self._lib.pbpic_get_counters(self._h, ctypes.byref(steps), je))
+        return steps.value, (je[0], je[1])
+
+    def set_counters(self, steps, je):
+        arr = (ctypes.c_double * 2)(*je)
+        self._chk(self._lib.pbpic_set_counters(self._h, int(steps), arr))
 
     def diag(self):
         out = (ctypes.c_double * 16)()

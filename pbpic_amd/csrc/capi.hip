@@ -43,6 +43,7 @@ struct pbpic_ctx {
     double *acc = nullptr;
     int *err = nullptr;
     double *rho_scratch = nullptr, *gauss0 = nullptr;
+    int32_t *diag_counts = nullptr;   // spectrum / number-density scratch
     bool gauss_first = true;
     std::vector<Species> sp;
     SortScratch sort;
@@ -563,7 +564,7 @@ int pbpic_ctx_destroy(pbpic_ctx *c)
     for (int k = 0; k < 3; k++) { f(c->E[k]); f(c->B[k]); }
     for (int k = 0; k < 4; k++) f(c->J[k]);
     f(c->NF);
-    f(c->acc); f(c->err); f(c->rho_scratch); f(c->gauss0);
+    f(c->acc); f(c->err); f(c->rho_scratch); f(c->gauss0); f(c->diag_counts);
     for (auto &S : c->sp)
         for (int b = 0; b < 2; b++) {
             f(S.cell[b]); f(S.ox[b]); f(S.oy[b]); f(S.ux[b]); f(S.uy[b]); f(S.uz[b]); f(S.pid[b]);
@@ -1006,6 +1007,74 @@ int pbpic_diag(pbpic_ctx *c, double out[PBPIC_NDIAG])
     out[PBPIC_D_GAUSS_DLINF] = a[ACC_G_DLINF];
     out[PBPIC_D_STEPS] = (double)c->steps;
     out[PBPIC_D_TILES_SPILLED] = a[ACC_SPILL];
+    return PBPIC_OK;
+}
+
+static int diag_counts_buf(pbpic_ctx *c)
+{
+    if (!c->diag_counts)
+        HIPCHK(c, dalloc(&c->diag_counts,
+                         sizeof(int32_t) * (size_t)std::max<int64_t>(ncell(c), SPEC_BINS_MAX)));
+    return PBPIC_OK;
+}
+
+int pbpic_energy_spectrum(pbpic_ctx *c, int32_t s, int32_t nbins, double de, double emax,
+                          int32_t *counts)
+{
+    int rc = valid_species(c, s);
+    if (rc) return rc;
+    if (!counts || nbins < 1 || nbins > SPEC_BINS_MAX || !(de > 0) || !(emax > 0))
+        return fail(c, PBPIC_ERR_ARG, "bad spectrum arguments (1 <= nbins <= %d, de > 0)",
+                    SPEC_BINS_MAX);
+    if ((rc = check_flags(c)) || (rc = diag_counts_buf(c))) return rc;
+    HIPCHK(c, launch_fill_u32((uint32_t *)c->diag_counts, nbins, 0u, c->st));
+    HIPCHK(c, launch_spectrum(species_dev(c, s, c->sp[s].cur), c->p.precision, nbins, de, emax,
+                              (uint32_t *)c->diag_counts, c->st));
+    HIPCHK(c, hipMemcpyAsync(counts, c->diag_counts, sizeof(int32_t) * nbins,
+                             hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return PBPIC_OK;
+}
+
+int pbpic_number_density(pbpic_ctx *c, int32_t *counts)
+{
+    if (!c || !counts) return PBPIC_ERR_ARG;
+    int rc = check_flags(c);
+    if (rc || (rc = diag_counts_buf(c))) return rc;
+    const int64_t nc = ncell(c);
+    HIPCHK(c, launch_fill_u32((uint32_t *)c->diag_counts, nc, 0u, c->st));
+    for (int s = 0; s < (int)c->sp.size(); s++)
+        HIPCHK(c, launch_cell_counts(species_dev(c, s, c->sp[s].cur), c->p.nx, c->p.ny,
+                                     c->diag_counts, c->err, c->st));
+    HIPCHK(c, hipMemcpyAsync(counts, c->diag_counts, sizeof(int32_t) * nc, hipMemcpyDeviceToHost,
+                             c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return check_flags(c);
+}
+
+int pbpic_get_counters(pbpic_ctx *c, int64_t *steps, double je[2])
+{
+    if (!c || !steps || !je) return PBPIC_ERR_ARG;
+    double a[ACC_N];
+    HIPCHK(c, hipMemcpyAsync(a, c->acc, sizeof a, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    *steps = c->steps;
+    je[0] = a[ACC_JE_STEP];
+    je[1] = a[ACC_JE_INT];
+    return PBPIC_OK;
+}
+
+int pbpic_set_counters(pbpic_ctx *c, int64_t steps, const double je[2])
+{
+    if (!c || !je || steps < 0) return PBPIC_ERR_ARG;
+    double a[ACC_N];
+    HIPCHK(c, hipMemcpyAsync(a, c->acc, sizeof a, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    a[ACC_JE_STEP] = je[0];
+    a[ACC_JE_INT] = je[1];
+    HIPCHK(c, hipMemcpyAsync(c->acc, a, sizeof a, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    c->steps = steps;
     return PBPIC_OK;
 }
 

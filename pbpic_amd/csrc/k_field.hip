@@ -418,6 +418,49 @@ hipError_t launch_gauss(const PushParams &p, int precision, double *rho_scratch,
     return hipGetLastError();
 }
 
+// energyspectra (pic2d.f90:1957-1990): counts of particles by kinetic
+// energy ene = (gamma - 1) ms in FP64 (:1966-1983), bin k (0-based) holding
+// ceil(ene/de) = k + 1 for ene < emax.  ene = 0 (the reference's bin 0, out
+// of its array) and a quotient rounded past the last bin go to the end bins.
+// LDS histogram per workgroup, flushed with one global atomic per bin.
+template <typename R>
+__global__ __launch_bounds__(256) void spectrum_kernel(SpeciesDev S, int nbins, double de,
+                                                       double emax, uint32_t *counts)
+{
+    __shared__ uint32_t h[SPEC_BINS_MAX];
+    for (int k = threadIdx.x; k < nbins; k += 256) h[k] = 0u;
+    __syncthreads();
+    const R *ux = (const R *)S.ux, *uy = (const R *)S.uy, *uz = (const R *)S.uz;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < S.n;
+         i += (int64_t)gridDim.x * 256) {
+        const double a = ux[i], b = uy[i], c = uz[i];
+        const double gama = sqrt(1.0 + a * a + b * b + c * c);
+        const double ene = (gama - 1.0) * S.ms;
+        if (!(ene < emax)) continue;
+        double k = ceil(ene / de) - 1.0;
+        k = k < 0.0 ? 0.0 : (k > nbins - 1 ? nbins - 1 : k);
+        atomicAdd(&h[(int)k], 1u);
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < nbins; k += 256)
+        if (h[k]) atomicAdd(counts + k, h[k]);
+}
+
+hipError_t launch_spectrum(const SpeciesDev &S, int precision, int nbins, double de, double emax,
+                           uint32_t *counts, hipStream_t st)
+{
+    if (S.n == 0) return hipSuccess;
+    int64_t nb = (S.n + 255) / 256;
+    if (nb > 1024) nb = 1024;
+    if (precision == PBPIC_PREC_FP64)
+        hipLaunchKernelGGL(spectrum_kernel<double>, dim3((unsigned)nb), dim3(256), 0, st, S, nbins,
+                           de, emax, counts);
+    else
+        hipLaunchKernelGGL(spectrum_kernel<float>, dim3((unsigned)nb), dim3(256), 0, st, S, nbins,
+                           de, emax, counts);
+    return hipGetLastError();
+}
+
 // ------------------------------------------------------ conducting y walls
 template <typename R>
 __global__ void wall_fold_kernel(R *jz, int nx, int nyp)

@@ -298,6 +298,26 @@ __global__ __launch_bounds__(256) void gather_kernel(const uint32_t *perm, int64
     if (src.pid) gmove<G>(dst.pid + i0, src.pid, p, contig);
 }
 
+// Adds species sp's particles per cell (compact cell id) to counts, as
+// numdensity (pic2d.f90:1343-1354): through the tile boxes of the last push
+// when they are current, else with run-aggregated atomics.
+hipError_t launch_cell_counts(const SpeciesDev &sp, int nx, int ny, int32_t *counts, int *err,
+                              hipStream_t st)
+{
+    if (sp.n == 0) return hipSuccess;
+    CellKey key;
+    key.init(nx, ny, 0);
+    const int64_t ncell = (int64_t)nx * ny;
+    if (sp.tbox_ok)
+        hipLaunchKernelGGL(cell_hist_tiles_kernel, dim3((unsigned)((sp.n + PUSH_TILE - 1) / PUSH_TILE)),
+                           dim3(256), 0, st, sp.cell, sp.n, nx, ncell, key,
+                           (const int32_t *)sp.tbox, counts, err);
+    else
+        hipLaunchKernelGGL(cell_hist_kernel, dim3((unsigned)((sp.n + 255) / 256)), dim3(256), 0, st,
+                           sp.cell, sp.n, ncell, key, counts, err);
+    return hipGetLastError();
+}
+
 // Sorts species sp into dst (a second buffer set of the same capacity).
 hipError_t sort_species(SortScratch &s, const SpeciesDev &sp, int precision, int push_order,
                         const CellKey &key, int64_t ncell,

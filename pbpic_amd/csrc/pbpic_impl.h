@@ -207,6 +207,14 @@ hipError_t launch_init_maxwellian(const SpeciesDev &s, int precision, int nx,
 hipError_t launch_check_cells(const int32_t *cell, int64_t n, int64_t ncell, int ny,
                               int nyp, int *err, hipStream_t st);
 
+// Diagnostics: energy spectrum of a species (SPEC_BINS_MAX bins at most) and
+// per-cell particle counts.
+constexpr int SPEC_BINS_MAX = 4096;
+hipError_t launch_spectrum(const SpeciesDev &S, int precision, int nbins, double de, double emax,
+                           uint32_t *counts, hipStream_t st);
+hipError_t launch_cell_counts(const SpeciesDev &sp, int nx, int ny, int32_t *counts, int *err,
+                              hipStream_t st);
+
 // Sort (stable LSD radix over the cell id, 8-bit digits).
 struct SortScratch {
     uint32_t *keys[2] = {nullptr, nullptr};
