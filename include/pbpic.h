@@ -87,6 +87,19 @@ extern "C" {
                                         and counts describe that order.
                                         Clear it for the reference's exact
                                         order.  Set by pbpic_params_default */
+#define PBPIC_FLAG_MARDER       0x8  /* divergenceclean after each E update
+                                        (pic2d.f90:867-885, commented out at
+                                        :754): E += d grad(div E^n -
+                                        c^2 rho^n), d = 0.001 dt; needs
+                                        PBPIC_FLAG_DEPOSIT_RHO              */
+#define PBPIC_FLAG_CHERENKOV    0x10 /* suppresscerenkov after each E update
+                                        (:890-907, commented out at :755):
+                                        E -= c^2 dt tau curl curl E^n,
+                                        tau = dt/8.  Both filters use the
+                                        time-n quantities the reference's
+                                        calls see, on the whole periodic
+                                        lattice (the reference's ghost rows
+                                        and columns are stale there)        */
 
 /* ------------------------------------------------------- y boundary kinds
  * The reference is periodic in y (the MPI ring of pic2d.f90:295-298 closes on

@@ -60,6 +60,7 @@ def lib():
                 "diag": (None, [vp, dbl, pdbl]),
                 "set_threads": (None, [vp, i32]),
                 "set_ybound": (i32, [vp, i32]),
+                "set_filters": (i32, [vp, i32, i32]),
             }
             for name, (res, args) in sig.items():
                 f = getattr(L, f"{pre}_{name}")
@@ -90,7 +91,7 @@ class Oracle:
     charge q (`type particle`, pic2d.f90:47-50)."""
 
     def __init__(self, nx, ny, dx, dy, dt, cva, mime, in0, prec=8, compat=COMPAT_REFERENCE,
-                 threads=1, ybound=0):
+                 threads=1, ybound=0, marder=False, cherenkov=False):
         self.L = lib()
         self.pre = "orc8" if prec == 8 else "orc4"
         self.nx, self.ny = nx, ny
@@ -100,6 +101,8 @@ class Oracle:
         if ybound:
             if self._f("set_ybound")(self.h, int(ybound)):
                 raise ValueError("conducting y walls need ny >= 4")
+        if marder or cherenkov:
+            self._f("set_filters")(self.h, int(marder), int(cherenkov))
 
     def _f(self, name):
         return getattr(self.L, f"{self.pre}_{name}")
@@ -114,6 +117,9 @@ class Oracle:
             self.close()
         except Exception:
             pass
+
+    def set_filters(self, marder, cherenkov):
+        self._f("set_filters")(self.h, int(marder), int(cherenkov))
 
     def set_threads(self, n):
         self._f("set_threads")(self.h, n)

@@ -56,7 +56,8 @@ void P##_countingsort(orc_ctx *c, int32_t *counts, int32_t *perm);             \
 void P##_step(orc_ctx *c, int nsteps, int sort_every);                         \
 void P##_diag(orc_ctx *c, double rho_bg, double out[16]);                      \
 void P##_set_threads(orc_ctx *c, int nthreads);                                \
-int  P##_set_ybound(orc_ctx *c, int kind);
+int  P##_set_ybound(orc_ctx *c, int kind);                                      \
+int  P##_set_filters(orc_ctx *c, int marder, int cherenkov);
 
 ORC_DECL(orc8)
 ORC_DECL(orc4)

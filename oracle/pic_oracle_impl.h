@@ -20,6 +20,7 @@ typedef struct {
     REAL dx, dy, idx, idy, dt, dth, cva, cva2, mime, in0;
     int compat;
     int ywall;                 /* conducting walls in y (rows my-1, my: guards) */
+    int marder, cherenkov;     /* the filters of :867-907 (off in the reference) */
     int nthreads;
     /* emf(0:mx+1,0:my+1) components (type fields, :52-57) */
     REAL *ex, *dex_dy, *ey, *dey_dx, *ez, *dez_dx, *dez_dy, *dive;
@@ -299,6 +300,11 @@ static void PFX(rebuild)(CT *c)
         F(c->dby_dx, mx + 1, j) = F(c->dby_dx, 1, j);
         F(c->dey_dx, mx + 1, j) = F(c->dey_dx, 1, j);
     }
+    /* dive as :1915 rebuilds it (y range 0:my; see fieldsolver) */
+    for (int j = 1; j <= my + 1; j++)
+        for (int i = 1; i <= mx + 1; i++)
+            F(c->dive, i, j) = (F(c->ex, i, j) - F(c->ex, i - 1, j)) * idx +
+                               (F(c->ey, i, j) - F(c->ey, i, j - 1)) * idy;
     PFX(emfgrid)(c);
 }
 
@@ -673,6 +679,80 @@ static void PFX(halo2)(CT *c, REAL *a, REAL *b)
     }
 }
 
+/* divergenceclean (pic2d.f90:867-885), Marder's pass: E += d grad(dive -
+ * c^2 rho) with d = dt*0.001, dive and rho the values held when it is
+ * called: div E^n from the end of the previous solve (or from the fields as
+ * set) and the rho the push just deposited (normalized).  The source updates
+ * ex(1:mx,1:my+1), ey(1:mx+1,1:my) and leaves the ghost copies stale (column
+ * 1 is then overwritten from column mx+1 at the next solve, :747-752); here
+ * the owned region is updated with periodic neighbours and the ghosts copied. */
+static void PFX(divergenceclean)(CT *c)
+{
+    const int mx = c->mx, my = c->my;
+    const REAL dconst = (REAL)0.001;
+    const REAL dtdx = c->dt * dconst / c->dx, dtdy = c->dt * dconst / c->dy;
+    REAL *mf = (REAL *)calloc((size_t)(mx + 2) * (my + 2), sizeof(REAL));
+#define MF(i, j) mf[(size_t)(i) + (size_t)(mx + 2) * (size_t)(j)]
+    for (int j = 1; j <= my; j++)
+        for (int i = 1; i <= mx; i++)
+            MF(i, j) = F(c->dive, i, j) - c->cva2 * D(c->rho, i, j);
+    for (int j = 1; j <= my; j++)
+        for (int i = 1; i <= mx; i++) {
+            const int ip = i == mx ? 1 : i + 1, jp = j == my ? 1 : j + 1;
+            F(c->ex, i, j) = F(c->ex, i, j) + dtdx * (MF(ip, j) - MF(i, j));
+            F(c->ey, i, j) = F(c->ey, i, j) + dtdy * (MF(i, jp) - MF(i, j));
+        }
+#undef MF
+    free(mf);
+    PFX(ghost_f)(c, c->ex);
+    PFX(ghost_f)(c, c->ey);
+}
+
+/* suppresscerenkov (pic2d.f90:890-907): E -= c^2 dt tau curl curl E with
+ * tau = dt/8, built from the curl terms of E^n the previous solve stored:
+ * w = dey_dx - dex_dy at the cell centres, and dez_dx, dez_dy.  The source
+ * reads rows of dex_dy its curl loop (:790) never writes (0 and my+1) and
+ * leaves the ghosts stale; here the owned region is updated with periodic
+ * neighbours and the ghosts copied. */
+static void PFX(suppresscerenkov)(CT *c)
+{
+    const int mx = c->mx, my = c->my;
+    const REAL tau = (REAL)0.125 * c->dt;
+    const REAL dtdx = c->dt * c->cva2 * tau / c->dx, dtdy = c->dt * c->cva2 * tau / c->dy;
+#define W(i, j) (F(c->dey_dx, i, j) - F(c->dex_dy, i, j))
+    REAL *t = (REAL *)malloc(sizeof(REAL) * 3 * (size_t)mx * my);
+    for (int j = 1; j <= my; j++)
+        for (int i = 1; i <= mx; i++) {
+            const int im = i == 1 ? mx : i - 1, jm = j == 1 ? my : j - 1;
+            const int ip = i == mx ? 1 : i + 1, jp = j == my ? 1 : j + 1;
+            const size_t k = (size_t)(i - 1) + (size_t)mx * (j - 1);
+            t[k] = -(dtdy * (W(i, j) - W(i, jm)));
+            t[(size_t)mx * my + k] = dtdx * (W(i, j) - W(im, j));
+            t[2 * (size_t)mx * my + k] = dtdx * (F(c->dez_dx, ip, j) - F(c->dez_dx, i, j)) +
+                                         dtdy * (F(c->dez_dy, i, jp) - F(c->dez_dy, i, j));
+        }
+#undef W
+    for (int j = 1; j <= my; j++)
+        for (int i = 1; i <= mx; i++) {
+            const size_t k = (size_t)(i - 1) + (size_t)mx * (j - 1);
+            F(c->ex, i, j) = F(c->ex, i, j) + t[k];
+            F(c->ey, i, j) = F(c->ey, i, j) + t[(size_t)mx * my + k];
+            F(c->ez, i, j) = F(c->ez, i, j) + t[2 * (size_t)mx * my + k];
+        }
+    free(t);
+    PFX(ghost_f)(c, c->ex);
+    PFX(ghost_f)(c, c->ey);
+    PFX(ghost_f)(c, c->ez);
+}
+
+int PFX(set_filters)(orc_ctx *oc, int marder, int cherenkov)
+{
+    CT *c = (CT *)oc;
+    c->marder = marder != 0;
+    c->cherenkov = cherenkov != 0;
+    return 0;
+}
+
 /* fieldsolver (pic2d.f90:678-860). */
 void PFX(fieldsolver)(orc_ctx *oc)
 {
@@ -729,6 +809,8 @@ void PFX(fieldsolver)(orc_ctx *oc)
         F(c->ey, 0, j) = F(c->ey, mx, j); F(c->ey, 1, j) = F(c->ey, mx + 1, j);
         F(c->ez, 0, j) = F(c->ez, mx, j); F(c->ez, 1, j) = F(c->ez, mx + 1, j);
     }
+    if (c->marder) PFX(divergenceclean)(c);                     /* :754 */
+    if (c->cherenkov) PFX(suppresscerenkov)(c);                 /* :755 */
     PFX(halo2)(c, c->ey, c->ez);                                /* :757-787 */
     if (c->ywall) PFX(wall_images_e)(c);
     for (int j = 1; j <= my; j++)                               /* :790 */

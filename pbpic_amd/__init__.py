@@ -24,6 +24,7 @@ PREC_FP64, PREC_SPLIT32 = 0, 1
 COMPAT_BORIS_OLD_U, COMPAT_RHO_MIRROR, COMPAT_JY_DROP = 1, 2, 4
 COMPAT_REFERENCE = 7
 FLAG_DEPOSIT_RHO, FLAG_TRACK_PID, FLAG_PUSH_ORDER = 1, 2, 4
+FLAG_MARDER, FLAG_CHERENKOV = 8, 16
 YBOUND_PERIODIC, YBOUND_CONDUCTING = 0, 1
 HARRIS_DOUBLE_X, HARRIS_SINGLE_Y = 0, 1
 FIELDS = {"ex": 0, "ey": 1, "ez": 2, "bx": 3, "by": 4, "bz": 5, "jx": 6,
@@ -151,7 +152,8 @@ class Context:
 
     def __init__(self, nx, ny, dx, dy, dt, cva, mime, in0, *, precision=PREC_FP64,
                  compat=COMPAT_REFERENCE, sort_every=50, deposit_rho=True,
-                 track_pid=False, push_order=True, rho_background=0.0, ybound=YBOUND_PERIODIC):
+                 track_pid=False, push_order=True, rho_background=0.0, ybound=YBOUND_PERIODIC,
+                 marder=False, cherenkov=False):
         self._lib = load_library()
         p = Params()
         self._lib.pbpic_params_default(ctypes.byref(p))
@@ -159,7 +161,8 @@ class Context:
         p.cva, p.mime, p.in0 = cva, mime, in0
         p.precision, p.compat, p.sort_every = precision, compat, sort_every
         p.flags = ((FLAG_DEPOSIT_RHO if deposit_rho else 0) | (FLAG_TRACK_PID if track_pid else 0) |
-                   (FLAG_PUSH_ORDER if push_order else 0))
+                   (FLAG_PUSH_ORDER if push_order else 0) | (FLAG_MARDER if marder else 0) |
+                   (FLAG_CHERENKOV if cherenkov else 0))
         p.rho_background = rho_background
         p.ybound = {"periodic": YBOUND_PERIODIC, "conducting": YBOUND_CONDUCTING}.get(ybound, ybound)
         self.params = p

@@ -44,6 +44,7 @@ struct pbpic_ctx {
     int *err = nullptr;
     double *rho_scratch = nullptr, *gauss0 = nullptr;
     int32_t *diag_counts = nullptr;   // spectrum / number-density scratch
+    double *filt[3] = {nullptr, nullptr, nullptr};   // filter increments of E
     bool gauss_first = true;
     std::vector<Species> sp;
     SortScratch sort;
@@ -168,6 +169,7 @@ FdtdParams fdtd_params(pbpic_ctx *c)
     F.ex = c->E[0]; F.ey = c->E[1]; F.ez = c->E[2];
     F.bx = c->B[0]; F.by = c->B[1]; F.bz = c->B[2];
     F.jx = c->J[0]; F.jy = c->J[1]; F.jz = c->J[2];
+    F.fex = c->filt[0]; F.fey = c->filt[1]; F.fez = c->filt[2];
     F.acc = c->acc;
     return F;
 }
@@ -273,6 +275,22 @@ int fieldsolve_impl(pbpic_ctx *c)
 {
     HIPCHK(c, launch_acc_op(c->acc, 1, c->p.dt * c->p.mime / (c->p.cva * c->p.cva), c->st));
     HIPCHK(c, prof_mark(c, PBPIC_PROF_FIELD, true));
+    if (c->filt[0]) {
+        const double cva2 = c->p.cva * c->p.cva, dt = c->p.dt;
+        const double tau = 0.125 * dt;                       // pic2d.f90:895
+        FilterParams q{};
+        q.nx = c->p.nx; q.ny = c->p.ny;
+        q.marder = (c->p.flags & PBPIC_FLAG_MARDER) ? 1 : 0;
+        q.cherenkov = (c->p.flags & PBPIC_FLAG_CHERENKOV) ? 1 : 0;
+        q.idx = c->idx; q.idy = c->idy;
+        q.cva2in0 = cva2 * c->p.in0;
+        q.mdx = dt * 0.001 / c->p.dx; q.mdy = dt * 0.001 / c->p.dy;   // :876-878
+        q.cdx = dt * cva2 * tau / c->p.dx; q.cdy = dt * cva2 * tau / c->p.dy;   // :896-897
+        q.ex = c->E[0]; q.ey = c->E[1]; q.ez = c->E[2];
+        q.rho = c->J[3];
+        q.fex = c->filt[0]; q.fey = c->filt[1]; q.fez = c->filt[2];
+        HIPCHK(c, launch_filters(q, c->p.precision, c->st));
+    }
     HIPCHK(c, launch_fdtd(fdtd_params(c), c->p.precision, c->st));
     if (walls(c)) HIPCHK(c, launch_wall_images(fdtd_params(c), 0x3f, c->st));
     HIPCHK(c, prof_mark(c, PBPIC_PROF_FIELD, false));
@@ -502,6 +520,7 @@ int pbpic_ctx_create(const pbpic_params *pp, pbpic_ctx **out)
         if (p.reserved[k]) return PBPIC_ERR_ARG;
     if (p.ybound != PBPIC_YBOUND_PERIODIC && p.ybound != PBPIC_YBOUND_CONDUCTING) return PBPIC_ERR_ARG;
     if (p.ybound == PBPIC_YBOUND_CONDUCTING && p.ny < 4) return PBPIC_ERR_ARG;
+    if ((p.flags & PBPIC_FLAG_MARDER) && !(p.flags & PBPIC_FLAG_DEPOSIT_RHO)) return PBPIC_ERR_ARG;
     pbpic_ctx *c = new pbpic_ctx();
     c->p = p;
     c->rsz = p.precision == PBPIC_PREC_FP64 ? 8 : 4;
@@ -526,6 +545,8 @@ int pbpic_ctx_create(const pbpic_params *pp, pbpic_ctx **out)
     }
     for (int k = 0; k < 4; k++) chk(dalloc(&c->J[k], nc * c->rsz), "alloc J");
     chk(dalloc(&c->NF, nodefield_bytes(c->p.precision, nc)), "alloc node fields");
+    if (p.flags & (PBPIC_FLAG_MARDER | PBPIC_FLAG_CHERENKOV))
+        for (int k = 0; k < 3; k++) chk(dalloc(&c->filt[k], nc * 8), "alloc filter planes");
     chk(dalloc(&c->acc, ACC_N * sizeof(double)), "alloc acc");
     chk(dalloc(&c->err, sizeof(int) * 4), "alloc err");
     {
@@ -565,6 +586,7 @@ int pbpic_ctx_destroy(pbpic_ctx *c)
     for (int k = 0; k < 4; k++) f(c->J[k]);
     f(c->NF);
     f(c->acc); f(c->err); f(c->rho_scratch); f(c->gauss0); f(c->diag_counts);
+    f(c->filt[0]); f(c->filt[1]); f(c->filt[2]);
     for (auto &S : c->sp)
         for (int b = 0; b < 2; b++) {
             f(S.cell[b]); f(S.ox[b]); f(S.oy[b]); f(S.ux[b]); f(S.uy[b]); f(S.uz[b]); f(S.pid[b]);
@@ -1138,6 +1160,8 @@ int pbpic_adjoint_forward(pbpic_ctx *c, int32_t nsteps, int32_t ckpt, double *lo
 {
     if (!c || nsteps < 1 || ckpt < 1) return PBPIC_ERR_ARG;
     if (walls(c)) return fail(c, PBPIC_ERR_STATE, "the adjoint does not support conducting y walls");
+    if (c->p.flags & (PBPIC_FLAG_MARDER | PBPIC_FLAG_CHERENKOV))
+        return fail(c, PBPIC_ERR_STATE, "the adjoint does not support the Marder/Cherenkov filters");
     int rc = check_flags(c);
     if (rc) return rc;
     adj_free(c);

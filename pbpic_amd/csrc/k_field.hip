@@ -117,11 +117,16 @@ __global__ __launch_bounds__(FT_THREADS) void fdtd_kernel(FdtdParams P)
             ezn = ezo + (double)(dtc2f * (((byx * idxf) - (bxy * idyf)) - (float)jzr * in0f));
         }
         if (P.nyp && (gy == 0 || gy == P.nyp)) exn = 0.0;   // tangential E on a wall
-        se[0][c] = exn; se[1][c] = eyn; se[2][c] = ezn;
         const bool own = u >= FT_H && u < FT_H + FT_X && v >= FT_H && v < FT_H + FT_Y &&
                          x0 + u < nx && y0 + v < ny;
         if (own)
             je += jxr * 0.5 * (exo + exn) + jyr * 0.5 * (eyo + eyn) + jzr * 0.5 * (ezo + ezn);
+        if (P.fex) {
+            // Marder / Cherenkov filters (:754-755), increments from E^n
+            exn += P.fex[g]; eyn += P.fey[g]; ezn += P.fez[g];
+            if (P.nyp && (gy == 0 || gy == P.nyp)) exn = 0.0;
+        }
+        se[0][c] = exn; se[1][c] = eyn; se[2][c] = ezn;
     }
     __syncthreads();
 
@@ -458,6 +463,71 @@ hipError_t launch_spectrum(const SpeciesDev &S, int precision, int nbins, double
     else
         hipLaunchKernelGGL(spectrum_kernel<float>, dim3((unsigned)nb), dim3(256), 0, st, S, nbins,
                            de, emax, counts);
+    return hipGetLastError();
+}
+
+// divergenceclean (pic2d.f90:867-885) and suppresscerenkov (:890-907) as
+// increments of E computed from E^n and rho^n before the field solve (the
+// reference calls them after the E update, but both read only the dive and
+// curl terms of E^n and the rho of this step's push), periodic throughout:
+//   Marder:    F = div E^n - c^2 in0 rho_raw at the nodes,
+//              dEx = dt 0.001/dx (F(x+1) - F(x)), dEy likewise in y
+//   Cherenkov: w = dEy/dx - dEx/dy at the cell centres,
+//              dEx = -cdy (w - w(y-1)), dEy = cdx (w - w(x-1)),
+//              dEz = cdx (d2x Ez) + cdy (d2y Ez) with cdx = dt c^2 (dt/8)/dx
+template <typename RJ>
+__global__ __launch_bounds__(256) void filter_kernel(FilterParams P)
+{
+    const int nx = P.nx, ny = P.ny;
+    const int64_t n = (int64_t)nx * ny;
+    const RJ *rho = (const RJ *)P.rho;
+    for (int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x; k < n;
+         k += (int64_t)gridDim.x * 256) {
+        const int gy = (int)(k / nx), gx = (int)(k - (int64_t)gy * nx);
+        const int xm = gx == 0 ? nx - 1 : gx - 1, xp = gx + 1 == nx ? 0 : gx + 1;
+        const int ym = gy == 0 ? ny - 1 : gy - 1, yp = gy + 1 == ny ? 0 : gy + 1;
+        auto at = [&](int x, int y) { return (size_t)y * nx + x; };
+        const double *ex = P.ex, *ey = P.ey, *ez = P.ez;
+        double dex = 0.0, dey = 0.0, dez = 0.0;
+        if (P.marder) {
+            auto mf = [&](int x, int y) {
+                const int xl = x == 0 ? nx - 1 : x - 1, yl = y == 0 ? ny - 1 : y - 1;
+                const double div = (ex[at(x, y)] - ex[at(xl, y)]) * P.idx +
+                                   (ey[at(x, y)] - ey[at(x, yl)]) * P.idy;
+                return div - P.cva2in0 * (double)rho[at(x, y)];
+            };
+            const double f0 = mf(gx, gy);
+            dex = P.mdx * (mf(xp, gy) - f0);
+            dey = P.mdy * (mf(gx, yp) - f0);
+        }
+        if (P.cherenkov) {
+            auto w = [&](int x, int y) {
+                const int xr = x + 1 == nx ? 0 : x + 1, yr = y + 1 == ny ? 0 : y + 1;
+                return (ey[at(xr, y)] - ey[at(x, y)]) * P.idx - (ex[at(x, yr)] - ex[at(x, y)]) * P.idy;
+            };
+            const double w0 = w(gx, gy);
+            dex += -(P.cdy * (w0 - w(gx, ym)));
+            dey += P.cdx * (w0 - w(xm, gy));
+            const double e0 = ez[at(gx, gy)];
+            const double dzx0 = (e0 - ez[at(xm, gy)]) * P.idx, dzx1 = (ez[at(xp, gy)] - e0) * P.idx;
+            const double dzy0 = (e0 - ez[at(gx, ym)]) * P.idy, dzy1 = (ez[at(gx, yp)] - e0) * P.idy;
+            dez = P.cdx * (dzx1 - dzx0) + P.cdy * (dzy1 - dzy0);
+        }
+        P.fex[k] = dex;
+        P.fey[k] = dey;
+        P.fez[k] = dez;
+    }
+}
+
+hipError_t launch_filters(const FilterParams &p, int precision, hipStream_t st)
+{
+    const int64_t n = (int64_t)p.nx * p.ny;
+    int nb = (int)((n + 255) / 256);
+    if (nb > 8192) nb = 8192;
+    if (precision == PBPIC_PREC_FP64)
+        hipLaunchKernelGGL(filter_kernel<double>, dim3(nb), dim3(256), 0, st, p);
+    else
+        hipLaunchKernelGGL(filter_kernel<float>, dim3(nb), dim3(256), 0, st, p);
     return hipGetLastError();
 }
 

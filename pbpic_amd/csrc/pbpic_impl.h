@@ -126,8 +126,19 @@ struct FdtdParams {
     double dth, dtc2, idx, idy, in0;   // dt/2, dt*cva^2, 1/dx, 1/dy, in0
     double *ex, *ey, *ez, *bx, *by, *bz;
     const void *jx, *jy, *jz;
+    const double *fex, *fey, *fez;     // filter increments added to E^{n+1} (or null)
     double *acc;
 };
+
+// Marder / Cherenkov increments of E from E^n and rho^n (k_field.hip).
+struct FilterParams {
+    int nx, ny, marder, cherenkov;
+    double idx, idy, cva2in0, mdx, mdy, cdx, cdy;   // mdx = dt 0.001/dx, cdx = dt c^2 tau/dx
+    const double *ex, *ey, *ez;
+    const void *rho;                               // raw charge, precision of J
+    double *fex, *fey, *fez;
+};
+hipError_t launch_filters(const FilterParams &p, int precision, hipStream_t st);
 
 // Adjoint of the push for one step (k_adjoint.hip).  The particle state of
 // step n comes from the tape (cell, ox..uz); xa..uza hold the adjoint of the

@@ -16,6 +16,7 @@ module pbpic_gpu
     integer(c_int32_t), parameter :: PBPIC_COMPAT_REFERENCE = 7
     integer(c_int32_t), parameter :: PBPIC_FLAG_DEPOSIT_RHO = 1, PBPIC_FLAG_TRACK_PID = 2
     integer(c_int32_t), parameter :: PBPIC_FLAG_PUSH_ORDER = 4
+    integer(c_int32_t), parameter :: PBPIC_FLAG_MARDER = 8, PBPIC_FLAG_CHERENKOV = 16
     integer(c_int32_t), parameter :: PBPIC_YBOUND_PERIODIC = 0, PBPIC_YBOUND_CONDUCTING = 1
     integer(c_int32_t), parameter :: PBPIC_HARRIS_DOUBLE_X = 0, PBPIC_HARRIS_SINGLE_Y = 1
     integer, parameter :: PBPIC_NDIAG = 16

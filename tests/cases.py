@@ -65,12 +65,13 @@ def oracle_species(o, case):
     return out
 
 
-def gpu_for(case, precision=0, compat=oracle.COMPAT_REFERENCE, sort_every=0, deposit_rho=True):
+def gpu_for(case, precision=0, compat=oracle.COMPAT_REFERENCE, sort_every=0, deposit_rho=True,
+            **kw):
     import pbpic_amd
     ctx = pbpic_amd.Context(case["nx"], case["ny"], case["dx"], case["dy"], case["dt"],
                             case["cva"], case["mime"], case["in0"], precision=precision,
                             compat=compat, sort_every=sort_every, deposit_rho=deposit_rho,
-                            track_pid=True, ybound=case.get("ybound", 0))
+                            track_pid=True, ybound=case.get("ybound", 0), **kw)
     for q, s in zip(SPECIES_Q, case["species"]):
         k = ctx.add_species(q, len(s["grid"]))
         ctx.set_particles(k, s["grid"], s["dx"], s["dy"], s["ux"], s["uy"], s["uz"], s["pid"])

@@ -275,3 +275,91 @@ def test_harris_loader_statistics(oracle_lib, geometry):
         x = np.arange(nx) * d
         lx = nx * d
         assert np.allclose(by[0], np.tanh((x - lx / 4) / 0.5) - np.tanh((x - 3 * lx / 4) / 0.5) - 1)
+
+
+# -------------------------------------------------------------- filters
+def _vacuum_periodic(nx, ny, d, dt, **kw):
+    return oracle.Oracle(nx, ny, d, d, dt, CVA, MIME, 0.1, compat=0, **kw)
+
+
+def test_marder_damps_a_gauss_error_mode(oracle_lib):
+    """divergenceclean (pic2d.f90:867-885): with B = J = rho = 0 and
+    Ex = sin(k x) (so dive = F is a single cosine mode), one solve maps F to
+    (1 - dt d kappa) F, d = 0.001, kappa = 4 sin^2(k dx/2)/dx^2 -- the
+    discrete Laplacian's eigenvalue; Ey stays zero."""
+    nx, ny, d = 32, 8, 0.1
+    dt = 0.5 * d / CVA
+    for m in (1, 5, 12):
+        k = 2 * np.pi * m / (nx * d)
+        ex = np.sin(k * (np.arange(nx) + 0.5) * d)[None, :] * np.ones((ny, 1))
+        divs = []
+        for marder in (False, True):
+            o = _vacuum_periodic(nx, ny, d, dt, marder=marder)
+            o.set_field("ex", ex)
+            o.zero_current()
+            o.fieldsolver()
+            e = o.get_field("ex")
+            divs.append((e - np.roll(e, 1, axis=1)) / d)
+            if marder:
+                assert np.max(np.abs(o.get_field("ey"))) == 0
+        kappa = 4 * np.sin(k * d / 2) ** 2 / d ** 2
+        f0 = (ex - np.roll(ex, 1, axis=1)) / d
+        # without Marder the divergence is unchanged (curl B = J = 0)
+        assert np.max(np.abs(divs[0] - f0)) < 1e-12 * np.max(np.abs(f0))
+        assert np.max(np.abs(divs[1] - (1 - dt * 0.001 * kappa) * f0)) < 1e-12 * np.max(np.abs(f0))
+
+
+@pytest.mark.parametrize("comp", ["ez", "ex", "ey"])
+def test_cherenkov_damps_transverse_modes(oracle_lib, comp):
+    """suppresscerenkov (pic2d.f90:890-907): a transverse E mode (Ez(x),
+    Ex(y) or Ey(x) = cos) in vacuum with B = 0 changes in one solve by the
+    factor 1 - c^2 kappa (dt^2/2 + dt tau), tau = dt/8: the Yee part
+    (-c^2 dt^2/2 curl curl E) plus the filter's -c^2 dt tau curl curl E."""
+    nx = ny = 32
+    d = 0.1
+    dt = 0.4 * d / CVA
+    k = 2 * np.pi * 3 / (nx * d)
+    g = np.arange(nx)
+    if comp == "ez":        # Ez at x = gx + 1/2
+        plane = np.cos(k * (g + 0.5) * d)[None, :] * np.ones((ny, 1))
+    elif comp == "ex":      # Ex at y = gy
+        plane = np.cos(k * g * d)[:, None] * np.ones((1, nx))
+    else:                   # Ey at x = gx
+        plane = np.cos(k * g * d)[None, :] * np.ones((ny, 1))
+    kappa = 4 * np.sin(k * d / 2) ** 2 / d ** 2
+    for cher, fac in ((False, 1 - CVA ** 2 * kappa * dt * dt / 2),
+                      (True, 1 - CVA ** 2 * kappa * (dt * dt / 2 + dt * dt / 8))):
+        o = _vacuum_periodic(nx, ny, d, dt, cherenkov=cher)
+        o.set_field(comp, plane)
+        o.zero_current()
+        o.fieldsolver()
+        assert np.max(np.abs(o.get_field(comp) - fac * plane)) < 1e-12, (comp, cher)
+
+
+@pytest.mark.parametrize("shift", [(0, 5), (7, 0), (3, 11)])
+def test_filters_translation_invariant(oracle_lib, shift):
+    """A full step with both filters commutes with a periodic shift of fields
+    and particles: the ghost cells hold periodic copies wherever the filter
+    stencils read them."""
+    sx, sy = shift
+    case = make_case(nx=16, ny=14, ppc=6, seed=3, efield=0.3, bfield=0.8)
+    shifted = dict(case)
+    shifted["fields"] = {k: np.roll(np.roll(v, sy, axis=0), sx, axis=1)
+                         for k, v in case["fields"].items()}
+    sp = []
+    for p in case["species"]:
+        g = p["grid"] - 1
+        cx, cy = g // case["ny"], g % case["ny"]
+        q = dict(p)
+        q["grid"] = (((cx + sx) % case["nx"]) * case["ny"] + (cy + sy) % case["ny"] + 1).astype(np.int32)
+        sp.append(q)
+    shifted["species"] = sp
+    out = []
+    for c in (case, shifted):
+        o = oracle_for(c, compat=0)
+        o.set_filters(True, True)
+        o.step(2, sort_every=0)
+        out.append({k: o.get_field(k) for k in ("ex", "ey", "ez", "bx", "by", "bz")})
+    for k in out[0]:
+        ref = np.roll(np.roll(out[0][k], sy, axis=0), sx, axis=1)
+        assert np.max(np.abs(out[1][k] - ref)) < 1e-12 * np.max(np.abs(ref)), k
