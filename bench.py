@@ -156,6 +156,41 @@ def adjoint_leg(precision, seed, nsteps=200, ckpt=20):
                 particles=sum(npart), steps=nsteps, ckpt_every=ckpt)
 
 
+def harris_leg(precision, steps, warmup, seed):
+    """The Harris current-sheet configuration at its full size: 1024 x 512
+    domain cells (plus two guard rows) over 25.6 x 12.8 d_i with conducting y
+    walls, 64 particles per cell and species (67,108,864 particles), the psi
+    perturbation, sort every 50 steps.  Returns particle-steps/s and the
+    total-energy change over the timed steps."""
+    import torch
+    import pbpic_amd
+    from pbpic_amd import configs
+
+    cfg = configs.harris_c1()
+    h = dict(cfg["harris"], seed=seed)
+    n = h["ppc"] * cfg["nx"] * (cfg["ny"] - 2)
+    ctx = pbpic_amd.Context(cfg["nx"], cfg["ny"], cfg["dx"], cfg["dy"], cfg["dt"], cfg["cva"],
+                            cfg["mime"], cfg["in0"], precision=precision, sort_every=50,
+                            deposit_rho=False, ybound="conducting")
+    si = ctx.add_species(1, n)
+    se = ctx.add_species(-1, n)
+    ctx.init_harris(si, se, **h)
+    ctx.countingsort()
+    ctx.step(warmup)
+    d0 = ctx.diag()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ctx.step(steps)
+    ctx.sync()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    d1 = ctx.diag()
+    ctx.close()
+    tot = lambda d: d["ke_e"] + d["ke_i"] + d["w_e"] + d["w_b"]
+    return dict(particles=2 * n, ps=2 * n * steps / wall, ms_per_step=1e3 * wall / steps,
+                energy_change=(tot(d1) - tot(d0)) / tot(d0), tiles_spilled=d1["tiles_spilled"])
+
+
 def cpu_baseline(budget_s, seed):
     """The oracle (CPU restatement of pic2d.f90, AoS `particle` records in the
     reference's default FP32) on a bounded sample of the same workload: the
@@ -204,6 +239,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-harris", action="store_true")
     ap.add_argument("--seed", type=int, default=20260101)
     ap.add_argument("--calib-copy-gib", type=float, default=0.0,
                     help="first run one plain device copy of this many GiB (a known-size "
@@ -283,6 +319,17 @@ def main():
                               **adj)
         # the larger of the forms' adjoint-sweep / plain-forward time ratios
         out["adjoint_forward_ratio"] = max(a["ratio"] for a in adj.values())
+    if not args.no_harris:
+        hr = {}
+        for f in forms:
+            log(f"[bench] Harris-sheet leg form={f}")
+            hr[f] = harris_leg(1 if f == "split" else 0, 100, 10, args.seed)
+            log(f"[bench] Harris form={f}: {hr[f]}")
+        out["harris_c1"] = dict(workload="Harris sheet 1024x512 (+2 guard rows), 25.6x12.8 d_i, "
+                                         "conducting y walls, 64 ppc/species, psi perturbation, "
+                                         "100 timed steps, K_sort=50", **hr)
+        if "split" in hr and "fp64" in hr:
+            out["harris_c1"]["ratio_split_vs_fp64"] = hr["split"]["ps"] / hr["fp64"]["ps"]
     out["diag"] = {f: {k: res[f]["diag1"][k] for k in ("ke_e", "w_e", "w_b", "je_int", "gauss_dlinf", "tiles_spilled")}
                    for f in res}
     if not args.no_cpu:

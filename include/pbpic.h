@@ -114,7 +114,8 @@ extern "C" {
  * a wall are reflected (offset and u_y change sign) before their deposit;
  * Jz deposited in a guard row is folded back with its image sign.  Field
  * energies weight the two wall rows by 1/2; the Gauss residual covers the
- * rows 1..ny-3 between the walls.  Not available with the adjoint.          */
+ * rows 1..ny-3 between the walls.  The adjoint covers the walls: its field
+ * gradients are with respect to the domain rows (guard rows zero).           */
 #define PBPIC_YBOUND_PERIODIC    0
 #define PBPIC_YBOUND_CONDUCTING  1
 

@@ -13,6 +13,11 @@ with 0-based cell = cx*ny + cy and offsets in (-1/2, 1/2].
                              absolute-index form of the source)
   x fold of jy column mx+2   :1156 (compat bit 4)
   fieldsolver                :686-803
+With P["nyp"] set (conducting y walls, no counterpart in the reference): the
+domain is rows 0..nyp-1 of the ny = nyp + 2 rows, particles are reflected at
+the walls before their deposit, Jz beyond a wall folds back with sign -1,
+Ex vanishes on the walls and the two guard rows hold the fields' mirror
+images (the same construction as the C oracle's wall_image).
 """
 import math
 
@@ -75,6 +80,14 @@ def push_deposit(P, sp, N, J, compat):
     a = torch.where(dx1.abs() > 0.5, torch.sign(dx1), torch.zeros_like(dx1)).detach()
     b = torch.where(dy1.abs() > 0.5, torch.sign(dy1), torch.zeros_like(dy1)).detach()
     ox2, oy2 = dx1 - a, dy1 - b
+    nyp = P.get("nyp")
+    if nyp:
+        # specular reflection at a wall: the offset mirrored into the cell
+        # the particle left, u_y reversed
+        refl = ((cy == 0) & (b < 0)) | ((cy == nyp - 1) & (b > 0))
+        b = torch.where(refl, torch.zeros_like(b), b)
+        oy2 = torch.where(refl, -oy2, oy2)
+        un = torch.stack([un[0], torch.where(refl, -un[1], un[1]), un[2]])
     vz = un[2] * P["cva"] / gn
     qx, qy = q * P["dx"] / P["dt"], q * P["dy"] / P["dt"]
 
@@ -119,6 +132,31 @@ def push_deposit(P, sp, N, J, compat):
     return new
 
 
+def _image_rows(a, nyp, sign):
+    """Guard rows nyp, nyp+1 of a half-row plane <- sign x rows nyp-1, 0."""
+    return torch.cat([a[:nyp], sign * a[nyp - 1:nyp], sign * a[0:1]], dim=0)
+
+
+def wall_images(E, B, nyp):
+    ex, ey, ez = E
+    bx, by, bz = B
+    z = torch.zeros_like(ex[0:1])
+    ex = torch.cat([z, ex[1:nyp], z, -ex[1:2]], dim=0)
+    bx = torch.cat([bx[:nyp + 1], bx[1:2]], dim=0)
+    return (torch.stack([ex, _image_rows(ey, nyp, 1.0), _image_rows(ez, nyp, -1.0)]),
+            torch.stack([bx, _image_rows(by, nyp, -1.0), _image_rows(bz, nyp, 1.0)]))
+
+
+def fold_jz(jz, nyp):
+    """Jz of the guard rows folded back with sign -1 (jz: (ny, nx) plane)."""
+    top, bot = jz[nyp], jz[nyp + 1]
+    rows = [jz[0] - bot] + [jz[r] for r in range(1, nyp - 1)] + [jz[nyp - 1] - top]
+    if nyp == 1:
+        rows = [jz[0] - bot - top]
+    z = torch.zeros_like(jz[0])
+    return torch.stack(rows + [z, z])
+
+
 def fieldsolve(P, E, B, J):
     """fieldsolver (pic2d.f90:686-803) on compact periodic planes; J raw."""
     ex, ey, ez = E
@@ -133,10 +171,18 @@ def fieldsolve(P, E, B, J):
     exn = ex + dtc2 * ((bz - _ry(bz, 1)) * idy - jx)
     eyn = ey - dtc2 * ((bz - _rx(bz, 1)) * idx + jy)
     ezn = ez + dtc2 * ((_rx(by, -1) - by) * idx - (_ry(bx, -1) - bx) * idy - jz)
+    nyp = P.get("nyp")
+    if nyp:
+        z = torch.zeros_like(exn[0:1])
+        exn = torch.cat([z, exn[1:nyp], z, exn[nyp + 1:]], dim=0)   # tangential E on the walls
+        ezn = _image_rows(ezn, nyp, -1.0)
     bx = bx - dth * ((ezn - _ry(ezn, 1)) * idy)
     by = by + dth * ((ezn - _rx(ezn, 1)) * idx)
     bz = bz + dth * ((_ry(exn, -1) - exn) * idy - (_rx(eyn, -1) - eyn) * idx)
-    return torch.stack([exn, eyn, ezn]), torch.stack([bx, by, bz])
+    E1, B1 = torch.stack([exn, eyn, ezn]), torch.stack([bx, by, bz])
+    if nyp:
+        E1, B1 = wall_images(E1, B1, nyp)
+    return E1, B1
 
 
 def run(P, E, B, species, nsteps, compat=7, sort_every=0, step0=0):
@@ -149,6 +195,8 @@ def run(P, E, B, species, nsteps, compat=7, sort_every=0, step0=0):
         J = [torch.zeros(nx * ny, dtype=torch.float64) for _ in range(3)]
         species = [push_deposit(P, sp, N, J, compat) for sp in species]
         Jp = torch.stack([j.reshape(ny, nx) for j in J])
+        if P.get("nyp"):
+            Jp = torch.stack([Jp[0], Jp[1], fold_jz(Jp[2], P["nyp"])])
         E1, B1 = fieldsolve(P, E, B, Jp)
         loss = loss + w * (Jp * (E + E1) * 0.5).sum()
         E, B = E1, B1

@@ -45,6 +45,8 @@ struct pbpic_ctx {
     double *rho_scratch = nullptr, *gauss0 = nullptr;
     int32_t *diag_counts = nullptr;   // spectrum / number-density scratch
     double *filt[3] = {nullptr, nullptr, nullptr};   // filter increments of E
+    int32_t *big = nullptr;    // deferred push tiles (count + list)
+    int64_t big_cap = 0;
     bool gauss_first = true;
     std::vector<Species> sp;
     SortScratch sort;
@@ -144,6 +146,9 @@ PushParams push_params(pbpic_ctx *c)
     P.compat = c->p.compat;
     P.deposit_rho = (c->p.flags & PBPIC_FLAG_DEPOSIT_RHO) ? 1 : 0;
     P.maxw = c->maxw;
+    P.maxw_big = push_maxw_big(c->p.precision, P.deposit_rho);
+    P.big = c->big;
+    P.pass = 0;
     P.dt = c->p.dt;
     P.cva = c->p.cva;
     P.idx = c->idx;
@@ -258,6 +263,14 @@ int kinetic_impl(pbpic_ctx *c)
 
 int push_impl(pbpic_ctx *c)
 {
+    int64_t ntiles = 0;
+    for (auto &S : c->sp) ntiles += (S.n + PUSH_TILE - 1) / PUSH_TILE;
+    if (ntiles + 1 > c->big_cap) {
+        // the deferred-tile list of the two-pass push: one entry per tile
+        if (c->big) { HIPCHK(c, hipStreamSynchronize(c->st)); (void)hipFree(c->big); c->big = nullptr; }
+        HIPCHK(c, dalloc(&c->big, sizeof(int32_t) * (size_t)(ntiles + 1)));
+        c->big_cap = ntiles + 1;
+    }
     const PushParams P = push_params(c);
     // node fields (emfgrid, pic2d.f90:837-845) are field work: timed as such
     HIPCHK(c, prof_mark(c, PBPIC_PROF_FIELD, true));
@@ -586,7 +599,7 @@ int pbpic_ctx_destroy(pbpic_ctx *c)
     for (int k = 0; k < 4; k++) f(c->J[k]);
     f(c->NF);
     f(c->acc); f(c->err); f(c->rho_scratch); f(c->gauss0); f(c->diag_counts);
-    f(c->filt[0]); f(c->filt[1]); f(c->filt[2]);
+    f(c->filt[0]); f(c->filt[1]); f(c->filt[2]); f(c->big);
     for (auto &S : c->sp)
         for (int b = 0; b < 2; b++) {
             f(S.cell[b]); f(S.ox[b]); f(S.oy[b]); f(S.ux[b]); f(S.uy[b]); f(S.uz[b]); f(S.pid[b]);
@@ -1159,7 +1172,6 @@ int pbpic_set_field_device(pbpic_ctx *c, int32_t which, const double *plane)
 int pbpic_adjoint_forward(pbpic_ctx *c, int32_t nsteps, int32_t ckpt, double *loss)
 {
     if (!c || nsteps < 1 || ckpt < 1) return PBPIC_ERR_ARG;
-    if (walls(c)) return fail(c, PBPIC_ERR_STATE, "the adjoint does not support conducting y walls");
     if (c->p.flags & (PBPIC_FLAG_MARDER | PBPIC_FLAG_CHERENKOV))
         return fail(c, PBPIC_ERR_STATE, "the adjoint does not support the Marder/Cherenkov filters");
     int rc = check_flags(c);
@@ -1256,6 +1268,7 @@ int pbpic_adjoint_backward(pbpic_ctx *c, double *const *fg, void *const *pg)
                 }
                 AdjFieldParams F{};
                 F.nx = c->p.nx; F.ny = c->p.ny;
+                F.nyp = walls(c) ? nyp_of(c) : 0;
                 F.dth = c->dth; F.dtc2 = c->dtc2; F.idx = c->idx; F.idy = c->idy;
                 F.in0 = c->p.in0; F.w = c->p.dt * c->p.mime / (c->p.cva * c->p.cva);
                 F.exa = fa[0]; F.eya = fa[1]; F.eza = fa[2];
@@ -1270,6 +1283,7 @@ int pbpic_adjoint_backward(pbpic_ctx *c, double *const *fg, void *const *pg)
                 HIPCHK(c, launch_fdtd_adj(F, c->p.precision, c->st));
                 AdjParams A{};
                 A.nx = c->p.nx; A.ny = c->p.ny; A.compat = c->p.compat;
+                A.nyp = walls(c) ? nyp_of(c) : 0;
                 A.maxw = c->p.precision == PBPIC_PREC_FP64 ? 640 : 1280;
                 A.nspecies = ns;
                 A.dtcva = c->dtcva; A.idx = c->idx; A.idy = c->idy; A.cva = c->p.cva;
@@ -1292,6 +1306,15 @@ int pbpic_adjoint_backward(pbpic_ctx *c, double *const *fg, void *const *pg)
                 HIPCHK(c, launch_push_adj(A, c->p.precision, c->st));
                 HIPCHK(c, launch_node_transpose(F, fa[12], c->st));
             }
+        }
+        if (walls(c)) {
+            // gradients with respect to the domain values: the guard rows of
+            // the initial fields are images of them
+            AdjFieldParams F{};
+            F.nx = c->p.nx; F.ny = c->p.ny; F.nyp = nyp_of(c);
+            F.exa = fa[0]; F.eya = fa[1]; F.eza = fa[2];
+            F.bxa = fa[3]; F.bya = fa[4]; F.bza = fa[5];
+            HIPCHK(c, launch_wall_images_adj(F, c->st));
         }
         HIPCHK(c, prof_mark(c, PBPIC_PROF_ADJ, false));
         for (int q = 0; q < 6; q++)

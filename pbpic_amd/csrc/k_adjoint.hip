@@ -243,8 +243,13 @@ __global__ __launch_bounds__(ADJ_THREADS, 2) void push_adj_kernel(AdjParams P)
         const R dx1 = ox + c3 * un[0] * idx;
         const R dy1 = oy + c3 * un[1] * idy;
         const int a = fabs(dx1) > Hh ? (int)a_sgn(dx1) : 0;
-        const int b = fabs(dy1) > Hh ? (int)a_sgn(dy1) : 0;
-        const R ox2 = dx1 + (R)(-a), oy2 = dy1 + (R)(-b);
+        int b = fabs(dy1) > Hh ? (int)a_sgn(dy1) : 0;
+        const R ox2 = dx1 + (R)(-a);
+        R oy2 = dy1 + (R)(-b);
+        // conducting walls: reflected before the deposit (offset mirrored,
+        // u_y reversed), so the deposit saw b = 0 and -oy2
+        const bool refl = P.nyp && ((cy == 0 && b < 0) || (cy == P.nyp - 1 && b > 0));
+        if (refl) { b = 0; oy2 = -oy2; }
         const R vz = un[2] * cva / gn;
         // deposit terms (relative form of :1237-1282)
         R fx1, fx2, wx1, wx2, gx, fy1, fy2, wy1, wy2, gy;
@@ -289,11 +294,13 @@ __global__ __launch_bounds__(ADJ_THREADS, 2) void push_adj_kernel(AdjParams P)
         const int64_t ia = i;
         R xa = ((R *)P.xa[s])[ia], ya = ((R *)P.ya[s])[ia];
         R ua[3] = {((R *)P.uxa[s])[ia], ((R *)P.uya[s])[ia], ((R *)P.uza[s])[ia]};
-        const R xn = xa + bo2x, yn = ya + bo2y;         // adjoint of the new offsets
+        const R xn = xa + bo2x;                         // adjoint of the new offsets
+        const R yn = refl ? -(ya + bo2y) : ya + bo2y;   // (before a reflection)
         R xo = xn + bo1x, yo = yn + bo1y;               // adjoint of the old offsets
         // Q_i = k_i un_i / gamma: dQ_i/dun_j = k_i (delta_ij - v_i v_j)/gamma
         const R kk[3] = {xn * dtcva * idx, yn * dtcva * idy, bvz * cva};
         const R vk = v[0] * kk[0] + v[1] * kk[1] + v[2] * kk[2];
+        if (refl) ua[1] = -ua[1];                       // stored u_y = -un_y
         R bun[3];
         for (int k = 0; k < 3; k++) bun[k] = ua[k] + ig * (kk[k] - v[k] * vk);
         // ---- Boris reverse
@@ -407,8 +414,10 @@ __device__ __forceinline__ void cbT(const double *ex, const double *ey, const do
 
 // Pass A: Ebar' = Ebar^{n+1} + (w/2) J - dth CE^T(Bbar^{n+1});
 //         Jbar  = -dtc2 in0 Ebar' + (w/2)(E^n + E^{n+1}).
+// With conducting walls (mode 1 then 2) the two halves run apart: Ebar' goes
+// through the transpose of the forward's wall projection before Jbar.
 template <typename RJ>
-__global__ void fdtd_adj_a(AdjFieldParams P)
+__global__ void fdtd_adj_a(AdjFieldParams P, int mode)
 {
     const int64_t n = (int64_t)P.nx * P.ny;
     for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
@@ -422,11 +431,63 @@ __global__ void fdtd_adj_a(AdjFieldParams P)
         double *Jb[3] = {P.jxa, P.jya, P.jza};
         double *Ep[3] = {P.etx, P.ety, P.etz};
         for (int q = 0; q < 3; q++) {
-            const double jr = (double)J[q][k];
-            const double e = Ea[q][k] + 0.5 * P.w * jr - P.dth * o[q];
-            Ep[q][k] = e;
-            Jb[q][k] = -P.dtc2 * P.in0 * e + 0.5 * P.w * (E0[q][k] + E1[q][k]);
+            double e;
+            if (mode != 2) {
+                const double jr = (double)J[q][k];
+                e = Ea[q][k] + 0.5 * P.w * jr - P.dth * o[q];
+                Ep[q][k] = e;
+            } else {
+                e = Ep[q][k];
+            }
+            if (mode != 1) Jb[q][k] = -P.dtc2 * P.in0 * e + 0.5 * P.w * (E0[q][k] + E1[q][k]);
         }
+    }
+}
+
+// Conducting walls, per column gx.  Forward maps and their transposes:
+//   imaging W (after the solve): guard rows of the half-row planes <- +-
+//     mirror (Ey, Bz even; Ez, By odd), Ex <- 0 on the walls and Ex, Bx of
+//     row nyp+1 <- -Ex, +Bx of row 1; transpose: the guard adjoint is added
+//     (with the sign) to the mirror and cleared, the wall Ex adjoint cleared;
+//   projection P (inside the solve, on E'): Ex <- 0 on the walls, Ez guards
+//     <- -mirror; transpose likewise on Ebar';
+//   Jz fold (after the push): row 0 -= row nyp+1, row nyp-1 -= row nyp,
+//     guards <- 0; transpose: guard adjoint <- -(mirror adjoint).
+__device__ __forceinline__ void half_T(double *a, size_t rt, size_t gt, size_t r0, size_t gb,
+                                       double sgn)
+{
+    a[rt] += sgn * a[gt];
+    a[r0] += sgn * a[gb];
+    a[gt] = 0.0;
+    a[gb] = 0.0;
+}
+
+// which: 0 = W^T on (exa..bza), 1 = P^T on (etx..etz), 2 = fold^T on jza
+__global__ void wall_adj_kernel(AdjFieldParams P, int which)
+{
+    const int gx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gx >= P.nx) return;
+    const int nx = P.nx, nyp = P.nyp;
+    const size_t r0 = gx, r1 = (size_t)nx + gx, rt = (size_t)(nyp - 1) * nx + gx,
+                 gt = (size_t)nyp * nx + gx, gb = (size_t)(nyp + 1) * nx + gx;
+    if (which == 0) {
+        half_T(P.eya, rt, gt, r0, gb, 1.0);
+        half_T(P.eza, rt, gt, r0, gb, -1.0);
+        half_T(P.bya, rt, gt, r0, gb, -1.0);
+        half_T(P.bza, rt, gt, r0, gb, 1.0);
+        P.exa[r1] -= P.exa[gb];
+        P.exa[gb] = 0.0;
+        P.exa[r0] = 0.0;
+        P.exa[gt] = 0.0;
+        P.bxa[r1] += P.bxa[gb];
+        P.bxa[gb] = 0.0;
+    } else if (which == 1) {
+        P.etx[r0] = 0.0;
+        P.etx[gt] = 0.0;
+        half_T(P.etz, rt, gt, r0, gb, -1.0);
+    } else {
+        P.jza[gb] = -P.jza[r0];
+        P.jza[gt] = -P.jza[rt];
     }
 }
 
@@ -532,15 +593,29 @@ hipError_t launch_push_adj(const AdjParams &p0, int precision, hipStream_t st)
     return hipGetLastError();
 }
 
+hipError_t launch_wall_images_adj(const AdjFieldParams &p, hipStream_t st)
+{
+    if (!p.nyp) return hipSuccess;
+    hipLaunchKernelGGL(wall_adj_kernel, dim3((p.nx + 255) / 256), dim3(256), 0, st, p, 0);
+    return hipGetLastError();
+}
+
 hipError_t launch_fdtd_adj(const AdjFieldParams &p, int precision, hipStream_t st)
 {
     const int64_t n = (int64_t)p.nx * p.ny;
     int nb = (int)((n + 255) / 256);
     if (nb > 8192) nb = 8192;
-    if (precision == PBPIC_PREC_FP64)
-        hipLaunchKernelGGL(fdtd_adj_a<double>, dim3(nb), dim3(256), 0, st, p);
-    else
-        hipLaunchKernelGGL(fdtd_adj_a<float>, dim3(nb), dim3(256), 0, st, p);
+    const dim3 wb((p.nx + 255) / 256);
+    if (p.nyp) hipLaunchKernelGGL(wall_adj_kernel, wb, dim3(256), 0, st, p, 0);
+    for (int mode : {p.nyp ? 1 : 0, p.nyp ? 2 : -1}) {
+        if (mode < 0) break;
+        if (precision == PBPIC_PREC_FP64)
+            hipLaunchKernelGGL(fdtd_adj_a<double>, dim3(nb), dim3(256), 0, st, p, mode);
+        else
+            hipLaunchKernelGGL(fdtd_adj_a<float>, dim3(nb), dim3(256), 0, st, p, mode);
+        if (mode == 1) hipLaunchKernelGGL(wall_adj_kernel, wb, dim3(256), 0, st, p, 1);
+    }
+    if (p.nyp) hipLaunchKernelGGL(wall_adj_kernel, wb, dim3(256), 0, st, p, 2);
     hipLaunchKernelGGL(fdtd_adj_b, dim3(nb), dim3(256), 0, st, p);
     if (precision == PBPIC_PREC_FP64)
         hipLaunchKernelGGL(fdtd_adj_c<double>, dim3(nb), dim3(256), 0, st, p);

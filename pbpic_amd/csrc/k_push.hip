@@ -662,7 +662,14 @@ __global__ __launch_bounds__(PUSH_THREADS, 3) void push_kernel(PushParams P)
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ int red[4][5];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int t = xcd_remap(blockIdx.x, gridDim.x);
+    int t;
+    if (P.pass == 2) {
+        // the tiles the first launch deferred; the rest of the grid is idle
+        if ((int)blockIdx.x >= P.big[0]) return;
+        t = P.big[1 + blockIdx.x];
+    } else {
+        t = xcd_remap(blockIdx.x, gridDim.x);
+    }
     int s = 0;
     while (s + 1 < P.nspecies && t >= P.sp[s + 1].tile0) s++;
     const SpeciesDev S = P.sp[s];
@@ -730,6 +737,11 @@ __global__ __launch_bounds__(PUSH_THREADS, 3) void push_kernel(PushParams P)
     const int wx0 = mnx - 1, wy0 = mny - 1;
     const int W = mxx - mnx + 4, H = mxy - mny + 4;
     const bool inlds = mnx <= mxx && (int64_t)W * H <= P.maxw;
+    if (P.pass == 1 && !inlds && mnx <= mxx && (int64_t)W * H <= P.maxw_big) {
+        // the box is block-uniform: the whole workgroup leaves together
+        if (tid == 0) P.big[1 + atomicAdd(P.big, 1)] = t;
+        return;
+    }
     if (!inlds && mnx <= mxx && tid == 0) unsafeAtomicAdd(P.acc + ACC_SPILL, 1.0);
     const int WH = inlds ? W * H : 0;
     // fast path: a full tile of valid cells, at least one cell away from every
@@ -1179,14 +1191,35 @@ static void launch_push_f(const PushParams &q, int nt, size_t lds, hipStream_t s
     else launch_push_t<R, SPLIT, false, false>(q, nt, lds, st);
 }
 
+// Largest window whose LDS fits one workgroup per CU (150 KiB).
+int push_maxw_big(int precision, int deposit_rho)
+{
+    const size_t budget = 150 * 1024, per = push_lds_bytes(precision, 1, deposit_rho) -
+                                             push_lds_bytes(precision, 0, deposit_rho);
+    return (int)((budget - push_lds_bytes(precision, 0, deposit_rho)) / per);
+}
+
 hipError_t launch_push(const PushParams &p, int precision, hipStream_t st)
 {
     PushParams q;
     const int nt = push_tiles(p, q);
     if (nt == 0) return hipSuccess;
     const size_t lds = push_lds_bytes(precision, p.maxw, p.deposit_rho);
+    q.pass = p.big && p.maxw_big > p.maxw ? 1 : 0;
+    if (q.pass) {
+        hipError_t e = launch_fill_u32((uint32_t *)p.big, 1, 0u, st);
+        if (e != hipSuccess) return e;
+    }
     if (precision == PBPIC_PREC_FP64) launch_push_f<double, false>(q, nt, lds, st);
     else launch_push_f<float, true>(q, nt, lds, st);
+    if (q.pass) {
+        PushParams b = q;
+        b.pass = 2;
+        b.maxw = p.maxw_big;
+        const size_t lb = push_lds_bytes(precision, b.maxw, p.deposit_rho);
+        if (precision == PBPIC_PREC_FP64) launch_push_f<double, false>(b, nt, lb, st);
+        else launch_push_f<float, true>(b, nt, lb, st);
+    }
     return hipGetLastError();
 }
 

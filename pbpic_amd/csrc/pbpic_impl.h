@@ -109,6 +109,12 @@ struct PushParams {
     FastDiv dny;
     int compat, deposit_rho;
     int maxw;                  // LDS window capacity in cells
+    // Tiles whose window exceeds maxw but fits maxw_big are deferred by the
+    // first launch (pass 1) to a second one with the larger window (pass 2);
+    // big[0] counts them, big[1..] lists their tile indices.  pass 0: one
+    // launch, oversized tiles deposit with global atomics.
+    int maxw_big, pass;
+    int *big;
     int ntiles;
     int nspecies;
     double dt, cva, idx, idy, dtcva;   // dtcva = dt*cva (:1003)
@@ -153,6 +159,7 @@ struct AdjSpecies {
 
 struct AdjParams {
     int nx, ny, compat, maxw, nspecies;
+    int nyp;                                     // conducting y walls (0: periodic)
     double dtcva, idx, idy, cva;
     const double *ex, *ey, *ez, *bx, *by, *bz;   // fields at step n
     const double *jadj[3];                       // current adjoint (raw J)
@@ -164,6 +171,7 @@ struct AdjParams {
 
 struct AdjFieldParams {
     int nx, ny;
+    int nyp;                                     // conducting y walls (0: periodic)
     double dth, dtc2, idx, idy, in0, w;          // w = dt*mime/cva^2
     double *exa, *eya, *eza, *bxa, *bya, *bza;   // field adjoints (in/out)
     double *etx, *ety, *etz;                     // scratch: Ebar'
@@ -174,6 +182,8 @@ struct AdjFieldParams {
 
 // Launchers (one per translation unit family).
 hipError_t launch_push(const PushParams &p, int precision, hipStream_t st);
+// Window capacity of the second push pass (one workgroup per CU).
+int push_maxw_big(int precision, int deposit_rho);
 size_t nodefield_bytes(int precision, int64_t ncell);
 // Fills p.nf from the E and B planes; launch_push reads p.nf.
 hipError_t launch_nodefield(const PushParams &p, int precision, hipStream_t st);
@@ -239,6 +249,9 @@ struct SortScratch {
 };
 hipError_t launch_push_adj(const AdjParams &p, int precision, hipStream_t st);
 hipError_t launch_fdtd_adj(const AdjFieldParams &p, int precision, hipStream_t st);
+// Conducting y walls: transpose of the guard-row imaging applied to the field
+// adjoints (exa..bza) in place.
+hipError_t launch_wall_images_adj(const AdjFieldParams &p, hipStream_t st);
 hipError_t launch_node_transpose(const AdjFieldParams &p, double *nadj, hipStream_t st);
 hipError_t launch_unpermute(const uint32_t *perm, int64_t n, int precision,
                             void *const *post, void *const *pre, hipStream_t st);

@@ -22,10 +22,15 @@ def _ref_grads(case, A0, ud0, nsteps, ckpt_sort, compat):
     nx, ny = case["nx"], case["ny"]
     P = dict(nx=nx, ny=ny, dx=case["dx"], dy=case["dy"], dt=case["dt"], cva=case["cva"],
              mime=case["mime"], in0=case["in0"])
+    if case.get("ybound"):
+        P["nyp"] = ny - 2
     x = torch.arange(nx, dtype=torch.float64)[None, :].expand(ny, nx)
     F = torch.zeros(6, ny, nx, dtype=torch.float64)
     F[1] = A0 * torch.sin(2 * np.pi * x / nx)
     F.requires_grad_(True)
+    FE, FB = F[:3], F[3:]
+    if P.get("nyp"):
+        FE, FB = tr.wall_images(FE, FB, P["nyp"])   # as set_field does
     sps = []
     for q, s in zip((1, -1), case["species"]):
         d = dict(q=q, cell=torch.tensor(s["grid"] - 1, dtype=torch.long))
@@ -35,7 +40,7 @@ def _ref_grads(case, A0, ud0, nsteps, ckpt_sort, compat):
                 v = v + ud0
             d[k] = v.clone().requires_grad_(True)
         sps.append(d)
-    loss, *_ = tr.run(P, F[:3], F[3:], sps, nsteps, compat=compat, sort_every=ckpt_sort)
+    loss, *_ = tr.run(P, FE, FB, sps, nsteps, compat=compat, sort_every=ckpt_sort)
     loss.backward()
     return loss.item(), F.grad.numpy(), [{k: sp[k].grad.numpy() for k in PART} for sp in sps]
 
@@ -46,7 +51,7 @@ def _gpu_grads(case, A0, ud0, nsteps, ckpt, sort_every, compat, precision):
     nx, ny = case["nx"], case["ny"]
     sim = pbpic_amd.Context(nx, ny, case["dx"], case["dy"], case["dt"], case["cva"], case["mime"],
                             case["in0"], precision=precision, compat=compat, sort_every=sort_every,
-                            deposit_rho=False)
+                            deposit_rho=False, ybound=case.get("ybound", 0))
     dt = torch.float64 if precision == 0 else torch.float32
     species = []
     for q, s in zip((1, -1), case["species"]):
@@ -97,3 +102,28 @@ def test_adjoint_split32_near_fp64(gpu):
     for s in range(2):
         for k in PART:
             assert rel_norm(gPg[s][k], gP[s][k]) < 1e-3, (s, k)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_adjoint_walls_matches_reverse_mode(gpu, precision):
+    """Conducting y walls with a plasma hot enough that particles reflect:
+    the adjoint through the reflection, the Jz fold and the guard-row imaging
+    against torch's reverse mode (field gradients with respect to the domain
+    rows, as set_field images the guards)."""
+    print(f"adjoint walls precision={precision}", flush=True)
+    case = make_case(nx=24, ny=16, ppc=6, seed=4, efield=0.0, bfield=0.0, vth=(0.05, 0.3),
+                     walls=True)
+    L, gF, gP = _ref_grads(case, 0.01, 0.05, 10, 5, 7)
+    Lg, gFg, gPg = _gpu_grads(case, 0.01, 0.05, 10, 4, 5, 7, precision)
+    tol_l, tol = (1e-10, 1e-8) if precision == 0 else (1e-4, 1e-3)
+    assert abs(Lg - L) <= tol_l * abs(L)
+    for k in range(6):
+        assert rel_norm(gFg[k], gF[k]) < tol, k
+        # guard rows carry no gradient: rows 14, 15 of the half-row planes
+        # (Ey, Ez, By, Bz); row 15 of Ex, Bx, whose row 14 is the top wall
+        assert not gFg[k][14 if k in (1, 2, 4, 5) else 15:].any(), k
+        if k == 0:
+            assert not gFg[0][0].any() and not gFg[0][14].any()   # Ex = 0 on the walls
+    for s in range(2):
+        for k in PART:
+            assert rel_norm(gPg[s][k], gP[s][k]) < tol, (s, k)

@@ -271,6 +271,32 @@ def test_one_step_full_tiles(gpu, precision, compat):
 
 
 @pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("nx,ny,path", [(40, 36, "second pass"), (72, 70, "global atomics")])
+def test_one_step_oversized_windows(gpu, precision, nx, ny, path):
+    """Unsorted particles: every 4096-particle tile spans the whole domain.  At
+    40 x 36 the window exceeds the first launch's LDS window but fits the
+    second pass's (no tile spills); at 72 x 70 it fits neither and the tiles
+    deposit with global atomics.  Both against the oracle."""
+    _say(f"oversized windows precision={precision} {nx}x{ny} ({path})")
+    case = make_case(nx=nx, ny=ny, ppc=12, seed=13, shuffle=True)
+    ctx = gpu_for(case, precision=precision, compat=7)
+    o = oracle_for(case, compat=7)
+    ctx.zero_current(); ctx.pusher()
+    o.zero_current(); o.pusher()
+    tol = FP64_TOL if precision == 0 else 1e-5
+    for f in ("jx", "jy", "jz", "rho"):
+        assert rel_norm(ctx.get_field(f), o.get_field(f)) < tol, f
+    osp = oracle_species(o, case)
+    for k in range(2):
+        g = ctx.get_particles(k)
+        if precision == 0:
+            assert np.array_equal(g["grid"], osp[k]["grid"])
+            assert np.max(np.abs(g["ux"] - osp[k]["ux"])) < 1e-12 * np.max(np.abs(osp[k]["ux"]))
+    spilled = ctx.diag()["tiles_spilled"]
+    assert (spilled == 0) if path == "second pass" else (spilled > 0), spilled
+
+
+@pytest.mark.parametrize("precision", [0, 1])
 def test_gauss_residual_tiled_charge(gpu, precision):
     """The Gauss residual's charge deposit, per push tile: after a sort (no
     recorded tile boxes: per-particle atomics) and after a push (tile boxes

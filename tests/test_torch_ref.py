@@ -14,6 +14,12 @@ from oracle import torch_ref as tr
 def torch_inputs(case):
     P = dict(nx=case["nx"], ny=case["ny"], dx=case["dx"], dy=case["dy"], dt=case["dt"],
              cva=case["cva"], mime=case["mime"], in0=case["in0"])
+    if case.get("ybound"):
+        P["nyp"] = case["ny"] - 2
+        # the fields as the oracle holds them: guard rows imaged
+        o = oracle_for(dict(case, species=[]), compat=0)
+        case = dict(case, fields={k: o.get_field(k) for k in case["fields"]})
+        o.close()
     E = torch.tensor(np.stack([case["fields"][k] for k in ("ex", "ey", "ez")]))
     B = torch.tensor(np.stack([case["fields"][k] for k in ("bx", "by", "bz")]))
     sps = []
@@ -23,9 +29,10 @@ def torch_inputs(case):
     return P, E, B, sps
 
 
-@pytest.mark.parametrize("compat", [7, 0])
-def test_forward_matches_oracle(oracle_lib, compat):
-    case = make_case(nx=16, ny=12, ppc=4, seed=3)
+@pytest.mark.parametrize("compat,walls", [(7, False), (0, False), (0, True), (7, True)])
+def test_forward_matches_oracle(oracle_lib, compat, walls):
+    case = make_case(nx=16, ny=12, ppc=4, seed=3, vth=(0.1, 0.45) if walls else (0.05, 0.3),
+                     walls=walls)
     P, E, B, sps = torch_inputs(case)
     loss, E1, B1, sp1 = tr.run(P, E, B, sps, 12, compat=compat, sort_every=5)
     o = oracle_for(case, compat=compat)
@@ -38,11 +45,13 @@ def test_forward_matches_oracle(oracle_lib, compat):
     assert np.allclose(E1[1].numpy(), o.get_field("ey"), rtol=1e-11, atol=1e-12)
 
 
-def _c3_like(nx=24, ny=16, ppc=6, seed=4):
+def _c3_like(nx=24, ny=16, ppc=6, seed=4, walls=False):
     """The differentiable-PIC configuration in miniature: thermal plasma, an
     initial Ey = A sin(2 pi x/Lx), an electron drift u_d along x and a per-cell
-    Bz perturbation dBz, all zero except A."""
-    case = make_case(nx=nx, ny=ny, ppc=ppc, seed=seed, efield=0.0, bfield=0.0, vth=(0.02, 0.1))
+    Bz perturbation dBz, all zero except A.  walls: conducting y walls and a
+    hotter plasma, so that particles reflect during the run."""
+    case = make_case(nx=nx, ny=ny, ppc=ppc, seed=seed, efield=0.0, bfield=0.0,
+                     vth=(0.05, 0.3) if walls else (0.02, 0.1), walls=walls)
     return case
 
 
@@ -60,9 +69,9 @@ def _oracle_loss(case, A, ud, dbz, nsteps, compat):
     return o.diag()["je_int"]
 
 
-@pytest.mark.parametrize("compat", [7, 0])
-def test_gradient_matches_finite_differences(oracle_lib, compat):
-    case = _c3_like()
+@pytest.mark.parametrize("compat,walls", [(7, False), (0, False), (7, True)])
+def test_gradient_matches_finite_differences(oracle_lib, compat, walls):
+    case = _c3_like(walls=walls)
     nx, ny, nsteps = case["nx"], case["ny"], 10
     A0, ud0 = 0.01, 0.05
     P, E, B, sps = torch_inputs(case)
@@ -72,6 +81,9 @@ def test_gradient_matches_finite_differences(oracle_lib, compat):
     x = torch.arange(nx, dtype=torch.float64)[None, :].expand(ny, nx)
     E = torch.stack([E[0], A * torch.sin(2 * np.pi * x / nx), E[2]])
     B = torch.stack([B[0], B[1], dbz])
+    if walls:
+        # set_field images the guard rows: the free variables are the domain rows
+        E, B = tr.wall_images(E, B, P["nyp"])
     sps[1] = dict(sps[1], ux=sps[1]["ux"] + ud)
     loss, *_ = tr.run(P, E, B, sps, nsteps, compat=compat, sort_every=5)
     loss.backward()
@@ -85,7 +97,7 @@ def test_gradient_matches_finite_differences(oracle_lib, compat):
             _oracle_loss(case, A0, ud0 - eps, z, nsteps, compat)) / (2 * eps)
     assert abs(A.grad.item() - fd_A) <= 1e-4 * abs(fd_A), (A.grad.item(), fd_A)
     assert abs(ud.grad.item() - fd_u) <= 1e-4 * abs(fd_u), (ud.grad.item(), fd_u)
-    for (j, i) in ((3, 5), (10, 17)):
+    for (j, i) in ((3, 5), (10, 17)) + (((0, 7), (13, 2)) if walls else ()):
         e = np.zeros((ny, nx))
         e[j, i] = eps
         fd = (_oracle_loss(case, A0, ud0, e, nsteps, compat) -
