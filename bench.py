@@ -191,6 +191,70 @@ def harris_leg(precision, steps, warmup, seed):
                 energy_change=(tot(d1) - tot(d0)) / tot(d0), tiles_spilled=d1["tiles_spilled"])
 
 
+def adjoint_c4_leg(seed, nsteps=1000, ckpt=50, precision=1):
+    """The large differentiable configuration: the Harris sheet at 1024 x 1024
+    domain cells (conducting y walls), 64 particles per cell and species
+    (134,217,728 particles), 1000 steps, loss = J.E over the central 4 x 2 d_i
+    diffusion region, gradient with respect to the initial Bz (and every
+    other initial field and particle), checkpoints every 50 steps, split
+    precision.  Times the plain forward against the checkpointed forward and
+    the adjoint sweep, and reports the device memory the sweep needed."""
+    import numpy as np
+    import torch
+    import pbpic_amd
+    from pbpic_amd import configs
+
+    cfg = configs.harris_c4()
+    h = dict(cfg["harris"], seed=seed)
+    n = h["ppc"] * cfg["nx"] * (cfg["ny"] - 2)
+
+    def make():
+        ctx = pbpic_amd.Context(cfg["nx"], cfg["ny"], cfg["dx"], cfg["dy"], cfg["dt"],
+                                cfg["cva"], cfg["mime"], cfg["in0"], precision=precision,
+                                sort_every=50, deposit_rho=False, ybound="conducting")
+        si = ctx.add_species(1, n)
+        se = ctx.add_species(-1, n)
+        ctx.init_harris(si, se, **h)
+        ctx.countingsort()
+        ctx.sync()
+        return ctx
+
+    ctx = make()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ctx.step(nsteps)
+    ctx.sync()
+    t_fwd = time.perf_counter() - t0
+    ctx.close()
+    torch.cuda.empty_cache()
+
+    ctx = make()
+    ctx.set_loss_region(configs.loss_region(cfg))
+    pdt = torch.float64 if precision == 0 else torch.float32
+    fg = torch.zeros(6, cfg["ny"], cfg["nx"], dtype=torch.float64, device="cuda")
+    pg = [torch.zeros(n, dtype=pdt, device="cuda") for _ in range(2) for _ in range(5)]
+    free0 = torch.cuda.mem_get_info()[0]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    loss = ctx.adjoint_forward(nsteps, ckpt)
+    ctx.sync()
+    t1 = time.perf_counter()
+    ctx.adjoint_backward([fg[k].data_ptr() for k in range(6)], [t.data_ptr() for t in pg])
+    ctx.sync()
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    ctx.close()
+    gbz = fg[5].cpu().numpy()
+    del fg, pg
+    torch.cuda.empty_cache()
+    return dict(particles=2 * n, steps=nsteps, ckpt_every=ckpt, forward_s=t_fwd,
+                ckpt_forward_s=t1 - t0, adjoint_s=t2 - t1, ratio=(t2 - t1) / t_fwd,
+                ratio_total=(t2 - t0) / t_fwd, loss=loss,
+                dloss_dbz_l2=float(np.sqrt((gbz ** 2).sum())),
+                dloss_dbz_maxabs=float(np.abs(gbz).max()),
+                device_free_gb_before=free0 / 1e9)
+
+
 def cpu_baseline(budget_s, seed):
     """The oracle (CPU restatement of pic2d.f90, AoS `particle` records in the
     reference's default FP32) on a bounded sample of the same workload: the
@@ -240,6 +304,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-harris", action="store_true")
+    ap.add_argument("--no-c4", action="store_true",
+                    help="skip the 1024x1024 Harris adjoint leg (about 240 GB of HBM)")
     ap.add_argument("--seed", type=int, default=20260101)
     ap.add_argument("--calib-copy-gib", type=float, default=0.0,
                     help="first run one plain device copy of this many GiB (a known-size "
@@ -319,6 +385,14 @@ def main():
                               **adj)
         # the larger of the forms' adjoint-sweep / plain-forward time ratios
         out["adjoint_forward_ratio"] = max(a["ratio"] for a in adj.values())
+        if not args.no_c4:
+            log("[bench] adjoint leg: Harris 1024x1024, 1000 steps, split")
+            c4 = adjoint_c4_leg(args.seed)
+            log(f"[bench] adjoint C4: {c4}")
+            out["adjoint_c4"] = dict(workload="differentiable PIC, large: Harris sheet 1024x1024 "
+                                              "(+2 guard rows), conducting y walls, 64 ppc/species, "
+                                              "1000 steps, checkpoints every 50, loss = J.E over the "
+                                              "central 4x2 d_i, split precision", **c4)
     if not args.no_harris:
         hr = {}
         for f in forms:

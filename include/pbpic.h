@@ -401,6 +401,12 @@ int pbpic_set_field_device(pbpic_ctx *ctx, int32_t which, const double *plane);
  * after the forward.                                                         */
 int pbpic_adjoint_forward(pbpic_ctx *ctx, int32_t nsteps, int32_t ckpt_every,
                           double *loss);
+
+/* Restricts the adjoint's loss to a region: weights is an nx*ny host plane
+ * (x fastest) multiplying each cell's J.(E^n+E^{n+1})/2 term -- e.g. 1 inside
+ * a diffusion region and 0 elsewhere; NULL restores the whole domain.  The
+ * diagnostics PBPIC_D_JE_* stay unweighted.  Discards a pending forward.    */
+int pbpic_set_loss_region(pbpic_ctx *ctx, const double *weights);
 int pbpic_adjoint_backward(pbpic_ctx *ctx, double *const *field_grads,
                            void *const *particle_grads);
 

@@ -185,8 +185,9 @@ def fieldsolve(P, E, B, J):
     return E1, B1
 
 
-def run(P, E, B, species, nsteps, compat=7, sort_every=0, step0=0):
-    """nsteps steps; returns (loss = time-integrated J.E, final state)."""
+def run(P, E, B, species, nsteps, compat=7, sort_every=0, step0=0, loss_mask=None):
+    """nsteps steps; returns (loss = time-integrated J.E, final state).
+    loss_mask: (ny, nx) weights of each cell's J.E term (None: all 1)."""
     nx, ny = P["nx"], P["ny"]
     w = P["dt"] * P["mime"] / (P["cva"] * P["cva"])
     loss = torch.zeros((), dtype=torch.float64)
@@ -198,7 +199,10 @@ def run(P, E, B, species, nsteps, compat=7, sort_every=0, step0=0):
         if P.get("nyp"):
             Jp = torch.stack([Jp[0], Jp[1], fold_jz(Jp[2], P["nyp"])])
         E1, B1 = fieldsolve(P, E, B, Jp)
-        loss = loss + w * (Jp * (E + E1) * 0.5).sum()
+        je = Jp * (E + E1) * 0.5
+        if loss_mask is not None:
+            je = je * loss_mask
+        loss = loss + w * je.sum()
         E, B = E1, B1
         if sort_every and (step0 + n + 1) % sort_every == 0:
             new = []

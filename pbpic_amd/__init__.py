@@ -123,6 +123,7 @@ def load_library(path=None):
         "pbpic_set_particles_device": (i32, [vp, i32, i64, vp, vp, vp, vp, vp, vp]),
         "pbpic_set_field_device": (i32, [vp, i32, vp]),
         "pbpic_adjoint_forward": (i32, [vp, i32, i32, pdbl]),
+        "pbpic_set_loss_region": (i32, [vp, pdbl]),
         "pbpic_adjoint_backward": (i32, [vp, P(vp), P(vp)]),
     }
     for name, (res, args) in sig.items():
@@ -417,6 +418,17 @@ class Context:
         self._chk(self._lib.pbpic_adjoint_forward(self._h, int(nsteps), int(ckpt_every),
                                                   ctypes.byref(out)))
         return out.value
+
+    def set_loss_region(self, weights=None):
+        """Weights (ny, nx) of each cell's J.E term in the adjoint's loss
+        (None: the whole domain)."""
+        if weights is None:
+            self._chk(self._lib.pbpic_set_loss_region(self._h, None))
+            return
+        a = _c(weights, np.float64)
+        if a.shape != (self.ny, self.nx):
+            raise ValueError(f"weights must have shape (ny, nx)={(self.ny, self.nx)}")
+        self._chk(self._lib.pbpic_set_loss_region(self._h, _ptr(a, ctypes.c_double)))
 
     def adjoint_backward(self, field_grad_ptrs, particle_grad_ptrs):
         """Writes d(loss)/d(initial state) into caller device buffers: 6 field

@@ -54,7 +54,31 @@ def harris_reference(nx=256, ny=256, ppg=64):
                 in0=1.0 / 291.0, ybound="periodic", harris=harris, steps=30)
 
 
-def harris_c1(nx=1024, ny=512, ppc=64):
+def harris_c4(nx=1024, ny=1024, ppc=64):
+    """The large differentiable configuration: the Harris sheet of harris_c1
+    (same physics and cell size, 0.025 d_i) on nx x ny = 1024 x 1024 domain
+    cells, i.e. 25.6 x 25.6 d_i; the adjoint's loss is J.E over the central
+    4 x 2 d_i diffusion region (loss_region), 1000 steps, checkpoints every
+    50."""
+    p = harris_c1(nx, ny, ppc, lx=0.025 * nx, ly=0.025 * ny)
+    p.update(steps=1000, ckpt_every=50)
+    return p
+
+
+def loss_region(cfg, wx=4.0, wy=2.0):
+    """Weights (ny, nx) of the loss: 1 for the cells whose centre lies in the
+    central wx x wy box (d_i), 0 elsewhere (guard rows included)."""
+    import numpy as np
+    nx, ny = cfg["nx"], cfg["ny"]
+    nyp = ny - 2 if cfg.get("ybound") == "conducting" else ny
+    x = (np.arange(nx) + 0.5) * cfg["dx"] - 0.5 * nx * cfg["dx"]
+    y = (np.arange(ny) + 0.5) * cfg["dy"] - 0.5 * nyp * cfg["dy"]
+    m = (np.abs(x)[None, :] < 0.5 * wx) & (np.abs(y)[:, None] < 0.5 * wy)
+    m[nyp:] = False
+    return m.astype(np.float64)
+
+
+def harris_c1(nx=1024, ny=512, ppc=64, lx=25.6, ly=12.8):
     """Harris current sheet with conducting y walls: nx x ny domain cells over
     25.6 x 12.8 d_i (the lattice gets ny + 2 rows: two guard rows), half
     thickness 0.5 d_i, mass ratio 25, Ti/Te = 5, background 0.2 n0,
@@ -67,7 +91,6 @@ def harris_c1(nx=1024, ny=512, ppc=64):
     cell at n0)."""
     mime, ti_te, L, nb = MIME, 5.0, 0.5, 0.2
     cva = 2.0 * math.sqrt(mime)
-    lx, ly = 25.6, 12.8
     dx, dy = lx / nx, ly / ny
     te = 0.5 / (1.0 + ti_te)
     ti = ti_te * te

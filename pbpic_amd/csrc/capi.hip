@@ -46,6 +46,7 @@ struct pbpic_ctx {
     int32_t *diag_counts = nullptr;   // spectrum / number-density scratch
     double *filt[3] = {nullptr, nullptr, nullptr};   // filter increments of E
     int32_t *big = nullptr;    // deferred push tiles (count + list)
+    double *lmask = nullptr;   // loss-region weights of the adjoint (nx*ny) or null
     int64_t big_cap = 0;
     bool gauss_first = true;
     std::vector<Species> sp;
@@ -175,6 +176,7 @@ FdtdParams fdtd_params(pbpic_ctx *c)
     F.bx = c->B[0]; F.by = c->B[1]; F.bz = c->B[2];
     F.jx = c->J[0]; F.jy = c->J[1]; F.jz = c->J[2];
     F.fex = c->filt[0]; F.fey = c->filt[1]; F.fez = c->filt[2];
+    F.lmask = c->lmask;
     F.acc = c->acc;
     return F;
 }
@@ -403,6 +405,13 @@ double je_total(pbpic_ctx *c, const double *a)
     return a[ACC_JE_INT] + a[ACC_JE_STEP] * (c->p.mime / (c->p.cva * c->p.cva)) * c->p.dt;
 }
 
+// The adjoint's loss: J.E weighted by the loss region when one is set.
+double loss_total(pbpic_ctx *c, const double *a)
+{
+    if (!c->lmask) return je_total(c, a);
+    return a[ACC_JEM_INT] + a[ACC_JEM_STEP] * (c->p.mime / (c->p.cva * c->p.cva)) * c->p.dt;
+}
+
 void adj_free(pbpic_ctx *c)
 {
     for (auto p : c->adj.ckpts) (void)hipFree(p);
@@ -599,7 +608,7 @@ int pbpic_ctx_destroy(pbpic_ctx *c)
     for (int k = 0; k < 4; k++) f(c->J[k]);
     f(c->NF);
     f(c->acc); f(c->err); f(c->rho_scratch); f(c->gauss0); f(c->diag_counts);
-    f(c->filt[0]); f(c->filt[1]); f(c->filt[2]); f(c->big);
+    f(c->filt[0]); f(c->filt[1]); f(c->filt[2]); f(c->big); f(c->lmask);
     for (auto &S : c->sp)
         for (int b = 0; b < 2; b++) {
             f(S.cell[b]); f(S.ox[b]); f(S.oy[b]); f(S.ux[b]); f(S.uy[b]); f(S.uz[b]); f(S.pid[b]);
@@ -1201,7 +1210,7 @@ int pbpic_adjoint_forward(pbpic_ctx *c, int32_t nsteps, int32_t ckpt, double *lo
     memcpy(c->adj.acc_after, a1, sizeof a1);
     c->adj.nsteps = nsteps;
     c->adj.ckpt = ckpt;
-    c->adj.loss = je_total(c, a1) - je_total(c, a0);
+    c->adj.loss = loss_total(c, a1) - loss_total(c, a0);
     c->adj.valid = true;
     if (loss) *loss = c->adj.loss;
     return PBPIC_OK;
@@ -1219,11 +1228,16 @@ int pbpic_adjoint_backward(pbpic_ctx *c, double *const *fg, void *const *pg)
     // one allocation for the tape and the adjoint buffers
     const size_t tape_states = (size_t)(K + 1) * L.total;
     const size_t tape_j = (size_t)K * 3 * al256(rz * nc);
-    const size_t tape_perm = (size_t)K * al256(4 * (size_t)ntot + 256 * ns);
+    // permutations only for the steps that sort: at most K/sort_every + 1 per segment
+    const int nperm = c->p.sort_every > 0 ? std::min(K, K / c->p.sort_every + 1) : 0;
+    const size_t tape_perm = (size_t)nperm * al256(4 * (size_t)ntot + 256 * ns);
     const size_t fadj = 18 * al256(8 * nc);                    // Ebar, Bbar, E', Jbar, node x6
     const size_t padj = 5 * al256(rz * (size_t)ntot + 256 * ns) + 5 * al256(rz * (size_t)nmax);
+    int64_t atiles = 0;
+    for (auto &S : c->sp) atiles += adj_tiles(S.n);
+    const size_t abig = al256(sizeof(int32_t) * (size_t)(atiles + 1));
     char *base = nullptr;
-    hipError_t e = hipMalloc(&base, tape_states + tape_j + tape_perm + fadj + padj);
+    hipError_t e = hipMalloc(&base, tape_states + tape_j + tape_perm + fadj + padj + abig);
     if (e != hipSuccess) return fail(c, PBPIC_ERR_NOMEM, "adjoint tape allocation: %s", hipGetErrorString(e));
     int rc = PBPIC_OK;
     char *p = base;
@@ -1232,13 +1246,15 @@ int pbpic_adjoint_backward(pbpic_ctx *c, double *const *fg, void *const *pg)
     for (int k = 0; k <= K; k++) tstate[k] = take(L.total);
     std::vector<std::array<void *, 3>> tj(K);
     for (int k = 0; k < K; k++) for (int q = 0; q < 3; q++) tj[k][q] = take(rz * nc);
-    std::vector<std::vector<uint32_t *>> tperm(K, std::vector<uint32_t *>(ns));
-    for (int k = 0; k < K; k++) for (int s = 0; s < ns; s++) tperm[k][s] = (uint32_t *)take(4 * (size_t)c->sp[s].n);
+    std::vector<std::vector<uint32_t *>> tperm(nperm, std::vector<uint32_t *>(ns));
+    for (int k = 0; k < nperm; k++) for (int s = 0; s < ns; s++) tperm[k][s] = (uint32_t *)take(4 * (size_t)c->sp[s].n);
+    std::vector<int> pslot(K, -1);
     double *fa[18];
     for (int q = 0; q < 18; q++) fa[q] = (double *)take(8 * nc);
     std::vector<std::array<void *, 5>> pa(ns), tmp(1);
     for (int s = 0; s < ns; s++) for (int q = 0; q < 5; q++) pa[s][q] = take(rz * (size_t)c->sp[s].n);
     for (int q = 0; q < 5; q++) tmp[0][q] = take(rz * (size_t)nmax);
+    int32_t *adj_big = (int32_t *)take(sizeof(int32_t) * (size_t)(atiles + 1));
     std::vector<bool> sorted(K);
     auto run = [&]() -> int {
         HIPCHK(c, hipMemsetAsync(fa[0], 0, (size_t)(p - (char *)fa[0]), c->st));
@@ -1249,17 +1265,26 @@ int pbpic_adjoint_backward(pbpic_ctx *c, double *const *fg, void *const *pg)
             int r = state_copy(c, L, c->adj.ckpts[seg], false);
             if (r) return r;
             c->steps = c->adj.step0 + s0;
+            int used = 0;
             for (int k = 0; k < kk; k++) {
                 if ((r = state_copy(c, L, tstate[k], true))) return r;
                 bool so = false;
-                if ((r = step_impl(c, tj[k].data(), tperm[k].data(), &so))) return r;
+                const bool will = c->p.sort_every > 0 && (c->steps + 1) % c->p.sort_every == 0;
+                uint32_t *const *pd = nullptr;
+                pslot[k] = -1;
+                if (will) {
+                    if (used >= nperm) return fail(c, PBPIC_ERR_STATE, "adjoint permutation tape overflow");
+                    pslot[k] = used;
+                    pd = tperm[used++].data();
+                }
+                if ((r = step_impl(c, tj[k].data(), pd, &so))) return r;
                 sorted[k] = so;
             }
             if ((r = state_copy(c, L, tstate[kk], true))) return r;
             for (int k = kk - 1; k >= 0; k--) {
                 if (sorted[k]) {
                     for (int s = 0; s < ns; s++) {
-                        HIPCHK(c, launch_unpermute(tperm[k][s], c->sp[s].n, c->p.precision,
+                        HIPCHK(c, launch_unpermute(tperm[pslot[k]][s], c->sp[s].n, c->p.precision,
                                                    pa[s].data(), tmp[0].data(), c->st));
                         for (int q = 0; q < 5; q++)
                             HIPCHK(c, hipMemcpyAsync(pa[s][q], tmp[0][q], rz * c->sp[s].n,
@@ -1280,11 +1305,15 @@ int pbpic_adjoint_backward(pbpic_ctx *c, double *const *fg, void *const *pg)
                 F.e1x = state_field(c, L, tstate[k + 1], 0); F.e1y = state_field(c, L, tstate[k + 1], 1);
                 F.e1z = state_field(c, L, tstate[k + 1], 2);
                 F.jx = tj[k][0]; F.jy = tj[k][1]; F.jz = tj[k][2];
+                F.lmask = c->lmask;
                 HIPCHK(c, launch_fdtd_adj(F, c->p.precision, c->st));
                 AdjParams A{};
                 A.nx = c->p.nx; A.ny = c->p.ny; A.compat = c->p.compat;
                 A.nyp = walls(c) ? nyp_of(c) : 0;
-                A.maxw = c->p.precision == PBPIC_PREC_FP64 ? 640 : 1280;
+                A.tv = (c->p.flags & PBPIC_FLAG_PUSH_ORDER) ? push_vec(c->p.precision) : 0;
+                A.dny.init((uint32_t)c->p.ny);
+                // windows of two workgroups per CU (72 or 96 B of LDS per node)
+                A.maxw = c->p.precision == PBPIC_PREC_FP64 ? 768 : 1024;
                 A.nspecies = ns;
                 A.dtcva = c->dtcva; A.idx = c->idx; A.idy = c->idy; A.cva = c->p.cva;
                 A.ex = state_field(c, L, tstate[k], 0); A.ey = state_field(c, L, tstate[k], 1);
@@ -1292,6 +1321,7 @@ int pbpic_adjoint_backward(pbpic_ctx *c, double *const *fg, void *const *pg)
                 A.by = state_field(c, L, tstate[k], 4); A.bz = state_field(c, L, tstate[k], 5);
                 A.jadj[0] = fa[9]; A.jadj[1] = fa[10]; A.jadj[2] = fa[11];
                 A.nadj = fa[12];
+                A.big = adj_big;
                 for (int s = 0; s < ns; s++) {
                     Species &S = c->sp[s];
                     AdjSpecies &T = A.sp[s];
@@ -1333,6 +1363,21 @@ int pbpic_adjoint_backward(pbpic_ctx *c, double *const *fg, void *const *pg)
     (void)hipStreamSynchronize(c->st);
     (void)hipFree(base);
     return rc;
+}
+
+int pbpic_set_loss_region(pbpic_ctx *c, const double *weights)
+{
+    if (!c) return PBPIC_ERR_ARG;
+    adj_free(c);
+    if (!weights) {
+        if (c->lmask) { HIPCHK(c, hipStreamSynchronize(c->st)); (void)hipFree(c->lmask); }
+        c->lmask = nullptr;
+        return PBPIC_OK;
+    }
+    const size_t nc = (size_t)ncell(c);
+    if (!c->lmask) HIPCHK(c, dalloc(&c->lmask, nc * 8));
+    HIPCHK(c, hipMemcpy(c->lmask, weights, nc * 8, hipMemcpyHostToDevice));
+    return PBPIC_OK;
 }
 
 int pbpic_profile(pbpic_ctx *c, int32_t enable)

@@ -29,9 +29,7 @@ namespace {
 
 constexpr int ADJ_THREADS = 256;
 constexpr int ADJ_ROUNDS = 8;
-constexpr int ADJ_GROUP = 8;
-constexpr int ADJ_RUN = ADJ_ROUNDS * ADJ_GROUP;         // 64
-constexpr int ADJ_TILE = ADJ_THREADS * ADJ_ROUNDS;      // 2048
+constexpr int ADJ_TILE = ADJ_THREADS * ADJ_ROUNDS;      // 2048, half a push tile
 
 __device__ __forceinline__ int a_xcd(int b, int nb)
 {
@@ -44,6 +42,12 @@ __device__ __forceinline__ int a_wrap(int g, int n)
     while (g < 0) g += n;
     while (g >= n) g -= n;
     return g;
+}
+
+// LDS bytes of the node-field window, rounded up to 8-byte alignment.
+__host__ __device__ constexpr size_t adj_nf_bytes(size_t r, int maxw)
+{
+    return ((size_t)maxw * 6 * r + 7) & ~(size_t)7;
 }
 
 template <typename R>
@@ -112,29 +116,48 @@ __global__ __launch_bounds__(ADJ_THREADS, 2) void push_adj_kernel(AdjParams P)
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ int red[4][4];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int t = a_xcd(blockIdx.x, gridDim.x);
+    int t;
+    if (P.pass == 2) {
+        if ((int)blockIdx.x >= P.big[0]) return;     // deferred tiles only
+        t = P.big[1 + blockIdx.x];
+    } else {
+        t = a_xcd(blockIdx.x, gridDim.x);
+    }
     int s = 0;
     while (s + 1 < P.nspecies && t >= P.sp[s + 1].tile0) s++;
     const AdjSpecies S = P.sp[s];
     const int64_t tstart = (int64_t)(t - S.tile0) * ADJ_TILE;
-    const int G = wave * 8 + (lane >> 3), gp = lane & 7;
-    const int64_t pbase = tstart + (int64_t)G * ADJ_RUN + gp;
+    // Each thread takes 8 consecutive ranks of the sort order -- mostly one
+    // cell, so its node adjoints add up in registers and reach LDS once per
+    // cell.  In push order (full 4096-particle tiles laid out by tile_rank)
+    // this tile is half k of push tile T; ranks l*64 + 32k + [0, 32) of it,
+    // thread tid taking l = tid/4 and 8 ranks from 32k + 8(tid%4).
+    const int64_t T0 = (tstart / PUSH_TILE) * PUSH_TILE;
+    const bool porder = P.tv > 0 && T0 + PUSH_TILE <= S.n;
+    auto slot_of = [&](int r) -> int64_t {
+        if (!porder) return tstart + (int64_t)tid * ADJ_ROUNDS + r;
+        const int k = (int)((tstart - T0) / ADJ_TILE);
+        const int rank = (tid >> 2) * 64 + 32 * k + (tid & 3) * ADJ_ROUNDS + r;
+        const int V = P.tv, l = rank >> 6, c = (rank & 63) / V, j = rank % V;
+        return T0 + (int64_t)c * 64 * V + l * V + j;
+    };
     const int nx = P.nx, ny = P.ny, nxh = nx >> 1, nyh = ny >> 1;
 
     const int c00 = S.cell[tstart];
     const bool c00ok = (unsigned)c00 < (unsigned)(nx * ny);
-    const int x0 = c00ok ? c00 / ny : 0, y0 = c00ok ? c00 - (c00 / ny) * ny : 0;
+    const int x0 = c00ok ? (int)P.dny.div31((uint32_t)c00) : 0;
+    const int y0 = c00ok ? c00 - x0 * ny : 0;
     int cl[ADJ_ROUNDS];
     int mnx = 1 << 30, mxx = -(1 << 30), mny = 1 << 30, mxy = -(1 << 30);
 #pragma unroll
     for (int r = 0; r < ADJ_ROUNDS; r++) {
-        const int64_t i = pbase + r * ADJ_GROUP;
+        const int64_t i = slot_of(r);
         cl[r] = -1;
         if (i < S.n) {
             const int c = S.cell[i];
             if ((unsigned)c >= (unsigned)(nx * ny)) continue;
             cl[r] = c;
-            const int cx = c / ny, cy = c - cx * ny;
+            const int cx = (int)P.dny.div31((uint32_t)c), cy = c - cx * ny;
             int rx = cx - x0, ry = cy - y0;
             if (rx > nxh) rx -= nx; else if (rx < -nxh) rx += nx;
             if (ry > nyh) ry -= ny; else if (ry < -nyh) ry += ny;
@@ -159,9 +182,17 @@ __global__ __launch_bounds__(ADJ_THREADS, 2) void push_adj_kernel(AdjParams P)
     const int wx0 = mnx, wy0 = mny;
     const int W = mxx - mnx + 2, H = mxy - mny + 2;
     const bool inlds = mnx <= mxx && (int64_t)W * H <= P.maxw;
+    if (P.pass == 1 && !inlds && mnx <= mxx && (int64_t)W * H <= P.maxw_big) {
+        if (tid == 0) P.big[1 + atomicAdd(P.big, 1)] = t;   // block-uniform exit
+        return;
+    }
     const int WH = inlds ? W * H : 0;
+    const bool interior = mnx <= mxx && x0 + mnx >= 1 && x0 + mxx <= nx - 3 && y0 + mny >= 1 &&
+                          y0 + mxy <= ny - 3;
     R *nf = (R *)smem;                      // [WH][6] node fields
-    R *na = nf + (size_t)6 * P.maxw;        // [6][WH] node adjoint accumulators
+    // [6][WH] node adjoint accumulators, FP64 in both forms: LDS FP64 atomic
+    // adds run at about ten times the rate of FP32 ones on this hardware
+    double *na = (double *)(smem + adj_nf_bytes(sizeof(R), P.maxw));
     if (inlds) {
         for (int k = tid; k < WH; k += ADJ_THREADS) {
             const int wj = k / W, wi = k - wj * W;
@@ -170,7 +201,7 @@ __global__ __launch_bounds__(ADJ_THREADS, 2) void push_adj_kernel(AdjParams P)
 #pragma unroll
             for (int q = 0; q < 6; q++) nf[(size_t)k * 6 + q] = (R)f[q];
         }
-        for (int k = tid; k < 6 * WH; k += ADJ_THREADS) na[k] = (R)0;
+        for (int k = tid; k < 6 * WH; k += ADJ_THREADS) na[k] = 0.0;
     }
     __syncthreads();
 
@@ -180,14 +211,39 @@ __global__ __launch_bounds__(ADJ_THREADS, 2) void push_adj_kernel(AdjParams P)
     const double *ja = P.jadj[0], *jb = P.jadj[1], *jc = P.jadj[2];
     double *nadj = P.nadj;
     const size_t NC = (size_t)nx * ny;
+    // node adjoints of the current cell, flushed when the cell changes
+    R nacc[6][4];
+    int acell = -1, abase = 0;
+    size_t ag[4] = {0, 0, 0, 0};
+    auto flush = [&]() {
+        if (acell < 0) return;
+        if (inlds) {
+#pragma unroll
+            for (int q = 0; q < 6; q++) {
+                double *pq = na + (size_t)q * WH;
+                a_lds_add(pq + abase, (double)nacc[q][0]);
+                a_lds_add(pq + abase + 1, (double)nacc[q][1]);
+                a_lds_add(pq + abase + W, (double)nacc[q][2]);
+                a_lds_add(pq + abase + W + 1, (double)nacc[q][3]);
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < 6; q++) {
+                double *pq = nadj + q * NC;
+#pragma unroll
+                for (int m = 0; m < 4; m++) unsafeAtomicAdd(pq + ag[m], (double)nacc[q][m]);
+            }
+        }
+        acell = -1;
+    };
 #pragma unroll 1
     for (int r = 0; r < ADJ_ROUNDS; r++) {
         const int c = cl[r];
         if (c < 0) continue;
-        const int64_t i = pbase + r * ADJ_GROUP;
+        const int64_t i = slot_of(r);
         const R ox = ((const R *)S.ox)[i], oy = ((const R *)S.oy)[i];
         const R u[3] = {((const R *)S.ux)[i], ((const R *)S.uy)[i], ((const R *)S.uz)[i]};
-        const int cx = c / ny, cy = c - cx * ny;
+        const int cx = (int)P.dny.div31((uint32_t)c), cy = c - cx * ny;
         int rx = cx - x0, ry = cy - y0;
         if (rx > nxh) rx -= nx; else if (rx < -nxh) rx += nx;
         if (ry > nyh) ry -= ny; else if (ry < -nyh) ry += ny;
@@ -260,7 +316,11 @@ __global__ __launch_bounds__(ADJ_THREADS, 2) void push_adj_kernel(AdjParams P)
         const R A = fabs(dimid), Bq = fabs(djmid);
         const R quz = qr * vz;
         // ---- reverse: deposit = gather of the current adjoint at the targets
+        // deposit targets lie within one cell below/left and two above/right
+        // of the cell: no periodic wrap for a tile away from every edge
+        const size_t g0 = (size_t)cy * nx + cx;
         auto at = [&](int ddx, int ddy) -> size_t {
+            if (interior) return g0 + (ptrdiff_t)ddy * nx + ddx;
             return (size_t)a_wrap(cy + ddy, ny) * nx + a_wrap(cx + ddx, nx);
         };
         const R l1 = (R)ja[at(0, 0)], l2 = (R)ja[at(0, 1)];
@@ -336,27 +396,23 @@ __global__ __launch_bounds__(ADJ_THREADS, 2) void push_adj_kernel(AdjParams P)
         }
         xo += dFx;
         yo += dFy;
-        if (inlds) {
-            const int base = ly * W + lx;
+        if (c != acell) {
+            flush();
+            acell = c;
+            abase = ly * W + lx;
+            ag[0] = (size_t)cy * nx + cx; ag[1] = (size_t)cy * nx + cx1;
+            ag[2] = (size_t)cy1 * nx + cx; ag[3] = (size_t)cy1 * nx + cx1;
 #pragma unroll
-            for (int q = 0; q < 6; q++) {
-                R *pq = na + (size_t)q * WH;
-                a_lds_add(pq + base, s00 * bF[q]);
-                a_lds_add(pq + base + 1, s10 * bF[q]);
-                a_lds_add(pq + base + W, s01 * bF[q]);
-                a_lds_add(pq + base + W + 1, s11 * bF[q]);
-            }
-        } else {
-            const size_t g00 = (size_t)cy * nx + cx, g10 = (size_t)cy * nx + cx1;
-            const size_t g01 = (size_t)cy1 * nx + cx, g11 = (size_t)cy1 * nx + cx1;
+            for (int q = 0; q < 6; q++)
 #pragma unroll
-            for (int q = 0; q < 6; q++) {
-                double *pq = nadj + q * NC;
-                unsafeAtomicAdd(pq + g00, (double)(s00 * bF[q]));
-                unsafeAtomicAdd(pq + g10, (double)(s10 * bF[q]));
-                unsafeAtomicAdd(pq + g01, (double)(s01 * bF[q]));
-                unsafeAtomicAdd(pq + g11, (double)(s11 * bF[q]));
-            }
+                for (int m = 0; m < 4; m++) nacc[q][m] = (R)0;
+        }
+#pragma unroll
+        for (int q = 0; q < 6; q++) {
+            nacc[q][0] += s00 * bF[q];
+            nacc[q][1] += s10 * bF[q];
+            nacc[q][2] += s01 * bF[q];
+            nacc[q][3] += s11 * bF[q];
         }
         ((R *)P.xa[s])[ia] = xo;
         ((R *)P.ya[s])[ia] = yo;
@@ -364,6 +420,7 @@ __global__ __launch_bounds__(ADJ_THREADS, 2) void push_adj_kernel(AdjParams P)
         ((R *)P.uya[s])[ia] = bu[1];
         ((R *)P.uza[s])[ia] = bu[2];
     }
+    flush();
     __syncthreads();
     if (inlds) {
         for (int k = tid; k < WH; k += ADJ_THREADS) {
@@ -371,8 +428,8 @@ __global__ __launch_bounds__(ADJ_THREADS, 2) void push_adj_kernel(AdjParams P)
             const size_t g = (size_t)a_wrap(y0 + wy0 + wj, ny) * nx + a_wrap(x0 + wx0 + wi, nx);
 #pragma unroll
             for (int q = 0; q < 6; q++) {
-                const R vq = na[(size_t)q * WH + k];
-                if (vq != (R)0) unsafeAtomicAdd(nadj + q * NC + g, (double)vq);
+                const double vq = na[(size_t)q * WH + k];
+                if (vq != 0.0) unsafeAtomicAdd(nadj + q * NC + g, vq);
             }
         }
     }
@@ -430,16 +487,17 @@ __global__ void fdtd_adj_a(AdjFieldParams P, int mode)
         const double *E0[3] = {P.e0x, P.e0y, P.e0z}, *E1[3] = {P.e1x, P.e1y, P.e1z};
         double *Jb[3] = {P.jxa, P.jya, P.jza};
         double *Ep[3] = {P.etx, P.ety, P.etz};
+        const double wm = P.lmask ? P.w * P.lmask[k] : P.w;   // loss weight of this cell
         for (int q = 0; q < 3; q++) {
             double e;
             if (mode != 2) {
                 const double jr = (double)J[q][k];
-                e = Ea[q][k] + 0.5 * P.w * jr - P.dth * o[q];
+                e = Ea[q][k] + 0.5 * wm * jr - P.dth * o[q];
                 Ep[q][k] = e;
             } else {
                 e = Ep[q][k];
             }
-            if (mode != 1) Jb[q][k] = -P.dtc2 * P.in0 * e + 0.5 * P.w * (E0[q][k] + E1[q][k]);
+            if (mode != 1) Jb[q][k] = -P.dtc2 * P.in0 * e + 0.5 * wm * (E0[q][k] + E1[q][k]);
         }
     }
 }
@@ -519,7 +577,8 @@ __global__ void fdtd_adj_c(AdjFieldParams P)
         const RJ *J[3] = {(const RJ *)P.jx, (const RJ *)P.jy, (const RJ *)P.jz};
         double *Ea[3] = {P.exa, P.eya, P.eza};
         const double *Ep[3] = {P.etx, P.ety, P.etz};
-        for (int q = 0; q < 3; q++) Ea[q][k] = Ep[q][k] - P.dth * o[q] + 0.5 * P.w * (double)J[q][k];
+        const double wm = P.lmask ? P.w * P.lmask[k] : P.w;
+        for (int q = 0; q < 3; q++) Ea[q][k] = Ep[q][k] - P.dth * o[q] + 0.5 * wm * (double)J[q][k];
     }
 }
 
@@ -568,7 +627,24 @@ __global__ void unpermute_kernel(const uint32_t *perm, int64_t n, Ptr5 post, Ptr
 
 size_t adj_lds_bytes(int precision, int maxw)
 {
-    return (size_t)maxw * 12 * (precision == PBPIC_PREC_FP64 ? 8 : 4);
+    const size_t r = precision == PBPIC_PREC_FP64 ? 8 : 4;
+    return adj_nf_bytes(r, maxw) + (size_t)maxw * 6 * 8;
+}
+
+int64_t adj_tiles(int64_t n) { return (n + ADJ_TILE - 1) / ADJ_TILE; }
+
+static void launch_push_adj_once(const AdjParams &p, int precision, int t, hipStream_t st)
+{
+    const size_t lds = adj_lds_bytes(precision, p.maxw);
+    if (precision == PBPIC_PREC_FP64) {
+        (void)hipFuncSetAttribute((const void *)push_adj_kernel<double, false>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        hipLaunchKernelGGL((push_adj_kernel<double, false>), dim3(t), dim3(ADJ_THREADS), lds, st, p);
+    } else {
+        (void)hipFuncSetAttribute((const void *)push_adj_kernel<float, true>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        hipLaunchKernelGGL((push_adj_kernel<float, true>), dim3(t), dim3(ADJ_THREADS), lds, st, p);
+    }
 }
 
 hipError_t launch_push_adj(const AdjParams &p0, int precision, hipStream_t st)
@@ -580,15 +656,19 @@ hipError_t launch_push_adj(const AdjParams &p0, int precision, hipStream_t st)
         t += (int)((p.sp[s].n + ADJ_TILE - 1) / ADJ_TILE);
     }
     if (t == 0) return hipSuccess;
-    const size_t lds = adj_lds_bytes(precision, p.maxw);
-    if (precision == PBPIC_PREC_FP64) {
-        (void)hipFuncSetAttribute((const void *)push_adj_kernel<double, false>,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        hipLaunchKernelGGL((push_adj_kernel<double, false>), dim3(t), dim3(ADJ_THREADS), lds, st, p);
-    } else {
-        (void)hipFuncSetAttribute((const void *)push_adj_kernel<float, true>,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        hipLaunchKernelGGL((push_adj_kernel<float, true>), dim3(t), dim3(ADJ_THREADS), lds, st, p);
+    // the second pass gets the window of one workgroup per CU (150 KiB)
+    p.maxw_big = (int)((150 * 1024) / adj_lds_bytes(precision, 1));
+    p.pass = p.big && p.maxw_big > p.maxw ? 1 : 0;
+    if (p.pass) {
+        hipError_t e = launch_fill_u32((uint32_t *)p.big, 1, 0u, st);
+        if (e != hipSuccess) return e;
+    }
+    launch_push_adj_once(p, precision, t, st);
+    if (p.pass) {
+        AdjParams b = p;
+        b.pass = 2;
+        b.maxw = p.maxw_big;
+        launch_push_adj_once(b, precision, t, st);
     }
     return hipGetLastError();
 }

@@ -50,7 +50,7 @@ __global__ __launch_bounds__(FT_THREADS) void fdtd_kernel(FdtdParams P)
 {
     __shared__ double se[3][FS_N];
     __shared__ double sb[3][FS_N];
-    __shared__ double jred[FT_THREADS / 64];
+    __shared__ double jred[2][FT_THREADS / 64];
     const int tid = threadIdx.x;
     const int nx = P.nx, ny = P.ny;
     const int ntx = (nx + FT_X - 1) / FT_X;
@@ -93,7 +93,7 @@ __global__ __launch_bounds__(FT_THREADS) void fdtd_kernel(FdtdParams P)
     const double dtc2 = P.dtc2, in0 = P.in0;
     const float dtc2f = (float)P.dtc2, in0f = (float)P.in0;
     const RJ *jx = (const RJ *)P.jx, *jy = (const RJ *)P.jy, *jz = (const RJ *)P.jz;
-    double je = 0;
+    double je = 0, jem = 0;
     for (int k = tid; k < (FS_X - 4) * (FS_Y - 4); k += FT_THREADS) {
         const int v = k / (FS_X - 4) + 2, u = k - (v - 2) * (FS_X - 4) + 2;
         const int c = v * FS_X + u;
@@ -119,8 +119,11 @@ __global__ __launch_bounds__(FT_THREADS) void fdtd_kernel(FdtdParams P)
         if (P.nyp && (gy == 0 || gy == P.nyp)) exn = 0.0;   // tangential E on a wall
         const bool own = u >= FT_H && u < FT_H + FT_X && v >= FT_H && v < FT_H + FT_Y &&
                          x0 + u < nx && y0 + v < ny;
-        if (own)
-            je += jxr * 0.5 * (exo + exn) + jyr * 0.5 * (eyo + eyn) + jzr * 0.5 * (ezo + ezn);
+        if (own) {
+            const double jp = jxr * 0.5 * (exo + exn) + jyr * 0.5 * (eyo + eyn) + jzr * 0.5 * (ezo + ezn);
+            je += jp;
+            if (P.lmask) jem += P.lmask[g] * jp;
+        }
         if (P.fex) {
             // Marder / Cherenkov filters (:754-755), increments from E^n
             exn += P.fex[g]; eyn += P.fey[g]; ezn += P.fez[g];
@@ -179,10 +182,17 @@ __global__ __launch_bounds__(FT_THREADS) void fdtd_kernel(FdtdParams P)
     }
 
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) je += __shfl_xor(je, o);
-    if ((tid & 63) == 0) jred[tid >> 6] = je;
+    for (int o = 32; o > 0; o >>= 1) {
+        je += __shfl_xor(je, o);
+        jem += __shfl_xor(jem, o);
+    }
+    if ((tid & 63) == 0) { jred[0][tid >> 6] = je; jred[1][tid >> 6] = jem; }
     __syncthreads();
-    if (tid == 0) unsafeAtomicAdd(P.acc + ACC_JE_STEP, (jred[0] + jred[1]) + (jred[2] + jred[3]));
+    if (tid == 0) {
+        unsafeAtomicAdd(P.acc + ACC_JE_STEP, (jred[0][0] + jred[0][1]) + (jred[0][2] + jred[0][3]));
+        if (P.lmask)
+            unsafeAtomicAdd(P.acc + ACC_JEM_STEP, (jred[1][0] + jred[1][1]) + (jred[1][2] + jred[1][3]));
+    }
 }
 
 hipError_t launch_fdtd(const FdtdParams &p, int precision, hipStream_t st)
@@ -643,7 +653,10 @@ __global__ void acc_op_kernel(double *acc, int op, double a)
 {
     if (threadIdx.x != 0) return;
     if (op == 0) { acc[ACC_KE_E] = 0; acc[ACC_KE_I] = 0; }
-    if (op == 1) { acc[ACC_JE_INT] += acc[ACC_JE_STEP] * a; acc[ACC_JE_STEP] = 0; }
+    if (op == 1) {
+        acc[ACC_JE_INT] += acc[ACC_JE_STEP] * a; acc[ACC_JE_STEP] = 0;
+        acc[ACC_JEM_INT] += acc[ACC_JEM_STEP] * a; acc[ACC_JEM_STEP] = 0;
+    }
     if (op == 2) {
         acc[ACC_WE] = 0; acc[ACC_WB] = 0; acc[ACC_G_L2] = 0; acc[ACC_G_LINF] = 0;
         acc[ACC_G_DL2] = 0; acc[ACC_G_DLINF] = 0;

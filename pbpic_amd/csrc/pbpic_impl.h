@@ -51,6 +51,8 @@ enum : int {
     ACC_G_DL2 = 8,     // sum of squares of the residual change
     ACC_G_DLINF = 9,   // max of the residual change
     ACC_SPILL = 10,    // push tiles whose window did not fit in LDS (global atomics)
+    ACC_JEM_STEP = 11, // as ACC_JE_STEP with the loss-region weights
+    ACC_JEM_INT = 12,  // as ACC_JE_INT with the loss-region weights
     ACC_N = 16,
 };
 
@@ -133,6 +135,7 @@ struct FdtdParams {
     double *ex, *ey, *ez, *bx, *by, *bz;
     const void *jx, *jy, *jz;
     const double *fex, *fey, *fez;     // filter increments added to E^{n+1} (or null)
+    const double *lmask;               // loss-region weights per cell (or null)
     double *acc;
 };
 
@@ -159,7 +162,13 @@ struct AdjSpecies {
 
 struct AdjParams {
     int nx, ny, compat, maxw, nspecies;
+    int tv;                                      // push order vector width (0: cell order)
+    FastDiv dny;                                 // division by ny
     int nyp;                                     // conducting y walls (0: periodic)
+    // two-pass launch as PushParams: tiles whose window exceeds maxw but fits
+    // maxw_big are deferred to a second launch (big: count + tile list)
+    int maxw_big, pass;
+    int *big;
     double dtcva, idx, idy, cva;
     const double *ex, *ey, *ez, *bx, *by, *bz;   // fields at step n
     const double *jadj[3];                       // current adjoint (raw J)
@@ -178,6 +187,7 @@ struct AdjFieldParams {
     double *jxa, *jya, *jza;                     // current adjoint (out)
     const double *e0x, *e0y, *e0z, *e1x, *e1y, *e1z;   // E^n, E^{n+1}
     const void *jx, *jy, *jz;                    // J^{n+1/2} (raw)
+    const double *lmask;                         // loss-region weights (or null)
 };
 
 // Launchers (one per translation unit family).
@@ -248,6 +258,8 @@ struct SortScratch {
     int64_t cap = 0, status_cap = 0;
 };
 hipError_t launch_push_adj(const AdjParams &p, int precision, hipStream_t st);
+// Tiles of the adjoint push (2048 particles each) for n particles.
+int64_t adj_tiles(int64_t n);
 hipError_t launch_fdtd_adj(const AdjFieldParams &p, int precision, hipStream_t st);
 // Conducting y walls: transpose of the guard-row imaging applied to the field
 // adjoints (exa..bza) in place.

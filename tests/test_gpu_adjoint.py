@@ -18,7 +18,7 @@ def _case():
     return make_case(nx=24, ny=16, ppc=6, seed=4, efield=0.0, bfield=0.0, vth=(0.02, 0.1))
 
 
-def _ref_grads(case, A0, ud0, nsteps, ckpt_sort, compat):
+def _ref_grads(case, A0, ud0, nsteps, ckpt_sort, compat, mask=None):
     nx, ny = case["nx"], case["ny"]
     P = dict(nx=nx, ny=ny, dx=case["dx"], dy=case["dy"], dt=case["dt"], cva=case["cva"],
              mime=case["mime"], in0=case["in0"])
@@ -40,18 +40,21 @@ def _ref_grads(case, A0, ud0, nsteps, ckpt_sort, compat):
                 v = v + ud0
             d[k] = v.clone().requires_grad_(True)
         sps.append(d)
-    loss, *_ = tr.run(P, FE, FB, sps, nsteps, compat=compat, sort_every=ckpt_sort)
+    loss, *_ = tr.run(P, FE, FB, sps, nsteps, compat=compat, sort_every=ckpt_sort,
+                      loss_mask=None if mask is None else torch.tensor(mask))
     loss.backward()
     return loss.item(), F.grad.numpy(), [{k: sp[k].grad.numpy() for k in PART} for sp in sps]
 
 
-def _gpu_grads(case, A0, ud0, nsteps, ckpt, sort_every, compat, precision):
+def _gpu_grads(case, A0, ud0, nsteps, ckpt, sort_every, compat, precision, mask=None):
     import pbpic_amd
     from pbpic_amd.autograd import jdote
     nx, ny = case["nx"], case["ny"]
     sim = pbpic_amd.Context(nx, ny, case["dx"], case["dy"], case["dt"], case["cva"], case["mime"],
                             case["in0"], precision=precision, compat=compat, sort_every=sort_every,
                             deposit_rho=False, ybound=case.get("ybound", 0))
+    if mask is not None:
+        sim.set_loss_region(mask)
     dt = torch.float64 if precision == 0 else torch.float32
     species = []
     for q, s in zip((1, -1), case["species"]):
@@ -127,3 +130,24 @@ def test_adjoint_walls_matches_reverse_mode(gpu, precision):
     for s in range(2):
         for k in PART:
             assert rel_norm(gPg[s][k], gP[s][k]) < tol, (s, k)
+
+
+@pytest.mark.parametrize("walls", [False, True])
+def test_adjoint_loss_region(gpu, walls):
+    """The loss restricted to a region of cells (weights 1 inside a central
+    box, 0.5 on its border, 0 elsewhere), the way the large Harris
+    configuration restricts J.E to the diffusion region."""
+    print(f"adjoint loss region walls={walls}", flush=True)
+    case = make_case(nx=24, ny=16, ppc=6, seed=4, efield=0.0, bfield=0.0,
+                     vth=(0.05, 0.3) if walls else (0.02, 0.1), walls=walls)
+    mask = np.zeros((16, 24))
+    mask[5:11, 8:16] = 0.5
+    mask[6:10, 9:15] = 1.0
+    L, gF, gP = _ref_grads(case, 0.01, 0.05, 10, 5, 7, mask)
+    Lg, gFg, gPg = _gpu_grads(case, 0.01, 0.05, 10, 4, 5, 7, 0, mask)
+    assert abs(Lg - L) <= 1e-10 * abs(L)
+    for k in range(6):
+        assert rel_norm(gFg[k], gF[k]) < 1e-8, k
+    for s in range(2):
+        for k in PART:
+            assert rel_norm(gPg[s][k], gP[s][k]) < 1e-8, (s, k)
