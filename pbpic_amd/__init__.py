@@ -81,6 +81,15 @@ def load_library(path=None):
     p = path or LIB_PATH
     if not os.path.exists(p):
         raise PbpicError(-7, f"{p} is missing; build it with `python -m pbpic_amd.build`")
+    # One HIP runtime per process.  PyTorch wheels bundle their own
+    # libamdhip64.so.7 (same soname as ROCm's): loaded first, it is the one the
+    # library binds to.  Loaded after the library, it would be a second
+    # runtime beside /opt/rocm's, and device pointers, streams and contexts
+    # would not be shared between the two.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = ctypes.CDLL(p)
     P = ctypes.POINTER
     vp, i32, i64, dbl = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_double

@@ -50,6 +50,10 @@ struct pbpic_ctx {
     int64_t big_cap = 0;
     bool gauss_first = true;
     std::vector<Species> sp;
+    // One-shot push destination per species (cell, dx, dy, ux, uy, uz), set
+    // by the adjoint replay: the next push writes there and those arrays
+    // become the species' current ones.  Empty: the push works in place.
+    std::vector<std::array<void *, 6>> push_out;
     SortScratch sort;
     int64_t steps = 0;
     int maxw = 0;
@@ -122,6 +126,8 @@ SpeciesDev species_dev(pbpic_ctx *c, int s, int buf)
     d.ox = S.ox[buf]; d.oy = S.oy[buf];
     d.ux = S.ux[buf]; d.uy = S.uy[buf]; d.uz = S.uz[buf];
     d.pid = S.pid[buf];
+    d.cell_o = d.cell;
+    d.ox_o = d.ox; d.oy_o = d.oy; d.ux_o = d.ux; d.uy_o = d.uy; d.uz_o = d.uz;
     d.n = S.n;
     d.q = S.q;
     d.ms = S.ms; d.c0 = S.c0; d.c1 = S.c1; d.qx = S.qx; d.qy = S.qy;
@@ -163,6 +169,12 @@ PushParams push_params(pbpic_ctx *c)
     P.err = c->err;
     P.nspecies = (int)c->sp.size();
     for (int s = 0; s < P.nspecies; s++) P.sp[s] = species_dev(c, s, c->sp[s].cur);
+    for (int s = 0; s < (int)c->push_out.size() && s < P.nspecies; s++) {
+        SpeciesDev &d = P.sp[s];
+        d.cell_o = (int32_t *)c->push_out[s][0];
+        d.ox_o = c->push_out[s][1]; d.oy_o = c->push_out[s][2];
+        d.ux_o = c->push_out[s][3]; d.uy_o = c->push_out[s][4]; d.uz_o = c->push_out[s][5];
+    }
     return P;
 }
 
@@ -280,6 +292,15 @@ int push_impl(pbpic_ctx *c)
     HIPCHK(c, prof_mark(c, PBPIC_PROF_FIELD, false));
     HIPCHK(c, prof_mark(c, PBPIC_PROF_PUSH, true));
     HIPCHK(c, launch_push(P, c->p.precision, c->st));
+    for (int s = 0; s < (int)c->push_out.size(); s++) {
+        // the pushed state lives where the push wrote it
+        Species &S = c->sp[s];
+        const int b = S.cur;
+        S.cell[b] = (int32_t *)c->push_out[s][0];
+        S.ox[b] = c->push_out[s][1]; S.oy[b] = c->push_out[s][2];
+        S.ux[b] = c->push_out[s][3]; S.uy[b] = c->push_out[s][4]; S.uz[b] = c->push_out[s][5];
+    }
+    c->push_out.clear();
     if (walls(c)) HIPCHK(c, launch_wall_fold(c->J[2], c->p.nx, c->p.ny, P.nyp, c->rsz, c->st));
     for (auto &S : c->sp) S.tbox_ok = true;
     HIPCHK(c, prof_mark(c, PBPIC_PROF_PUSH, false));
@@ -376,10 +397,12 @@ double *state_field(pbpic_ctx *c, const StateLayout &L, char *buf, int f)
     return (double *)(buf + L.field_off + (size_t)f * al256(8 * (size_t)ncell(c)));
 }
 
-int state_copy(pbpic_ctx *c, const StateLayout &L, char *buf, bool save)
+enum { STATE_PARTICLES = 1, STATE_FIELDS = 2, STATE_ALL = 3 };
+
+int state_copy(pbpic_ctx *c, const StateLayout &L, char *buf, bool save, int what = STATE_ALL)
 {
     const size_t nc = (size_t)ncell(c);
-    for (int s = 0; s < (int)c->sp.size(); s++) {
+    for (int s = 0; s < (int)c->sp.size() && (what & STATE_PARTICLES); s++) {
         Species &S = c->sp[s];
         const int b = S.cur;
         if (!save) S.tbox_ok = false;
@@ -392,13 +415,62 @@ int state_copy(pbpic_ctx *c, const StateLayout &L, char *buf, bool save)
                                      hipMemcpyDeviceToDevice, c->st));
         }
     }
-    for (int f = 0; f < 6; f++) {
+    for (int f = 0; f < 6 && (what & STATE_FIELDS); f++) {
         double *dev = f < 3 ? c->E[f] : c->B[f - 3];
         double *st = state_field(c, L, buf, f);
         HIPCHK(c, hipMemcpyAsync(save ? st : dev, save ? dev : st, nc * 8, hipMemcpyDeviceToDevice, c->st));
     }
     return PBPIC_OK;
 }
+
+// The adjoint replay points each species' current arrays into its tape (the
+// push reads slot k and writes slot k+1, so the tape needs no copy per step).
+// TapeGuard keeps the context's own arrays and puts them back: between steps
+// (so that a sort writes into the context's spare arrays, never into the tape)
+// and when the replay ends or fails.
+struct TapeGuard {
+    pbpic_ctx *c;
+    std::vector<std::array<void *, 12>> own;
+    explicit TapeGuard(pbpic_ctx *cc) : c(cc)
+    {
+        for (auto &S : c->sp)
+            own.push_back({S.cell[0], S.cell[1], S.ox[0], S.ox[1], S.oy[0], S.oy[1],
+                           S.ux[0], S.ux[1], S.uy[0], S.uy[1], S.uz[0], S.uz[1]});
+    }
+    void restore()
+    {
+        for (size_t s = 0; s < own.size(); s++) {
+            Species &S = c->sp[s];
+            const auto &o = own[s];
+            for (int b = 0; b < 2; b++) {
+                S.cell[b] = (int32_t *)o[0 + b];
+                S.ox[b] = o[2 + b]; S.oy[b] = o[4 + b];
+                S.ux[b] = o[6 + b]; S.uy[b] = o[8 + b]; S.uz[b] = o[10 + b];
+            }
+        }
+    }
+    // current arrays <- tape slot `from`; the next push writes slot `to`
+    void step(const StateLayout &L, char *from, char *to)
+    {
+        restore();
+        c->push_out.assign(c->sp.size(), {});
+        for (int s = 0; s < (int)c->sp.size(); s++) {
+            Species &S = c->sp[s];
+            const int b = S.cur;
+            S.cell[b] = (int32_t *)state_ptr(c, L, from, s, 0);
+            S.ox[b] = state_ptr(c, L, from, s, 1); S.oy[b] = state_ptr(c, L, from, s, 2);
+            S.ux[b] = state_ptr(c, L, from, s, 3); S.uy[b] = state_ptr(c, L, from, s, 4);
+            S.uz[b] = state_ptr(c, L, from, s, 5);
+            for (int q = 0; q < 6; q++) c->push_out[s][q] = state_ptr(c, L, to, s, q);
+        }
+    }
+    ~TapeGuard()
+    {
+        restore();
+        c->push_out.clear();
+        for (auto &S : c->sp) S.tbox_ok = false;
+    }
+};
 
 double je_total(pbpic_ctx *c, const double *a)
 {
@@ -1265,9 +1337,11 @@ int pbpic_adjoint_backward(pbpic_ctx *c, double *const *fg, void *const *pg)
             int r = state_copy(c, L, c->adj.ckpts[seg], false);
             if (r) return r;
             c->steps = c->adj.step0 + s0;
+            if ((r = state_copy(c, L, tstate[0], true))) return r;
             int used = 0;
+            TapeGuard guard(c);
             for (int k = 0; k < kk; k++) {
-                if ((r = state_copy(c, L, tstate[k], true))) return r;
+                guard.step(L, tstate[k], tstate[k + 1]);
                 bool so = false;
                 const bool will = c->p.sort_every > 0 && (c->steps + 1) % c->p.sort_every == 0;
                 uint32_t *const *pd = nullptr;
@@ -1279,8 +1353,12 @@ int pbpic_adjoint_backward(pbpic_ctx *c, double *const *fg, void *const *pg)
                 }
                 if ((r = step_impl(c, tj[k].data(), pd, &so))) return r;
                 sorted[k] = so;
+                // the push wrote slot k+1; a sort moved the particles into the
+                // context's own arrays, and the fields are always the context's
+                if ((r = state_copy(c, L, tstate[k + 1], true, so ? STATE_ALL : STATE_FIELDS)))
+                    return r;
             }
-            if ((r = state_copy(c, L, tstate[kk], true))) return r;
+            guard.restore();
             for (int k = kk - 1; k >= 0; k--) {
                 if (sorted[k]) {
                     for (int s = 0; s < ns; s++) {

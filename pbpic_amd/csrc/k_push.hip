@@ -440,42 +440,54 @@ __device__ __forceinline__ void vst(int32_t *p, int64_t i0, int64_t n, const int
     }
 }
 
-// Unchecked 16-byte loads and stores for full tiles (16-byte aligned).
+// Unchecked 16-byte loads and stores for full tiles (16-byte aligned).  The
+// particle streams are read and written once per push and are loaded and
+// stored non-temporally (measured on MI355X: a 3-4% faster push in both forms
+// than plain loads and stores; non-temporal stores alone were slower).
+typedef float nt_f4 __attribute__((ext_vector_type(4)));
+typedef double nt_d2 __attribute__((ext_vector_type(2)));
+typedef int nt_i4 __attribute__((ext_vector_type(4)));
+typedef int nt_i2 __attribute__((ext_vector_type(2)));
+
 __device__ __forceinline__ void ldu(const double *p, double (&o)[2])
 {
-    const double2 t = *reinterpret_cast<const double2 *>(p);
+    const nt_d2 t = __builtin_nontemporal_load(reinterpret_cast<const nt_d2 *>(p));
     o[0] = t.x; o[1] = t.y;
 }
 __device__ __forceinline__ void ldu(const float *p, float (&o)[4])
 {
-    const float4 t = *reinterpret_cast<const float4 *>(p);
+    const nt_f4 t = __builtin_nontemporal_load(reinterpret_cast<const nt_f4 *>(p));
     o[0] = t.x; o[1] = t.y; o[2] = t.z; o[3] = t.w;
 }
 __device__ __forceinline__ void ldu(const int32_t *p, int (&o)[2])
 {
-    const int2 t = *reinterpret_cast<const int2 *>(p);
+    const nt_i2 t = __builtin_nontemporal_load(reinterpret_cast<const nt_i2 *>(p));
     o[0] = t.x; o[1] = t.y;
 }
 __device__ __forceinline__ void ldu(const int32_t *p, int (&o)[4])
 {
-    const int4 t = *reinterpret_cast<const int4 *>(p);
+    const nt_i4 t = __builtin_nontemporal_load(reinterpret_cast<const nt_i4 *>(p));
     o[0] = t.x; o[1] = t.y; o[2] = t.z; o[3] = t.w;
 }
 __device__ __forceinline__ void stu(double *p, const double (&v)[2])
 {
-    *reinterpret_cast<double2 *>(p) = make_double2(v[0], v[1]);
+    const nt_d2 t = {v[0], v[1]};
+    __builtin_nontemporal_store(t, reinterpret_cast<nt_d2 *>(p));
 }
 __device__ __forceinline__ void stu(float *p, const float (&v)[4])
 {
-    *reinterpret_cast<float4 *>(p) = make_float4(v[0], v[1], v[2], v[3]);
+    const nt_f4 t = {v[0], v[1], v[2], v[3]};
+    __builtin_nontemporal_store(t, reinterpret_cast<nt_f4 *>(p));
 }
 __device__ __forceinline__ void stu(int32_t *p, const int (&v)[2])
 {
-    *reinterpret_cast<int2 *>(p) = make_int2(v[0], v[1]);
+    const nt_i2 t = {v[0], v[1]};
+    __builtin_nontemporal_store(t, reinterpret_cast<nt_i2 *>(p));
 }
 __device__ __forceinline__ void stu(int32_t *p, const int (&v)[4])
 {
-    *reinterpret_cast<int4 *>(p) = make_int4(v[0], v[1], v[2], v[3]);
+    const nt_i4 t = {v[0], v[1], v[2], v[3]};
+    __builtin_nontemporal_store(t, reinterpret_cast<nt_i4 *>(p));
 }
 
 // One contribution to the LDS current window.  FP64 form: FP64 atomic add.
@@ -770,8 +782,10 @@ __global__ __launch_bounds__(PUSH_THREADS, 3) void push_kernel(PushParams P)
 
     // ---- C: push, V particles per lane per round, next round prefetched
     const R c0 = (R)S.c0, c1 = (R)S.c1, qx = (R)S.qx, qy = (R)S.qy, qr = (R)S.q;
-    R *ox_p = (R *)S.ox, *oy_p = (R *)S.oy, *ux_p = (R *)S.ux, *uy_p = (R *)S.uy,
-      *uz_p = (R *)S.uz;
+    const R *ox_p = (const R *)S.ox, *oy_p = (const R *)S.oy, *ux_p = (const R *)S.ux,
+            *uy_p = (const R *)S.uy, *uz_p = (const R *)S.uz;
+    R *ox_q = (R *)S.ox_o, *oy_q = (R *)S.oy_o, *ux_q = (R *)S.ux_o, *uy_q = (R *)S.uy_o,
+      *uz_q = (R *)S.uz_o;
     R *gjx = (R *)P.jx, *gjy = (R *)P.jy, *gjz = (R *)P.jz, *grho = (R *)P.rho;
     constexpr bool want_rho = RHO;
     // split form with an LDS window: deposit contributions come out of
@@ -882,9 +896,12 @@ __global__ __launch_bounds__(PUSH_THREADS, 3) void push_kernel(PushParams P)
 
     if (fast) {
         const int XW = x0 + wx0, YW = y0 + wy0;        // window origin, absolute
-        int32_t *cp = S.cell + tstart;
-        R *op = ox_p + tstart, *pp = oy_p + tstart, *up = ux_p + tstart, *vp = uy_p + tstart,
-          *wp = uz_p + tstart;
+        const int32_t *cp = S.cell + tstart;
+        const R *op = ox_p + tstart, *pp = oy_p + tstart, *up = ux_p + tstart,
+                *vp = uy_p + tstart, *wp = uz_p + tstart;
+        int32_t *cq = S.cell_o + tstart;
+        R *oq = ox_q + tstart, *pq = oy_q + tstart, *uq = ux_q + tstart, *vq = uy_q + tstart,
+          *wq = uz_q + tstart;
         const int lo = wave * ROUNDS * CH + lane * V;
         CrossQueue Q;
         Q.init(smem + push_queue_offset(P.maxw, RHO ? 1 : 0), wave, lane);
@@ -944,9 +961,9 @@ __global__ __launch_bounds__(PUSH_THREADS, 3) void push_kernel(PushParams P)
 #pragma unroll
             for (int j = 0; j < V; j++)
                 nc[j] = onef(ccur[j], cur[0][j], cur[1][j], cur[2][j], cur[3][j], cur[4][j]);
-            stu(cp + o, nc);
-            stu(op + o, cur[0]); stu(pp + o, cur[1]); stu(up + o, cur[2]);
-            stu(vp + o, cur[3]); stu(wp + o, cur[4]);
+            stu(cq + o, nc);
+            stu(oq + o, cur[0]); stu(pq + o, cur[1]); stu(uq + o, cur[2]);
+            stu(vq + o, cur[3]); stu(wq + o, cur[4]);
 #pragma unroll
             for (int j = 0; j < V; j++) {
                 ccur[j] = cnxt[j];
@@ -983,9 +1000,9 @@ __global__ __launch_bounds__(PUSH_THREADS, 3) void push_kernel(PushParams P)
                 if (i0 + j < n && (unsigned)c < (unsigned)(nx * ny))
                     nc[j] = one(c, cur[0][j], cur[1][j], cur[2][j], cur[3][j], cur[4][j]);
             }
-            vst(S.cell, i0, n, nc);
-            vst(ox_p, i0, n, cur[0]); vst(oy_p, i0, n, cur[1]); vst(ux_p, i0, n, cur[2]);
-            vst(uy_p, i0, n, cur[3]); vst(uz_p, i0, n, cur[4]);
+            vst(S.cell_o, i0, n, nc);
+            vst(ox_q, i0, n, cur[0]); vst(oy_q, i0, n, cur[1]); vst(ux_q, i0, n, cur[2]);
+            vst(uy_q, i0, n, cur[3]); vst(uz_q, i0, n, cur[4]);
         }
 #pragma unroll
         for (int j = 0; j < V; j++) {

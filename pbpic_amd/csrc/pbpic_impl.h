@@ -88,6 +88,11 @@ struct SpeciesDev {
     int32_t *cell;             // 0-based compact id cx*ny + cy (= grid - 1)
     void *ox, *oy, *ux, *uy, *uz;
     int32_t *pid;              // may be null
+    // Where the push writes the pushed cell, offsets and momenta: the arrays
+    // above (in place) unless a caller keeps the input state (the adjoint's
+    // replay writes each step into the next slot of its tape).
+    int32_t *cell_o;
+    void *ox_o, *oy_o, *ux_o, *uy_o, *uz_o;
     int64_t n;
     int32_t q;                 // +1 ion, -1 electron
     int32_t tile0;             // first tile of this species in a push launch

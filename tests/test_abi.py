@@ -80,3 +80,19 @@ def test_missing_library_fails_loudly(tmp_path):
     import pbpic_amd
     with pytest.raises(pbpic_amd.PbpicError):
         pbpic_amd.load_library(str(tmp_path / "absent.so"))
+
+
+def test_one_hip_runtime_whatever_the_import_order():
+    """Loading the library before torch must not map a second HIP runtime:
+    torch bundles libamdhip64.so.7 under the same soname as ROCm's."""
+    pytest.importorskip("torch")
+    import pbpic_amd
+    if not os.path.exists(pbpic_amd.LIB_PATH):
+        pytest.skip("library not built")
+    code = ("import sys; sys.path.insert(0, %r)\n"
+            "import pbpic_amd; pbpic_amd.load_library()\n"
+            "import torch\n"
+            "m = open('/proc/self/maps').read().splitlines()\n"
+            "print(len({l.split()[-1] for l in m if 'libamdhip64' in l}))\n") % ROOT
+    out = subprocess.run(["python", "-c", code], capture_output=True, text=True, check=True)
+    assert out.stdout.strip().splitlines()[-1] == "1", out.stdout
