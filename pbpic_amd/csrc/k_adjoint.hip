@@ -19,9 +19,12 @@
 // state (checkpointing happens in the host code of capi.hip).
 #include "pbpic_impl.h"
 
-// the per-particle forward recompute follows the forward kernel's arithmetic
-// exactly (same expression order, no contraction)
-#pragma clang fp contract(off)
+// Multiply-adds are contracted.  The per-particle forward recompute keeps
+// the forward kernel's expression order; the FP64 forward runs uncontracted,
+// so its recompute can differ in the last bit, which matters only for a
+// particle within one rounding error of a cell face (it is differentiated on
+// the other side of it).
+#pragma clang fp contract(fast)
 
 namespace pbpic {
 
@@ -134,12 +137,15 @@ __global__ __launch_bounds__(ADJ_THREADS, 2) void push_adj_kernel(AdjParams P)
     // thread tid taking l = tid/4 and 8 ranks from 32k + 8(tid%4).
     const int64_t T0 = (tstart / PUSH_TILE) * PUSH_TILE;
     const bool porder = P.tv > 0 && T0 + PUSH_TILE <= S.n;
+    // the push's vector width of this precision (P.tv is it or 0): a
+    // constant, so the rank -> slot map compiles to shifts
+    constexpr int TV = sizeof(R) == 8 ? push_vec(PBPIC_PREC_FP64) : push_vec(PBPIC_PREC_SPLIT32);
     auto slot_of = [&](int r) -> int64_t {
         if (!porder) return tstart + (int64_t)tid * ADJ_ROUNDS + r;
         const int k = (int)((tstart - T0) / ADJ_TILE);
         const int rank = (tid >> 2) * 64 + 32 * k + (tid & 3) * ADJ_ROUNDS + r;
-        const int V = P.tv, l = rank >> 6, c = (rank & 63) / V, j = rank % V;
-        return T0 + (int64_t)c * 64 * V + l * V + j;
+        const int l = rank >> 6, c = (rank & 63) / TV, j = rank % TV;
+        return T0 + (int64_t)c * 64 * TV + l * TV + j;
     };
     const int nx = P.nx, ny = P.ny, nxh = nx >> 1, nyh = ny >> 1;
 
@@ -278,10 +284,18 @@ __global__ __launch_bounds__(ADJ_THREADS, 2) void push_adj_kernel(AdjParams P)
         // Boris (:964-1000)
         R um[3], t[3], sv[3], u0[3], up[3], un[3];
         for (int k = 0; k < 3; k++) um[k] = u[k] + c1 * E[k];
-        const R g1 = sqrt(ONE + um[0] * um[0] + um[1] * um[1] + um[2] * um[2]);
-        const R c2 = c0 / g1;
+        // 1/gamma and 2/(1+t.t): IEEE in FP64; in the split form the
+        // hardware reciprocal (square root) the forward split push uses
+        const R q1 = ONE + um[0] * um[0] + um[1] * um[1] + um[2] * um[2];
+        R g1, ig1;
+        if constexpr (SPLIT) { ig1 = __builtin_amdgcn_rsqf(q1); g1 = q1 * ig1; }
+        else { g1 = sqrt(q1); ig1 = ONE / g1; }
+        const R c2 = SPLIT ? c0 * ig1 : c0 / g1;
         for (int k = 0; k < 3; k++) t[k] = c2 * Bf[k];
-        const R t2 = (R)2 / (ONE + t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
+        const R qt = ONE + t[0] * t[0] + t[1] * t[1] + t[2] * t[2];
+        R t2;
+        if constexpr (SPLIT) t2 = (R)2 * __builtin_amdgcn_rcpf(qt);
+        else t2 = (R)2 / qt;
         for (int k = 0; k < 3; k++) sv[k] = t2 * t[k];
         R uq[3];
         for (int k = 0; k < 3; k++) uq[k] = oldu ? u[k] : um[k];
@@ -291,11 +305,13 @@ __global__ __launch_bounds__(ADJ_THREADS, 2) void push_adj_kernel(AdjParams P)
         cross3(u0, sv, cr);
         for (int k = 0; k < 3; k++) up[k] = um[k] + cr[k];
         for (int k = 0; k < 3; k++) un[k] = up[k] + c1 * E[k];
-        const R gn = sqrt(ONE + un[0] * un[0] + un[1] * un[1] + un[2] * un[2]);
-        const R ig = ONE / gn;
+        const R q2 = ONE + un[0] * un[0] + un[1] * un[1] + un[2] * un[2];
+        R gn, ig;
+        if constexpr (SPLIT) { ig = __builtin_amdgcn_rsqf(q2); gn = q2 * ig; }
+        else { gn = sqrt(q2); ig = ONE / gn; }
         const R v[3] = {un[0] * ig, un[1] * ig, un[2] * ig};
         // move (:1002-1026)
-        const R c3 = dtcva / gn;
+        const R c3 = SPLIT ? dtcva * ig : dtcva / gn;
         const R dx1 = ox + c3 * un[0] * idx;
         const R dy1 = oy + c3 * un[1] * idy;
         const int a = fabs(dx1) > Hh ? (int)a_sgn(dx1) : 0;
@@ -306,7 +322,7 @@ __global__ __launch_bounds__(ADJ_THREADS, 2) void push_adj_kernel(AdjParams P)
         // u_y reversed), so the deposit saw b = 0 and -oy2
         const bool refl = P.nyp && ((cy == 0 && b < 0) || (cy == P.nyp - 1 && b > 0));
         if (refl) { b = 0; oy2 = -oy2; }
-        const R vz = un[2] * cva / gn;
+        const R vz = SPLIT ? un[2] * cva * ig : un[2] * cva / gn;
         // deposit terms (relative form of :1237-1282)
         R fx1, fx2, wx1, wx2, gx, fy1, fy2, wy1, wy2, gy;
         zig_axis<R>(a, ox, ox2, qx, fx1, fx2, wx1, wx2, gx, false);
@@ -379,12 +395,12 @@ __global__ __launch_bounds__(ADJ_THREADS, 2) void push_adj_kernel(AdjParams P)
         // t = c2 B, c2 = c0/g1, g1 = sqrt(1+um.um)
         const R bc2 = bt[0] * Bf[0] + bt[1] * Bf[1] + bt[2] * Bf[2];
         for (int k = 0; k < 3; k++) bB3[k] = c2 * bt[k];
-        const R bg1 = -c2 / g1 * bc2;
+        const R bg1 = SPLIT ? -c2 * ig1 * bc2 : -c2 / g1 * bc2;
         for (int k = 0; k < 3; k++) {
             if (oldu) bu[k] = buq[k];
             else { bu[k] = 0; bum[k] += buq[k]; }
         }
-        for (int k = 0; k < 3; k++) bum[k] += bg1 * um[k] / g1;
+        for (int k = 0; k < 3; k++) bum[k] += SPLIT ? bg1 * um[k] * ig1 : bg1 * um[k] / g1;
         for (int k = 0; k < 3; k++) { bu[k] += bum[k]; bE[k] += c1 * bum[k]; }
         // ---- gather reverse: force adjoint -> nodes; weights -> position
         const R bF[6] = {bE[0], bE[1], bE[2], bB3[0], bB3[1], bB3[2]};
