@@ -306,6 +306,8 @@ def main():
     ap.add_argument("--no-harris", action="store_true")
     ap.add_argument("--no-c4", action="store_true",
                     help="skip the 1024x1024 Harris adjoint leg (about 240 GB of HBM)")
+    ap.add_argument("--c4-steps", type=int, default=1000,
+                    help="steps of the 1024x1024 Harris adjoint leg (its configuration: 1000)")
     ap.add_argument("--seed", type=int, default=20260101)
     ap.add_argument("--calib-copy-gib", type=float, default=0.0,
                     help="first run one plain device copy of this many GiB (a known-size "
@@ -386,13 +388,13 @@ def main():
         # the larger of the forms' adjoint-sweep / plain-forward time ratios
         out["adjoint_forward_ratio"] = max(a["ratio"] for a in adj.values())
         if not args.no_c4:
-            log("[bench] adjoint leg: Harris 1024x1024, 1000 steps, split")
-            c4 = adjoint_c4_leg(args.seed)
+            log(f"[bench] adjoint leg: Harris 1024x1024, {args.c4_steps} steps, split")
+            c4 = adjoint_c4_leg(args.seed, nsteps=args.c4_steps)
             log(f"[bench] adjoint C4: {c4}")
             out["adjoint_c4"] = dict(workload="differentiable PIC, large: Harris sheet 1024x1024 "
                                               "(+2 guard rows), conducting y walls, 64 ppc/species, "
-                                              "1000 steps, checkpoints every 50, loss = J.E over the "
-                                              "central 4x2 d_i, split precision", **c4)
+                                              f"{args.c4_steps} steps, checkpoints every 50, loss = "
+                                              "J.E over the central 4x2 d_i, split precision", **c4)
     if not args.no_harris:
         hr = {}
         for f in forms:

@@ -153,6 +153,7 @@ PushParams push_params(pbpic_ctx *c)
     P.compat = c->p.compat;
     P.deposit_rho = (c->p.flags & PBPIC_FLAG_DEPOSIT_RHO) ? 1 : 0;
     P.maxw = c->maxw;
+    P.maxw_mid = push_maxw_mid(c->p.precision, P.deposit_rho);
     P.maxw_big = push_maxw_big(c->p.precision, P.deposit_rho);
     P.big = c->big;
     P.pass = 0;
@@ -279,11 +280,12 @@ int push_impl(pbpic_ctx *c)
 {
     int64_t ntiles = 0;
     for (auto &S : c->sp) ntiles += (S.n + PUSH_TILE - 1) / PUSH_TILE;
-    if (ntiles + 1 > c->big_cap) {
-        // the deferred-tile list of the two-pass push: one entry per tile
+    if (2 * ntiles + 2 > c->big_cap) {
+        // the deferred-tile lists of the tiered push: two counts, one entry
+        // per tile in each list
         if (c->big) { HIPCHK(c, hipStreamSynchronize(c->st)); (void)hipFree(c->big); c->big = nullptr; }
-        HIPCHK(c, dalloc(&c->big, sizeof(int32_t) * (size_t)(ntiles + 1)));
-        c->big_cap = ntiles + 1;
+        HIPCHK(c, dalloc(&c->big, sizeof(int32_t) * (size_t)(2 * ntiles + 2)));
+        c->big_cap = 2 * ntiles + 2;
     }
     const PushParams P = push_params(c);
     // node fields (emfgrid, pic2d.f90:837-845) are field work: timed as such

@@ -675,10 +675,12 @@ __global__ __launch_bounds__(PUSH_THREADS, 3) void push_kernel(PushParams P)
     __shared__ int red[4][5];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     int t;
-    if (P.pass == 2) {
-        // the tiles the first launch deferred; the rest of the grid is idle
-        if ((int)blockIdx.x >= P.big[0]) return;
-        t = P.big[1 + blockIdx.x];
+    if (P.pass >= 2) {
+        // the tiles the first launch deferred to this one; the rest of the
+        // grid is idle
+        const int which = P.pass - 2;
+        if ((int)blockIdx.x >= P.big[which]) return;
+        t = P.big[2 + which * P.ntiles + blockIdx.x];
     } else {
         t = xcd_remap(blockIdx.x, gridDim.x);
     }
@@ -751,7 +753,8 @@ __global__ __launch_bounds__(PUSH_THREADS, 3) void push_kernel(PushParams P)
     const bool inlds = mnx <= mxx && (int64_t)W * H <= P.maxw;
     if (P.pass == 1 && !inlds && mnx <= mxx && (int64_t)W * H <= P.maxw_big) {
         // the box is block-uniform: the whole workgroup leaves together
-        if (tid == 0) P.big[1 + atomicAdd(P.big, 1)] = t;
+        const int which = (int64_t)W * H <= P.maxw_mid ? 0 : 1;
+        if (tid == 0) P.big[2 + which * P.ntiles + atomicAdd(P.big + which, 1)] = t;
         return;
     }
     if (!inlds && mnx <= mxx && tid == 0) unsafeAtomicAdd(P.acc + ACC_SPILL, 1.0);
@@ -1060,11 +1063,13 @@ __global__ __launch_bounds__(PUSH_THREADS, 3) void push_kernel(PushParams P)
         // cells read must lie inside the window's interior [1, W-3] x [1, H-3]
         if (any && (v[0] < 1 || -v[1] > W - 3 || v[2] < 1 || -v[3] > H - 3))
             atomicOr(P.err, ERRF_STATE);
-        // a box wider than half the domain would be ambiguous under the
-        // periodic unwrap: leave it unknown (the next push scans the cells)
+        // the next push unwraps cells into (-n/2, n/2] around the box origin:
+        // a box spanning half the domain or more would be ambiguous (for even
+        // n the offset n/2 unwraps to -n/2), so it is left unknown and the
+        // next push scans the cells
         int4 nb = make_int4(0, -1, 0, -1);
         const int ex = -v[5] - v[4], ey = -v[7] - v[6];
-        if (any && aerr == 0 && ex <= nxh && ey <= nyh) {
+        if (any && aerr == 0 && ex < nxh && ey < nyh) {
             nb.x = wrapi(x0 + wx0 + v[4], nx);
             nb.y = ex;
             nb.z = wrapi(y0 + wy0 + v[6], ny);
@@ -1209,11 +1214,22 @@ static void launch_push_f(const PushParams &q, int nt, size_t lds, hipStream_t s
 }
 
 // Largest window whose LDS fits one workgroup per CU (150 KiB).
+// Window capacity (nodes) of a launch whose workgroups take `budget` bytes of LDS.
+static int push_maxw_for(int precision, int deposit_rho, size_t budget)
+{
+    const size_t per = push_lds_bytes(precision, 1, deposit_rho) -
+                       push_lds_bytes(precision, 0, deposit_rho);
+    return (int)((budget - push_lds_bytes(precision, 0, deposit_rho)) / per);
+}
+
+// one workgroup per CU (150 KiB) and two (78 KiB) of the 160 KiB
 int push_maxw_big(int precision, int deposit_rho)
 {
-    const size_t budget = 150 * 1024, per = push_lds_bytes(precision, 1, deposit_rho) -
-                                             push_lds_bytes(precision, 0, deposit_rho);
-    return (int)((budget - push_lds_bytes(precision, 0, deposit_rho)) / per);
+    return push_maxw_for(precision, deposit_rho, 150 * 1024);
+}
+int push_maxw_mid(int precision, int deposit_rho)
+{
+    return push_maxw_for(precision, deposit_rho, 78 * 1024);
 }
 
 hipError_t launch_push(const PushParams &p, int precision, hipStream_t st)
@@ -1223,19 +1239,23 @@ hipError_t launch_push(const PushParams &p, int precision, hipStream_t st)
     if (nt == 0) return hipSuccess;
     const size_t lds = push_lds_bytes(precision, p.maxw, p.deposit_rho);
     q.pass = p.big && p.maxw_big > p.maxw ? 1 : 0;
+    q.maxw_mid = std::max(p.maxw, std::min(p.maxw_mid, p.maxw_big));
     if (q.pass) {
-        hipError_t e = launch_fill_u32((uint32_t *)p.big, 1, 0u, st);
+        hipError_t e = launch_fill_u32((uint32_t *)p.big, 2, 0u, st);
         if (e != hipSuccess) return e;
     }
     if (precision == PBPIC_PREC_FP64) launch_push_f<double, false>(q, nt, lds, st);
     else launch_push_f<float, true>(q, nt, lds, st);
     if (q.pass) {
-        PushParams b = q;
-        b.pass = 2;
-        b.maxw = p.maxw_big;
-        const size_t lb = push_lds_bytes(precision, b.maxw, p.deposit_rho);
-        if (precision == PBPIC_PREC_FP64) launch_push_f<double, false>(b, nt, lb, st);
-        else launch_push_f<float, true>(b, nt, lb, st);
+        for (int pass = 2; pass <= 3; pass++) {
+            PushParams b = q;
+            b.pass = pass;
+            b.maxw = pass == 2 ? q.maxw_mid : p.maxw_big;
+            if (pass == 2 && b.maxw == p.maxw) continue;       // no middle tier
+            const size_t lb = push_lds_bytes(precision, b.maxw, p.deposit_rho);
+            if (precision == PBPIC_PREC_FP64) launch_push_f<double, false>(b, nt, lb, st);
+            else launch_push_f<float, true>(b, nt, lb, st);
+        }
     }
     return hipGetLastError();
 }

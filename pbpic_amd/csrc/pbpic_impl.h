@@ -116,11 +116,13 @@ struct PushParams {
     FastDiv dny;
     int compat, deposit_rho;
     int maxw;                  // LDS window capacity in cells
-    // Tiles whose window exceeds maxw but fits maxw_big are deferred by the
-    // first launch (pass 1) to a second one with the larger window (pass 2);
-    // big[0] counts them, big[1..] lists their tile indices.  pass 0: one
-    // launch, oversized tiles deposit with global atomics.
-    int maxw_big, pass;
+    // Tiles whose window exceeds maxw are deferred by the first launch
+    // (pass 1): to a second launch with windows of maxw_mid nodes (pass 2,
+    // two workgroups per CU) if they fit it, else to a third with maxw_big
+    // (pass 3, one per CU).  big[0] and big[1] count the two lists, which
+    // start at big + 2 and big + 2 + ntiles.  pass 0: one launch; tiles that
+    // do not fit deposit with global atomics, as do those wider than maxw_big.
+    int maxw_mid, maxw_big, pass;
     int *big;
     int ntiles;
     int nspecies;
@@ -199,6 +201,7 @@ struct AdjFieldParams {
 hipError_t launch_push(const PushParams &p, int precision, hipStream_t st);
 // Window capacity of the second push pass (one workgroup per CU).
 int push_maxw_big(int precision, int deposit_rho);
+int push_maxw_mid(int precision, int deposit_rho);
 size_t nodefield_bytes(int precision, int64_t ncell);
 // Fills p.nf from the E and B planes; launch_push reads p.nf.
 hipError_t launch_nodefield(const PushParams &p, int precision, hipStream_t st);
