@@ -663,10 +663,11 @@ __device__ __forceinline__ void deposit_split_fast(double *jw, int WH, int W, in
     Q.push(cross, base, a, b, dx1, dx2, dy1, dy2);
 }
 
-// OLDU: the reference's Boris rotation with the pre-kick momentum
-// (PBPIC_COMPAT_BORIS_OLD_U); RHO: deposit rho as well (PBPIC_FLAG_DEPOSIT_RHO).
+// One push tile t (one workgroup).  OLDU: the reference's Boris rotation
+// with the pre-kick momentum (PBPIC_COMPAT_BORIS_OLD_U); RHO: deposit rho as
+// well (PBPIC_FLAG_DEPOSIT_RHO).
 template <typename R, bool SPLIT, bool OLDU, bool RHO>
-__global__ __launch_bounds__(PUSH_THREADS, 3) void push_kernel(PushParams P)
+__device__ __forceinline__ void push_tile(const PushParams &P, int t)
 {
     constexpr int V = Vw<R>::V;
     constexpr int CH = 64 * V;                                   // slots per wave-wide load
@@ -674,16 +675,6 @@ __global__ __launch_bounds__(PUSH_THREADS, 3) void push_kernel(PushParams P)
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ int red[4][5];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    int t;
-    if (P.pass >= 2) {
-        // the tiles the first launch deferred to this one; the rest of the
-        // grid is idle
-        const int which = P.pass - 2;
-        if ((int)blockIdx.x >= P.big[which]) return;
-        t = P.big[2 + which * P.ntiles + blockIdx.x];
-    } else {
-        t = xcd_remap(blockIdx.x, gridDim.x);
-    }
     int s = 0;
     while (s + 1 < P.nspecies && t >= P.sp[s + 1].tile0) s++;
     const SpeciesDev S = P.sp[s];
@@ -1079,6 +1070,29 @@ __global__ __launch_bounds__(PUSH_THREADS, 3) void push_kernel(PushParams P)
     }
 }
 
+template <typename R, bool SPLIT, bool OLDU, bool RHO>
+__global__ __launch_bounds__(PUSH_THREADS, 3) void push_kernel(PushParams P)
+{
+    push_tile<R, SPLIT, OLDU, RHO>(P, xcd_remap(blockIdx.x, gridDim.x));
+}
+
+// Passes 2 and 3: the tiles the first launch deferred, list entry k taken by
+// workgroup k mod gridDim.x.  The grid is an eighth of the tiles, so a pass
+// with nothing to do costs little, and one with fewer deferred tiles than
+// workgroups takes one per workgroup in list order (the hardware's dispatch
+// order; taking them from a shared counter instead was 6% slower on a
+// Harris sheet).  Their LDS windows allow at most two workgroups per CU, so
+// the register budget is that of two.
+template <typename R, bool SPLIT, bool OLDU, bool RHO>
+__global__ __launch_bounds__(PUSH_THREADS, 2) void push_kernel_listed(PushParams P)
+{
+    const int which = P.pass - 2, cnt = P.big[which];
+    for (int k = blockIdx.x; k < cnt; k += gridDim.x) {
+        push_tile<R, SPLIT, OLDU, RHO>(P, P.big[2 + which * P.ntiles + k]);
+        __syncthreads();
+    }
+}
+
 // Batched currentdeposit with global atomics (unit-test entry point).
 template <typename R, bool SPLIT>
 __global__ void deposit_batch_kernel(PushParams P, int64_t n, const int32_t *i1,
@@ -1197,6 +1211,13 @@ size_t push_lds_bytes(int precision, int maxw, int deposit_rho)
 template <typename R, bool SPLIT, bool OLDU, bool RHO>
 static void launch_push_t(const PushParams &q, int nt, size_t lds, hipStream_t st)
 {
+    if (q.pass >= 2) {
+        (void)hipFuncSetAttribute((const void *)push_kernel_listed<R, SPLIT, OLDU, RHO>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        hipLaunchKernelGGL((push_kernel_listed<R, SPLIT, OLDU, RHO>), dim3(nt),
+                           dim3(PUSH_THREADS), lds, st, q);
+        return;
+    }
     (void)hipFuncSetAttribute((const void *)push_kernel<R, SPLIT, OLDU, RHO>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     hipLaunchKernelGGL((push_kernel<R, SPLIT, OLDU, RHO>), dim3(nt), dim3(PUSH_THREADS), lds, st,
@@ -1241,6 +1262,7 @@ hipError_t launch_push(const PushParams &p, int precision, hipStream_t st)
     q.pass = p.big && p.maxw_big > p.maxw ? 1 : 0;
     q.maxw_mid = std::max(p.maxw, std::min(p.maxw_mid, p.maxw_big));
     if (q.pass) {
+        // the two list counts
         hipError_t e = launch_fill_u32((uint32_t *)p.big, 2, 0u, st);
         if (e != hipSuccess) return e;
     }
@@ -1253,8 +1275,9 @@ hipError_t launch_push(const PushParams &p, int precision, hipStream_t st)
             b.maxw = pass == 2 ? q.maxw_mid : p.maxw_big;
             if (pass == 2 && b.maxw == p.maxw) continue;       // no middle tier
             const size_t lb = push_lds_bytes(precision, b.maxw, p.deposit_rho);
-            if (precision == PBPIC_PREC_FP64) launch_push_f<double, false>(b, nt, lb, st);
-            else launch_push_f<float, true>(b, nt, lb, st);
+            const int ng = std::max(1, (nt + 7) / 8);
+            if (precision == PBPIC_PREC_FP64) launch_push_f<double, false>(b, ng, lb, st);
+            else launch_push_f<float, true>(b, ng, lb, st);
         }
     }
     return hipGetLastError();
