@@ -64,7 +64,7 @@ def run_form(precision, cfg, steps, warmup, sort_every, seed, push_order=True):
 # HBM bytes per push launch at 2048x2048 x 100 ppc, steady state: rocprofv3
 # FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (factors from a 4 GiB copy), from the
 # commands in profiles/push_kernel_traffic_two_stream_2048x2048_ppc100.md.
-TRAFFIC_2048_PPC100 = {1: 10.301e9 + 10.339e9, 0: 19.225e9 + 19.228e9}
+TRAFFIC_2048_PPC100 = {1: 10.248e9 + 10.336e9, 0: 18.713e9 + 18.807e9}
 TRAFFIC_SOURCE = "profiles/push_kernel_traffic_two_stream_2048x2048_ppc100.md"
 
 
