@@ -151,3 +151,23 @@ def test_adjoint_loss_region(gpu, walls):
     for s in range(2):
         for k in PART:
             assert rel_norm(gPg[s][k], gP[s][k]) < 1e-8, (s, k)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_adjoint_full_tiles(gpu, precision):
+    """131,072 particles per species (32 full 4096-particle tiles): after the
+    sort at step 5 the particles are in push order, so the adjoint takes its
+    push-order rank map and the replayed push its fast path, writing each
+    step into the next tape slot.  10 steps, checkpoints every 4, against
+    torch's reverse mode."""
+    print(f"adjoint full tiles precision={precision}", flush=True)
+    case = make_case(nx=64, ny=64, ppc=32, seed=11, efield=0.0, bfield=0.0, vth=(0.02, 0.1))
+    L, gF, gP = _ref_grads(case, 0.01, 0.05, 10, 5, 7)
+    Lg, gFg, gPg = _gpu_grads(case, 0.01, 0.05, 10, 4, 5, 7, precision)
+    tol_l, tol = (1e-10, 1e-8) if precision == 0 else (1e-4, 1e-3)
+    assert abs(Lg - L) <= tol_l * abs(L)
+    for k in range(6):
+        assert rel_norm(gFg[k], gF[k]) < tol, k
+    for s in range(2):
+        for k in PART:
+            assert rel_norm(gPg[s][k], gP[s][k]) < tol, (s, k)
