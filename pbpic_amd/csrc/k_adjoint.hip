@@ -412,23 +412,46 @@ __global__ __launch_bounds__(ADJ_THREADS, 2) void push_adj_kernel(AdjParams P)
         }
         xo += dFx;
         yo += dFy;
-        if (c != acell) {
-            flush();
-            acell = c;
-            abase = ly * W + lx;
-            ag[0] = (size_t)cy * nx + cx; ag[1] = (size_t)cy * nx + cx1;
-            ag[2] = (size_t)cy1 * nx + cx; ag[3] = (size_t)cy1 * nx + cx1;
+        if constexpr (SPLIT) {
+            if (c != acell) {
+                flush();
+                acell = c;
+                abase = ly * W + lx;
+                ag[0] = (size_t)cy * nx + cx; ag[1] = (size_t)cy * nx + cx1;
+                ag[2] = (size_t)cy1 * nx + cx; ag[3] = (size_t)cy1 * nx + cx1;
+#pragma unroll
+                for (int q = 0; q < 6; q++)
+#pragma unroll
+                    for (int m = 0; m < 4; m++) nacc[q][m] = (R)0;
+            }
+#pragma unroll
+            for (int q = 0; q < 6; q++) {
+                nacc[q][0] += s00 * bF[q];
+                nacc[q][1] += s10 * bF[q];
+                nacc[q][2] += s01 * bF[q];
+                nacc[q][3] += s11 * bF[q];
+            }
+        } else if (inlds) {
+            // FP64: each particle's node adjoints go straight to the window
+            // (register sums per cell would need 48 more registers and spill)
+            const int base = ly * W + lx;
+#pragma unroll
+            for (int q = 0; q < 6; q++) {
+                double *pq = na + (size_t)q * WH;
+                a_lds_add(pq + base, (double)(s00 * bF[q]));
+                a_lds_add(pq + base + 1, (double)(s10 * bF[q]));
+                a_lds_add(pq + base + W, (double)(s01 * bF[q]));
+                a_lds_add(pq + base + W + 1, (double)(s11 * bF[q]));
+            }
+        } else {
+            const size_t g[4] = {(size_t)cy * nx + cx, (size_t)cy * nx + cx1,
+                                 (size_t)cy1 * nx + cx, (size_t)cy1 * nx + cx1};
+            const R w[4] = {s00, s10, s01, s11};
 #pragma unroll
             for (int q = 0; q < 6; q++)
 #pragma unroll
-                for (int m = 0; m < 4; m++) nacc[q][m] = (R)0;
-        }
-#pragma unroll
-        for (int q = 0; q < 6; q++) {
-            nacc[q][0] += s00 * bF[q];
-            nacc[q][1] += s10 * bF[q];
-            nacc[q][2] += s01 * bF[q];
-            nacc[q][3] += s11 * bF[q];
+                for (int m = 0; m < 4; m++)
+                    unsafeAtomicAdd(nadj + q * NC + g[m], (double)(w[m] * bF[q]));
         }
         ((R *)P.xa[s])[ia] = xo;
         ((R *)P.ya[s])[ia] = yo;
