@@ -34,12 +34,6 @@ constexpr int ADJ_THREADS = 256;
 constexpr int ADJ_ROUNDS = 8;
 constexpr int ADJ_TILE = ADJ_THREADS * ADJ_ROUNDS;      // 2048, half a push tile
 
-__device__ __forceinline__ int a_xcd(int b, int nb)
-{
-    const int q = nb >> 3, r = nb & 7, x = b & 7, k = b >> 3;
-    return x * q + (x < r ? x : r) + k;
-}
-
 __device__ __forceinline__ int a_wrap(int g, int n)
 {
     while (g < 0) g += n;
@@ -124,7 +118,7 @@ __global__ __launch_bounds__(ADJ_THREADS, 2) void push_adj_kernel(AdjParams P)
         if ((int)blockIdx.x >= P.big[0]) return;     // deferred tiles only
         t = P.big[1 + blockIdx.x];
     } else {
-        t = a_xcd(blockIdx.x, gridDim.x);
+        t = xcd_remap(blockIdx.x, gridDim.x);
     }
     int s = 0;
     while (s + 1 < P.nspecies && t >= P.sp[s + 1].tile0) s++;

@@ -37,12 +37,6 @@ __device__ __forceinline__ int fwrap(int g, int n)
     return g;
 }
 
-__device__ __forceinline__ int fxcd_remap(int b, int nb)
-{
-    const int q = nb >> 3, r = nb & 7, x = b & 7, k = b >> 3;
-    return x * q + (x < r ? x : r) + k;
-}
-
 // SPLIT: the stencil differences are taken on the FP64 state, rounded to FP32,
 // and the update increments are formed in FP32 and accumulated into FP64.
 template <typename RJ, bool SPLIT>
@@ -54,7 +48,7 @@ __global__ __launch_bounds__(FT_THREADS) void fdtd_kernel(FdtdParams P)
     const int tid = threadIdx.x;
     const int nx = P.nx, ny = P.ny;
     const int ntx = (nx + FT_X - 1) / FT_X;
-    const int t = fxcd_remap(blockIdx.x, gridDim.x);
+    const int t = xcd_remap(blockIdx.x, gridDim.x);
     const int tx = t % ntx, ty = t / ntx;
     const int x0 = tx * FT_X - FT_H, y0 = ty * FT_Y - FT_H;
 

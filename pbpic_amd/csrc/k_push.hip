@@ -34,14 +34,6 @@
 
 namespace pbpic {
 
-__device__ __forceinline__ int xcd_remap(int b, int nb)
-{
-    // Workgroups are dispatched round-robin over the 8 XCDs; give each XCD a
-    // contiguous range of tiles so halo atomics stay in one L2.
-    const int q = nb >> 3, r = nb & 7, x = b & 7, k = b >> 3;
-    return x * q + (x < r ? x : r) + k;
-}
-
 __device__ __forceinline__ int wrapi(int g, int n)
 {
     while (g < 0) g += n;

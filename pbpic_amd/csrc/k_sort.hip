@@ -256,7 +256,9 @@ __global__ __launch_bounds__(256) void gather_kernel(const uint32_t *perm, int64
                                                      SpeciesDev dst)
 {
     constexpr int G = 16 / (int)sizeof(R);
-    const int64_t i0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * G;
+    // XCD-contiguous destination ranges: the sources of neighbouring
+    // destinations overlap, and stay in one L2
+    const int64_t i0 = ((int64_t)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x) * G;
     if (i0 >= n) return;
     if (i0 + G > n) {                        // ragged end: element by element
         for (int64_t i = i0; i < n; i++) {

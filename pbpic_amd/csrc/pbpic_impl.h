@@ -58,6 +58,16 @@ enum : int {
 
 constexpr int MAX_SPECIES = 4;
 
+// Workgroups are dispatched round-robin over the 8 XCDs.  Block b of nb is
+// given work item xcd_remap(b, nb) so that each XCD takes a contiguous range
+// (a bijection of [0, nb)): neighbouring items, which share field halos or
+// source cache lines, then meet in one L2.
+__device__ __forceinline__ int xcd_remap(int b, int nb)
+{
+    const int q = nb >> 3, r = nb & 7, x = b & 7, k = b >> 3;
+    return x * q + (x < r ? x : r) + k;
+}
+
 // Division by a run-time constant d >= 1 for 32-bit unsigned n
 // (Granlund & Montgomery, "Division by invariant integers using
 // multiplication", PLDI'94, round-up variant with a 33-bit magic).
