@@ -154,9 +154,11 @@ extern "C" {
 #define PBPIC_D_GAUSS_DLINF 9  /* max norm of that change                   */
 #define PBPIC_D_STEPS       10 /* steps taken                               */
 #define PBPIC_D_TILES_SPILLED 11 /* push tiles (4096 particles) whose window
-                                    exceeded LDS and deposited with global
-                                    atomics, since creation: a performance
-                                    counter, zero in normal runs            */
+                                    exceeded even the largest LDS window
+                                    (one workgroup per CU) and deposited
+                                    with global atomics, since creation: a
+                                    performance counter, zero in normal
+                                    runs                                    */
 #define PBPIC_NDIAG         16
 
 typedef struct pbpic_ctx pbpic_ctx;
