@@ -34,13 +34,6 @@ constexpr int ADJ_THREADS = 256;
 constexpr int ADJ_ROUNDS = 8;
 constexpr int ADJ_TILE = ADJ_THREADS * ADJ_ROUNDS;      // 2048, half a push tile
 
-__device__ __forceinline__ int a_wrap(int g, int n)
-{
-    while (g < 0) g += n;
-    while (g >= n) g -= n;
-    return g;
-}
-
 // LDS bytes of the node-field window, rounded up to 8-byte alignment.
 __host__ __device__ constexpr size_t adj_nf_bytes(size_t r, int maxw)
 {
@@ -197,7 +190,7 @@ __global__ __launch_bounds__(ADJ_THREADS, 2) void push_adj_kernel(AdjParams P)
         for (int k = tid; k < WH; k += ADJ_THREADS) {
             const int wj = k / W, wi = k - wj * W;
             double f[6];
-            a_node_fields(P, a_wrap(x0 + wx0 + wi, nx), a_wrap(y0 + wy0 + wj, ny), f);
+            a_node_fields(P, wrap_index(x0 + wx0 + wi, nx), wrap_index(y0 + wy0 + wj, ny), f);
 #pragma unroll
             for (int q = 0; q < 6; q++) nf[(size_t)k * 6 + q] = (R)f[q];
         }
@@ -331,7 +324,7 @@ __global__ __launch_bounds__(ADJ_THREADS, 2) void push_adj_kernel(AdjParams P)
         const size_t g0 = (size_t)cy * nx + cx;
         auto at = [&](int ddx, int ddy) -> size_t {
             if (interior) return g0 + (ptrdiff_t)ddy * nx + ddx;
-            return (size_t)a_wrap(cy + ddy, ny) * nx + a_wrap(cx + ddx, nx);
+            return (size_t)wrap_index(cy + ddy, ny) * nx + wrap_index(cx + ddx, nx);
         };
         const R l1 = (R)ja[at(0, 0)], l2 = (R)ja[at(0, 1)];
         const R l3 = (R)jb[at(0, 0)], l4 = (R)jb[at(1, 0)];
@@ -458,7 +451,7 @@ __global__ __launch_bounds__(ADJ_THREADS, 2) void push_adj_kernel(AdjParams P)
     if (inlds) {
         for (int k = tid; k < WH; k += ADJ_THREADS) {
             const int wj = k / W, wi = k - wj * W;
-            const size_t g = (size_t)a_wrap(y0 + wy0 + wj, ny) * nx + a_wrap(x0 + wx0 + wi, nx);
+            const size_t g = (size_t)wrap_index(y0 + wy0 + wj, ny) * nx + wrap_index(x0 + wx0 + wi, nx);
 #pragma unroll
             for (int q = 0; q < 6; q++) {
                 const double vq = na[(size_t)q * WH + k];

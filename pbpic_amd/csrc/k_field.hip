@@ -30,13 +30,6 @@ constexpr int FS_X = FT_X + 2 * FT_H, FS_Y = FT_Y + 2 * FT_H;   // 38 x 22
 constexpr int FS_N = FS_X * FS_Y;
 constexpr int FT_THREADS = 256;
 
-__device__ __forceinline__ int fwrap(int g, int n)
-{
-    while (g < 0) g += n;
-    while (g >= n) g -= n;
-    return g;
-}
-
 // SPLIT: the stencil differences are taken on the FP64 state, rounded to FP32,
 // and the update increments are formed in FP32 and accumulated into FP64.
 template <typename RJ, bool SPLIT>
@@ -54,7 +47,7 @@ __global__ __launch_bounds__(FT_THREADS) void fdtd_kernel(FdtdParams P)
 
     for (int k = tid; k < FS_N; k += FT_THREADS) {
         const int v = k / FS_X, u = k - v * FS_X;
-        const size_t g = (size_t)fwrap(y0 + v, ny) * nx + fwrap(x0 + u, nx);
+        const size_t g = (size_t)wrap_index(y0 + v, ny) * nx + wrap_index(x0 + u, nx);
         se[0][k] = P.ex[g]; se[1][k] = P.ey[g]; se[2][k] = P.ez[g];
         sb[0][k] = P.bx[g]; sb[1][k] = P.by[g]; sb[2][k] = P.bz[g];
     }
@@ -91,7 +84,7 @@ __global__ __launch_bounds__(FT_THREADS) void fdtd_kernel(FdtdParams P)
     for (int k = tid; k < (FS_X - 4) * (FS_Y - 4); k += FT_THREADS) {
         const int v = k / (FS_X - 4) + 2, u = k - (v - 2) * (FS_X - 4) + 2;
         const int c = v * FS_X + u;
-        const int gx = fwrap(x0 + u, nx), gy = fwrap(y0 + v, ny);
+        const int gx = wrap_index(x0 + u, nx), gy = wrap_index(y0 + v, ny);
         const size_t g = (size_t)gy * nx + gx;
         const double jxr = (double)jx[g], jyr = (double)jy[g], jzr = (double)jz[g];
         const double exo = se[0][c], eyo = se[1][c], ezo = se[2][c];
@@ -134,13 +127,13 @@ __global__ __launch_bounds__(FT_THREADS) void fdtd_kernel(FdtdParams P)
         // mirror lies outside it feeds only guard rows of B^{n+1}.
         bool any = false;
         for (int v = 2; v < FS_Y - 2; v++) {
-            const int gy = fwrap(y0 + v, ny);
+            const int gy = wrap_index(y0 + v, ny);
             any |= gy >= P.nyp;
         }
         if (any) {
             for (int k = tid; k < (FS_X - 4) * (FS_Y - 4); k += FT_THREADS) {
                 const int v = k / (FS_X - 4) + 2, u = k - (v - 2) * (FS_X - 4) + 2;
-                const int gy = fwrap(y0 + v, ny);
+                const int gy = wrap_index(y0 + v, ny);
                 const int c = v * FS_X + u;
                 if (gy == P.nyp && v - 1 >= 2) se[2][c] = -se[2][c - FS_X];
                 else if (gy == P.nyp + 1 && v + 1 < FS_Y - 2) se[2][c] = -se[2][c + FS_X];

@@ -34,13 +34,6 @@
 
 namespace pbpic {
 
-__device__ __forceinline__ int wrapi(int g, int n)
-{
-    while (g < 0) g += n;
-    while (g >= n) g -= n;
-    return g;
-}
-
 template <typename R>
 __device__ __forceinline__ void lds_add(R *p, R v)
 {
@@ -83,7 +76,7 @@ __global__ __launch_bounds__(256) void nodefield_kernel(PushParams P, R *out)
     const double *pl[6] = {P.ex, P.ey, P.ez, P.bx, P.by, P.bz};
     for (int k = threadIdx.x; k < (NF_T + 1) * (NF_T + 1); k += 256) {
         const int r = k / (NF_T + 1), cc = k - r * (NF_T + 1);
-        const size_t g = (size_t)wrapi(Y0 - 1 + r, ny) * nx + wrapi(X0 - 1 + cc, nx);
+        const size_t g = (size_t)wrap_index(Y0 - 1 + r, ny) * nx + wrap_index(X0 - 1 + cc, nx);
 #pragma unroll
         for (int q = 0; q < 6; q++) t[q][r][cc] = pl[q][g];
     }
@@ -759,7 +752,7 @@ __device__ __forceinline__ void push_tile(const PushParams &P, int t)
         const R *gnf = (const R *)P.nf;
         for (int k = tid; k < WH; k += PUSH_THREADS) {
             const int wi = k / H, wj = k - wi * H;
-            const int gx = wrapi(x0 + wx0 + wi, nx), gy = wrapi(y0 + wy0 + wj, ny);
+            const int gx = wrap_index(x0 + wx0 + wi, nx), gy = wrap_index(y0 + wy0 + wj, ny);
             copy_node<R>(nf + (size_t)k * NS, gnf + ((size_t)gx * ny + gy) * NS);
         }
         for (int k = tid; k < njc * WH; k += PUSH_THREADS) jw[k] = 0.0;
@@ -1005,7 +998,7 @@ __device__ __forceinline__ void push_tile(const PushParams &P, int t)
         const double isx = 1.0 / S.jsx, isy = 1.0 / S.jsy, isz = 1.0 / S.jsz, isr = 1.0 / S.jsr;
         for (int k = tid; k < WH; k += PUSH_THREADS) {
             const int wj = k / W, wi = k - wj * W;
-            const int gx = wrapi(x0 + wx0 + wi, nx), gy = wrapi(y0 + wy0 + wj, ny);
+            const int gx = wrap_index(x0 + wx0 + wi, nx), gy = wrap_index(y0 + wy0 + wj, ny);
             const size_t g = (size_t)gy * nx + gx;
             if constexpr (SPLIT) {
                 const long long *jb = reinterpret_cast<const long long *>(jw);
@@ -1053,9 +1046,9 @@ __device__ __forceinline__ void push_tile(const PushParams &P, int t)
         int4 nb = make_int4(0, -1, 0, -1);
         const int ex = -v[5] - v[4], ey = -v[7] - v[6];
         if (any && aerr == 0 && ex < nxh && ey < nyh) {
-            nb.x = wrapi(x0 + wx0 + v[4], nx);
+            nb.x = wrap_index(x0 + wx0 + v[4], nx);
             nb.y = ex;
-            nb.z = wrapi(y0 + wy0 + v[6], ny);
+            nb.z = wrap_index(y0 + wy0 + v[6], ny);
             nb.w = ey;
         }
         *reinterpret_cast<int4 *>(S.tbox + 4 * (t - S.tile0)) = nb;
@@ -1109,7 +1102,7 @@ __global__ void deposit_batch_kernel(PushParams P, int64_t n, const int32_t *i1,
     if ((P.compat & PBPIC_COMPAT_JY_DROP) && i2[k] == nx + 1) d.jyb1 = (R)0;
     const int cx = i1[k] - 1, cy = j1[k] - 1;
     auto at = [&](int ddx, int ddy) -> size_t {
-        return (size_t)wrapi(cy + ddy, ny) * nx + wrapi(cx + ddx, nx);
+        return (size_t)wrap_index(cy + ddy, ny) * nx + wrap_index(cx + ddx, nx);
     };
     R *gjx = (R *)P.jx, *gjy = (R *)P.jy, *gjz = (R *)P.jz, *grho = (R *)P.rho;
     gadd(gjx + at(0, 0), (double)d.jxa0);

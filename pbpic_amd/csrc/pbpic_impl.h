@@ -58,6 +58,14 @@ enum : int {
 
 constexpr int MAX_SPECIES = 4;
 
+// g mod n for any int g (periodic lattice index).
+__device__ __forceinline__ int wrap_index(int g, int n)
+{
+    while (g < 0) g += n;
+    while (g >= n) g -= n;
+    return g;
+}
+
 // Workgroups are dispatched round-robin over the 8 XCDs.  Block b of nb is
 // given work item xcd_remap(b, nb) so that each XCD takes a contiguous range
 // (a bijection of [0, nb)): neighbouring items, which share field halos or
