@@ -522,14 +522,19 @@ int set_particles_t(pbpic_ctx *c, int32_t s, int64_t n, const int32_t *grid, con
     std::vector<double> vd;
     std::vector<float> vf;
     for (int k = 0; k < 5; k++) {
-        if (c->rsz == 8) {
+        if ((int)sizeof(H) == c->rsz) {
+            // the caller's arrays have the device type: copied as they are
+            // (the stream is synchronized before returning)
+            HIPCHK(c, hipMemcpyAsync(dst[k], src[k], sizeof(H) * n, hipMemcpyHostToDevice, c->st));
+        } else if (c->rsz == 8) {
             vd.assign(src[k], src[k] + n);
             HIPCHK(c, hipMemcpyAsync(dst[k], vd.data(), 8 * n, hipMemcpyHostToDevice, c->st));
+            HIPCHK(c, hipStreamSynchronize(c->st));
         } else {
             vf.assign(src[k], src[k] + n);
             HIPCHK(c, hipMemcpyAsync(dst[k], vf.data(), 4 * n, hipMemcpyHostToDevice, c->st));
+            HIPCHK(c, hipStreamSynchronize(c->st));
         }
-        HIPCHK(c, hipStreamSynchronize(c->st));
     }
     if (S.pid[b]) {
         std::vector<int32_t> ids((size_t)n);
@@ -563,7 +568,9 @@ int get_particles_t(pbpic_ctx *c, int32_t s, int64_t cap, int64_t *nout, int32_t
     std::vector<float> vf;
     for (int k = 0; k < 5; k++) {
         if (!dst[k]) continue;
-        if (c->rsz == 8) {
+        if ((int)sizeof(H) == c->rsz) {
+            HIPCHK(c, hipMemcpy(dst[k], src[k], sizeof(H) * n, hipMemcpyDeviceToHost));
+        } else if (c->rsz == 8) {
             vd.resize((size_t)n);
             HIPCHK(c, hipMemcpy(vd.data(), src[k], 8 * n, hipMemcpyDeviceToHost));
             for (int64_t i = 0; i < n; i++) dst[k][i] = (H)vd[(size_t)i];
