@@ -445,3 +445,30 @@ def test_tile_box_of_half_the_domain(gpu, precision):
     scale = max(np.linalg.norm(o.get_field(f)) for f in ("jx", "jy", "jz"))
     for f in ("jx", "jy", "jz"):
         assert np.linalg.norm(ctx.get_field(f) - o.get_field(f)) < tol * scale, f
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_steps_with_an_empty_species(gpu, precision):
+    """A species with no particles beside one with a ragged last tile (3840
+    electrons): 12 steps with a sort every 5 against the oracle, and the
+    empty species' particle I/O and counts."""
+    _say(f"empty species precision={precision}")
+    case = make_case(nx=24, ny=20, ppc=8, seed=3)
+    e = case["species"][0]
+    case["species"][0] = {k: v[:0] for k, v in e.items()}
+    ctx = gpu_for(case, precision=precision, compat=7, sort_every=5)
+    o = oracle_for(case, compat=7)
+    assert ctx.num_particles(0) == 0
+    ctx.countingsort(); o.countingsort()
+    ctx.step(12)
+    o.step(12, sort_every=5)
+    d, od = ctx.diag(), o.diag()
+    tol = 1e-10 if precision == 0 else 1e-4
+    for key in ("ke_e", "w_e", "w_b", "je_int"):
+        assert abs(d[key] - od[key]) <= tol * abs(od[key]), (key, d[key], od[key])
+    assert d["ke_i"] == 0.0
+    g = ctx.get_particles(0)
+    assert len(g["grid"]) == 0
+    assert ctx.energy_spectrum(0).sum() == 0
+    counts, perm = ctx.countingsort(0, want=True)
+    assert counts.sum() == 0 and len(perm) == 0
