@@ -230,15 +230,17 @@ def _by_pid(sp):
     return {k: v[order] for k, v in sp.items()}
 
 
+@pytest.mark.parametrize("nx,ny", [(32, 64), (45, 37)])
 @pytest.mark.parametrize("precision,compat", [(0, 7), (0, 0), (1, 7)])
-def test_one_step_full_tiles(gpu, precision, compat):
-    """Sorted species of 32 full 4096-particle tiles (32 x 64 cells, 64 per
-    cell: one 8 x 8 supertile per tile), 12 of them interior: the push of
-    whole tiles in the push order against the oracle's push of the same
-    particles in cell order, matched by particle id.  FP64 to 1e-12; split
-    precision to the bars of test_one_step_split32."""
-    _say(f"one step full tiles precision={precision} compat={compat}")
-    case = make_case(nx=32, ny=64, ppc=64, seed=31, efield=0.05, bfield=0.5)
+def test_one_step_full_tiles(gpu, precision, compat, nx, ny):
+    """Sorted species of full 4096-particle tiles, 64 particles per cell: at
+    32 x 64 cells 32 tiles of one 8 x 8 supertile each, 12 of them interior;
+    at 45 x 37 (odd sizes: partial supertiles at the far edges) 26 full tiles
+    and a ragged one.  The push of whole tiles in the push order against the
+    oracle's push of the same particles in cell order, matched by particle
+    id.  FP64 to 1e-12; split precision to the bars of test_one_step_split32."""
+    _say(f"one step full tiles precision={precision} compat={compat} {nx}x{ny}")
+    case = make_case(nx=nx, ny=ny, ppc=64, seed=31, efield=0.05, bfield=0.5)
     ctx = gpu_for(case, precision=precision, compat=compat)
     o = oracle_for(case, compat=compat)
     ctx.countingsort(); o.countingsort()
