@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "pbpic_impl.h"
@@ -74,6 +75,11 @@ struct pbpic_ctx {
     std::vector<hipEvent_t> ev_pool;
     size_t ev_used = 0;
     std::vector<std::pair<int, size_t>> ev_marks;   // (slot, index of start event)
+    // host particle I/O: two pinned staging buffers (allocated on first use)
+    // and the events of the copies that last used them
+    char *stage[2] = {nullptr, nullptr};
+    hipEvent_t stage_ev[2] = {nullptr, nullptr};
+    bool stage_busy[2] = {false, false};
 };
 
 namespace {
@@ -495,6 +501,97 @@ void adj_free(pbpic_ctx *c)
     c->adj.valid = false;
 }
 
+// Host particle I/O.  The caller's arrays are pageable memory: they are
+// moved in chunks through two pinned staging buffers, host threads filling or
+// draining one buffer (with the grid <-> cell shift, the validation and any
+// precision conversion) while the other is on its way over the link.
+constexpr size_t STAGE_BYTES = (size_t)64 << 20;
+
+int stage_init(pbpic_ctx *c)
+{
+    for (int k = 0; k < 2; k++) {
+        if (c->stage[k]) continue;
+        HIPCHK(c, hipHostMalloc((void **)&c->stage[k], STAGE_BYTES, hipHostMallocDefault));
+        HIPCHK(c, hipEventCreateWithFlags(&c->stage_ev[k], hipEventDisableTiming));
+        c->stage_busy[k] = false;
+    }
+    return PBPIC_OK;
+}
+
+// Host threads for m elements: up to 8, one per 256 Ki elements.
+int64_t host_threads(int64_t m)
+{
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    return std::max<int64_t>(1, std::min<int64_t>((int64_t)std::min(8u, hw), m / (1 << 18) + 1));
+}
+
+// Runs f(t, i0, i1) over [0, m) split in host_threads(m) parts (t = part).
+template <typename F>
+void host_parallel(int64_t m, F f)
+{
+    const int64_t nt = host_threads(m);
+    if (nt <= 1) { f(0, (int64_t)0, m); return; }
+    std::vector<std::thread> th;
+    const int64_t per = (m + nt - 1) / nt;
+    for (int64_t t = 0; t < nt; t++) {
+        const int64_t i0 = t * per, i1 = std::min(m, i0 + per);
+        if (i0 < i1) th.emplace_back(f, (int)t, i0, i1);
+    }
+    for (auto &x : th) x.join();
+}
+
+// n elements of type T to the device array dst; fill(T *stage, e0, i0, i1)
+// writes elements e0+i0 .. e0+i1-1 to stage[i0 .. i1).  The copies are
+// queued on the context's stream; the caller synchronizes it before the
+// caller's arrays may change.
+template <typename T, typename Fill>
+int stage_h2d(pbpic_ctx *c, T *dst, int64_t n, Fill fill)
+{
+    int rc = stage_init(c);
+    if (rc) return rc;
+    const int64_t per = (int64_t)(STAGE_BYTES / sizeof(T));
+    int k = 0;
+    for (int64_t e0 = 0; e0 < n; e0 += per, k ^= 1) {
+        const int64_t m = std::min(per, n - e0);
+        if (c->stage_busy[k]) HIPCHK(c, hipEventSynchronize(c->stage_ev[k]));
+        T *st = (T *)c->stage[k];
+        host_parallel(m, [&, st, e0](int, int64_t i0, int64_t i1) { fill(st, e0, i0, i1); });
+        HIPCHK(c, hipMemcpyAsync(dst + e0, st, sizeof(T) * m, hipMemcpyHostToDevice, c->st));
+        HIPCHK(c, hipEventRecord(c->stage_ev[k], c->st));
+        c->stage_busy[k] = true;
+    }
+    return PBPIC_OK;
+}
+
+// n elements of type T from the device array src; drain(const T *stage, e0,
+// i0, i1) consumes elements e0+i0 .. e0+i1-1 from stage[i0 .. i1).
+template <typename T, typename Drain>
+int stage_d2h(pbpic_ctx *c, const T *src, int64_t n, Drain drain)
+{
+    int rc = stage_init(c);
+    if (rc) return rc;
+    const int64_t per = (int64_t)(STAGE_BYTES / sizeof(T));
+    int64_t pend0 = -1, pendm = 0;       // chunk waiting in buffer k ^ 1
+    int k = 0;
+    auto flush = [&](int b, int64_t e0, int64_t m) -> int {
+        HIPCHK(c, hipEventSynchronize(c->stage_ev[b]));
+        const T *st = (const T *)c->stage[b];
+        host_parallel(m, [&, st, e0](int, int64_t i0, int64_t i1) { drain(st, e0, i0, i1); });
+        return PBPIC_OK;
+    };
+    for (int64_t e0 = 0; e0 < n; e0 += per, k ^= 1) {
+        const int64_t m = std::min(per, n - e0);
+        HIPCHK(c, hipMemcpyAsync(c->stage[k], src + e0, sizeof(T) * m, hipMemcpyDeviceToHost, c->st));
+        HIPCHK(c, hipEventRecord(c->stage_ev[k], c->st));
+        c->stage_busy[k] = true;
+        if (pend0 >= 0 && (rc = flush(k ^ 1, pend0, pendm))) return rc;
+        pend0 = e0;
+        pendm = m;
+    }
+    if (pend0 >= 0 && (rc = flush(k ^ 1, pend0, pendm))) return rc;
+    return PBPIC_OK;
+}
+
 template <typename H>
 int set_particles_t(pbpic_ctx *c, int32_t s, int64_t n, const int32_t *grid, const H *dx,
                     const H *dy, const H *ux, const H *uy, const H *uz, const int32_t *pid)
@@ -505,41 +602,49 @@ int set_particles_t(pbpic_ctx *c, int32_t s, int64_t n, const int32_t *grid, con
     if (n < 0 || n > S.cap) return fail(c, PBPIC_ERR_CAPACITY, "n=%lld exceeds capacity %lld", (long long)n, (long long)S.cap);
     if (n > 0 && (!grid || !dx || !dy || !ux || !uy || !uz)) return fail(c, PBPIC_ERR_ARG, "null particle array");
     const int64_t nc = ncell(c);
-    std::vector<int32_t> cell((size_t)n);
     const int ny = c->p.ny, nyp = nyp_of(c);
-    for (int64_t i = 0; i < n; i++) {
-        if (grid[i] < 1 || grid[i] > nc) return fail(c, PBPIC_ERR_CELL, "grid[%lld]=%d out of 1..%lld", (long long)i, grid[i], (long long)nc);
-        if ((grid[i] - 1) % ny >= nyp)
+    // every grid value checked before any state changes: in 1..nc and, with
+    // conducting walls, not in a guard row (first offender per host thread)
+    std::vector<int64_t> first((size_t)host_threads(n), -1);
+    host_parallel(n, [&](int t, int64_t i0, int64_t i1) {
+        for (int64_t i = i0; i < i1; i++) {
+            const int32_t g = grid[i];
+            if (g < 1 || g > nc || (g - 1) % ny >= nyp) { first[(size_t)t] = i; return; }
+        }
+    });
+    for (int64_t bad : first) {
+        if (bad < 0) continue;
+        if (grid[bad] >= 1 && grid[bad] <= nc)
             return fail(c, PBPIC_ERR_CELL, "grid[%lld]=%d lies in a guard row of the conducting walls",
-                        (long long)i, grid[i]);
-        cell[(size_t)i] = grid[i] - 1;
+                        (long long)bad, grid[bad]);
+        return fail(c, PBPIC_ERR_CELL, "grid[%lld]=%d out of 1..%lld", (long long)bad, grid[bad], (long long)nc);
     }
     const int b = S.cur;
     S.tbox_ok = false;
-    HIPCHK(c, hipMemcpyAsync(S.cell[b], cell.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    rc = stage_h2d<int32_t>(c, S.cell[b], n, [grid](int32_t *st, int64_t e0, int64_t i0, int64_t i1) {
+        for (int64_t i = i0; i < i1; i++) st[i] = grid[e0 + i] - 1;
+    });
+    if (rc) return rc;
     const H *src[5] = {dx, dy, ux, uy, uz};
     void *dst[5] = {S.ox[b], S.oy[b], S.ux[b], S.uy[b], S.uz[b]};
-    std::vector<double> vd;
-    std::vector<float> vf;
     for (int k = 0; k < 5; k++) {
-        if ((int)sizeof(H) == c->rsz) {
-            // the caller's arrays have the device type: copied as they are
-            // (the stream is synchronized before returning)
-            HIPCHK(c, hipMemcpyAsync(dst[k], src[k], sizeof(H) * n, hipMemcpyHostToDevice, c->st));
-        } else if (c->rsz == 8) {
-            vd.assign(src[k], src[k] + n);
-            HIPCHK(c, hipMemcpyAsync(dst[k], vd.data(), 8 * n, hipMemcpyHostToDevice, c->st));
-            HIPCHK(c, hipStreamSynchronize(c->st));
-        } else {
-            vf.assign(src[k], src[k] + n);
-            HIPCHK(c, hipMemcpyAsync(dst[k], vf.data(), 4 * n, hipMemcpyHostToDevice, c->st));
-            HIPCHK(c, hipStreamSynchronize(c->st));
-        }
+        const H *a = src[k];
+        if (c->rsz == 8)
+            rc = stage_h2d<double>(c, (double *)dst[k], n, [a](double *st, int64_t e0, int64_t i0, int64_t i1) {
+                for (int64_t i = i0; i < i1; i++) st[i] = (double)a[e0 + i];
+            });
+        else
+            rc = stage_h2d<float>(c, (float *)dst[k], n, [a](float *st, int64_t e0, int64_t i0, int64_t i1) {
+                for (int64_t i = i0; i < i1; i++) st[i] = (float)a[e0 + i];
+            });
+        if (rc) return rc;
     }
     if (S.pid[b]) {
-        std::vector<int32_t> ids((size_t)n);
-        for (int64_t i = 0; i < n; i++) ids[(size_t)i] = pid ? pid[i] : (int32_t)(i + 1);
-        HIPCHK(c, hipMemcpyAsync(S.pid[b], ids.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, c->st));
+        rc = stage_h2d<int32_t>(c, S.pid[b], n, [pid](int32_t *st, int64_t e0, int64_t i0, int64_t i1) {
+            for (int64_t i = i0; i < i1; i++) st[i] = pid ? pid[e0 + i] : (int32_t)(e0 + i + 1);
+        });
+        if (rc) return rc;
     }
     HIPCHK(c, hipStreamSynchronize(c->st));
     S.n = n;
@@ -558,31 +663,35 @@ int get_particles_t(pbpic_ctx *c, int32_t s, int64_t cap, int64_t *nout, int32_t
     const int64_t n = S.n;
     if (cap < n) return fail(c, PBPIC_ERR_CAPACITY, "host capacity %lld < %lld particles", (long long)cap, (long long)n);
     const int b = S.cur;
-    if (grid) {
-        HIPCHK(c, hipMemcpy(grid, S.cell[b], sizeof(int32_t) * n, hipMemcpyDeviceToHost));
-        for (int64_t i = 0; i < n; i++) grid[i] += 1;
-    }
+    if (grid)
+        if ((rc = stage_d2h<int32_t>(c, S.cell[b], n, [grid](const int32_t *st, int64_t e0, int64_t i0, int64_t i1) {
+                 for (int64_t i = i0; i < i1; i++) grid[e0 + i] = st[i] + 1;
+             })))
+            return rc;
     H *dst[5] = {dx, dy, ux, uy, uz};
     void *src[5] = {S.ox[b], S.oy[b], S.ux[b], S.uy[b], S.uz[b]};
-    std::vector<double> vd;
-    std::vector<float> vf;
     for (int k = 0; k < 5; k++) {
-        if (!dst[k]) continue;
-        if ((int)sizeof(H) == c->rsz) {
-            HIPCHK(c, hipMemcpy(dst[k], src[k], sizeof(H) * n, hipMemcpyDeviceToHost));
-        } else if (c->rsz == 8) {
-            vd.resize((size_t)n);
-            HIPCHK(c, hipMemcpy(vd.data(), src[k], 8 * n, hipMemcpyDeviceToHost));
-            for (int64_t i = 0; i < n; i++) dst[k][i] = (H)vd[(size_t)i];
-        } else {
-            vf.resize((size_t)n);
-            HIPCHK(c, hipMemcpy(vf.data(), src[k], 4 * n, hipMemcpyDeviceToHost));
-            for (int64_t i = 0; i < n; i++) dst[k][i] = (H)vf[(size_t)i];
-        }
+        H *a = dst[k];
+        if (!a) continue;
+        if (c->rsz == 8)
+            rc = stage_d2h<double>(c, (const double *)src[k], n, [a](const double *st, int64_t e0, int64_t i0, int64_t i1) {
+                for (int64_t i = i0; i < i1; i++) a[e0 + i] = (H)st[i];
+            });
+        else
+            rc = stage_d2h<float>(c, (const float *)src[k], n, [a](const float *st, int64_t e0, int64_t i0, int64_t i1) {
+                for (int64_t i = i0; i < i1; i++) a[e0 + i] = (H)st[i];
+            });
+        if (rc) return rc;
     }
     if (pid) {
-        if (S.pid[b]) HIPCHK(c, hipMemcpy(pid, S.pid[b], sizeof(int32_t) * n, hipMemcpyDeviceToHost));
-        else for (int64_t i = 0; i < n; i++) pid[i] = 0;
+        if (S.pid[b]) {
+            if ((rc = stage_d2h<int32_t>(c, S.pid[b], n, [pid](const int32_t *st, int64_t e0, int64_t i0, int64_t i1) {
+                     for (int64_t i = i0; i < i1; i++) pid[e0 + i] = st[i];
+                 })))
+                return rc;
+        } else {
+            for (int64_t i = 0; i < n; i++) pid[i] = 0;
+        }
     }
     return PBPIC_OK;
 }
@@ -700,6 +809,10 @@ int pbpic_ctx_destroy(pbpic_ctx *c)
     for (auto p : c->adj.ckpts) f(p);
     f(c->adj.final_state);
     for (auto e : c->ev_pool) (void)hipEventDestroy(e);
+    for (int k = 0; k < 2; k++) {
+        if (c->stage[k]) (void)hipHostFree(c->stage[k]);
+        if (c->stage_ev[k]) (void)hipEventDestroy(c->stage_ev[k]);
+    }
     if (c->st) (void)hipStreamDestroy(c->st);
     delete c;
     return PBPIC_OK;
