@@ -90,21 +90,22 @@ def test_reference_files_from_a_run(gpu, tmp_path):
     assert np.array_equal(p["ux"][:n0], g0["ux"].astype(np.float32))
 
 
+@pytest.mark.parametrize("walls", [False, True])
 @pytest.mark.parametrize("precision", [0, 1])
-def test_checkpoint_restart(gpu, tmp_path, precision):
+def test_checkpoint_restart(gpu, tmp_path, precision, walls):
     """save_checkpoint after 12 steps, load into a fresh context: identical
     state (fields, particles at the device precision, step count, J.E); 8 more
     steps from both agree (FP64 to 1e-12; the split form to FP32 rounding),
     including the sort at step 20 that the restored step count schedules."""
-    _say(f"checkpoint precision={precision}")
-    case = make_case(nx=24, ny=20, ppc=8, seed=9)
+    _say(f"checkpoint precision={precision} walls={walls}")
+    case = make_case(nx=24, ny=20, ppc=8, seed=9, walls=walls)
     ctx = gpu_for(case, precision=precision, compat=0, sort_every=10)
     ctx.countingsort()
     ctx.step(12)
     path = str(tmp_path / "ck.h5")
     dg.save_checkpoint(ctx, path)
-    ctx2 = gpu_for(make_case(nx=24, ny=20, ppc=8, seed=99), precision=precision, compat=0,
-                   sort_every=10)
+    ctx2 = gpu_for(make_case(nx=24, ny=20, ppc=8, seed=99, walls=walls), precision=precision,
+                   compat=0, sort_every=10)
     dg.load_checkpoint(ctx2, path)
     assert ctx2.counters() == ctx.counters()
     for k in ("ex", "ey", "ez", "bx", "by", "bz"):
