@@ -27,10 +27,13 @@ def _spectrum_np(p, ms, nbins, de, emax):
     return np.bincount(k, minlength=nbins)
 
 
+@pytest.mark.parametrize("walls", [False, True])
 @pytest.mark.parametrize("precision", [0, 1])
-def test_energy_spectrum_and_density(gpu, precision):
-    _say(f"spectrum/density precision={precision}")
-    case = make_case(nx=40, ny=36, ppc=20, seed=4, vth=(0.1, 0.5))
+def test_energy_spectrum_and_density(gpu, precision, walls):
+    """With conducting walls the guard rows hold no particles: their counts
+    are zero."""
+    _say(f"spectrum/density precision={precision} walls={walls}")
+    case = make_case(nx=40, ny=36, ppc=20, seed=4, vth=(0.1, 0.5), walls=walls)
     ctx = gpu_for(case, precision=precision, compat=0)
     for stage in ("loaded", "pushed"):
         if stage == "pushed":
@@ -47,6 +50,8 @@ def test_energy_spectrum_and_density(gpu, precision):
         for s in range(2):
             nd += np.bincount(ctx.get_particles(s)["grid"] - 1, minlength=nd.size)
         assert np.array_equal(ctx.number_density(), nd.reshape(case["nx"], case["ny"]).T), stage
+        if walls:
+            assert not ctx.number_density()[case["ny"] - 2:].any(), stage
 
 
 def test_reference_files_from_a_run(gpu, tmp_path):
