@@ -75,9 +75,11 @@ struct pbpic_ctx {
     std::vector<hipEvent_t> ev_pool;
     size_t ev_used = 0;
     std::vector<std::pair<int, size_t>> ev_marks;   // (slot, index of start event)
-    // host particle I/O: two pinned staging buffers (allocated on first use)
-    // and the events of the copies that last used them
+    // host particle I/O: two pinned staging buffers of stage_bytes each
+    // (grown on demand up to STAGE_BYTES) and the events of the copies that
+    // last used them
     char *stage[2] = {nullptr, nullptr};
+    size_t stage_bytes = 0;
     hipEvent_t stage_ev[2] = {nullptr, nullptr};
     bool stage_busy[2] = {false, false};
 };
@@ -503,18 +505,27 @@ void adj_free(pbpic_ctx *c)
 
 // Host particle I/O.  The caller's arrays are pageable memory: they are
 // moved in chunks through two pinned staging buffers, host threads filling or
-// draining one buffer (with the grid <-> cell shift, the validation and any
-// precision conversion) while the other is on its way over the link.
+// draining one buffer (with the grid <-> cell shift and any precision
+// conversion) while the other is on its way over the link.
 constexpr size_t STAGE_BYTES = (size_t)64 << 20;
 
-int stage_init(pbpic_ctx *c)
+// Staging buffers of at least min(want, STAGE_BYTES) bytes each (1 MiB
+// granules), so that a small context pins little host memory.
+int stage_init(pbpic_ctx *c, size_t want)
 {
+    const size_t need = std::min(STAGE_BYTES, ((want + (1u << 20) - 1) >> 20) << 20);
+    if (c->stage[0] && c->stage_bytes >= need) return PBPIC_OK;
     for (int k = 0; k < 2; k++) {
-        if (c->stage[k]) continue;
-        HIPCHK(c, hipHostMalloc((void **)&c->stage[k], STAGE_BYTES, hipHostMallocDefault));
-        HIPCHK(c, hipEventCreateWithFlags(&c->stage_ev[k], hipEventDisableTiming));
+        if (c->stage[k]) {
+            if (c->stage_busy[k]) HIPCHK(c, hipEventSynchronize(c->stage_ev[k]));
+            HIPCHK(c, hipHostFree(c->stage[k]));
+            c->stage[k] = nullptr;
+        }
+        HIPCHK(c, hipHostMalloc((void **)&c->stage[k], need, hipHostMallocDefault));
+        if (!c->stage_ev[k]) HIPCHK(c, hipEventCreateWithFlags(&c->stage_ev[k], hipEventDisableTiming));
         c->stage_busy[k] = false;
     }
+    c->stage_bytes = need;
     return PBPIC_OK;
 }
 
@@ -547,9 +558,10 @@ void host_parallel(int64_t m, F f)
 template <typename T, typename Fill>
 int stage_h2d(pbpic_ctx *c, T *dst, int64_t n, Fill fill)
 {
-    int rc = stage_init(c);
+    if (n <= 0) return PBPIC_OK;
+    int rc = stage_init(c, sizeof(T) * (size_t)n);
     if (rc) return rc;
-    const int64_t per = (int64_t)(STAGE_BYTES / sizeof(T));
+    const int64_t per = (int64_t)(c->stage_bytes / sizeof(T));
     int k = 0;
     for (int64_t e0 = 0; e0 < n; e0 += per, k ^= 1) {
         const int64_t m = std::min(per, n - e0);
@@ -568,9 +580,10 @@ int stage_h2d(pbpic_ctx *c, T *dst, int64_t n, Fill fill)
 template <typename T, typename Drain>
 int stage_d2h(pbpic_ctx *c, const T *src, int64_t n, Drain drain)
 {
-    int rc = stage_init(c);
+    if (n <= 0) return PBPIC_OK;
+    int rc = stage_init(c, sizeof(T) * (size_t)n);
     if (rc) return rc;
-    const int64_t per = (int64_t)(STAGE_BYTES / sizeof(T));
+    const int64_t per = (int64_t)(c->stage_bytes / sizeof(T));
     int64_t pend0 = -1, pendm = 0;       // chunk waiting in buffer k ^ 1
     int k = 0;
     auto flush = [&](int b, int64_t e0, int64_t m) -> int {
