@@ -198,7 +198,10 @@ int pbpic_sync(pbpic_ctx *ctx);
 
 /* Add a particle species.  q = +1 (ion, mass mime) or -1 (electron, mass 1),
  * the reference's per-particle charge `ptl%q` and mass rule (:964-969).
- * capacity = maximum number of particles.  *species receives its index.     */
+ * capacity = maximum number of particles.  *species receives its index.
+ * Device memory per particle of capacity: 2 x 44 B (FP64) or 2 x 24 B
+ * (split) of particle arrays, plus sort scratch of 16 B, and in the split
+ * form a further 24 B (the sort's packed records).                        */
 int pbpic_add_species(pbpic_ctx *ctx, int32_t q, int64_t capacity,
                       int32_t *species);
 int pbpic_num_particles(pbpic_ctx *ctx, int32_t species, int64_t *n);

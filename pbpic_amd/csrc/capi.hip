@@ -818,7 +818,7 @@ int pbpic_ctx_destroy(pbpic_ctx *c)
         }
     for (auto &S : c->sp) f(S.tbox);
     for (int b = 0; b < 2; b++) { f(c->sort.keys[b]); f(c->sort.vals[b]); }
-    f(c->sort.status); f(c->sort.dhist); f(c->sort.ticket); f(c->sort.counts);
+    f(c->sort.status); f(c->sort.dhist); f(c->sort.ticket); f(c->sort.counts); f(c->sort.rec);
     for (auto p : c->adj.ckpts) f(p);
     f(c->adj.final_state);
     for (auto e : c->ev_pool) (void)hipEventDestroy(e);
@@ -872,6 +872,11 @@ int pbpic_add_species(pbpic_ctx *c, int32_t q, int64_t capacity, int32_t *specie
             if (c->sort.vals[b]) (void)hipFree(c->sort.vals[b]);
             HIPCHK(c, dalloc(&c->sort.keys[b], n * 4));
             HIPCHK(c, dalloc(&c->sort.vals[b], n * 4));
+        }
+        if (c->rsz == 4) {
+            // split form: the sort's gather reads 24-byte particle records
+            if (c->sort.rec) (void)hipFree(c->sort.rec);
+            HIPCHK(c, dalloc(&c->sort.rec, n * 24));
         }
         if (c->sort.status) (void)hipFree(c->sort.status);
         const int64_t ntile = (capacity + 4095) / 4096 + 1;

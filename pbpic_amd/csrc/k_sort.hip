@@ -300,6 +300,76 @@ __global__ __launch_bounds__(256) void gather_kernel(const uint32_t *perm, int64
     if (src.pid) gmove<G>(dst.pid + i0, src.pid, p, contig);
 }
 
+// Split form: the gather reads each source particle as one 24-byte record
+// (cell and the five FP32 values), packed by a coalesced pass, instead of six
+// scattered 4-byte reads from six arrays.  15% faster sorts at 2048^2 x 100
+// ppc; the FP64 form keeps the array-wise gather (its 48-byte records were
+// slower).
+struct PartRec {
+    int32_t c;
+    float a[5];
+};
+
+__global__ __launch_bounds__(256) void pack_records_kernel(SpeciesDev src, PartRec *rec, int64_t n)
+{
+    const int64_t i = (int64_t)xcd_remap(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
+    if (i >= n) return;
+    PartRec r;
+    r.c = src.cell[i];
+    r.a[0] = ((const float *)src.ox)[i];
+    r.a[1] = ((const float *)src.oy)[i];
+    r.a[2] = ((const float *)src.ux)[i];
+    r.a[3] = ((const float *)src.uy)[i];
+    r.a[4] = ((const float *)src.uz)[i];
+    rec[i] = r;
+}
+
+// As gather_kernel<float>, reading the packed records.
+__global__ __launch_bounds__(256) void gather_records_kernel(const uint32_t *perm, int64_t n,
+                                                             int tv, uint32_t *perm_out,
+                                                             const PartRec *rec, SpeciesDev src,
+                                                             SpeciesDev dst)
+{
+    constexpr int G = 4;
+    const int64_t i0 = ((int64_t)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x) * G;
+    if (i0 >= n) return;
+    float *out[5] = {(float *)dst.ox, (float *)dst.oy, (float *)dst.ux, (float *)dst.uy,
+                     (float *)dst.uz};
+    if (i0 + G > n) {                        // ragged end: element by element
+        for (int64_t i = i0; i < n; i++) {
+            const uint32_t p = perm[i];
+            if (perm_out) perm_out[i] = p;
+            const PartRec r = rec[p];
+            dst.cell[i] = r.c;
+            for (int q = 0; q < 5; q++) out[q][i] = r.a[q];
+            if (src.pid) dst.pid[i] = src.pid[p];
+        }
+        return;
+    }
+    int64_t k0 = i0;
+    if (tv) {
+        const int64_t t0 = i0 & ~(int64_t)(PUSH_TILE - 1);
+        if (t0 + PUSH_TILE <= n) k0 = t0 + tile_rank((int)(i0 - t0), tv);
+    }
+    uint32_t p[G];
+    *reinterpret_cast<uint4 *>(p) = *reinterpret_cast<const uint4 *>(perm + k0);
+    if (perm_out) *reinterpret_cast<uint4 *>(perm_out + i0) = *reinterpret_cast<const uint4 *>(p);
+    int c[G];
+    float a[5][G];
+#pragma unroll
+    for (int j = 0; j < G; j++) {
+        const PartRec r = rec[p[j]];
+        c[j] = r.c;
+#pragma unroll
+        for (int q = 0; q < 5; q++) a[q][j] = r.a[q];
+    }
+    *reinterpret_cast<int4 *>(dst.cell + i0) = make_int4(c[0], c[1], c[2], c[3]);
+#pragma unroll
+    for (int q = 0; q < 5; q++)
+        *reinterpret_cast<float4 *>(out[q] + i0) = make_float4(a[q][0], a[q][1], a[q][2], a[q][3]);
+    if (src.pid) gmove<G>(dst.pid + i0, src.pid, p, false);
+}
+
 // Adds species sp's particles per cell (compact cell id) to counts, as
 // numdensity (pic2d.f90:1343-1354): through the tile boxes of the last push
 // when they are current, else with run-aggregated atomics.
@@ -388,10 +458,17 @@ hipError_t sort_species(SortScratch &s, const SpeciesDev &sp, int precision, int
     if (tv) s.perm = pout = s.vals[pb];
     const int G = precision == PBPIC_PREC_FP64 ? 2 : 4;        // 16 bytes of R per lane
     const unsigned nb = (unsigned)(((n + G - 1) / G + 255) / 256);
-    if (precision == PBPIC_PREC_FP64)
+    if (precision == PBPIC_PREC_FP64) {
         hipLaunchKernelGGL(gather_kernel<double>, dim3(nb), dim3(256), 0, st, perm, n, tv, pout, sp, dst);
-    else
+    } else if (s.rec) {
+        PartRec *rec = (PartRec *)s.rec;
+        hipLaunchKernelGGL(pack_records_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
+                           sp, rec, n);
+        hipLaunchKernelGGL(gather_records_kernel, dim3(nb), dim3(256), 0, st, perm, n, tv, pout,
+                           (const PartRec *)rec, sp, dst);
+    } else {
         hipLaunchKernelGGL(gather_kernel<float>, dim3(nb), dim3(256), 0, st, perm, n, tv, pout, sp, dst);
+    }
     return hipGetLastError();
 }
 

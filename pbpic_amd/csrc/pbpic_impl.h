@@ -281,6 +281,7 @@ struct SortScratch {
     uint32_t *ticket = nullptr;     // one counter per pass
     int32_t *counts = nullptr;      // ncell
     const uint32_t *perm = nullptr; // permutation of the last sort (in vals)
+    void *rec = nullptr;            // split form: one 24-byte record per particle
     int64_t cap = 0, status_cap = 0;
 };
 hipError_t launch_push_adj(const AdjParams &p, int precision, hipStream_t st);
