@@ -28,6 +28,7 @@ struct Species {
     int cur = 0;
     int32_t *tbox = nullptr;   // per push tile bounding boxes (SpeciesDev::tbox)
     bool tbox_ok = false;      // cleared by anything that moves or rewrites particles
+    void *rec = nullptr;       // split form: the sort's PartRec buffer (cap records)
 };
 
 }  // namespace
@@ -151,7 +152,9 @@ SpeciesDev species_dev(pbpic_ctx *c, int s, int buf)
     return d;
 }
 
-PushParams push_params(pbpic_ctx *c)
+// recs: the push writes the split form's particles as the sort's records
+// (the sort follows in the same step).
+PushParams push_params(pbpic_ctx *c, bool recs = false)
 {
     PushParams P{};
     P.nx = c->p.nx;
@@ -184,6 +187,8 @@ PushParams push_params(pbpic_ctx *c)
         d.ox_o = c->push_out[s][1]; d.oy_o = c->push_out[s][2];
         d.ux_o = c->push_out[s][3]; d.uy_o = c->push_out[s][4]; d.uz_o = c->push_out[s][5];
     }
+    if (recs && c->rsz == 4)
+        for (int s = 0; s < P.nspecies; s++) P.sp[s].rec_o = c->sp[s].rec;
     return P;
 }
 
@@ -230,7 +235,9 @@ int valid_species(pbpic_ctx *c, int32_t s)
 
 hipError_t prof_mark(pbpic_ctx *c, int slot, bool begin);
 
-int sort_one(pbpic_ctx *c, int s, int32_t *counts, int32_t *perm, uint32_t *perm_dev = nullptr)
+// rec_written: the push of this step wrote the species' records (step_impl).
+int sort_one(pbpic_ctx *c, int s, int32_t *counts, int32_t *perm, uint32_t *perm_dev = nullptr,
+             bool rec_written = false)
 {
     Species &S = c->sp[s];
     SpeciesDev src = species_dev(c, s, S.cur), dst = species_dev(c, s, S.cur ^ 1);
@@ -239,7 +246,7 @@ int sort_one(pbpic_ctx *c, int s, int32_t *counts, int32_t *perm, uint32_t *perm
     CellKey key;
     key.init(c->p.nx, c->p.ny, po ? 1 : 0);
     HIPCHK(c, sort_species(c->sort, src, c->p.precision, po ? 1 : 0, key, ncell(c), dst, c->err,
-                           counts != nullptr, c->st));
+                           counts != nullptr, S.rec, rec_written && S.rec, c->st));
     HIPCHK(c, prof_mark(c, PBPIC_PROF_SORT, false));
     S.cur ^= 1;
     S.tbox_ok = false;
@@ -284,7 +291,7 @@ int kinetic_impl(pbpic_ctx *c)
     return PBPIC_OK;
 }
 
-int push_impl(pbpic_ctx *c)
+int push_impl(pbpic_ctx *c, bool recs = false)
 {
     int64_t ntiles = 0;
     for (auto &S : c->sp) ntiles += (S.n + PUSH_TILE - 1) / PUSH_TILE;
@@ -295,7 +302,7 @@ int push_impl(pbpic_ctx *c)
         HIPCHK(c, dalloc(&c->big, sizeof(int32_t) * (size_t)(2 * ntiles + 2)));
         c->big_cap = 2 * ntiles + 2;
     }
-    const PushParams P = push_params(c);
+    const PushParams P = push_params(c, recs);
     // node fields (emfgrid, pic2d.f90:837-845) are field work: timed as such
     HIPCHK(c, prof_mark(c, PBPIC_PROF_FIELD, true));
     HIPCHK(c, launch_nodefield(P, c->p.precision, c->st));
@@ -352,7 +359,12 @@ int step_impl(pbpic_ctx *c, void *const *jcopy, uint32_t *const *perm_dev, bool 
     HIPCHK(c, prof_mark(c, PBPIC_PROF_ZERO, true));
     HIPCHK(c, launch_zero_current(c->J[0], c->J[1], c->J[2], c->J[3], ncell(c), c->rsz, c->acc, 1, c->st));
     HIPCHK(c, prof_mark(c, PBPIC_PROF_ZERO, false));
-    int rc = push_impl(c);
+    // split form: when this step sorts, the push writes the particles as the
+    // sort's records, so the sort skips its packing pass.  Until the sort
+    // has run, the species' value arrays are stale.
+    const bool recs = c->rsz == 4 && c->p.sort_every > 0 &&
+                      (c->steps + 1) % c->p.sort_every == 0;
+    int rc = push_impl(c, recs);
     if (rc) return rc;
     if (jcopy)
         for (int k = 0; k < 3; k++)
@@ -362,7 +374,8 @@ int step_impl(pbpic_ctx *c, void *const *jcopy, uint32_t *const *perm_dev, bool 
     if (sorted) *sorted = false;
     if (c->p.sort_every > 0 && c->steps % c->p.sort_every == 0) {
         for (int s = 0; s < (int)c->sp.size(); s++)
-            if ((rc = sort_one(c, s, nullptr, nullptr, perm_dev ? perm_dev[s] : nullptr))) return rc;
+            if ((rc = sort_one(c, s, nullptr, nullptr, perm_dev ? perm_dev[s] : nullptr, recs)))
+                return rc;
         if (sorted) *sorted = true;
     }
     return PBPIC_OK;
@@ -816,9 +829,9 @@ int pbpic_ctx_destroy(pbpic_ctx *c)
         for (int b = 0; b < 2; b++) {
             f(S.cell[b]); f(S.ox[b]); f(S.oy[b]); f(S.ux[b]); f(S.uy[b]); f(S.uz[b]); f(S.pid[b]);
         }
-    for (auto &S : c->sp) f(S.tbox);
+    for (auto &S : c->sp) { f(S.tbox); f(S.rec); }
     for (int b = 0; b < 2; b++) { f(c->sort.keys[b]); f(c->sort.vals[b]); }
-    f(c->sort.status); f(c->sort.dhist); f(c->sort.ticket); f(c->sort.counts); f(c->sort.rec);
+    f(c->sort.status); f(c->sort.dhist); f(c->sort.ticket); f(c->sort.counts);
     for (auto p : c->adj.ckpts) f(p);
     f(c->adj.final_state);
     for (auto e : c->ev_pool) (void)hipEventDestroy(e);
@@ -865,6 +878,8 @@ int pbpic_add_species(pbpic_ctx *c, int32_t q, int64_t capacity, int32_t *specie
         if (p.flags & PBPIC_FLAG_TRACK_PID) HIPCHK(c, dalloc(&S.pid[b], n * 4));
     }
     HIPCHK(c, dalloc(&S.tbox, (n / PUSH_TILE + 1) * 4 * sizeof(int32_t)));
+    // split form: the sort's gather reads 24-byte particle records
+    if (c->rsz == 4) HIPCHK(c, dalloc(&S.rec, n * sizeof(PartRec)));
     // sort scratch sized for the largest species
     if (capacity > c->sort.cap) {
         for (int b = 0; b < 2; b++) {
@@ -872,11 +887,6 @@ int pbpic_add_species(pbpic_ctx *c, int32_t q, int64_t capacity, int32_t *specie
             if (c->sort.vals[b]) (void)hipFree(c->sort.vals[b]);
             HIPCHK(c, dalloc(&c->sort.keys[b], n * 4));
             HIPCHK(c, dalloc(&c->sort.vals[b], n * 4));
-        }
-        if (c->rsz == 4) {
-            // split form: the sort's gather reads 24-byte particle records
-            if (c->sort.rec) (void)hipFree(c->sort.rec);
-            HIPCHK(c, dalloc(&c->sort.rec, n * 24));
         }
         if (c->sort.status) (void)hipFree(c->sort.status);
         const int64_t ntile = (capacity + 4095) / 4096 + 1;

@@ -475,6 +475,43 @@ __device__ __forceinline__ void stu(int32_t *p, const int (&v)[4])
     __builtin_nontemporal_store(t, reinterpret_cast<nt_i4 *>(p));
 }
 
+// The split form's four particles of a lane as four PartRec records (96
+// contiguous bytes at slot i0, a multiple of 4): six 16-byte stores.  One
+// store instruction covers 16 of every 96 bytes of the wave's 6 KiB, so the
+// stores go through L2, which merges the lines (non-temporal ones ran the
+// push at a third of its speed).
+__device__ __forceinline__ void st_rec4(void *rec, int64_t i0, const int (&c)[4],
+                                        const float (&v)[5][4])
+{
+    float w[24];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        w[6 * j] = __int_as_float(c[j]);
+#pragma unroll
+        for (int q = 0; q < 5; q++) w[6 * j + 1 + q] = v[q][j];
+    }
+    float4 *p = reinterpret_cast<float4 *>((PartRec *)rec + i0);
+#pragma unroll
+    for (int k = 0; k < 6; k++) p[k] = make_float4(w[4 * k], w[4 * k + 1], w[4 * k + 2], w[4 * k + 3]);
+}
+// Bounds-checked form for the generic path.
+__device__ __forceinline__ void vst_rec4(void *rec, int64_t i0, int64_t n, const int (&c)[4],
+                                         const float (&v)[5][4])
+{
+    if (i0 + 4 <= n) {
+        st_rec4(rec, i0, c, v);
+        return;
+    }
+    PartRec *r = (PartRec *)rec;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        if (i0 + j >= n) break;
+        r[i0 + j].c = c[j];
+#pragma unroll
+        for (int q = 0; q < 5; q++) r[i0 + j].a[q] = v[q][j];
+    }
+}
+
 // One contribution to the LDS current window.  FP64 form: FP64 atomic add.
 // Split form: the contribution arrives pre-multiplied by its plane's scale
 // (a power of two, so the product is exact) and below 2^30 in magnitude; it
@@ -650,8 +687,9 @@ __device__ __forceinline__ void deposit_split_fast(double *jw, int WH, int W, in
 
 // One push tile t (one workgroup).  OLDU: the reference's Boris rotation
 // with the pre-kick momentum (PBPIC_COMPAT_BORIS_OLD_U); RHO: deposit rho as
-// well (PBPIC_FLAG_DEPOSIT_RHO).
-template <typename R, bool SPLIT, bool OLDU, bool RHO>
+// well (PBPIC_FLAG_DEPOSIT_RHO); REC (split form): write the pushed particles
+// as the sort's records at S.rec_o in place of the five value arrays.
+template <typename R, bool SPLIT, bool OLDU, bool RHO, bool REC>
 __device__ __forceinline__ void push_tile(const PushParams &P, int t)
 {
     constexpr int V = Vw<R>::V;
@@ -941,8 +979,12 @@ __device__ __forceinline__ void push_tile(const PushParams &P, int t)
             for (int j = 0; j < V; j++)
                 nc[j] = onef(ccur[j], cur[0][j], cur[1][j], cur[2][j], cur[3][j], cur[4][j]);
             stu(cq + o, nc);
-            stu(oq + o, cur[0]); stu(pq + o, cur[1]); stu(uq + o, cur[2]);
-            stu(vq + o, cur[3]); stu(wq + o, cur[4]);
+            if constexpr (REC) {
+                st_rec4(S.rec_o, tstart + o, nc, cur);
+            } else {
+                stu(oq + o, cur[0]); stu(pq + o, cur[1]); stu(uq + o, cur[2]);
+                stu(vq + o, cur[3]); stu(wq + o, cur[4]);
+            }
 #pragma unroll
             for (int j = 0; j < V; j++) {
                 ccur[j] = cnxt[j];
@@ -980,8 +1022,12 @@ __device__ __forceinline__ void push_tile(const PushParams &P, int t)
                     nc[j] = one(c, cur[0][j], cur[1][j], cur[2][j], cur[3][j], cur[4][j]);
             }
             vst(S.cell_o, i0, n, nc);
-            vst(ox_q, i0, n, cur[0]); vst(oy_q, i0, n, cur[1]); vst(ux_q, i0, n, cur[2]);
-            vst(uy_q, i0, n, cur[3]); vst(uz_q, i0, n, cur[4]);
+            if constexpr (REC) {
+                vst_rec4(S.rec_o, i0, n, nc, cur);
+            } else {
+                vst(ox_q, i0, n, cur[0]); vst(oy_q, i0, n, cur[1]); vst(ux_q, i0, n, cur[2]);
+                vst(uy_q, i0, n, cur[3]); vst(uz_q, i0, n, cur[4]);
+            }
         }
 #pragma unroll
         for (int j = 0; j < V; j++) {
@@ -1055,10 +1101,10 @@ __device__ __forceinline__ void push_tile(const PushParams &P, int t)
     }
 }
 
-template <typename R, bool SPLIT, bool OLDU, bool RHO>
+template <typename R, bool SPLIT, bool OLDU, bool RHO, bool REC>
 __global__ __launch_bounds__(PUSH_THREADS, 3) void push_kernel(PushParams P)
 {
-    push_tile<R, SPLIT, OLDU, RHO>(P, xcd_remap(blockIdx.x, gridDim.x));
+    push_tile<R, SPLIT, OLDU, RHO, REC>(P, xcd_remap(blockIdx.x, gridDim.x));
 }
 
 // Passes 2 and 3: the tiles the first launch deferred, list entry k taken by
@@ -1068,12 +1114,12 @@ __global__ __launch_bounds__(PUSH_THREADS, 3) void push_kernel(PushParams P)
 // order; taking them from a shared counter instead was 6% slower on a
 // Harris sheet).  Their LDS windows allow at most two workgroups per CU, so
 // the register budget is that of two.
-template <typename R, bool SPLIT, bool OLDU, bool RHO>
+template <typename R, bool SPLIT, bool OLDU, bool RHO, bool REC>
 __global__ __launch_bounds__(PUSH_THREADS, 2) void push_kernel_listed(PushParams P)
 {
     const int which = P.pass - 2, cnt = P.big[which];
     for (int k = blockIdx.x; k < cnt; k += gridDim.x) {
-        push_tile<R, SPLIT, OLDU, RHO>(P, P.big[2 + which * P.ntiles + k]);
+        push_tile<R, SPLIT, OLDU, RHO, REC>(P, P.big[2 + which * P.ntiles + k]);
         __syncthreads();
     }
 }
@@ -1193,19 +1239,19 @@ size_t push_lds_bytes(int precision, int maxw, int deposit_rho)
                                         : windows + (PUSH_THREADS / 64) * XQ_BYTES;
 }
 
-template <typename R, bool SPLIT, bool OLDU, bool RHO>
+template <typename R, bool SPLIT, bool OLDU, bool RHO, bool REC>
 static void launch_push_t(const PushParams &q, int nt, size_t lds, hipStream_t st)
 {
     if (q.pass >= 2) {
-        (void)hipFuncSetAttribute((const void *)push_kernel_listed<R, SPLIT, OLDU, RHO>,
+        (void)hipFuncSetAttribute((const void *)push_kernel_listed<R, SPLIT, OLDU, RHO, REC>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        hipLaunchKernelGGL((push_kernel_listed<R, SPLIT, OLDU, RHO>), dim3(nt),
+        hipLaunchKernelGGL((push_kernel_listed<R, SPLIT, OLDU, RHO, REC>), dim3(nt),
                            dim3(PUSH_THREADS), lds, st, q);
         return;
     }
-    (void)hipFuncSetAttribute((const void *)push_kernel<R, SPLIT, OLDU, RHO>,
+    (void)hipFuncSetAttribute((const void *)push_kernel<R, SPLIT, OLDU, RHO, REC>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    hipLaunchKernelGGL((push_kernel<R, SPLIT, OLDU, RHO>), dim3(nt), dim3(PUSH_THREADS), lds, st,
+    hipLaunchKernelGGL((push_kernel<R, SPLIT, OLDU, RHO, REC>), dim3(nt), dim3(PUSH_THREADS), lds, st,
                        q);
 }
 
@@ -1213,10 +1259,21 @@ template <typename R, bool SPLIT>
 static void launch_push_f(const PushParams &q, int nt, size_t lds, hipStream_t st)
 {
     const bool oldu = (q.compat & PBPIC_COMPAT_BORIS_OLD_U) != 0, rho = q.deposit_rho != 0;
-    if (oldu && rho) launch_push_t<R, SPLIT, true, true>(q, nt, lds, st);
-    else if (oldu) launch_push_t<R, SPLIT, true, false>(q, nt, lds, st);
-    else if (rho) launch_push_t<R, SPLIT, false, true>(q, nt, lds, st);
-    else launch_push_t<R, SPLIT, false, false>(q, nt, lds, st);
+    if constexpr (SPLIT) {
+        // the step's sort follows: a separate build of the kernel writes the
+        // sort's records, so the other steps' kernel carries none of it
+        if (q.sp[0].rec_o) {
+            if (oldu && rho) launch_push_t<R, SPLIT, true, true, true>(q, nt, lds, st);
+            else if (oldu) launch_push_t<R, SPLIT, true, false, true>(q, nt, lds, st);
+            else if (rho) launch_push_t<R, SPLIT, false, true, true>(q, nt, lds, st);
+            else launch_push_t<R, SPLIT, false, false, true>(q, nt, lds, st);
+            return;
+        }
+    }
+    if (oldu && rho) launch_push_t<R, SPLIT, true, true, false>(q, nt, lds, st);
+    else if (oldu) launch_push_t<R, SPLIT, true, false, false>(q, nt, lds, st);
+    else if (rho) launch_push_t<R, SPLIT, false, true, false>(q, nt, lds, st);
+    else launch_push_t<R, SPLIT, false, false, false>(q, nt, lds, st);
 }
 
 // Largest window whose LDS fits one workgroup per CU (150 KiB).

@@ -301,15 +301,10 @@ __global__ __launch_bounds__(256) void gather_kernel(const uint32_t *perm, int64
 }
 
 // Split form: the gather reads each source particle as one 24-byte record
-// (cell and the five FP32 values), packed by a coalesced pass, instead of six
-// scattered 4-byte reads from six arrays.  15% faster sorts at 2048^2 x 100
-// ppc; the FP64 form keeps the array-wise gather (its 48-byte records were
-// slower).
-struct PartRec {
-    int32_t c;
-    float a[5];
-};
-
+// (cell and the five FP32 values) instead of six scattered 4-byte reads from
+// six arrays.  The records come from a coalesced packing pass, or straight
+// from the push on a step whose sort follows.  The FP64 form keeps the
+// array-wise gather (its 48-byte records were slower).
 __global__ __launch_bounds__(256) void pack_records_kernel(SpeciesDev src, PartRec *rec, int64_t n)
 {
     const int64_t i = (int64_t)xcd_remap(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
@@ -393,7 +388,8 @@ hipError_t launch_cell_counts(const SpeciesDev &sp, int nx, int ny, int32_t *cou
 // Sorts species sp into dst (a second buffer set of the same capacity).
 hipError_t sort_species(SortScratch &s, const SpeciesDev &sp, int precision, int push_order,
                         const CellKey &key, int64_t ncell,
-                        SpeciesDev &dst, int *err, int want_counts, hipStream_t st)
+                        SpeciesDev &dst, int *err, int want_counts, void *rec,
+                        bool rec_written, hipStream_t st)
 {
     (void)want_counts;
     const int64_t n = sp.n;
@@ -460,10 +456,10 @@ hipError_t sort_species(SortScratch &s, const SpeciesDev &sp, int precision, int
     const unsigned nb = (unsigned)(((n + G - 1) / G + 255) / 256);
     if (precision == PBPIC_PREC_FP64) {
         hipLaunchKernelGGL(gather_kernel<double>, dim3(nb), dim3(256), 0, st, perm, n, tv, pout, sp, dst);
-    } else if (s.rec) {
-        PartRec *rec = (PartRec *)s.rec;
-        hipLaunchKernelGGL(pack_records_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
-                           sp, rec, n);
+    } else if (rec) {
+        if (!rec_written)
+            hipLaunchKernelGGL(pack_records_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                               st, sp, (PartRec *)rec, n);
         hipLaunchKernelGGL(gather_records_kernel, dim3(nb), dim3(256), 0, st, perm, n, tv, pout,
                            (const PartRec *)rec, sp, dst);
     } else {

@@ -111,6 +111,10 @@ struct SpeciesDev {
     // replay writes each step into the next slot of its tape).
     int32_t *cell_o;
     void *ox_o, *oy_o, *ux_o, *uy_o, *uz_o;
+    // Split form, on a step whose sort follows: the push writes the pushed
+    // particles here as PartRec records (index = slot) and writes cell_o, but
+    // not ox_o..uz_o.  Null: the arrays as above.
+    void *rec_o;
     int64_t n;
     int32_t q;                 // +1 ion, -1 electron
     int32_t tile0;             // first tile of this species in a push launch
@@ -281,7 +285,6 @@ struct SortScratch {
     uint32_t *ticket = nullptr;     // one counter per pass
     int32_t *counts = nullptr;      // ncell
     const uint32_t *perm = nullptr; // permutation of the last sort (in vals)
-    void *rec = nullptr;            // split form: one 24-byte record per particle
     int64_t cap = 0, status_cap = 0;
 };
 hipError_t launch_push_adj(const AdjParams &p, int precision, hipStream_t st);
@@ -321,10 +324,21 @@ struct CellKey {
     }
 };
 
+// One particle of the split form as the sort's gather reads it: 24 bytes,
+// record i holding slot i.
+struct PartRec {
+    int32_t c;
+    float a[5];                // ox, oy, ux, uy, uz
+};
+static_assert(sizeof(PartRec) == 24, "PartRec is 24 bytes");
+
 // push_order != 0: cells in CellKey's supertile order and full tiles in the
 // push order (tile_rank).  counts are indexed by key.(s.counts holds nkeys).
+// rec (split form): the species' record buffer (PartRec per particle), or
+// null for the array-wise gather; rec_written: the push already wrote sp's
+// records there (sp's value arrays are then stale and not read).
 hipError_t sort_species(SortScratch &s, const SpeciesDev &sp, int precision, int push_order,
                         const CellKey &key, int64_t ncell, SpeciesDev &dst, int *err,
-                        int want_counts, hipStream_t st);
+                        int want_counts, void *rec, bool rec_written, hipStream_t st);
 
 }  // namespace pbpic
