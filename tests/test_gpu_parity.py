@@ -449,6 +449,37 @@ def test_tile_box_of_half_the_domain(gpu, precision):
         assert np.linalg.norm(ctx.get_field(f) - o.get_field(f)) < tol * scale, f
 
 
+def test_record_writing_push_matches_packing_sort(gpu):
+    """Split form: on a step whose sort follows, the push writes the sort's
+    24-byte particle records and the sort skips its packing pass.  One step
+    with a sort every step (its push writes the records) against one step
+    without an in-step sort and then the sort on its own (which packs the
+    records).  The pushed particles depend only on the initial fields, so
+    they, their order and the cell counts agree bit for bit.  The fields
+    agree to FP32 rounding: tiles add their current windows into J with
+    FP32 atomics in no fixed order.  45 x 37 cells at 64 particles per cell:
+    full tiles on the fast path, edge tiles on the generic one and a ragged
+    last tile."""
+    _say("record-writing push vs packing sort")
+    case = make_case(nx=45, ny=37, ppc=64, seed=5, efield=0.05, bfield=0.5)
+    a = gpu_for(case, precision=1, compat=7, sort_every=1)
+    b = gpu_for(case, precision=1, compat=7, sort_every=0)
+    a.countingsort(); b.countingsort()
+    a.step(1)
+    b.step(1)
+    b.countingsort()
+    for k in range(2):
+        pa, pb = a.get_particles(k, native=True), b.get_particles(k, native=True)
+        for f in ("grid", "dx", "dy", "ux", "uy", "uz", "pid"):
+            assert np.array_equal(pa[f], pb[f]), (k, f)
+        ca, _ = a.countingsort(k, want=True)
+        cb, _ = b.countingsort(k, want=True)
+        assert np.array_equal(ca, cb), k
+    for f in ("ex", "ey", "ez", "bx", "by", "bz", "jx", "jy", "jz"):
+        assert rel_norm(a.get_field(f), b.get_field(f)) < 1e-6, f
+    assert a.counters()[0] == b.counters()[0] == 1
+
+
 @pytest.mark.parametrize("precision", [0, 1])
 def test_steps_with_an_empty_species(gpu, precision):
     """A species with no particles beside one with a ragged last tile (3840
