@@ -200,8 +200,9 @@ int pbpic_sync(pbpic_ctx *ctx);
  * the reference's per-particle charge `ptl%q` and mass rule (:964-969).
  * capacity = maximum number of particles.  *species receives its index.
  * Device memory per particle of capacity: 2 x 44 B (FP64) or 2 x 24 B
- * (split) of particle arrays, plus sort scratch of 16 B, and in the split
- * form a further 24 B (the sort's packed records).                        */
+ * (split) of particle arrays (+2 x 4 B with PBPIC_FLAG_TRACK_PID), and in
+ * the split form a further 24 B (the sort's particle records); plus 16 B of
+ * sort scratch per particle of the largest species.                        */
 int pbpic_add_species(pbpic_ctx *ctx, int32_t q, int64_t capacity,
                       int32_t *species);
 int pbpic_num_particles(pbpic_ctx *ctx, int32_t species, int64_t *n);
