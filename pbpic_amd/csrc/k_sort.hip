@@ -319,50 +319,80 @@ __global__ __launch_bounds__(256) void pack_records_kernel(SpeciesDev src, PartR
     rec[i] = r;
 }
 
-// As gather_kernel<float>, reading the packed records.
+// As gather_kernel<float>, reading the packed records; two 4-slot groups per
+// thread (slots i0 and i0 + 256 G), all eight record loads in flight.
+__device__ __forceinline__ void gather_rec_one(const uint32_t *perm, int64_t n, int tv,
+                                               uint32_t *perm_out, const PartRec *rec,
+                                               const SpeciesDev &src, const SpeciesDev &dst,
+                                               int64_t i0)
+{
+    float *out[5] = {(float *)dst.ox, (float *)dst.oy, (float *)dst.ux, (float *)dst.uy,
+                     (float *)dst.uz};
+    for (int64_t i = i0; i < n && i < i0 + 4; i++) {
+        int64_t k = i;
+        if (tv) {
+            const int64_t t0 = i & ~(int64_t)(PUSH_TILE - 1);
+            if (t0 + PUSH_TILE <= n) k = t0 + tile_rank((int)(i - t0), tv);
+        }
+        const uint32_t p = perm[k];
+        if (perm_out) perm_out[i] = p;
+        const PartRec r = rec[p];
+        dst.cell[i] = r.c;
+        for (int q = 0; q < 5; q++) out[q][i] = r.a[q];
+        if (src.pid) dst.pid[i] = src.pid[p];
+    }
+}
+
 __global__ __launch_bounds__(256) void gather_records_kernel(const uint32_t *perm, int64_t n,
                                                              int tv, uint32_t *perm_out,
                                                              const PartRec *rec, SpeciesDev src,
                                                              SpeciesDev dst)
 {
-    constexpr int G = 4;
-    const int64_t i0 = ((int64_t)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x) * G;
-    if (i0 >= n) return;
-    float *out[5] = {(float *)dst.ox, (float *)dst.oy, (float *)dst.ux, (float *)dst.uy,
-                     (float *)dst.uz};
-    if (i0 + G > n) {                        // ragged end: element by element
-        for (int64_t i = i0; i < n; i++) {
-            const uint32_t p = perm[i];
-            if (perm_out) perm_out[i] = p;
-            const PartRec r = rec[p];
-            dst.cell[i] = r.c;
-            for (int q = 0; q < 5; q++) out[q][i] = r.a[q];
-            if (src.pid) dst.pid[i] = src.pid[p];
-        }
+    constexpr int G = 4, H = 2;
+    const int64_t b0 = (int64_t)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x * H * G;
+    int64_t i0[H];
+#pragma unroll
+    for (int h = 0; h < H; h++) i0[h] = b0 + ((int64_t)h * blockDim.x + threadIdx.x) * G;
+    if (i0[H - 1] + G > n) {
+#pragma unroll
+        for (int h = 0; h < H; h++)
+            if (i0[h] < n) gather_rec_one(perm, n, tv, perm_out, rec, src, dst, i0[h]);
         return;
     }
-    int64_t k0 = i0;
-    if (tv) {
-        const int64_t t0 = i0 & ~(int64_t)(PUSH_TILE - 1);
-        if (t0 + PUSH_TILE <= n) k0 = t0 + tile_rank((int)(i0 - t0), tv);
+    float *out[5] = {(float *)dst.ox, (float *)dst.oy, (float *)dst.ux, (float *)dst.uy,
+                     (float *)dst.uz};
+    uint32_t p[H][G];
+#pragma unroll
+    for (int h = 0; h < H; h++) {
+        int64_t k0 = i0[h];
+        if (tv) {
+            const int64_t t0 = i0[h] & ~(int64_t)(PUSH_TILE - 1);
+            if (t0 + PUSH_TILE <= n) k0 = t0 + tile_rank((int)(i0[h] - t0), tv);
+        }
+        *reinterpret_cast<uint4 *>(p[h]) = *reinterpret_cast<const uint4 *>(perm + k0);
     }
-    uint32_t p[G];
-    *reinterpret_cast<uint4 *>(p) = *reinterpret_cast<const uint4 *>(perm + k0);
-    if (perm_out) *reinterpret_cast<uint4 *>(perm_out + i0) = *reinterpret_cast<const uint4 *>(p);
-    int c[G];
-    float a[5][G];
+    int c[H][G];
+    float a[H][5][G];
 #pragma unroll
-    for (int j = 0; j < G; j++) {
-        const PartRec r = rec[p[j]];
-        c[j] = r.c;
+    for (int h = 0; h < H; h++)
 #pragma unroll
-        for (int q = 0; q < 5; q++) a[q][j] = r.a[q];
+        for (int j = 0; j < G; j++) {
+            const PartRec r = rec[p[h][j]];
+            c[h][j] = r.c;
+#pragma unroll
+            for (int q = 0; q < 5; q++) a[h][q][j] = r.a[q];
+        }
+#pragma unroll
+    for (int h = 0; h < H; h++) {
+        if (perm_out)
+            *reinterpret_cast<uint4 *>(perm_out + i0[h]) = *reinterpret_cast<const uint4 *>(p[h]);
+        *reinterpret_cast<int4 *>(dst.cell + i0[h]) = make_int4(c[h][0], c[h][1], c[h][2], c[h][3]);
+#pragma unroll
+        for (int q = 0; q < 5; q++)
+            *reinterpret_cast<float4 *>(out[q] + i0[h]) =
+                make_float4(a[h][q][0], a[h][q][1], a[h][q][2], a[h][q][3]);
+        if (src.pid) gmove<G>(dst.pid + i0[h], src.pid, p[h], false);
     }
-    *reinterpret_cast<int4 *>(dst.cell + i0) = make_int4(c[0], c[1], c[2], c[3]);
-#pragma unroll
-    for (int q = 0; q < 5; q++)
-        *reinterpret_cast<float4 *>(out[q] + i0) = make_float4(a[q][0], a[q][1], a[q][2], a[q][3]);
-    if (src.pid) gmove<G>(dst.pid + i0, src.pid, p, false);
 }
 
 // Adds species sp's particles per cell (compact cell id) to counts, as
@@ -460,7 +490,7 @@ hipError_t sort_species(SortScratch &s, const SpeciesDev &sp, int precision, int
         if (!rec_written)
             hipLaunchKernelGGL(pack_records_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
                                st, sp, (PartRec *)rec, n);
-        hipLaunchKernelGGL(gather_records_kernel, dim3(nb), dim3(256), 0, st, perm, n, tv, pout,
+        hipLaunchKernelGGL(gather_records_kernel, dim3((nb + 1) / 2), dim3(256), 0, st, perm, n, tv, pout,
                            (const PartRec *)rec, sp, dst);
     } else {
         hipLaunchKernelGGL(gather_kernel<float>, dim3(nb), dim3(256), 0, st, perm, n, tv, pout, sp, dst);
