@@ -68,6 +68,9 @@ struct pbpic_ctx {
         int64_t step0 = 0;
         std::vector<char *> ckpts;     // particle+field state before step k*ckpt
         char *final_state = nullptr;   // state after the forward
+        // J after the push of every forward step (3 planes each), so that
+        // the backward's replay need not deposit; null if it did not fit
+        char *jstore = nullptr;
         double acc_after[ACC_N] = {0};
         double loss = 0;
     } adj;
@@ -154,7 +157,7 @@ SpeciesDev species_dev(pbpic_ctx *c, int s, int buf)
 
 // recs: the push writes the split form's particles as the sort's records
 // (the sort follows in the same step).
-PushParams push_params(pbpic_ctx *c, bool recs = false)
+PushParams push_params(pbpic_ctx *c, bool recs = false, bool nodep = false)
 {
     PushParams P{};
     P.nx = c->p.nx;
@@ -168,6 +171,7 @@ PushParams push_params(pbpic_ctx *c, bool recs = false)
     P.maxw_big = push_maxw_big(c->p.precision, P.deposit_rho);
     P.big = c->big;
     P.pass = 0;
+    P.nodep = nodep ? 1 : 0;
     P.dt = c->p.dt;
     P.cva = c->p.cva;
     P.idx = c->idx;
@@ -291,7 +295,8 @@ int kinetic_impl(pbpic_ctx *c)
     return PBPIC_OK;
 }
 
-int push_impl(pbpic_ctx *c, bool recs = false)
+// nodep: push without depositing J (the adjoint's replay supplies J).
+int push_impl(pbpic_ctx *c, bool recs = false, bool nodep = false)
 {
     int64_t ntiles = 0;
     for (auto &S : c->sp) ntiles += (S.n + PUSH_TILE - 1) / PUSH_TILE;
@@ -302,7 +307,7 @@ int push_impl(pbpic_ctx *c, bool recs = false)
         HIPCHK(c, dalloc(&c->big, sizeof(int32_t) * (size_t)(2 * ntiles + 2)));
         c->big_cap = 2 * ntiles + 2;
     }
-    const PushParams P = push_params(c, recs);
+    const PushParams P = push_params(c, recs, nodep);
     // node fields (emfgrid, pic2d.f90:837-845) are field work: timed as such
     HIPCHK(c, prof_mark(c, PBPIC_PROF_FIELD, true));
     HIPCHK(c, launch_nodefield(P, c->p.precision, c->st));
@@ -318,7 +323,8 @@ int push_impl(pbpic_ctx *c, bool recs = false)
         S.ux[b] = c->push_out[s][3]; S.uy[b] = c->push_out[s][4]; S.uz[b] = c->push_out[s][5];
     }
     c->push_out.clear();
-    if (walls(c)) HIPCHK(c, launch_wall_fold(c->J[2], c->p.nx, c->p.ny, P.nyp, c->rsz, c->st));
+    if (walls(c) && !nodep)
+        HIPCHK(c, launch_wall_fold(c->J[2], c->p.nx, c->p.ny, P.nyp, c->rsz, c->st));
     for (auto &S : c->sp) S.tbox_ok = true;
     HIPCHK(c, prof_mark(c, PBPIC_PROF_PUSH, false));
     return PBPIC_OK;
@@ -353,19 +359,28 @@ int fieldsolve_impl(pbpic_ctx *c)
 
 // One time-loop body (pic2d.f90:171-188).  Optionally copies J after the push
 // (jcopy: 3 device planes) and the sort permutations (perm_dev: one device
-// array per species) for the adjoint replay.
-int step_impl(pbpic_ctx *c, void *const *jcopy, uint32_t *const *perm_dev, bool *sorted)
+// array per species) for the adjoint replay.  jsrc (the replay of a step
+// whose J the checkpointed forward kept): the push does not deposit, and J
+// is copied from jsrc before the field solve.
+int step_impl(pbpic_ctx *c, void *const *jcopy, uint32_t *const *perm_dev, bool *sorted,
+              void *const *jsrc = nullptr)
 {
     HIPCHK(c, prof_mark(c, PBPIC_PROF_ZERO, true));
-    HIPCHK(c, launch_zero_current(c->J[0], c->J[1], c->J[2], c->J[3], ncell(c), c->rsz, c->acc, 1, c->st));
+    if (!jsrc)
+        HIPCHK(c, launch_zero_current(c->J[0], c->J[1], c->J[2], c->J[3], ncell(c), c->rsz, c->acc,
+                                      1, c->st));
     HIPCHK(c, prof_mark(c, PBPIC_PROF_ZERO, false));
     // split form: when this step sorts, the push writes the particles as the
     // sort's records, so the sort skips its packing pass.  Until the sort
     // has run, the species' value arrays are stale.
     const bool recs = c->rsz == 4 && c->p.sort_every > 0 &&
                       (c->steps + 1) % c->p.sort_every == 0;
-    int rc = push_impl(c, recs);
+    int rc = push_impl(c, recs, jsrc != nullptr);
     if (rc) return rc;
+    if (jsrc)
+        for (int k = 0; k < 3; k++)
+            HIPCHK(c, hipMemcpyAsync(c->J[k], jsrc[k], (size_t)c->rsz * ncell(c),
+                                     hipMemcpyDeviceToDevice, c->st));
     if (jcopy)
         for (int k = 0; k < 3; k++)
             HIPCHK(c, hipMemcpyAsync(jcopy[k], c->J[k], (size_t)c->rsz * ncell(c),
@@ -513,6 +528,8 @@ void adj_free(pbpic_ctx *c)
     c->adj.ckpts.clear();
     if (c->adj.final_state) (void)hipFree(c->adj.final_state);
     c->adj.final_state = nullptr;
+    if (c->adj.jstore) (void)hipFree(c->adj.jstore);
+    c->adj.jstore = nullptr;
     c->adj.valid = false;
 }
 
@@ -833,7 +850,7 @@ int pbpic_ctx_destroy(pbpic_ctx *c)
     for (int b = 0; b < 2; b++) { f(c->sort.keys[b]); f(c->sort.vals[b]); }
     f(c->sort.status); f(c->sort.dhist); f(c->sort.ticket); f(c->sort.counts);
     for (auto p : c->adj.ckpts) f(p);
-    f(c->adj.final_state);
+    f(c->adj.final_state); f(c->adj.jstore);
     for (auto e : c->ev_pool) (void)hipEventDestroy(e);
     for (int k = 0; k < 2; k++) {
         if (c->stage[k]) (void)hipHostFree(c->stage[k]);
@@ -1417,13 +1434,23 @@ int pbpic_adjoint_forward(pbpic_ctx *c, int32_t nsteps, int32_t ckpt, double *lo
         c->adj.ckpts.push_back(p);
     }
     HIPCHK(c, hipMalloc(&c->adj.final_state, L.total));
+    // J of every step (3 planes each), for a replay without deposit; the
+    // replay deposits again when this does not fit
+    const size_t jplane = al256((size_t)c->rsz * ncell(c));
+    if (hipMalloc(&c->adj.jstore, (size_t)nsteps * 3 * jplane) != hipSuccess) {
+        c->adj.jstore = nullptr;
+        (void)hipGetLastError();
+    }
     double a0[ACC_N], a1[ACC_N];
     HIPCHK(c, hipMemcpyAsync(a0, c->acc, sizeof a0, hipMemcpyDeviceToHost, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
     c->adj.step0 = c->steps;
     for (int k = 0; k < nsteps; k++) {
         if (k % ckpt == 0 && (rc = state_copy(c, L, c->adj.ckpts[k / ckpt], true))) return rc;
-        if ((rc = step_impl(c, nullptr, nullptr, nullptr))) return rc;
+        void *jk[3];
+        for (int q = 0; q < 3 && c->adj.jstore; q++)
+            jk[q] = c->adj.jstore + ((size_t)k * 3 + q) * jplane;
+        if ((rc = step_impl(c, c->adj.jstore ? jk : nullptr, nullptr, nullptr))) return rc;
     }
     if ((rc = state_copy(c, L, c->adj.final_state, true))) return rc;
     HIPCHK(c, hipMemcpyAsync(a1, c->acc, sizeof a1, hipMemcpyDeviceToHost, c->st));
@@ -1449,7 +1476,8 @@ int pbpic_adjoint_backward(pbpic_ctx *c, double *const *fg, void *const *pg)
     for (auto &S : c->sp) { ntot += S.n; nmax = std::max(nmax, S.n); }
     // one allocation for the tape and the adjoint buffers
     const size_t tape_states = (size_t)(K + 1) * L.total;
-    const size_t tape_j = (size_t)K * 3 * al256(rz * nc);
+    // J of each replayed step: kept by the forward, else recomputed into the tape
+    const size_t jplane = al256(rz * nc);
     // permutations only for the steps that sort: at most K/sort_every + 1 per segment
     const int nperm = c->p.sort_every > 0 ? std::min(K, K / c->p.sort_every + 1) : 0;
     const size_t tape_perm = (size_t)nperm * al256(4 * (size_t)ntot + 256 * ns);
@@ -1459,7 +1487,16 @@ int pbpic_adjoint_backward(pbpic_ctx *c, double *const *fg, void *const *pg)
     for (auto &S : c->sp) atiles += adj_tiles(S.n);
     const size_t abig = al256(sizeof(int32_t) * (size_t)(atiles + 1));
     char *base = nullptr;
-    hipError_t e = hipMalloc(&base, tape_states + tape_j + tape_perm + fadj + padj + abig);
+    hipError_t e = hipMalloc(&base, tape_states + (c->adj.jstore ? 0 : (size_t)K * 3 * jplane) +
+                                        tape_perm + fadj + padj + abig);
+    if (e != hipSuccess && c->adj.jstore) {
+        // the forward's kept J leaves too little room: drop it and replay
+        // with the deposit, keeping J in the tape instead
+        (void)hipGetLastError();
+        (void)hipFree(c->adj.jstore);
+        c->adj.jstore = nullptr;
+        e = hipMalloc(&base, tape_states + (size_t)K * 3 * jplane + tape_perm + fadj + padj + abig);
+    }
     if (e != hipSuccess) return fail(c, PBPIC_ERR_NOMEM, "adjoint tape allocation: %s", hipGetErrorString(e));
     int rc = PBPIC_OK;
     char *p = base;
@@ -1467,7 +1504,8 @@ int pbpic_adjoint_backward(pbpic_ctx *c, double *const *fg, void *const *pg)
     std::vector<char *> tstate(K + 1);
     for (int k = 0; k <= K; k++) tstate[k] = take(L.total);
     std::vector<std::array<void *, 3>> tj(K);
-    for (int k = 0; k < K; k++) for (int q = 0; q < 3; q++) tj[k][q] = take(rz * nc);
+    for (int k = 0; k < K && !c->adj.jstore; k++)
+        for (int q = 0; q < 3; q++) tj[k][q] = take(rz * nc);
     std::vector<std::vector<uint32_t *>> tperm(nperm, std::vector<uint32_t *>(ns));
     for (int k = 0; k < nperm; k++) for (int s = 0; s < ns; s++) tperm[k][s] = (uint32_t *)take(4 * (size_t)c->sp[s].n);
     std::vector<int> pslot(K, -1);
@@ -1501,7 +1539,13 @@ int pbpic_adjoint_backward(pbpic_ctx *c, double *const *fg, void *const *pg)
                     pslot[k] = used;
                     pd = tperm[used++].data();
                 }
-                if ((r = step_impl(c, tj[k].data(), pd, &so))) return r;
+                if (c->adj.jstore) {
+                    for (int q = 0; q < 3; q++)
+                        tj[k][q] = c->adj.jstore + ((size_t)(s0 + k) * 3 + q) * jplane;
+                    if ((r = step_impl(c, nullptr, pd, &so, tj[k].data()))) return r;
+                } else if ((r = step_impl(c, tj[k].data(), pd, &so))) {
+                    return r;
+                }
                 sorted[k] = so;
                 // the push wrote slot k+1; a sort moved the particles into the
                 // context's own arrays, and the fields are always the context's

@@ -688,8 +688,10 @@ __device__ __forceinline__ void deposit_split_fast(double *jw, int WH, int W, in
 // One push tile t (one workgroup).  OLDU: the reference's Boris rotation
 // with the pre-kick momentum (PBPIC_COMPAT_BORIS_OLD_U); RHO: deposit rho as
 // well (PBPIC_FLAG_DEPOSIT_RHO); REC (split form): write the pushed particles
-// as the sort's records at S.rec_o in place of the five value arrays.
-template <typename R, bool SPLIT, bool OLDU, bool RHO, bool REC>
+// as the sort's records at S.rec_o in place of the five value arrays; DEP:
+// deposit the current (false: the adjoint's replay, whose J comes from the
+// checkpointed forward).
+template <typename R, bool SPLIT, bool OLDU, bool RHO, bool REC, bool DEP>
 __device__ __forceinline__ void push_tile(const PushParams &P, int t)
 {
     constexpr int V = Vw<R>::V;
@@ -793,7 +795,8 @@ __device__ __forceinline__ void push_tile(const PushParams &P, int t)
             const int gx = wrap_index(x0 + wx0 + wi, nx), gy = wrap_index(y0 + wy0 + wj, ny);
             copy_node<R>(nf + (size_t)k * NS, gnf + ((size_t)gx * ny + gy) * NS);
         }
-        for (int k = tid; k < njc * WH; k += PUSH_THREADS) jw[k] = 0.0;
+        if constexpr (DEP)
+            for (int k = tid; k < njc * WH; k += PUSH_THREADS) jw[k] = 0.0;
     }
     __syncthreads();
 
@@ -862,45 +865,47 @@ __device__ __forceinline__ void push_tile(const PushParams &P, int t)
                          err);
         }
         wall_reflect(P.nyp, cy, dcy, oy, uy);
-        Dep<R> d;
-        deposit_terms<R, SPLIT>(cx + 1, cy + 1, dcx, dcy, dx0, ox, dy0, oy, dqx, dqy,
-                                dqz, vz, P.compat, want_rho, d);
-        if (fxw && want_rho) { d.r0 *= rsc; d.r1 *= rsc; d.r2 *= rsc; d.r3 *= rsc; }
-        // jy at column mx+2 is lost by the reference's fold (:1156, :1159)
-        if ((P.compat & PBPIC_COMPAT_JY_DROP) && cx == nx - 1 && dcx == 1) d.jyb1 = (R)0;
-        if (d.a == 0 && d.b == 0) {
-            d.jxa0 += d.jxb0; d.jxa1 += d.jxb1; d.jya0 += d.jyb0; d.jya1 += d.jyb1;
-        }
-        if (inlds) {
-            deposit_lds(jw, WH, W, ly * W + lx, d, want_rho);
-        } else {
-            const int xm = cx == 0 ? nx - 1 : cx - 1, xp = cx + 1 == nx ? 0 : cx + 1;
-            const int ym = cy == 0 ? ny - 1 : cy - 1, yp = cy + 1 == ny ? 0 : cy + 1;
-            const int xs[3] = {xm, cx, xp}, ys[3] = {ym, cy, yp};
-            auto at = [&](int ddx, int ddy) -> size_t {
-                int gx = ddx == 2 ? (xp + 1 == nx ? 0 : xp + 1) : xs[ddx + 1];
-                int gy = ddy == 2 ? (yp + 1 == ny ? 0 : yp + 1) : ys[ddy + 1];
-                return (size_t)gy * nx + gx;
-            };
-            gadd(gjx + at(0, 0), d.jxa0);
-            gadd(gjx + at(0, 1), d.jxa1);
-            gadd(gjy + at(0, 0), d.jya0);
-            gadd(gjy + at(1, 0), d.jya1);
-            if (d.a != 0 || d.b != 0) {
-                gadd(gjx + at(d.a, d.b), d.jxb0);
-                gadd(gjx + at(d.a, d.b + 1), d.jxb1);
-                gadd(gjy + at(d.a, d.b), d.jyb0);
-                gadd(gjy + at(d.a + 1, d.b), d.jyb1);
+        if constexpr (DEP) {
+            Dep<R> d;
+            deposit_terms<R, SPLIT>(cx + 1, cy + 1, dcx, dcy, dx0, ox, dy0, oy, dqx, dqy,
+                                    dqz, vz, P.compat, want_rho, d);
+            if (fxw && want_rho) { d.r0 *= rsc; d.r1 *= rsc; d.r2 *= rsc; d.r3 *= rsc; }
+            // jy at column mx+2 is lost by the reference's fold (:1156, :1159)
+            if ((P.compat & PBPIC_COMPAT_JY_DROP) && cx == nx - 1 && dcx == 1) d.jyb1 = (R)0;
+            if (d.a == 0 && d.b == 0) {
+                d.jxa0 += d.jxb0; d.jxa1 += d.jxb1; d.jya0 += d.jyb0; d.jya1 += d.jyb1;
             }
-            gadd(gjz + at(0, 0), d.jz0);
-            gadd(gjz + at(d.di, 0), d.jz1);
-            gadd(gjz + at(0, d.dj), d.jz2);
-            gadd(gjz + at(d.di, d.dj), d.jz3);
-            if (want_rho) {
-                gadd(grho + at(0, 0), d.r0);
-                gadd(grho + at(1, 0), d.r1);
-                gadd(grho + at(0, 1), d.r2);
-                gadd(grho + at(1, 1), d.r3);
+            if (inlds) {
+                deposit_lds(jw, WH, W, ly * W + lx, d, want_rho);
+            } else {
+                const int xm = cx == 0 ? nx - 1 : cx - 1, xp = cx + 1 == nx ? 0 : cx + 1;
+                const int ym = cy == 0 ? ny - 1 : cy - 1, yp = cy + 1 == ny ? 0 : cy + 1;
+                const int xs[3] = {xm, cx, xp}, ys[3] = {ym, cy, yp};
+                auto at = [&](int ddx, int ddy) -> size_t {
+                    int gx = ddx == 2 ? (xp + 1 == nx ? 0 : xp + 1) : xs[ddx + 1];
+                    int gy = ddy == 2 ? (yp + 1 == ny ? 0 : yp + 1) : ys[ddy + 1];
+                    return (size_t)gy * nx + gx;
+                };
+                gadd(gjx + at(0, 0), d.jxa0);
+                gadd(gjx + at(0, 1), d.jxa1);
+                gadd(gjy + at(0, 0), d.jya0);
+                gadd(gjy + at(1, 0), d.jya1);
+                if (d.a != 0 || d.b != 0) {
+                    gadd(gjx + at(d.a, d.b), d.jxb0);
+                    gadd(gjx + at(d.a, d.b + 1), d.jxb1);
+                    gadd(gjy + at(d.a, d.b), d.jyb0);
+                    gadd(gjy + at(d.a + 1, d.b), d.jyb1);
+                }
+                gadd(gjz + at(0, 0), d.jz0);
+                gadd(gjz + at(d.di, 0), d.jz1);
+                gadd(gjz + at(0, d.dj), d.jz2);
+                gadd(gjz + at(d.di, d.dj), d.jz3);
+                if (want_rho) {
+                    gadd(grho + at(0, 0), d.r0);
+                    gadd(grho + at(1, 0), d.r1);
+                    gadd(grho + at(0, 1), d.r2);
+                    gadd(grho + at(1, 1), d.r3);
+                }
             }
         }
         track(lx, ly, dcx, dcy);
@@ -945,7 +950,8 @@ __device__ __forceinline__ void push_tile(const PushParams &P, int t)
             }
             wall_reflect(P.nyp, cy, dcy, oy, uy);
             const int base = (cy - YW) * W + (cx - XW);
-            if constexpr (SPLIT) {
+            if constexpr (!DEP) {
+            } else if constexpr (SPLIT) {
                 deposit_split_fast(jw, WH, W, base, dcx, dcy, dx0, ox, dy0, oy, dqx, dqy, dqz,
                                    rsc, vz, P.compat, want_rho, Q);
                 if (Q.tail - Q.head >= 64) Q.drain(jw, WH, W, dqx, dqy, P.compat, 64);
@@ -992,7 +998,7 @@ __device__ __forceinline__ void push_tile(const PushParams &P, int t)
                 for (int q = 0; q < 5; q++) cur[q][j] = nxt[q][j];
             }
         }
-        if constexpr (SPLIT) Q.drain(jw, WH, W, dqx, dqy, P.compat, Q.tail - Q.head);
+        if constexpr (SPLIT && DEP) Q.drain(jw, WH, W, dqx, dqy, P.compat, Q.tail - Q.head);
     } else {
     // the tile's cells were read in phase A and are re-read here from L2
     R cur[5][V], nxt[5][V];
@@ -1040,7 +1046,7 @@ __device__ __forceinline__ void push_tile(const PushParams &P, int t)
 
     // ---- D: flush the window, reduce the kinetic energy and error flags
     __syncthreads();
-    if (inlds) {
+    if (DEP && inlds) {
         const double isx = 1.0 / S.jsx, isy = 1.0 / S.jsy, isz = 1.0 / S.jsz, isr = 1.0 / S.jsr;
         for (int k = tid; k < WH; k += PUSH_THREADS) {
             const int wj = k / W, wi = k - wj * W;
@@ -1101,10 +1107,10 @@ __device__ __forceinline__ void push_tile(const PushParams &P, int t)
     }
 }
 
-template <typename R, bool SPLIT, bool OLDU, bool RHO, bool REC>
+template <typename R, bool SPLIT, bool OLDU, bool RHO, bool REC, bool DEP>
 __global__ __launch_bounds__(PUSH_THREADS, 3) void push_kernel(PushParams P)
 {
-    push_tile<R, SPLIT, OLDU, RHO, REC>(P, xcd_remap(blockIdx.x, gridDim.x));
+    push_tile<R, SPLIT, OLDU, RHO, REC, DEP>(P, xcd_remap(blockIdx.x, gridDim.x));
 }
 
 // Passes 2 and 3: the tiles the first launch deferred, list entry k taken by
@@ -1114,12 +1120,12 @@ __global__ __launch_bounds__(PUSH_THREADS, 3) void push_kernel(PushParams P)
 // order; taking them from a shared counter instead was 6% slower on a
 // Harris sheet).  Their LDS windows allow at most two workgroups per CU, so
 // the register budget is that of two.
-template <typename R, bool SPLIT, bool OLDU, bool RHO, bool REC>
+template <typename R, bool SPLIT, bool OLDU, bool RHO, bool REC, bool DEP>
 __global__ __launch_bounds__(PUSH_THREADS, 2) void push_kernel_listed(PushParams P)
 {
     const int which = P.pass - 2, cnt = P.big[which];
     for (int k = blockIdx.x; k < cnt; k += gridDim.x) {
-        push_tile<R, SPLIT, OLDU, RHO, REC>(P, P.big[2 + which * P.ntiles + k]);
+        push_tile<R, SPLIT, OLDU, RHO, REC, DEP>(P, P.big[2 + which * P.ntiles + k]);
         __syncthreads();
     }
 }
@@ -1239,41 +1245,51 @@ size_t push_lds_bytes(int precision, int maxw, int deposit_rho)
                                         : windows + (PUSH_THREADS / 64) * XQ_BYTES;
 }
 
-template <typename R, bool SPLIT, bool OLDU, bool RHO, bool REC>
+template <typename R, bool SPLIT, bool OLDU, bool RHO, bool REC, bool DEP>
 static void launch_push_t(const PushParams &q, int nt, size_t lds, hipStream_t st)
 {
     if (q.pass >= 2) {
-        (void)hipFuncSetAttribute((const void *)push_kernel_listed<R, SPLIT, OLDU, RHO, REC>,
+        (void)hipFuncSetAttribute((const void *)push_kernel_listed<R, SPLIT, OLDU, RHO, REC, DEP>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        hipLaunchKernelGGL((push_kernel_listed<R, SPLIT, OLDU, RHO, REC>), dim3(nt),
+        hipLaunchKernelGGL((push_kernel_listed<R, SPLIT, OLDU, RHO, REC, DEP>), dim3(nt),
                            dim3(PUSH_THREADS), lds, st, q);
         return;
     }
-    (void)hipFuncSetAttribute((const void *)push_kernel<R, SPLIT, OLDU, RHO, REC>,
+    (void)hipFuncSetAttribute((const void *)push_kernel<R, SPLIT, OLDU, RHO, REC, DEP>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    hipLaunchKernelGGL((push_kernel<R, SPLIT, OLDU, RHO, REC>), dim3(nt), dim3(PUSH_THREADS), lds, st,
+    hipLaunchKernelGGL((push_kernel<R, SPLIT, OLDU, RHO, REC, DEP>), dim3(nt), dim3(PUSH_THREADS), lds, st,
                        q);
+}
+
+template <typename R, bool SPLIT, bool REC, bool DEP>
+static void launch_push_o(const PushParams &q, int nt, size_t lds, hipStream_t st)
+{
+    const bool oldu = (q.compat & PBPIC_COMPAT_BORIS_OLD_U) != 0, rho = q.deposit_rho != 0;
+    if constexpr (!DEP) {
+        // no deposit: rho is not deposited either
+        if (oldu) launch_push_t<R, SPLIT, true, false, REC, false>(q, nt, lds, st);
+        else launch_push_t<R, SPLIT, false, false, REC, false>(q, nt, lds, st);
+    } else {
+        if (oldu && rho) launch_push_t<R, SPLIT, true, true, REC, true>(q, nt, lds, st);
+        else if (oldu) launch_push_t<R, SPLIT, true, false, REC, true>(q, nt, lds, st);
+        else if (rho) launch_push_t<R, SPLIT, false, true, REC, true>(q, nt, lds, st);
+        else launch_push_t<R, SPLIT, false, false, REC, true>(q, nt, lds, st);
+    }
 }
 
 template <typename R, bool SPLIT>
 static void launch_push_f(const PushParams &q, int nt, size_t lds, hipStream_t st)
 {
-    const bool oldu = (q.compat & PBPIC_COMPAT_BORIS_OLD_U) != 0, rho = q.deposit_rho != 0;
-    if constexpr (SPLIT) {
-        // the step's sort follows: a separate build of the kernel writes the
-        // sort's records, so the other steps' kernel carries none of it
-        if (q.sp[0].rec_o) {
-            if (oldu && rho) launch_push_t<R, SPLIT, true, true, true>(q, nt, lds, st);
-            else if (oldu) launch_push_t<R, SPLIT, true, false, true>(q, nt, lds, st);
-            else if (rho) launch_push_t<R, SPLIT, false, true, true>(q, nt, lds, st);
-            else launch_push_t<R, SPLIT, false, false, true>(q, nt, lds, st);
-            return;
-        }
+    // the step's sort follows: a separate build of the kernel writes the
+    // sort's records, so the other steps' kernel carries none of it
+    const bool rec = SPLIT && q.sp[0].rec_o != nullptr;
+    if (q.nodep) {
+        if (rec) launch_push_o<R, SPLIT, SPLIT, false>(q, nt, lds, st);
+        else launch_push_o<R, SPLIT, false, false>(q, nt, lds, st);
+        return;
     }
-    if (oldu && rho) launch_push_t<R, SPLIT, true, true, false>(q, nt, lds, st);
-    else if (oldu) launch_push_t<R, SPLIT, true, false, false>(q, nt, lds, st);
-    else if (rho) launch_push_t<R, SPLIT, false, true, false>(q, nt, lds, st);
-    else launch_push_t<R, SPLIT, false, false, false>(q, nt, lds, st);
+    if (rec) launch_push_o<R, SPLIT, SPLIT, true>(q, nt, lds, st);
+    else launch_push_o<R, SPLIT, false, true>(q, nt, lds, st);
 }
 
 // Largest window whose LDS fits one workgroup per CU (150 KiB).

@@ -145,6 +145,7 @@ struct PushParams {
     // start at big + 2 and big + 2 + ntiles.  pass 0: one launch; tiles that
     // do not fit deposit with global atomics, as do those wider than maxw_big.
     int maxw_mid, maxw_big, pass;
+    int nodep;                 // 1: push without depositing J (adjoint replay)
     int *big;
     int ntiles;
     int nspecies;
