@@ -397,7 +397,9 @@ int pbpic_set_field_device(pbpic_ctx *ctx, int32_t which, const double *plane);
  * reference names but leaves empty (python/analysis_jdote.py:1-31); no
  * entry point of the reference corresponds.  pbpic_adjoint_forward runs
  * nsteps steps from the current state (as pbpic_step, sorting at its cadence),
- * keeping particle+field checkpoints every ckpt_every steps, and returns
+ * keeping particle+field checkpoints every ckpt_every steps and, when device
+ * memory allows, the three J planes of every step (the backward then replays
+ * without depositing), and returns
  * loss = sum_n dt (mime/c^2) sum_cells J^{n+1/2}.(E^n+E^{n+1})/2.
  * pbpic_adjoint_backward then writes dloss/d(initial state): field_grads holds
  * 6 device pointers (nx*ny doubles: Ex, Ey, Ez, Bx, By, Bz), particle_grads
