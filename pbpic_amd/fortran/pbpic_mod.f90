@@ -116,6 +116,11 @@ module pbpic_gpu
             import :: c_ptr, c_int32_t
             type(c_ptr), value :: ctx
         end function
+        integer(c_int32_t) function c_step(ctx, nsteps) bind(c, name="pbpic_step")
+            import :: c_ptr, c_int32_t
+            type(c_ptr), value :: ctx
+            integer(c_int32_t), value :: nsteps
+        end function
         integer(c_int32_t) function c_diag(ctx, out) bind(c, name="pbpic_diag")
             import :: c_ptr, c_int32_t, c_double
             type(c_ptr), value :: ctx
@@ -233,6 +238,13 @@ contains
     ! countingsort (pic2d.f90:1303-1329)
     subroutine countingsort
         call pbpic_check(c_countingsort(ctx), 'countingsort')
+    end subroutine
+
+    ! nsteps whole time-loop bodies (zerocurrent, pusher, fieldsolver and the
+    ! sort at the context's cadence, pic2d.f90:171-188) in one call
+    subroutine pbpic_step(nsteps)
+        integer, intent(in) :: nsteps
+        call pbpic_check(c_step(ctx, int(nsteps, c_int32_t)), 'pbpic_step')
     end subroutine
 
 end module pbpic_gpu

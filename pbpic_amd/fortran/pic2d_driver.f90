@@ -17,7 +17,7 @@ program pic2d_gpu
     integer, parameter :: nx = 128, ny = 128, ppg = 16, ntdiag = 10
     real(c_double), parameter :: mime = 25.0d0, cva = 15.0d0
     real(c_double) :: dx, dt, in0
-    integer :: itime, ntime, prec, nargs
+    integer :: itime, ntime, prec, nargs, mode
     character(len=32) :: arg
     real(c_double) :: d(16)
 
@@ -31,6 +31,12 @@ program pic2d_gpu
     if (nargs >= 2) then
         call get_command_argument(2, arg)
         read(arg, *) prec
+    end if
+    ! mode 0: the reference's loop body; mode 1: one pbpic_step call per step
+    mode = 0
+    if (nargs >= 3) then
+        call get_command_argument(3, arg)
+        read(arg, *) mode
     end if
 
     ! electron units mapped onto the reference normalization: dx = 0.5 d_e,
@@ -47,10 +53,14 @@ program pic2d_gpu
 
     call countingsort                                   ! pic2d.f90:161
     do itime = 1, ntime                                 ! :171
-        call zerocurrent                                ! :173-176
-        call pusher                                     ! :177
-        call fieldsolver                                ! :178
-        if (mod(itime, 50) == 0) call countingsort      ! :180-182
+        if (mode == 1) then
+            call pbpic_step(1)                          ! :173-182 in one call
+        else
+            call zerocurrent                            ! :173-176
+            call pusher                                 ! :177
+            call fieldsolver                            ! :178
+            if (mod(itime, 50) == 0) call countingsort  ! :180-182
+        end if
         if (mod(itime, ntdiag) == 0) then               ! :183-187
             call fieldsenergy
             call pbpic_check(c_diag(ctx, d), 'diag')

@@ -19,12 +19,12 @@ DRIVER = os.path.join(ROOT, "pbpic_amd", "fortran", "pic2d_gpu")
 pytestmark = pytest.mark.gpu
 
 
-def _run_driver(ntime, precision):
+def _run_driver(ntime, precision, mode=0):
     if not os.path.exists(DRIVER):
         pytest.skip("pic2d_gpu is not built (python -m pbpic_amd.build --fortran)")
-    print(f"fortran driver ntime={ntime} precision={precision}", flush=True)
-    out = subprocess.run([DRIVER, str(ntime), str(precision)], capture_output=True, text=True,
-                         timeout=120)
+    print(f"fortran driver ntime={ntime} precision={precision} mode={mode}", flush=True)
+    out = subprocess.run([DRIVER, str(ntime), str(precision), str(mode)], capture_output=True,
+                         text=True, timeout=120)
     assert out.returncode == 0, out.stderr[-2000:]
     rows = []
     for line in out.stdout.splitlines():
@@ -54,9 +54,12 @@ def _python_run(nsteps, precision):
     return [d["ke_e"], d["ke_i"], d["w_e"], d["w_b"]]
 
 
+@pytest.mark.parametrize("mode", [0, 1])
 @pytest.mark.parametrize("precision", [0, 1])
-def test_fortran_driver_matches_python_binding(gpu, precision):
-    rows = _run_driver(100, precision)
+def test_fortran_driver_matches_python_binding(gpu, precision, mode):
+    """Mode 0 runs the reference's loop body (zerocurrent, pusher,
+    fieldsolver, countingsort every 50); mode 1 one pbpic_step per step."""
+    rows = _run_driver(100, precision, mode)
     assert [r[0] for r in rows] == list(range(10, 101, 10))
     tot = np.array([sum(r[1]) for r in rows])
     assert np.all(np.isfinite(tot))
