@@ -376,7 +376,9 @@ int pbpic_set_counters(pbpic_ctx *ctx, int64_t steps, const double je[2]);
 int pbpic_species_device_ptrs(pbpic_ctx *ctx, int32_t species, void **ptrs);
 
 /* Device field planes: ptrs receives 10 pointers: ex, ey, ez, bx, by, bz
- * (double), jx, jy, jz, rho (double or float by precision, unnormalized).   */
+ * (double), jx, jy, jz, rho (double or float by precision, unnormalized).
+ * The E and B pointers are valid until the next field solve: the solve
+ * writes the new fields into a second set of planes and swaps the two.     */
 int pbpic_field_device_ptrs(pbpic_ctx *ctx, void **ptrs);
 
 /* The HIP stream (hipStream_t) the context orders its work on.              */

@@ -62,6 +62,9 @@ void orc_init_maxwellian(int nx, int ny, int ppc, double vth, double vdrift_x,
     const double ud = vdrift_x / sqrt(1.0 - vdrift_x * vdrift_x);
     const double twopi = 6.283185307179586;
     const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    /* every particle depends only on its own counter: any thread split gives
+       the same arrays */
+#pragma omp parallel for schedule(static)
     for (int64_t p = 0; p < n; p++) {
         uint32_t a[4], b[4];
         uint32_t ca[4] = {(uint32_t)p, (uint32_t)((uint64_t)p >> 32), stream, 0u};

@@ -129,6 +129,7 @@ int PFX(set_particles)(orc_ctx *oc, int64_t n, const int32_t *grid,
     c->ptlsort = (PFX(ptl_t) *)malloc(sizeof(PFX(ptl_t)) * (size_t)(n > 0 ? n : 1));
     if (!c->ptl || !c->ptlsort) return -1;
     c->nptl = n;
+#pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < n; i++) {
         PFX(ptl_t) *p = &c->ptl[i];
         p->dx = (REAL)dx[i]; p->dy = (REAL)dy[i];
@@ -893,6 +894,49 @@ void PFX(countingsort)(orc_ctx *oc, int32_t *counts, int32_t *perm)
     CT *c = (CT *)oc;
     const int64_t ncell = (int64_t)c->mx * c->my;
     int32_t *pa = c->ptlalloc;
+    const int nt = c->nthreads;
+    if (nt > 1 && c->nptl >= (int64_t)nt * 4096) {
+        /* The same stable counting sort with the particles cut into nt
+           contiguous chunks: per-chunk counts, the offset of (chunk t, cell)
+           = all particles of lower cells + those of cell in chunks < t, then
+           each chunk scatters in index order.  The result is the serial
+           loop's, element for element. */
+        int32_t *h = (int32_t *)calloc((size_t)nt * (size_t)ncell, sizeof(int32_t));
+        if (h) {
+            #pragma omp parallel num_threads(nt)
+            {
+                const int t = omp_get_thread_num();
+                int32_t *ht = h + (size_t)t * (size_t)ncell;
+                const int64_t lo = c->nptl * t / nt, hi = c->nptl * (t + 1) / nt;
+                for (int64_t i = lo; i < hi; i++) ht[c->ptl[i].grid - 1]++;
+                #pragma omp barrier
+                #pragma omp single
+                {
+                    int32_t k = 0;
+                    for (int64_t j = 0; j < ncell; j++) {
+                        int32_t tot = 0;
+                        for (int u = 0; u < nt; u++) {
+                            int32_t *hu = h + (size_t)u * (size_t)ncell + j;
+                            const int32_t v = *hu;
+                            *hu = k + tot;
+                            tot += v;
+                        }
+                        if (counts) counts[j] = tot;
+                        k += tot;
+                    }
+                }
+                for (int64_t i = lo; i < hi; i++) {
+                    const int32_t kk = ht[c->ptl[i].grid - 1]++;
+                    c->ptlsort[kk] = c->ptl[i];
+                    if (perm) perm[kk] = (int32_t)i;
+                }
+                #pragma omp barrier
+                memcpy(c->ptl + lo, c->ptlsort + lo, sizeof(PFX(ptl_t)) * (size_t)(hi - lo));
+            }
+            free(h);
+            return;
+        }
+    }
     memset(pa, 0, sizeof(int32_t) * (size_t)ncell);             /* :1309 */
     for (int64_t i = 0; i < c->nptl; i++) pa[c->ptl[i].grid - 1]++;
     if (counts) memcpy(counts, pa, sizeof(int32_t) * (size_t)ncell);

@@ -34,9 +34,26 @@ def hipcc():
 
 
 def build_library(verbose=True, out=None):
+    """One hipcc per translation unit, in parallel, then one link.  The
+    translation units share no device symbols, so each carries its own code
+    object."""
     out = out or LIB
-    cmd = [hipcc(), *HIPCC_FLAGS, "-I", os.path.join(ROOT, "include"),
-           *[os.path.join(CSRC, s) for s in SOURCES], "-o", out + ".tmp"]
+    objdir = os.path.join(HERE, "_build")
+    os.makedirs(objdir, exist_ok=True)
+    compile_flags = [f for f in HIPCC_FLAGS if f != "-shared"]
+    procs, objs = [], []
+    for s in SOURCES:
+        obj = os.path.join(objdir, os.path.splitext(s)[0] + ".o")
+        objs.append(obj)
+        cmd = [hipcc(), *compile_flags, "-I", os.path.join(ROOT, "include"), "-c",
+               os.path.join(CSRC, s), "-o", obj]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        procs.append((s, subprocess.Popen(cmd)))
+    failed = [s for s, p in procs if p.wait() != 0]
+    if failed:
+        raise subprocess.CalledProcessError(1, f"hipcc {' '.join(failed)}")
+    cmd = [hipcc(), "--offload-arch=gfx950", "-fPIC", "-shared", *objs, "-o", out + ".tmp"]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)

@@ -40,6 +40,10 @@ struct pbpic_ctx {
     hipStream_t st = nullptr;
     double *E[3] = {nullptr, nullptr, nullptr};
     double *B[3] = {nullptr, nullptr, nullptr};
+    // the field solve writes E^{n+1}, B^{n+1} into these and swaps them with
+    // E, B (an in-place stencil would read neighbours' new values)
+    double *E2[3] = {nullptr, nullptr, nullptr};
+    double *B2[3] = {nullptr, nullptr, nullptr};
     void *J[4] = {nullptr, nullptr, nullptr, nullptr};
     void *NF = nullptr;        // node fields for the push (rebuilt before each push)
     double *acc = nullptr;
@@ -204,6 +208,8 @@ FdtdParams fdtd_params(pbpic_ctx *c)
     F.dth = c->dth; F.dtc2 = c->dtc2; F.idx = c->idx; F.idy = c->idy; F.in0 = c->p.in0;
     F.ex = c->E[0]; F.ey = c->E[1]; F.ez = c->E[2];
     F.bx = c->B[0]; F.by = c->B[1]; F.bz = c->B[2];
+    F.nex = c->E2[0]; F.ney = c->E2[1]; F.nez = c->E2[2];
+    F.nbx = c->B2[0]; F.nby = c->B2[1]; F.nbz = c->B2[2];
     F.jx = c->J[0]; F.jy = c->J[1]; F.jz = c->J[2];
     F.fex = c->filt[0]; F.fey = c->filt[1]; F.fez = c->filt[2];
     F.lmask = c->lmask;
@@ -351,6 +357,10 @@ int fieldsolve_impl(pbpic_ctx *c)
         HIPCHK(c, launch_filters(q, c->p.precision, c->st));
     }
     HIPCHK(c, launch_fdtd(fdtd_params(c), c->p.precision, c->st));
+    for (int k = 0; k < 3; k++) {
+        std::swap(c->E[k], c->E2[k]);
+        std::swap(c->B[k], c->B2[k]);
+    }
     if (walls(c)) HIPCHK(c, launch_wall_images(fdtd_params(c), 0x3f, c->st));
     HIPCHK(c, prof_mark(c, PBPIC_PROF_FIELD, false));
     c->steps++;
@@ -797,6 +807,8 @@ int pbpic_ctx_create(const pbpic_params *pp, pbpic_ctx **out)
     for (int k = 0; k < 3; k++) {
         chk(dalloc(&c->E[k], nc * 8), "alloc E");
         chk(dalloc(&c->B[k], nc * 8), "alloc B");
+        chk(dalloc(&c->E2[k], nc * 8), "alloc E");
+        chk(dalloc(&c->B2[k], nc * 8), "alloc B");
     }
     for (int k = 0; k < 4; k++) chk(dalloc(&c->J[k], nc * c->rsz), "alloc J");
     chk(dalloc(&c->NF, nodefield_bytes(c->p.precision, nc)), "alloc node fields");
@@ -837,7 +849,7 @@ int pbpic_ctx_destroy(pbpic_ctx *c)
     if (!c) return PBPIC_ERR_ARG;
     if (c->st) (void)hipStreamSynchronize(c->st);
     auto f = [](void *p) { if (p) (void)hipFree(p); };
-    for (int k = 0; k < 3; k++) { f(c->E[k]); f(c->B[k]); }
+    for (int k = 0; k < 3; k++) { f(c->E[k]); f(c->B[k]); f(c->E2[k]); f(c->B2[k]); }
     for (int k = 0; k < 4; k++) f(c->J[k]);
     f(c->NF);
     f(c->acc); f(c->err); f(c->rho_scratch); f(c->gauss0); f(c->diag_counts);

@@ -6,7 +6,11 @@
 //   B^{n+1/2} on tile+2  (:688-690)
 //   E^{n+1}   on tile+1  (:742-744, current normalized by in0 as at :1066)
 //   B^{n+1}   on tile    (:801-803)
-// and writes E^{n+1}, B^{n+1} once: 9 planes read (E, B, J), 6 written.  The
+// and writes E^{n+1}, B^{n+1} once: 9 planes read (E, B, J), 6 written.
+// The new fields go to a second set of planes (FdtdParams::nex..nbz): every
+// workgroup reads a 3-cell halo of its neighbours' E^n and B^n, and nothing
+// orders workgroups, so writing in place would let a late tile read a
+// finished neighbour's E^{n+1} as its halo.  The
 // curl terms the reference stores between steps are recomputed from E and B,
 // which yields the same values (they are formed from the same fields by the
 // same expressions, :733-736, :790-793).
@@ -164,8 +168,8 @@ __global__ __launch_bounds__(FT_THREADS) void fdtd_kernel(FdtdParams P)
             bzn = sb[2][c] + (double)(dthf * ((dxy * idyf) - (dyx * idxf)));
         }
         const size_t g = (size_t)(y0 + v) * nx + (x0 + u);
-        P.ex[g] = se[0][c]; P.ey[g] = se[1][c]; P.ez[g] = se[2][c];
-        P.bx[g] = bxn; P.by[g] = byn; P.bz[g] = bzn;
+        P.nex[g] = se[0][c]; P.ney[g] = se[1][c]; P.nez[g] = se[2][c];
+        P.nbx[g] = bxn; P.nby[g] = byn; P.nbz[g] = bzn;
     }
 
 #pragma unroll

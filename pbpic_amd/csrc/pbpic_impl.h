@@ -163,6 +163,11 @@ struct FdtdParams {
     int nyp;                   // conducting y walls: domain rows [0, nyp); 0 = periodic
     double dth, dtc2, idx, idy, in0;   // dt/2, dt*cva^2, 1/dx, 1/dy, in0
     double *ex, *ey, *ez, *bx, *by, *bz;
+    // fdtd_kernel writes E^{n+1}, B^{n+1} here, never into the planes above:
+    // a workgroup reads its neighbours' cells as halo, and workgroups run in
+    // no fixed order, so an in-place update would mix time levels at tile
+    // seams.  The caller swaps the two plane sets after the launch.
+    double *nex, *ney, *nez, *nbx, *nby, *nbz;
     const void *jx, *jy, *jz;
     const double *fex, *fey, *fez;     // filter increments added to E^{n+1} (or null)
     const double *lmask;               // loss-region weights per cell (or null)

@@ -185,6 +185,36 @@ def test_fieldsolver_fp64(gpu):
     assert abs(d["je_step"] - od["je_step"]) <= 1e-12 * abs(od["je_step"])
 
 
+@pytest.mark.parametrize("precision", [0, 1])
+def test_fieldsolver_many_tiles(gpu, precision):
+    """fieldsolver (pic2d.f90:678-860) at 2048 x 2048 cells, where the FDTD
+    launch has 8192 workgroups, far more than are resident at once: tiles
+    that run late must still read their neighbours' E^n and B^n as halo, not
+    fields a finished neighbour has already advanced.  Three solves with
+    random E, B and J against the oracle: FP64 to 1e-14, the split form
+    (FP32 increments on FP64 fields) to 1e-6."""
+    nx = ny = 2048
+    _say(f"fieldsolver {nx}x{ny} precision={precision}")
+    rng = np.random.default_rng(31)
+    case = dict(nx=nx, ny=ny, dx=0.1, dy=0.1, dt=0.2 / (15.0 * 5.0), cva=15.0, mime=25.0,
+                in0=1.0 / 16, species=[], sort=False, ybound=0,
+                fields={f: rng.standard_normal((ny, nx)) for f in ("ex", "ey", "ez", "bx", "by", "bz")})
+    ctx = gpu_for(case, precision=precision)
+    o = oracle_for(case)
+    tol = 1e-14 if precision == 0 else 1e-6
+    for it in range(3):
+        for name in ("jx", "jy", "jz"):
+            plane = rng.standard_normal((ny, nx))
+            ctx.set_field(name, plane)
+            o.set_field(name, plane)
+        ctx.fieldsolver()
+        o.fieldsolver()
+    for f in ("ex", "ey", "ez", "bx", "by", "bz"):
+        err = rel_norm(ctx.get_field(f), o.get_field(f))
+        assert err < tol, (f, err)
+    ctx.close()
+
+
 def test_emfgrid_matches(gpu):
     case = make_case(nx=20, ny=16, ppc=1, seed=4)
     case["species"] = []
