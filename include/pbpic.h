@@ -315,6 +315,22 @@ int pbpic_deposit(pbpic_ctx *ctx, int64_t n, const int32_t *i1,
                   const double *dx1, const double *dx2, const double *dy1,
                   const double *dy2, const int32_t *q, const double *vz);
 
+/* pbpic_deposit without any device allocation, for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  It has no counterpart in the reference.  The inputs are
+ * staged in a fixed per-GPU scratch array of PBPIC_DEPOSIT_NOALLOC_MAX
+ * entries (3.9 MB, held until the process ends) instead of a buffer
+ * allocated per call; larger n fails with PBPIC_ERR_CAPACITY.  Every context
+ * on one GPU shares that array: calls that overlap on one GPU (contexts used
+ * from two host threads at once) race.  Results equal pbpic_deposit's.      */
+#define PBPIC_DEPOSIT_NOALLOC_MAX 65536
+int pbpic_deposit_noalloc(pbpic_ctx *ctx, int64_t n, const int32_t *i1,
+                          const int32_t *i2, const int32_t *j1,
+                          const int32_t *j2, const double *dx1,
+                          const double *dx2, const double *dy1,
+                          const double *dy2, const int32_t *q,
+                          const double *vz);
+
 /* `fieldsolver` (pic2d.f90:678-860): half-step B, full-step E with -J,
  * half-step B, all on the periodic Yee lattice.  Also accumulates
  * J.E (PBPIC_D_JE_STEP / PBPIC_D_JE_INT).                                    */

@@ -25,6 +25,7 @@ COMPAT_BORIS_OLD_U, COMPAT_RHO_MIRROR, COMPAT_JY_DROP = 1, 2, 4
 COMPAT_REFERENCE = 7
 FLAG_DEPOSIT_RHO, FLAG_TRACK_PID, FLAG_PUSH_ORDER = 1, 2, 4
 FLAG_MARDER, FLAG_CHERENKOV = 8, 16
+DEPOSIT_NOALLOC_MAX = 65536     # PBPIC_DEPOSIT_NOALLOC_MAX
 YBOUND_PERIODIC, YBOUND_CONDUCTING = 0, 1
 HARRIS_DOUBLE_X, HARRIS_SINGLE_Y = 0, 1
 FIELDS = {"ex": 0, "ey": 1, "ez": 2, "bx": 3, "by": 4, "bz": 5, "jx": 6,
@@ -117,6 +118,8 @@ def load_library(path=None):
         "pbpic_zero_current": (i32, [vp]),
         "pbpic_push": (i32, [vp]),
         "pbpic_deposit": (i32, [vp, i64, pi32, pi32, pi32, pi32, pdbl, pdbl, pdbl, pdbl, pi32, pdbl]),
+        "pbpic_deposit_noalloc": (i32, [vp, i64, pi32, pi32, pi32, pi32, pdbl, pdbl, pdbl, pdbl,
+                                        pi32, pdbl]),
         "pbpic_fieldsolve": (i32, [vp]),
         "pbpic_fieldsenergy": (i32, [vp, pdbl]),
         "pbpic_kineticenergy": (i32, [vp, pdbl]),
@@ -309,18 +312,20 @@ class Context:
         """`pusher` (pic2d.f90:912-1217)."""
         self._chk(self._lib.pbpic_push(self._h))
 
-    def currentdeposit(self, i1, i2, j1, j2, dx1, dx2, dy1, dy2, q0, uz):
+    def currentdeposit(self, i1, i2, j1, j2, dx1, dx2, dy1, dy2, q0, uz, noalloc=False):
         """`currentdeposit(i1,i2,j1,j2,dx1,dx2,dy1,dy2,q0,uz)` (pic2d.f90:1226),
-        batched over equal-length arrays; `uz` is the velocity vz passed at :1029."""
+        batched over equal-length arrays; `uz` is the velocity vz passed at :1029.
+        noalloc=True calls pbpic_deposit_noalloc (no device allocation, at most
+        DEPOSIT_NOALLOC_MAX entries)."""
         ints = [_c(np.atleast_1d(a), np.int32) for a in (i1, i2, j1, j2)]
         dbl = [_c(np.atleast_1d(a), np.float64) for a in (dx1, dx2, dy1, dy2)]
         q = _c(np.atleast_1d(q0), np.int32)
         vz = _c(np.atleast_1d(uz), np.float64)
         n = len(q)
-        self._chk(self._lib.pbpic_deposit(
-            self._h, n, *[_ptr(a, ctypes.c_int32) for a in ints],
-            *[_ptr(a, ctypes.c_double) for a in dbl], _ptr(q, ctypes.c_int32),
-            _ptr(vz, ctypes.c_double)))
+        f = self._lib.pbpic_deposit_noalloc if noalloc else self._lib.pbpic_deposit
+        self._chk(f(self._h, n, *[_ptr(a, ctypes.c_int32) for a in ints],
+                    *[_ptr(a, ctypes.c_double) for a in dbl], _ptr(q, ctypes.c_int32),
+                    _ptr(vz, ctypes.c_double)))
 
     def fieldsolver(self):
         """`fieldsolver` (pic2d.f90:678-860)."""

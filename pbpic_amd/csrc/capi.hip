@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <mutex>
 #include <thread>
 #include <vector>
 
@@ -14,7 +15,36 @@
 
 using namespace pbpic;
 
+// Staging area of pbpic_deposit_noalloc, the entry point for callers that
+// cannot allocate device memory: PBPIC_DEPOSIT_NOALLOC_MAX entries of 5 int32
+// and 5 double inputs (60 B each, plus alignment slack), 3.9 MB per GPU, held
+// until the process ends.  In host code the name stands for a host-side
+// shadow, not device memory: only its address from hipGetSymbolAddress (taken
+// per device, kept in scratch_dev) is handed to copies and kernels.
+static __device__ __align__(256) char scratch[PBPIC_DEPOSIT_NOALLOC_MAX * 60 + 256];
+
 namespace {
+
+constexpr int SCRATCH_MAX_DEVICES = 64;
+std::mutex scratch_mu;
+void *scratch_dev[SCRATCH_MAX_DEVICES] = {};
+
+// The device address of `scratch` on the current device dev, looked up once
+// per device (each GPU holds its own copy of the array at its own address).
+hipError_t scratch_address(int dev, void **out)
+{
+    *out = nullptr;
+    if (dev < 0 || dev >= SCRATCH_MAX_DEVICES) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(scratch_mu);
+    if (!scratch_dev[dev]) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        scratch_dev[dev] = ptr;
+    }
+    *out = scratch_dev[dev];
+    return hipSuccess;
+}
 
 struct Species {
     int32_t q = 0;
@@ -38,6 +68,7 @@ struct pbpic_ctx {
     int rsz = 8;
     int dev = 0;
     hipStream_t st = nullptr;
+    void *scratch = nullptr;   // this device's address of `scratch` (pbpic_deposit_noalloc)
     double *E[3] = {nullptr, nullptr, nullptr};
     double *B[3] = {nullptr, nullptr, nullptr};
     // the field solve writes E^{n+1}, B^{n+1} into these and swaps them with
@@ -749,6 +780,54 @@ int get_particles_t(pbpic_ctx *c, int32_t s, int64_t cap, int64_t *nout, int32_t
     return PBPIC_OK;
 }
 
+// pbpic_deposit's inputs as staged on the device: 5 int32 and 5 double arrays
+// per entry.
+constexpr size_t DEPOSIT_ENTRY_BYTES = 5 * 4 + 5 * 8;
+
+int deposit_check(pbpic_ctx *c, int64_t n, const int32_t *i1, const int32_t *i2,
+                  const int32_t *j1, const int32_t *j2, const double *dx1, const double *dx2,
+                  const double *dy1, const double *dy2, const int32_t *q, const double *vz)
+{
+    if (!c) return PBPIC_ERR_ARG;
+    if (n < 0) return fail(c, PBPIC_ERR_ARG, "deposit count %lld < 0", (long long)n);
+    if (n == 0) return PBPIC_OK;
+    if (!i1 || !i2 || !j1 || !j2 || !dx1 || !dx2 || !dy1 || !dy2 || !q || !vz)
+        return fail(c, PBPIC_ERR_ARG, "null deposit array");
+    const int nx = c->p.nx, ny = c->p.ny;
+    for (int64_t k = 0; k < n; k++) {
+        if (i1[k] < 1 || i1[k] > nx || j1[k] < 1 || j1[k] > ny || std::abs(i2[k] - i1[k]) > 1 ||
+            std::abs(j2[k] - j1[k]) > 1 || (q[k] != 1 && q[k] != -1))
+            return fail(c, PBPIC_ERR_ARG, "deposit entry %lld out of range", (long long)k);
+    }
+    return PBPIC_OK;
+}
+
+// Uploads n checked entries into buf (n * DEPOSIT_ENTRY_BYTES bytes of device
+// memory), deposits them into J (and rho) and waits for the stream.
+int deposit_staged(pbpic_ctx *c, void *buf, int64_t n, const int32_t *i1, const int32_t *i2,
+                   const int32_t *j1, const int32_t *j2, const double *dx1, const double *dx2,
+                   const double *dy1, const double *dy2, const int32_t *q, const double *vz)
+{
+    const size_t ib = (size_t)n * 4, db = (size_t)n * 8;
+    char *b = (char *)buf;
+    int32_t *d_i1 = (int32_t *)b, *d_i2 = (int32_t *)(b + ib), *d_j1 = (int32_t *)(b + 2 * ib),
+            *d_j2 = (int32_t *)(b + 3 * ib), *d_q = (int32_t *)(b + 4 * ib);
+    // the doubles start at an 8-byte boundary: 5 ib = 20 n bytes, padded
+    double *d_dx1 = (double *)(b + ((5 * ib + 7) & ~(size_t)7)), *d_dx2 = d_dx1 + n,
+           *d_dy1 = d_dx2 + n, *d_dy2 = d_dy1 + n, *d_vz = d_dy2 + n;
+    const void *src[10] = {i1, i2, j1, j2, q, dx1, dx2, dy1, dy2, vz};
+    void *dst[10] = {d_i1, d_i2, d_j1, d_j2, d_q, d_dx1, d_dx2, d_dy1, d_dy2, d_vz};
+    for (int k = 0; k < 10; k++)
+        HIPCHK(c, hipMemcpyAsync(dst[k], src[k], k < 5 ? ib : db, hipMemcpyHostToDevice, c->st));
+    PushParams P = push_params(c);
+    P.sp[0].qx = c->p.dx / c->p.dt;
+    P.sp[0].qy = c->p.dy / c->p.dt;
+    HIPCHK(c, launch_deposit_batch(P, c->p.precision, n, d_i1, d_i2, d_j1, d_j2, d_dx1, d_dx2,
+                                   d_dy1, d_dy2, d_q, d_vz, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return PBPIC_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -803,6 +882,7 @@ int pbpic_ctx_create(const pbpic_params *pp, pbpic_ctx **out)
     };
     chk(hipGetDevice(&c->dev), "hipGetDevice");
     chk(hipStreamCreate(&c->st), "hipStreamCreate");
+    chk(scratch_address(c->dev, &c->scratch), "hipGetSymbolAddress(scratch)");
     const size_t nc = (size_t)p.nx * p.ny;
     for (int k = 0; k < 3; k++) {
         chk(dalloc(&c->E[k], nc * 8), "alloc E");
@@ -1170,41 +1250,29 @@ int pbpic_deposit(pbpic_ctx *c, int64_t n, const int32_t *i1, const int32_t *i2,
                   const int32_t *j2, const double *dx1, const double *dx2, const double *dy1,
                   const double *dy2, const int32_t *q, const double *vz)
 {
-    if (!c) return PBPIC_ERR_ARG;
-    if (n <= 0) return PBPIC_OK;
-    if (!i1 || !i2 || !j1 || !j2 || !dx1 || !dx2 || !dy1 || !dy2 || !q || !vz) return fail(c, PBPIC_ERR_ARG, "null deposit array");
-    const int nx = c->p.nx, ny = c->p.ny;
-    for (int64_t k = 0; k < n; k++) {
-        if (i1[k] < 1 || i1[k] > nx || j1[k] < 1 || j1[k] > ny || std::abs(i2[k] - i1[k]) > 1 ||
-            std::abs(j2[k] - j1[k]) > 1 || (q[k] != 1 && q[k] != -1))
-            return fail(c, PBPIC_ERR_ARG, "deposit entry %lld out of range", (long long)k);
-    }
+    int rc = deposit_check(c, n, i1, i2, j1, j2, dx1, dx2, dy1, dy2, q, vz);
+    if (rc || n <= 0) return rc;
     void *buf = nullptr;
-    const size_t ib = (size_t)n * 4, db = (size_t)n * 8;
-    HIPCHK(c, hipMalloc(&buf, 5 * ib + 5 * db));
-    char *b = (char *)buf;
-    int32_t *d_i1 = (int32_t *)b, *d_i2 = (int32_t *)(b + ib), *d_j1 = (int32_t *)(b + 2 * ib),
-            *d_j2 = (int32_t *)(b + 3 * ib), *d_q = (int32_t *)(b + 4 * ib);
-    double *d_dx1 = (double *)(b + 5 * ib), *d_dx2 = d_dx1 + n, *d_dy1 = d_dx2 + n, *d_dy2 = d_dy1 + n,
-           *d_vz = d_dy2 + n;
-    int rc = PBPIC_OK;
-    const void *src[10] = {i1, i2, j1, j2, q, dx1, dx2, dy1, dy2, vz};
-    void *dst[10] = {d_i1, d_i2, d_j1, d_j2, d_q, d_dx1, d_dx2, d_dy1, d_dy2, d_vz};
-    for (int k = 0; k < 10 && rc == PBPIC_OK; k++) {
-        hipError_t e = hipMemcpyAsync(dst[k], src[k], k < 5 ? ib : db, hipMemcpyHostToDevice, c->st);
-        if (e != hipSuccess) rc = fail(c, PBPIC_ERR_HIP, "deposit upload: %s", hipGetErrorString(e));
-    }
-    if (rc == PBPIC_OK) {
-        PushParams P = push_params(c);
-        P.sp[0].qx = c->p.dx / c->p.dt;
-        P.sp[0].qy = c->p.dy / c->p.dt;
-        hipError_t e = launch_deposit_batch(P, c->p.precision, n, d_i1, d_i2, d_j1, d_j2, d_dx1,
-                                            d_dx2, d_dy1, d_dy2, d_q, d_vz, c->st);
-        if (e == hipSuccess) e = hipStreamSynchronize(c->st);
-        if (e != hipSuccess) rc = fail(c, PBPIC_ERR_HIP, "deposit: %s", hipGetErrorString(e));
-    }
+    HIPCHK(c, hipMalloc(&buf, (size_t)n * DEPOSIT_ENTRY_BYTES + 8));
+    rc = deposit_staged(c, buf, n, i1, i2, j1, j2, dx1, dx2, dy1, dy2, q, vz);
     (void)hipFree(buf);
     return rc;
+}
+
+int pbpic_deposit_noalloc(pbpic_ctx *c, int64_t n, const int32_t *i1, const int32_t *i2,
+                          const int32_t *j1, const int32_t *j2, const double *dx1,
+                          const double *dx2, const double *dy1, const double *dy2,
+                          const int32_t *q, const double *vz)
+{
+    int rc = deposit_check(c, n, i1, i2, j1, j2, dx1, dx2, dy1, dy2, q, vz);
+    if (rc || n <= 0) return rc;
+    if (n > PBPIC_DEPOSIT_NOALLOC_MAX)
+        return fail(c, PBPIC_ERR_CAPACITY,
+                    "pbpic_deposit_noalloc: n=%lld exceeds its fixed scratch of %d entries "
+                    "(use pbpic_deposit, or split the batch)",
+                    (long long)n, PBPIC_DEPOSIT_NOALLOC_MAX);
+    if (!c->scratch) return fail(c, PBPIC_ERR_STATE, "no deposit scratch on device %d", c->dev);
+    return deposit_staged(c, c->scratch, n, i1, i2, j1, j2, dx1, dx2, dy1, dy2, q, vz);
 }
 
 int pbpic_fieldsolve(pbpic_ctx *c)
