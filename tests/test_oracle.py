@@ -23,6 +23,13 @@ wrong sign or index) fails.
   fieldsenergy (:653-673)          closed form for uniform fields
   normalization (:15-43, :742)     Langmuir oscillation at omega_pe
   Philox loader                    Random123 known-answer vectors
+  kinetic tally (:1061-1062)       numpy sum of (gamma-1) ms over the pushed
+                                   momenta, ions at mime
+  mirrored rho (:1283-1295)        bit-identical to the corrected deposit at
+                                   the mirrored offset (-dx, -dy)
+  J.E (defined here)               closed form for uniform fields; discrete
+                                   Poynting theorem of the Yee scheme to
+                                   rounding over many steps
 """
 import numpy as np
 import pytest
@@ -54,6 +61,26 @@ def test_countingsort_is_stable_bincount(oracle_lib, prec):
     p = o.get_particles()
     assert np.array_equal(p["grid"], np.sort(grid, kind="stable"))
     assert np.allclose(p["ux"], ux[perm], rtol=1e-6 if prec == 4 else 0)
+
+
+def test_countingsort_threaded_equals_serial(oracle_lib):
+    """The chunk-parallel counting sort (per-chunk counts, offsets by chunk
+    order) gives the serial loop's stable result element for element."""
+    rng = np.random.default_rng(12)
+    nx, ny, n = 40, 30, 50000
+    grid = rng.integers(1, nx * ny + 1, n).astype(np.int32)
+    ux = rng.standard_normal(n)
+    out = []
+    for threads in (1, 4):
+        o = oracle.Oracle(nx, ny, 0.1, 0.1, 0.01, CVA, MIME, 0.1, prec=4, threads=threads)
+        o.set_particles(grid, rng.random(n) - 0.5, rng.random(n) - 0.5, ux, ux, ux,
+                        np.full(n, -1, np.int32))
+        out.append(o.countingsort(want=True) + (o.get_particles(),))
+    (c1, p1, s1), (c4, p4, s4) = out
+    assert np.array_equal(c1, c4) and np.array_equal(p1, p4)
+    assert np.array_equal(p1, np.argsort(grid, kind="stable"))
+    for k in ("grid", "ux"):
+        assert np.array_equal(s1[k], s4[k])
 
 
 def test_countingsort_empty(oracle_lib):
@@ -381,3 +408,150 @@ def test_loader_moments(oracle_lib):
     assert abs(np.std(p["uy"]) / uth - 1) < 0.01
     beam = np.where(np.arange(len(p["ux"])) % 2 == 0, 1, -1)
     assert abs(np.mean(p["ux"] * beam) / ud - 1) < 0.01
+
+
+# ------------------------------------------ kinetic energy, rho, J.E pins
+@pytest.mark.parametrize("prec", [8, 4])
+def test_kinetic_energy_tally(oracle_lib, prec):
+    """totene(j) += (gamma - 1) ms with j = 1 for electrons and 2 for ions,
+    gamma from the pushed momenta (pic2d.f90:1003, :1061-1062): equals a
+    numpy sum over the particles after the push, ions weighted by mime and
+    electrons by 1 (a wrong ion mass or a pre-push gamma fails)."""
+    case = make_case(nx=12, ny=10, ppc=9, seed=13, vth=(0.08, 0.4))
+    o = oracle_for(case, prec=prec)
+    o.zero_current()
+    o.pusher()
+    d = o.diag()
+    p = o.get_particles()
+    u2 = p["ux"] ** 2 + p["uy"] ** 2 + p["uz"] ** 2
+    ke = np.sqrt(1.0 + u2) - 1.0
+    ke_e = ke[p["q"] < 0].sum()
+    ke_i = MIME * ke[p["q"] > 0].sum()
+    tol = 1e-12 if prec == 8 else 2e-5
+    assert abs(d["ke_e"] - ke_e) <= tol * ke_e
+    assert abs(d["ke_i"] - ke_i) <= tol * ke_i
+    # the momenta before the push give a different tally: the test can tell
+    pre = case["species"]
+    u2_0 = np.concatenate([s["ux"] ** 2 + s["uy"] ** 2 + s["uz"] ** 2 for s in pre[1:]])
+    if prec == 8:
+        assert abs((np.sqrt(1 + u2_0) - 1).sum() - ke_e) > 1e4 * tol * ke_e
+
+
+@pytest.mark.parametrize("q", [1, -1])
+def test_mirrored_rho_is_the_deposit_at_the_mirrored_offset(oracle_lib, q):
+    """The reference deposits rho at the old position with the weight of node
+    i1 = (1/2 + dx1) (pic2d.f90:1283-1295), where the gather gives node ix the
+    weight (1/2 - dx) (:948-951).  The mirrored deposit at (dx, dy) therefore
+    equals, bit for bit, the corrected (CIC, charge-conserving) deposit at
+    (-dx, -dy); the corrected one is pinned by charge continuity
+    (test_deposit_charge_continuity)."""
+    rng = np.random.default_rng(17)
+    nx, ny, n = 11, 9, 600
+    i1 = rng.integers(1, nx + 1, n)
+    j1 = rng.integers(1, ny + 1, n)
+    dx1 = rng.random(n) - 0.5
+    dy1 = rng.random(n) - 0.5
+    qq = np.full(n, q)
+    zero = np.zeros(n)
+    out = {}
+    for compat, sx in ((oracle.COMPAT_REFERENCE & 2, 1.0), (0, -1.0)):
+        o = oracle.Oracle(nx, ny, 0.1, 0.1, 0.003, CVA, MIME, 0.25, compat=compat)
+        o.deposit_batch(i1, i1, j1, j1, sx * dx1, sx * dx1, sx * dy1, sx * dy1, qq, zero)
+        o.finish_current()
+        out[compat] = o.get_field("rho")
+    assert np.array_equal(out[2], out[0])
+    # and the mirror is not the identity: the two weightings differ
+    o = oracle.Oracle(nx, ny, 0.1, 0.1, 0.003, CVA, MIME, 0.25, compat=0)
+    o.deposit_batch(i1, i1, j1, j1, dx1, dx1, dy1, dy1, qq, zero)
+    o.finish_current()
+    assert not np.allclose(o.get_field("rho"), out[2])
+
+
+def test_jdote_closed_form_uniform_fields(oracle_lib):
+    """One field solve with uniform E^n, B = 0 and a uniform current: curl
+    terms vanish, so E^{n+1} = E^n - dt c^2 J (normalized J, :742-744), and
+    je_step = (mime/c^2) sum_cells J_raw . (E^n + E^{n+1})/2, the rate at
+    which field energy goes into the current.  The field energy then changes
+    by exactly -dt je_step (the discrete Poynting balance is exact here), and
+    J.E taken with E^n alone or E^{n+1} alone would miss both by
+    dt^2 c^2 |J|^2 terms far above rounding."""
+    nx, ny, dxg, dt, in0 = 10, 7, 0.1, 0.004, 0.2
+    o = oracle.Oracle(nx, ny, dxg, dxg, dt, CVA, MIME, in0)
+    E0 = np.array([0.3, -0.7, 0.45])
+    Jn = np.array([0.002, 0.0035, -0.0015])           # normalized (x in0) current
+    for k, v in zip(("ex", "ey", "ez"), E0):
+        o.set_field(k, np.full((ny, nx), v))
+    for k in ("bx", "by", "bz"):
+        o.set_field(k, np.zeros((ny, nx)))
+    for k, v in zip(("jx", "jy", "jz"), Jn):
+        o.set_field(k, np.full((ny, nx), v))
+    we0, wb0 = o.fieldsenergy()
+    o.fieldsolver()
+    E1 = E0 - dt * CVA ** 2 * Jn
+    for k, v in zip(("ex", "ey", "ez"), E1):
+        assert np.allclose(o.get_field(k), v, rtol=1e-14, atol=1e-16)
+    d = o.diag()
+    Jraw = Jn / in0
+    je = MIME / CVA ** 2 * nx * ny * (Jraw @ (0.5 * (E0 + E1)))
+    assert abs(d["je_step"] - je) <= 1e-12 * abs(je)
+    assert abs(d["je_int"] - dt * je) <= 1e-12 * abs(dt * je)
+    we1, wb1 = o.fieldsenergy()
+    assert wb0 == 0 and wb1 == 0
+    assert abs((we1 - we0) + dt * d["je_step"]) <= 1e-10 * abs(dt * d["je_step"])
+    for wrong in (E0, E1):
+        je_w = MIME / CVA ** 2 * nx * ny * (Jraw @ wrong)
+        assert abs((we1 - we0) + dt * je_w) > 1e3 * 1e-10 * abs(dt * je)
+
+
+def _yee_energy(o, case):
+    """The energy the Yee scheme conserves between integer times n:
+    (mime/2) [ |E^n|^2 / (c^4 in0) + B^{n-1/2}.B^{n+1/2} / (c^2 in0) ], with
+    B^{n+-1/2} = B^n -+ (dt/2) curl E^n, so B^{n-1/2}.B^{n+1/2} =
+    |B^n|^2 - (dt/2)^2 |curl E^n|^2, on the staggered planes (x fastest)."""
+    mime, cva, in0, dt, dx, dy = (case[k] for k in ("mime", "cva", "in0", "dt", "dx", "dy"))
+    ex, ey, ez, bx, by, bz = (o.get_field(k) for k in ("ex", "ey", "ez", "bx", "by", "bz"))
+    r = np.roll
+    cex = (ez - r(ez, 1, 0)) / dy                       # the curls of fieldsolver :688-690
+    cey = -(ez - r(ez, 1, 1)) / dx
+    cez = (r(ex, -1, 0) - ex) / dy - (r(ey, -1, 1) - ey) / dx
+    e2 = (ex ** 2 + ey ** 2 + ez ** 2).sum()
+    bb = (bx ** 2 + by ** 2 + bz ** 2).sum() - (0.5 * dt) ** 2 * (cex ** 2 + cey ** 2 + cez ** 2).sum()
+    return 0.5 * mime * (e2 / (cva ** 4 * in0) + bb / (cva ** 2 * in0))
+
+
+@pytest.mark.parametrize("compat", [0, 7])
+def test_jdote_poynting_balance_over_steps(oracle_lib, compat):
+    """The discrete Poynting theorem of the Yee scheme: with J^{n+1/2} taken
+    against (E^n + E^{n+1})/2, the curl terms cancel by summation by parts on
+    the periodic lattice and W_Yee(n) + integral of J.E is constant to
+    rounding over any number of full steps (push, deposit, fold, field solve),
+    whatever the deposit puts into J.  A thermal plasma with an Ey mode over
+    48 steps: the balance holds to 1e-12 of the energy exchanged.  The
+    node-averaged energies of fieldsenergy do not close it (they undercount
+    grid-scale modes), and neither does J.E with E^n alone."""
+    case = make_case(nx=32, ny=16, ppc=16, seed=29, efield=0.0, bfield=0.0, vth=(0.02, 0.1))
+    x = np.arange(case["nx"])[None, :] * np.ones((case["ny"], 1))
+    case["fields"]["ey"] = 2.0 * np.sin(2 * np.pi * x / case["nx"])
+    o = oracle_for(case, compat=compat)
+    w0 = _yee_energy(o, case)
+    peak = 0.0
+    for _ in range(6):
+        o.step(8, sort_every=20)
+        dw = _yee_energy(o, case) - w0
+        peak = max(peak, abs(dw))
+        je_int = o.diag()["je_int"]
+        assert abs(dw + je_int) <= 1e-12 * peak
+    assert peak > 0.5 * w0                         # a real exchange took place
+    # a plausible slip: J.E with the field before the update (je_step uses
+    # the average; its E^n form differs by dt c^2 J.(curl B - J)/2 per step)
+    o2 = oracle_for(case, compat=compat)
+    o2.step(1, sort_every=0)
+    d = o2.diag()
+    jn = [o2.get_field(k) for k in ("jx", "jy", "jz")]
+    e1 = [o2.get_field(k) for k in ("ex", "ey", "ez")]
+    e0 = [case["fields"][k] for k in ("ex", "ey", "ez")]
+    scale = case["mime"] / (case["cva"] ** 2 * case["in0"])
+    avg = scale * sum((j * 0.5 * (a + b)).sum() for j, a, b in zip(jn, e0, e1))
+    old = scale * sum((j * a).sum() for j, a in zip(jn, e0))
+    assert abs(d["je_step"] - avg) <= 1e-12 * abs(avg)
+    assert abs(old - avg) > 1e-6 * abs(avg)
