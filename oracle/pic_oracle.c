@@ -136,6 +136,8 @@ void orc_init_harris(int nx, int ny, int ywall, double dx, double dy, int geomet
     const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
     const int nn = geometry == 0 ? nx : nyp, nt = geometry == 0 ? nyp : nx;
     const double dn = geometry == 0 ? dx : dy;
+    /* each particle depends only on its own Philox counters */
+#pragma omp parallel for schedule(static)
     for (int64_t p = 0; p < n; p++) {
         uint32_t w[10][4];
         for (uint32_t b = 0; b < 10; b++) {

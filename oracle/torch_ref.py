@@ -190,10 +190,10 @@ def run(P, E, B, species, nsteps, compat=7, sort_every=0, step0=0, loss_mask=Non
     loss_mask: (ny, nx) weights of each cell's J.E term (None: all 1)."""
     nx, ny = P["nx"], P["ny"]
     w = P["dt"] * P["mime"] / (P["cva"] * P["cva"])
-    loss = torch.zeros((), dtype=torch.float64)
+    loss = torch.zeros((), dtype=torch.float64, device=E.device)
     for n in range(nsteps):
         N = node_fields(E, B)
-        J = [torch.zeros(nx * ny, dtype=torch.float64) for _ in range(3)]
+        J = [torch.zeros(nx * ny, dtype=torch.float64, device=E.device) for _ in range(3)]
         species = [push_deposit(P, sp, N, J, compat) for sp in species]
         Jp = torch.stack([j.reshape(ny, nx) for j in J])
         if P.get("nyp"):
