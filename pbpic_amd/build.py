@@ -74,6 +74,20 @@ def build_fortran(verbose=True):
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
+    # the test program that moves the reference's own host state (type
+    # particle / fields / density arrays) through the module, in the
+    # reference's default 4-byte real and in a -fdefault-real-8 build
+    tdir = os.path.join(ROOT, "tests", "fortran")
+    for kind, flags in (("r4", []), ("r8", ["-fdefault-real-8"])):
+        mdir = os.path.join(HERE, "_build", "fortran_" + kind)
+        os.makedirs(mdir, exist_ok=True)
+        cmd = [fc, "-O2", *flags, "-J", mdir, os.path.join(fdir, "pbpic_mod.f90"),
+               os.path.join(tdir, "ref_state_step.f90"), "-o",
+               os.path.join(tdir, "ref_state_step_" + kind), "-L" + HERE, "-lpbpic",
+               "-Wl,-rpath," + os.path.relpath(HERE, tdir).join(["$ORIGIN/", ""])]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True, cwd=mdir)
     return out
 
 
