@@ -159,6 +159,13 @@ extern "C" {
                                     with global atomics, since creation: a
                                     performance counter, zero in normal
                                     runs                                    */
+#define PBPIC_D_SORTS_RADIX 12 /* species sorts since creation that took the
+                                  radix path: every sort not right after a
+                                  push (no recorded tile boxes), one whose
+                                  per-cell counts are requested, or one with
+                                  a tile box above 4096 cells; the others
+                                  take the tile-box path.  A performance
+                                  counter                                  */
 #define PBPIC_NDIAG         16
 
 typedef struct pbpic_ctx pbpic_ctx;

@@ -32,7 +32,8 @@ FIELDS = {"ex": 0, "ey": 1, "ez": 2, "bx": 3, "by": 4, "bz": 5, "jx": 6,
           "jy": 7, "jz": 8, "rho": 9, "gex": 10, "gey": 11, "gez": 12,
           "gbx": 13, "gby": 14, "gbz": 15}
 DIAG_NAMES = ["ke_e", "ke_i", "w_e", "w_b", "je_step", "je_int", "gauss_l2",
-              "gauss_linf", "gauss_dl2", "gauss_dlinf", "steps", "tiles_spilled"]
+              "gauss_linf", "gauss_dl2", "gauss_dlinf", "steps", "tiles_spilled",
+              "sorts_radix"]
 _ERR = {-1: "invalid argument", -2: "HIP runtime error", -3: "capacity",
         -4: "CFL violation", -5: "non-finite value", -6: "cell out of range",
         -7: "bad state", -8: "out of device memory", -9: "sort spin limit"}
@@ -347,13 +348,20 @@ class Context:
     def countingsort(self, species=None, want=False):
         """`countingsort` (pic2d.f90:1303-1329).  With a species and want=True
         returns (counts, perm): per-cell counts indexed by grid-1 and the
-        0-based pre-sort index of each particle after the sort."""
+        0-based pre-sort index of each particle after the sort; want="perm"
+        returns the permutation alone (a sort right after a push then takes
+        the tile-box path, which computes no per-cell counts)."""
         if species is None:
             self._chk(self._lib.pbpic_countingsort(self._h))
             return None
         if not want:
             self._chk(self._lib.pbpic_countingsort_species(self._h, species, None, None))
             return None
+        if want == "perm":
+            perm = np.empty(max(self.num_particles(species), 1), np.int32)
+            self._chk(self._lib.pbpic_countingsort_species(self._h, species, None,
+                                                           _ptr(perm, ctypes.c_int32)))
+            return perm[: self.num_particles(species)]
         counts = np.empty(self.nx * self.ny, np.int32)
         perm = np.empty(max(self.num_particles(species), 1), np.int32)
         self._chk(self._lib.pbpic_countingsort_species(

@@ -93,6 +93,7 @@ struct pbpic_ctx {
     std::vector<std::array<void *, 6>> push_out;
     SortScratch sort;
     int64_t steps = 0;
+    int64_t sorts_radix = 0;   // sorts that took the radix path (PBPIC_D_SORTS_RADIX)
     int maxw = 0;
     double idx = 0, idy = 0, dth = 0, dtc2 = 0, dtcva = 0;
     std::string last_error;
@@ -286,8 +287,11 @@ int sort_one(pbpic_ctx *c, int s, int32_t *counts, int32_t *perm, uint32_t *perm
     const bool po = (c->p.flags & PBPIC_FLAG_PUSH_ORDER) != 0;
     CellKey key;
     key.init(c->p.nx, c->p.ny, po ? 1 : 0);
+    bool boxed = false;
     HIPCHK(c, sort_species(c->sort, src, c->p.precision, po ? 1 : 0, key, ncell(c), dst, c->err,
-                           counts != nullptr, S.rec, rec_written && S.rec, c->st));
+                           counts != nullptr, perm != nullptr || perm_dev != nullptr, S.rec,
+                           rec_written && S.rec, &boxed, c->st));
+    if (!boxed && S.n > 0) c->sorts_radix++;
     HIPCHK(c, prof_mark(c, PBPIC_PROF_SORT, false));
     S.cur ^= 1;
     S.tbox_ok = false;
@@ -941,6 +945,7 @@ int pbpic_ctx_destroy(pbpic_ctx *c)
     for (auto &S : c->sp) { f(S.tbox); f(S.rec); }
     for (int b = 0; b < 2; b++) { f(c->sort.keys[b]); f(c->sort.vals[b]); }
     f(c->sort.status); f(c->sort.dhist); f(c->sort.ticket); f(c->sort.counts);
+    f(c->sort.tarea); f(c->sort.toff); f(c->sort.bsum); f(c->sort.ctl); f(c->sort.rbuf);
     for (auto p : c->adj.ckpts) f(p);
     f(c->adj.final_state); f(c->adj.jstore);
     for (auto e : c->ev_pool) (void)hipEventDestroy(e);
@@ -1000,6 +1005,11 @@ int pbpic_add_species(pbpic_ctx *c, int32_t q, int64_t capacity, int32_t *specie
         if (c->sort.status) (void)hipFree(c->sort.status);
         const int64_t ntile = (capacity + 4095) / 4096 + 1;
         HIPCHK(c, dalloc(&c->sort.status, (size_t)ntile * 256 * 4));
+        for (uint32_t **q : {&c->sort.tarea, &c->sort.toff, &c->sort.bsum}) {
+            if (*q) (void)hipFree(*q);
+            HIPCHK(c, dalloc(q, (size_t)(ntile + 1) * 4));
+        }
+        if (!c->sort.ctl) HIPCHK(c, dalloc(&c->sort.ctl, 8 * sizeof(int)));
         c->sort.cap = capacity;
         c->sort.status_cap = ntile;
     }
@@ -1370,6 +1380,7 @@ int pbpic_diag(pbpic_ctx *c, double out[PBPIC_NDIAG])
     out[PBPIC_D_GAUSS_DLINF] = a[ACC_G_DLINF];
     out[PBPIC_D_STEPS] = (double)c->steps;
     out[PBPIC_D_TILES_SPILLED] = a[ACC_SPILL];
+    out[PBPIC_D_SORTS_RADIX] = (double)c->sorts_radix;
     return PBPIC_OK;
 }
 

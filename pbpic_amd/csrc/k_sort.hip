@@ -3,12 +3,15 @@
 // The reference is a stable counting sort of the particles by cell id:
 // histogram (:1310-1313), exclusive scan (:1315-1320), stable scatter
 // (:1322-1327), copy back (:1328).  With millions of cells a per-block x
-// per-cell count matrix does not fit, so the stable scatter is done as an LSD
-// radix sort of (cell, index) pairs over 8-bit digits; each digit pass is
-// itself a counting sort (a per-wave ballot histogram with in-order ranks,
-// then a decoupled look-back prefix scan across tiles).  A final gather moves
-// the particle payload once.  The result (permutation and per-cell counts) is
-// identical to the reference's: both are the unique stable sort by cell.
+// per-cell count matrix does not fit.  Right after a push (the sort of the
+// time loop, :180-182) the ranks come from the push's tile boxes (the
+// tile-box path below); otherwise the stable scatter is done as an LSD radix
+// sort of (cell, index) pairs over 8-bit digits, each digit pass itself a
+// counting sort (a per-wave ballot histogram with in-order ranks, then a
+// decoupled look-back prefix scan across tiles).  Either way a final gather
+// moves the particle payload once.  The result (permutation and per-cell
+// counts) is identical to the reference's: both are the unique stable sort
+// by cell.
 #include "pbpic_impl.h"
 
 namespace pbpic {
@@ -395,6 +398,445 @@ __global__ __launch_bounds__(256) void gather_records_kernel(const uint32_t *per
     }
 }
 
+// ---------------------------------------------------------------------
+// Tile-box path.  Right after a push every push tile (PUSH_TILE consecutive
+// particles) has the bounding box of its cells recorded (SpeciesDev::tbox),
+// a few hundred cells.  The stable rank of particle i is
+//   (particles of smaller keys) + (particles of key(i) in earlier tiles)
+//   + (particles of key(i) before i in its own tile).
+// The first two terms depend only on (tile, box cell): one record per box
+// cell of every tile, holding that cell's count in the tile.  Sorting the
+// records by key, stably (records are laid out tile by tile), and taking an
+// exclusive scan of their counts in that order gives both terms at once.
+// Each tile then orders its own particles by box cell (a stable in-tile
+// counting sort), one wave per 64 records expands the records into the
+// permutation (source of every rank), and the gather moves the payload as
+// after the radix path.  The records number sum(box areas), about 6% of the
+// particles at 100 per cell, so the three radix passes run over them
+// instead of over the particles.
+constexpr int BX_MAX = 4096;             // largest box (cells) a tile may have
+
+enum : int { BXF_FALLBACK = 1, BXF_OUTSIDE = 2 };
+
+__global__ __launch_bounds__(256) void box_area_kernel(const int32_t *tbox, int64_t ntiles,
+                                                       uint32_t *area, int *ctl)
+{
+    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (t >= ntiles) return;
+    const int4 b = *reinterpret_cast<const int4 *>(tbox + 4 * t);
+    uint32_t a = 0;
+    if (b.y < 0 || b.w < 0 || (int64_t)(b.y + 1) * (b.w + 1) > BX_MAX) {
+        atomicOr(ctl, BXF_FALLBACK);
+    } else {
+        a = (uint32_t)((b.y + 1) * (b.w + 1));
+        atomicMax(ctl + 1, (int)a);
+    }
+    area[t] = a;
+}
+
+// Exclusive scan of n u32 values (load(i)) into store(i, v), three kernels:
+// 4096 values per block are reduced, the block sums scanned by one block
+// (the total goes to *total), then each block scans its values again.
+constexpr int SC_T = 256, SC_E = 16, SC_B = SC_T * SC_E;
+
+struct LdArr {
+    const uint32_t *a;
+    __device__ uint32_t operator()(int64_t i) const { return a[i]; }
+};
+struct LdGather {           // a[idx[i]]
+    const uint32_t *a, *idx;
+    __device__ uint32_t operator()(int64_t i) const { return a[idx[i]]; }
+};
+struct StArr {
+    uint32_t *o;
+    __device__ void operator()(int64_t i, uint32_t v) const { o[i] = v; }
+};
+
+template <class L>
+__global__ __launch_bounds__(SC_T) void scan_reduce_kernel(L ld, int64_t n, uint32_t *bsum)
+{
+    __shared__ uint32_t w[SC_T / 64];
+    const int64_t b0 = (int64_t)blockIdx.x * SC_B;
+    uint32_t acc = 0;
+    for (int e = 0; e < SC_E; e++) {
+        const int64_t i = b0 + (int64_t)e * SC_T + threadIdx.x;
+        if (i < n) acc += ld(i);
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+    if ((threadIdx.x & 63) == 0) w[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) bsum[blockIdx.x] = (w[0] + w[1]) + (w[2] + w[3]);
+}
+
+// one block of 1024 threads: exclusive scan of bsum[0, nb) in place
+__global__ __launch_bounds__(1024) void scan_top_kernel(uint32_t *bsum, int64_t nb, int *total)
+{
+    __shared__ uint32_t part[1024];
+    const int64_t per = (nb + 1023) / 1024;
+    const int64_t a = threadIdx.x * per, e = a + per < nb ? a + per : nb;
+    uint32_t sum = 0;
+    for (int64_t i = a; i < e; i++) sum += bsum[i];
+    part[threadIdx.x] = sum;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t run = 0;
+        for (int k = 0; k < 1024; k++) {
+            const uint32_t v = part[k];
+            part[k] = run;
+            run += v;
+        }
+        if (total) *total = (int)run;
+    }
+    __syncthreads();
+    uint32_t run = part[threadIdx.x];
+    for (int64_t i = a; i < e; i++) {
+        const uint32_t v = bsum[i];
+        bsum[i] = run;
+        run += v;
+    }
+}
+
+template <class L, class S>
+__global__ __launch_bounds__(SC_T) void scan_down_kernel(L ld, S st, int64_t n,
+                                                         const uint32_t *boff)
+{
+    __shared__ uint32_t w[SC_T / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t b0 = (int64_t)blockIdx.x * SC_B;
+    uint32_t run = boff[blockIdx.x];
+    for (int e = 0; e < SC_E; e++) {
+        const int64_t i = b0 + (int64_t)e * SC_T + threadIdx.x;
+        const uint32_t v = i < n ? ld(i) : 0u;
+        // inclusive scan across the wave, then across the four waves
+        uint32_t x = v;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t y = __shfl_up(x, o);
+            if (lane >= o) x += y;
+        }
+        if (lane == 63) w[wave] = x;
+        __syncthreads();
+        uint32_t before = 0, all = 0;
+        for (int k = 0; k < SC_T / 64; k++) {
+            if (k < wave) before += w[k];
+            all += w[k];
+        }
+        if (i < n) st(i, run + before + x - v);
+        run += all;
+        __syncthreads();
+    }
+}
+
+template <class L, class S>
+static hipError_t exclusive_scan(L ld, S st, int64_t n, uint32_t *bsum, int *total,
+                                 hipStream_t stream)
+{
+    const int64_t nb = (n + SC_B - 1) / SC_B;
+    if (nb == 0) return hipSuccess;
+    hipLaunchKernelGGL(scan_reduce_kernel<L>, dim3((unsigned)nb), dim3(SC_T), 0, stream, ld, n, bsum);
+    hipLaunchKernelGGL(scan_top_kernel, dim3(1), dim3(1024), 0, stream, bsum, nb, total);
+    hipLaunchKernelGGL((scan_down_kernel<L, S>), dim3((unsigned)nb), dim3(SC_T), 0, stream, ld, st, n,
+                       bsum);
+    return hipGetLastError();
+}
+
+// Box cell of compact cell c in box b = {x0, W-1, y0, H-1} (y fastest), or -1.
+__device__ __forceinline__ int box_bin(int c, int4 b, int nx, int ny, const FastDiv &dny)
+{
+    const int cx = (int)dny.div31((uint32_t)c), cy = c - cx * ny;
+    int rx = cx - b.x, ry = cy - b.z;
+    if (rx < 0) rx += nx;
+    if (ry < 0) ry += ny;
+    if (rx > b.y || ry > b.w) return -1;
+    return rx * (b.w + 1) + ry;
+}
+
+// One push tile per workgroup: the count of every box cell (an LDS
+// histogram), its sort key and (tile, cell) as the tile's records at toff[t].
+__global__ __launch_bounds__(256) void tile_hist_kernel(const int32_t *cell, int64_t n, int nx,
+                                                        int ny, CellKey key, const int32_t *tbox,
+                                                        const uint32_t *toff, uint32_t *thist,
+                                                        uint32_t *rkey, uint32_t *rinfo, int *ctl)
+{
+    extern __shared__ uint32_t h[];
+    const int64_t t = blockIdx.x;
+    const int4 b = *reinterpret_cast<const int4 *>(tbox + 4 * t);
+    const int H = b.w + 1, A = (b.y + 1) * H;
+    for (int k = threadIdx.x; k < A; k += 256) h[k] = 0;
+    __syncthreads();
+    for (int q = threadIdx.x; q < PUSH_TILE; q += 256) {
+        const int64_t i = t * PUSH_TILE + q;
+        if (i >= n) break;
+        const int c = cell[i];
+        const int bin = (unsigned)c < (unsigned)(nx * ny) ? box_bin(c, b, nx, ny, key.dny) : -1;
+        if (bin < 0) { atomicOr(ctl, BXF_OUTSIDE); continue; }
+        atomicAdd(&h[bin], 1u);
+    }
+    __syncthreads();
+    const uint32_t o = toff[t];
+    for (int k = threadIdx.x; k < A; k += 256) {
+        int gx = b.x + k / H, gy = b.z + k % H;
+        if (gx >= nx) gx -= nx;
+        if (gy >= ny) gy -= ny;
+        thist[o + k] = h[k];
+        rkey[o + k] = key((uint32_t)(gx * ny + gy));
+        rinfo[o + k] = (uint32_t)(t << 12) | (uint32_t)k;       // tile, box cell (< 2^12)
+    }
+}
+
+// Digit histograms of all passes over the record keys.
+__global__ __launch_bounds__(256) void rec_digit_hist_kernel(const uint32_t *k, int64_t n,
+                                                             int npass, uint32_t *dhist)
+{
+    __shared__ uint32_t h[4][256];
+    for (int q = threadIdx.x; q < 4 * 256; q += 256) h[q >> 8][q & 255] = 0;
+    __syncthreads();
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * 256) {
+        const uint32_t v = k[i];
+        for (int p = 0; p < npass; p++) atomicAdd(&h[p][(v >> (8 * p)) & 255u], 1u);
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < npass * 256; q += 256)
+        if (h[q >> 8][q & 255]) atomicAdd(dhist + q, h[q >> 8][q & 255]);
+}
+
+// One push tile per workgroup: the tile's particles in order of box cell,
+// ties in slot order (a stable in-tile counting sort).  Each wave counts its
+// quarter of the tile per box cell; the counts become per-wave bases (the
+// cell's offset in the tile, an exclusive scan over the box, plus the earlier
+// waves' counts); each wave then walks its quarter again in slot order,
+// ranks equal box cells within a round by ballots and writes the slot (in the
+// tile) at its position, local[t*PUSH_TILE + pos].  loff receives each box
+// cell's offset in the tile, at the cell's record.
+__global__ __launch_bounds__(256) void tile_order_kernel(const int32_t *cell, int64_t n, int nx,
+                                                         int ny, FastDiv dny, const int32_t *tbox,
+                                                         const uint32_t *toff, uint32_t *loff,
+                                                         uint16_t *local, int *err)
+{
+    extern __shared__ uint32_t cnt[];    // [4][A]
+    __shared__ uint32_t part[256];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int64_t t = blockIdx.x;
+    const int4 b = *reinterpret_cast<const int4 *>(tbox + 4 * t);
+    const int A = (b.y + 1) * (b.w + 1);
+    for (int k = tid; k < 4 * A; k += 256) cnt[k] = 0;
+    __syncthreads();
+    const int q0 = wave * (PUSH_TILE / 4);
+    uint32_t *mine = cnt + wave * A;
+    for (int r = 0; r < PUSH_TILE / 256; r++) {
+        const int64_t i = t * PUSH_TILE + q0 + r * 64 + lane;
+        if (i < n) {
+            const int bin = box_bin(cell[i], b, nx, ny, dny);
+            if (bin >= 0) atomicAdd(&mine[bin], 1u);
+        }
+    }
+    __syncthreads();
+    // in-tile offset of every box cell: each thread scans a contiguous chunk
+    const int per = (A + 255) / 256, k0 = tid * per, k1 = min(A, k0 + per);
+    uint32_t sum = 0;
+    for (int k = k0; k < k1; k++) sum += cnt[k] + cnt[A + k] + cnt[2 * A + k] + cnt[3 * A + k];
+    part[tid] = sum;
+    __syncthreads();
+    if (tid < 64) {                      // exclusive scan of the 256 chunk sums
+        uint32_t v[4], tot = 0;
+        for (int j = 0; j < 4; j++) { v[j] = part[4 * tid + j]; tot += v[j]; }
+        uint32_t x = tot;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t y = __shfl_up(x, o);
+            if (lane >= o) x += y;
+        }
+        uint32_t run = x - tot;
+        for (int j = 0; j < 4; j++) { part[4 * tid + j] = run; run += v[j]; }
+    }
+    __syncthreads();
+    uint32_t run = part[tid];
+    const uint32_t o = toff[t];
+    for (int k = k0; k < k1; k++) {
+        const uint32_t c0 = cnt[k], c1 = cnt[A + k], c2 = cnt[2 * A + k], c3 = cnt[3 * A + k];
+        loff[o + k] = run;
+        cnt[k] = run;
+        cnt[A + k] = run + c0;
+        cnt[2 * A + k] = run + c0 + c1;
+        cnt[3 * A + k] = run + c0 + c1 + c2;
+        run += c0 + c1 + c2 + c3;
+    }
+    __syncthreads();
+    const uint64_t lt = (1ull << lane) - 1;
+    for (int r = 0; r < PUSH_TILE / 256; r++) {
+        const int q = q0 + r * 64 + lane;
+        const int64_t i = t * PUSH_TILE + q;
+        int bin = i < n ? box_bin(cell[i], b, nx, ny, dny) : 0;
+        // every cell lies in its box (tile_hist_kernel checked it)
+        const bool valid = i < n && bin >= 0;
+        if (bin < 0) {
+            atomicOr(err, ERRF_STATE);
+            bin = 0;
+        }
+        uint64_t m = __ballot(valid);
+#pragma unroll
+        for (int j = 0; j < 12; j++) {               // bins < BX_MAX = 2^12
+            const uint64_t bb = __ballot((bin >> j) & 1);
+            m &= ((bin >> j) & 1) ? bb : ~bb;
+        }
+        const uint32_t base = mine[bin];
+        if (valid && (m & lt) == 0) mine[bin] = base + (uint32_t)__popcll(m);
+        __builtin_amdgcn_wave_barrier();
+        const uint32_t pos = base + (uint32_t)__popcll(m & lt);
+        if (valid && pos < PUSH_TILE) local[t * PUSH_TILE + pos] = (uint16_t)q;
+    }
+}
+
+// The records of the tile-box path in key order, as expand_kernel reads
+// them: order[r] = index of the r-th record, sbase[r] = its first rank.
+struct BoxRecs {
+    const uint32_t *order, *sbase, *thist, *rinfo, *loff;
+    const uint16_t *local;
+    int64_t R;
+};
+
+// The permutation in rank order from the records in key order: one wave per
+// 64 consecutive records, whose ranks form one contiguous range.  Lane l
+// holds record r0 + l (first rank B, c particles, tile t, cell offset L in
+// the tile); the wave walks the range 64 ranks at a time, each lane finding
+// the record of its rank d (the last lane with B <= d) by a binary search
+// over the lanes, and writes perm[d] = t*PUSH_TILE + local[t*PUSH_TILE + L +
+// d - B]: coalesced writes, contiguous reads of each tile's local order.
+__global__ __launch_bounds__(256) void expand_kernel(BoxRecs B, int64_t n, uint32_t *perm,
+                                                     int *err)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t r0 = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * 64;
+    if (r0 >= B.R) return;
+    const int64_t r = r0 + lane;
+    uint32_t b0 = 0xFFFFFFFFu, c = 0, lb = 0, tb = 0;
+    if (r < B.R) {
+        const uint32_t idx = B.order[r];
+        b0 = B.sbase[r];
+        c = B.thist[idx];
+        tb = (B.rinfo[idx] >> 12) * (uint32_t)PUSH_TILE;
+        lb = tb + B.loff[idx] - b0;              // local index = lb + rank (mod 2^32)
+    }
+    const int last = B.R - r0 < 64 ? (int)(B.R - r0) - 1 : 63;
+    const int64_t D0 = (uint32_t)__shfl(b0, 0);
+    const int64_t D1 = (int64_t)(uint32_t)__shfl(b0, last) + (uint32_t)__shfl(c, last);
+    if (D1 > n) {
+        if (lane == 0) atomicOr(err, ERRF_STATE);
+        return;
+    }
+    for (int64_t base = D0; base < D1; base += 64) {     // uniform trip count for the shuffles
+        const int64_t d = base + lane;
+        int lo = 0, hi = last;
+#pragma unroll
+        for (int k = 0; k < 6; k++) {
+            const int mid = (lo + hi + 1) >> 1;
+            const uint32_t bm = (uint32_t)__shfl(b0, mid);
+            if (lo < hi) {
+                if ((int64_t)bm <= d) lo = mid; else hi = mid - 1;
+            }
+        }
+        const uint32_t l = (uint32_t)__shfl(lb, lo), t = (uint32_t)__shfl(tb, lo);
+        if (d < D1) perm[d] = t + B.local[(uint32_t)(l + (uint32_t)d)];
+    }
+}
+
+// The tile-box path of sort_species up to the records in key order (*ok), or
+// *ok = false when a tile's box is unknown, too large, or does not hold all
+// its particles (the caller then takes the radix path).
+static hipError_t box_records(SortScratch &s, const SpeciesDev &sp, const CellKey &key,
+                              int64_t ncell, int *err, BoxRecs *out, bool *ok, hipStream_t st)
+{
+    *ok = false;
+    hipError_t e;
+    const int64_t n = sp.n, nt = (n + PUSH_TILE - 1) / PUSH_TILE;
+    const int nx = (int)(ncell / key.ny), ny = (int)key.ny;
+    if ((e = launch_fill_u32((uint32_t *)s.ctl, 8, 0u, st)) != hipSuccess) return e;
+    hipLaunchKernelGGL(box_area_kernel, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, st,
+                       (const int32_t *)sp.tbox, nt, s.tarea, s.ctl);
+    if ((e = exclusive_scan(LdArr{s.tarea}, StArr{s.toff}, nt, s.bsum, s.ctl + 2, st)) != hipSuccess)
+        return e;
+    int ctl[4];
+    if ((e = hipMemcpyAsync(ctl, s.ctl, sizeof ctl, hipMemcpyDeviceToHost, st)) != hipSuccess) return e;
+    if ((e = hipStreamSynchronize(st)) != hipSuccess) return e;
+    const int64_t R = (uint32_t)ctl[2];
+    const int amax = ctl[1];
+    // the record radix reuses the particle look-back status (status_cap tiles)
+    if (ctl[0] || R == 0 || (R + ST_TILE - 1) / ST_TILE > s.status_cap) return hipSuccess;
+    constexpr int NREC = 9;                      // record arrays in rbuf
+    if (R > s.rcap) {
+        if (s.rbuf) (void)hipFree(s.rbuf);
+        s.rbuf = nullptr;
+        s.rcap = 0;
+        const int64_t cap = R + R / 4 + 4096;
+        if ((e = hipMalloc(&s.rbuf, (size_t)cap * NREC * sizeof(uint32_t))) != hipSuccess) {
+            (void)hipGetLastError();
+            return hipSuccess;                   // no room: the radix path
+        }
+        s.rcap = cap;
+    }
+    uint32_t *thist = s.rbuf, *rkey = s.rbuf + s.rcap, *sbase = s.rbuf + 2 * s.rcap;
+    uint32_t *rinfo = s.rbuf + 3 * s.rcap, *loff = s.rbuf + 4 * s.rcap;
+    uint32_t *kb[2] = {s.rbuf + 5 * s.rcap, s.rbuf + 6 * s.rcap};
+    uint32_t *vb[2] = {s.rbuf + 7 * s.rcap, s.rbuf + 8 * s.rcap};
+    hipLaunchKernelGGL(tile_hist_kernel, dim3((unsigned)nt), dim3(256), (size_t)amax * 4, st,
+                       sp.cell, n, nx, ny, key, (const int32_t *)sp.tbox, s.toff, thist, rkey, rinfo,
+                       s.ctl);
+    if ((e = hipMemcpyAsync(ctl, s.ctl, sizeof(int), hipMemcpyDeviceToHost, st)) != hipSuccess) return e;
+    if ((e = hipStreamSynchronize(st)) != hipSuccess) return e;
+    if (ctl[0]) return hipSuccess;
+    // stable radix sort of the records by key (record index = tile-major)
+    int bits = 0;
+    while ((1ll << bits) < key.nkeys) bits++;
+    const int npass = bits > 8 ? (bits + 7) / 8 : 1;
+    if ((e = launch_fill_u32(s.dhist, 4 * 256, 0u, st)) != hipSuccess) return e;
+    hipLaunchKernelGGL(rec_digit_hist_kernel, dim3(1024), dim3(256), 0, st, rkey, R, npass, s.dhist);
+    hipLaunchKernelGGL(digit_scan_kernel, dim3(1), dim3(64), 0, st, s.dhist, npass);
+    CellKey ident;
+    ident.init(1, 1, 0);                         // key(c) = c: the records carry their keys
+    const int64_t rtile = (R + ST_TILE - 1) / ST_TILE;
+    const uint32_t *kin = rkey, *vin = nullptr;
+    int pb = 0;
+    for (int p = 0; p < npass; p++) {
+        if ((e = launch_fill_u32(s.status, rtile * 256, 0u, st)) != hipSuccess) return e;
+        if ((e = launch_fill_u32(s.ticket + p, 1, 0u, st)) != hipSuccess) return e;
+        const bool first = p == 0, last = p == npass - 1;
+        const uint32_t *db = s.dhist + p * 256;
+        dim3 g((unsigned)rtile), bl(ST_THREADS);
+        if (first && last)
+            hipLaunchKernelGGL((radix_pass_kernel<true, true>), g, bl, 0, st, kin, vin, kb[pb], vb[pb],
+                               R, 8 * p, db, s.status, s.ticket + p, err, ident);
+        else if (first)
+            hipLaunchKernelGGL((radix_pass_kernel<true, false>), g, bl, 0, st, kin, vin, kb[pb], vb[pb],
+                               R, 8 * p, db, s.status, s.ticket + p, err, ident);
+        else if (last)
+            hipLaunchKernelGGL((radix_pass_kernel<false, true>), g, bl, 0, st, kin, vin, kb[pb], vb[pb],
+                               R, 8 * p, db, s.status, s.ticket + p, err, ident);
+        else
+            hipLaunchKernelGGL((radix_pass_kernel<false, false>), g, bl, 0, st, kin, vin, kb[pb], vb[pb],
+                               R, 8 * p, db, s.status, s.ticket + p, err, ident);
+        kin = kb[pb];
+        vin = vb[pb];
+        pb ^= 1;
+    }
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    // the records' counts scanned in key order: each record's first rank
+    if ((e = exclusive_scan(LdGather{thist, vin}, StArr{sbase}, R, s.bsum, nullptr, st)) !=
+        hipSuccess)
+        return e;
+    // the tiles' particles by box cell (u16 slots, in the first key buffer)
+    uint16_t *local = reinterpret_cast<uint16_t *>(s.keys[0]);
+    FastDiv dny;
+    dny.init((uint32_t)ny);
+    hipLaunchKernelGGL(tile_order_kernel, dim3((unsigned)nt), dim3(256), (size_t)amax * 16, st,
+                       sp.cell, n, nx, ny, dny, (const int32_t *)sp.tbox, s.toff, loff, local, err);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    *out = BoxRecs{vin, sbase, thist, rinfo, loff, local, R};
+    *ok = true;
+    return hipSuccess;
+}
+
 // Adds species sp's particles per cell (compact cell id) to counts, as
 // numdensity (pic2d.f90:1343-1354): through the tile boxes of the last push
 // when they are current, else with run-aggregated atomics.
@@ -418,15 +860,32 @@ hipError_t launch_cell_counts(const SpeciesDev &sp, int nx, int ny, int32_t *cou
 // Sorts species sp into dst (a second buffer set of the same capacity).
 hipError_t sort_species(SortScratch &s, const SpeciesDev &sp, int precision, int push_order,
                         const CellKey &key, int64_t ncell,
-                        SpeciesDev &dst, int *err, int want_counts, void *rec,
-                        bool rec_written, hipStream_t st)
+                        SpeciesDev &dst, int *err, int want_counts, int want_perm, void *rec,
+                        bool rec_written, bool *boxed, hipStream_t st)
 {
-    (void)want_counts;
+    *boxed = false;
     const int64_t n = sp.n;
     hipError_t e;
     dst.n = n;
     const int64_t nkeys = key.nkeys;
     if (n == 0) return launch_fill_u32((uint32_t *)s.counts, nkeys, 0u, st);
+    (void)want_perm;
+    const uint32_t *perm = nullptr;
+    int pb = 0;
+    if (sp.tbox_ok && !want_counts) {
+        BoxRecs br{};
+        bool ok = false;
+        if ((e = box_records(s, sp, key, ncell, err, &br, &ok, st)) != hipSuccess) return e;
+        if (ok) {
+            *boxed = true;
+            // the permutation goes to vals[1]; the slot map (push order) to vals[0]
+            hipLaunchKernelGGL(expand_kernel, dim3((unsigned)((br.R + 255) / 256)), dim3(256), 0, st,
+                               br, n, s.vals[1], err);
+            perm = s.vals[1];
+            pb = 0;
+        }
+    }
+    if (!perm) {
     int bits = 0;
     while ((1ll << bits) < nkeys) bits++;
     int npass = (bits + 7) / 8;
@@ -450,7 +909,6 @@ hipError_t sort_species(SortScratch &s, const SpeciesDev &sp, int precision, int
 
     const uint32_t *kin = (const uint32_t *)sp.cell;
     const uint32_t *vin = nullptr;
-    int pb = 0;
     for (int p = 0; p < npass; p++) {
         if ((e = launch_fill_u32(s.status, ntile * 256, 0u, st)) != hipSuccess) return e;
         if ((e = launch_fill_u32(s.ticket + p, 1, 0u, st)) != hipSuccess) return e;
@@ -475,10 +933,12 @@ hipError_t sort_species(SortScratch &s, const SpeciesDev &sp, int precision, int
         vin = vo;
         pb ^= 1;
     }
-    const uint32_t *perm = vin;
+    perm = vin;
+    }
     s.perm = perm;
     // the push order needs the slot->source map in a buffer of its own: the
-    // value buffer the last pass did not write is free
+    // value buffer the radix's last pass did not write is free (vals[0] after
+    // the tile-box path)
     const int tv = push_order && n >= PUSH_TILE ? push_vec(precision) : 0;
     uint32_t *pout = nullptr;
     if (tv) s.perm = pout = s.vals[pb];

@@ -292,6 +292,14 @@ struct SortScratch {
     int32_t *counts = nullptr;      // ncell
     const uint32_t *perm = nullptr; // permutation of the last sort (in vals)
     int64_t cap = 0, status_cap = 0;
+    // tile-box path (sort_species): per push tile the box area and offset
+    // (tiles + 1 entries each, grown with cap), scan block sums, a small
+    // control block read back by the host, and the record arrays (7 per
+    // record, grown on demand)
+    uint32_t *tarea = nullptr, *toff = nullptr, *bsum = nullptr;
+    int *ctl = nullptr;
+    uint32_t *rbuf = nullptr;
+    int64_t rcap = 0;
 };
 hipError_t launch_push_adj(const AdjParams &p, int precision, hipStream_t st);
 // Tiles of the adjoint push (2048 particles each) for n particles.
@@ -343,8 +351,12 @@ static_assert(sizeof(PartRec) == 24, "PartRec is 24 bytes");
 // rec (split form): the species' record buffer (PartRec per particle), or
 // null for the array-wise gather; rec_written: the push already wrote sp's
 // records there (sp's value arrays are then stale and not read).
+// want_counts: fill s.counts (per key); want_perm: leave the permutation at
+// s.perm.  Sorts right after a push (sp.tbox_ok) take the tile-box path when
+// every tile's box fits; otherwise, or when counts are wanted, the radix path.
 hipError_t sort_species(SortScratch &s, const SpeciesDev &sp, int precision, int push_order,
                         const CellKey &key, int64_t ncell, SpeciesDev &dst, int *err,
-                        int want_counts, void *rec, bool rec_written, hipStream_t st);
+                        int want_counts, int want_perm, void *rec, bool rec_written,
+                        bool *boxed, hipStream_t st);
 
 }  // namespace pbpic

@@ -120,6 +120,51 @@ def test_countingsort_push_order(gpu, precision, n):
     assert np.array_equal(g["ux"], want_ux)
 
 
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("push_order", [False, True])
+@pytest.mark.parametrize("nx,ny,ppc,boxed", [(128, 96, 30, True), (300, 257, 7, False),
+                                             (40, 40, 200, True)])
+def test_countingsort_after_push_bitexact(gpu, precision, push_order, nx, ny, ppc, boxed):
+    """A sort right after a push in push order takes the tile-box path
+    (records per tile box cell, radix-sorted by key, scanned, then one stable
+    scatter).  Its
+    permutation is the stable argsort of the keys (cell ids in reference
+    order, supertile keys in push order, laid out in the push kernel's lane
+    order inside full tiles), bit for bit, and the particles move with it."""
+    _say(f"sort after push precision={precision} push_order={push_order} {nx}x{ny}x{ppc}")
+    case = make_case(nx=nx, ny=ny, ppc=ppc, seed=nx + ppc, vth=(0.1, 0.45), efield=0.2,
+                     bfield=1.0)
+    ctx = gpu_for(case, precision=precision, push_order=push_order)
+    for k in range(2):
+        ctx.countingsort(k)
+    for _ in range(3):
+        ctx.zero_current()
+        ctx.pusher()
+        ctx.fieldsolver()
+    V = 2 if precision == 0 else 4
+    radix0 = ctx.diag()["sorts_radix"]
+    for k in range(2):
+        before = ctx.get_particles(k, native=True)
+        perm = ctx.countingsort(k, want="perm")
+        after = ctx.get_particles(k, native=True)
+        c = before["grid"] - 1
+        key = _supertile_key(c, nx, ny) if push_order else c
+        ref = np.argsort(key, kind="stable")
+        if push_order:
+            ref = _push_order(ref, V)
+        assert np.array_equal(perm, ref)
+        for f in ("grid", "dx", "dy", "ux", "uy", "uz", "pid"):
+            assert np.array_equal(after[f], before[f][perm]), f
+    if push_order and boxed:
+        # both took the tile-box path.  The push leaves a tile's box unknown
+        # when it spans half the domain or more: in reference order a tile is
+        # a strip of whole columns; at 300 x 257 x 7 a push-order tile spans
+        # 9 supertiles and some of them the height's half.  Those sorts take
+        # the radix path.
+        assert ctx.diag()["sorts_radix"] == radix0
+    ctx.close()
+
+
 def test_countingsort_empty(gpu):
     import pbpic_amd
     ctx = pbpic_amd.Context(8, 8, 0.1, 0.1, 0.01, 15.0, 25.0, 0.1)
