@@ -76,8 +76,13 @@ struct pbpic_ctx {
     double *E2[3] = {nullptr, nullptr, nullptr};
     double *B2[3] = {nullptr, nullptr, nullptr};
     void *J[4] = {nullptr, nullptr, nullptr, nullptr};
+    // split form: the push's 64-bit fixed-point current sums (scales js),
+    // converted into J after every deposit
+    void *J64[4] = {nullptr, nullptr, nullptr, nullptr};
+    float js[4] = {1, 1, 1, 1};
     void *NF = nullptr;        // node fields for the push (rebuilt before each push)
     double *acc = nullptr;
+    double *part = nullptr;    // partials of the fixed-order reductions
     int *err = nullptr;
     double *rho_scratch = nullptr, *gauss0 = nullptr;
     int32_t *diag_counts = nullptr;   // spectrum / number-density scratch
@@ -215,9 +220,12 @@ PushParams push_params(pbpic_ctx *c, bool recs = false, bool nodep = false)
     P.dtcva = c->dtcva;
     P.ex = c->E[0]; P.ey = c->E[1]; P.ez = c->E[2];
     P.bx = c->B[0]; P.by = c->B[1]; P.bz = c->B[2];
-    P.jx = c->J[0]; P.jy = c->J[1]; P.jz = c->J[2]; P.rho = c->J[3];
+    void *const *jp = c->rsz == 4 ? c->J64 : c->J;
+    P.jx = jp[0]; P.jy = jp[1]; P.jz = jp[2]; P.rho = jp[3];
+    P.jsx = c->js[0]; P.jsy = c->js[1]; P.jsz = c->js[2]; P.jsr = c->js[3];
     P.nf = c->NF;
     P.acc = c->acc;
+    P.part = c->part;
     P.err = c->err;
     P.nspecies = (int)c->sp.size();
     for (int s = 0; s < P.nspecies; s++) P.sp[s] = species_dev(c, s, c->sp[s].cur);
@@ -246,6 +254,7 @@ FdtdParams fdtd_params(pbpic_ctx *c)
     F.fex = c->filt[0]; F.fey = c->filt[1]; F.fez = c->filt[2];
     F.lmask = c->lmask;
     F.acc = c->acc;
+    F.part = c->part;
     return F;
 }
 
@@ -332,7 +341,17 @@ int kinetic_impl(pbpic_ctx *c)
 {
     HIPCHK(c, launch_acc_op(c->acc, 0, 0.0, c->st));
     for (int s = 0; s < (int)c->sp.size(); s++)
-        HIPCHK(c, launch_kinetic(species_dev(c, s, c->sp[s].cur), c->p.precision, c->acc, c->st));
+        HIPCHK(c, launch_kinetic(species_dev(c, s, c->sp[s].cur), c->p.precision, c->acc, c->part,
+                                 c->st));
+    return PBPIC_OK;
+}
+
+// Split form: the FP32 current planes from the fixed-point sums.
+int finalize_current(pbpic_ctx *c)
+{
+    if (c->rsz != 4) return PBPIC_OK;
+    HIPCHK(c, launch_finalize_current(c->J64, c->J, ncell(c), c->js,
+                                      (c->p.flags & PBPIC_FLAG_DEPOSIT_RHO) ? 1 : 0, c->st));
     return PBPIC_OK;
 }
 
@@ -355,6 +374,10 @@ int push_impl(pbpic_ctx *c, bool recs = false, bool nodep = false)
     HIPCHK(c, prof_mark(c, PBPIC_PROF_FIELD, false));
     HIPCHK(c, prof_mark(c, PBPIC_PROF_PUSH, true));
     HIPCHK(c, launch_push(P, c->p.precision, c->st));
+    if (!nodep) {
+        int rc = finalize_current(c);
+        if (rc) return rc;
+    }
     for (int s = 0; s < (int)c->push_out.size(); s++) {
         // the pushed state lives where the push wrote it
         Species &S = c->sp[s];
@@ -370,6 +393,7 @@ int push_impl(pbpic_ctx *c, bool recs = false, bool nodep = false)
     HIPCHK(c, prof_mark(c, PBPIC_PROF_PUSH, false));
     return PBPIC_OK;
 }
+
 
 int fieldsolve_impl(pbpic_ctx *c)
 {
@@ -411,9 +435,10 @@ int step_impl(pbpic_ctx *c, void *const *jcopy, uint32_t *const *perm_dev, bool 
               void *const *jsrc = nullptr)
 {
     HIPCHK(c, prof_mark(c, PBPIC_PROF_ZERO, true));
+    // the split form's push writes its FP32 planes whole from the sums
+    void *const *jz = c->rsz == 4 ? c->J64 : c->J;
     if (!jsrc)
-        HIPCHK(c, launch_zero_current(c->J[0], c->J[1], c->J[2], c->J[3], ncell(c), c->rsz, c->acc,
-                                      1, c->st));
+        HIPCHK(c, launch_zero_current(jz[0], jz[1], jz[2], jz[3], ncell(c), 8, c->acc, 1, c->st));
     HIPCHK(c, prof_mark(c, PBPIC_PROF_ZERO, false));
     // split form: when this step sorts, the push writes the particles as the
     // sort's records, so the sort skips its packing pass.  Until the sort
@@ -828,6 +853,8 @@ int deposit_staged(pbpic_ctx *c, void *buf, int64_t n, const int32_t *i1, const 
     P.sp[0].qy = c->p.dy / c->p.dt;
     HIPCHK(c, launch_deposit_batch(P, c->p.precision, n, d_i1, d_i2, d_j1, d_j2, d_dx1, d_dx2,
                                    d_dy1, d_dy2, d_q, d_vz, c->st));
+    int rc = finalize_current(c);
+    if (rc) return rc;
     HIPCHK(c, hipStreamSynchronize(c->st));
     return PBPIC_OK;
 }
@@ -895,10 +922,26 @@ int pbpic_ctx_create(const pbpic_params *pp, pbpic_ctx **out)
         chk(dalloc(&c->B2[k], nc * 8), "alloc B");
     }
     for (int k = 0; k < 4; k++) chk(dalloc(&c->J[k], nc * c->rsz), "alloc J");
+    if (c->rsz == 4) {
+        for (int k = 0; k < 4; k++) chk(dalloc(&c->J64[k], nc * 8), "alloc J sums");
+        // one contribution of each plane stays below 2^30 (bounds as in
+        // species_dev, |q| = 1): |jx| <= dx/dt, |jy| <= dy/dt, |jz| <= cva, |rho| <= 1
+        c->js[0] = fx_scale(p.dx / p.dt);
+        c->js[1] = fx_scale(p.dy / p.dt);
+        c->js[2] = fx_scale(p.cva);
+        c->js[3] = fx_scale(1.0);
+    }
     chk(dalloc(&c->NF, nodefield_bytes(c->p.precision, nc)), "alloc node fields");
     if (p.flags & (PBPIC_FLAG_MARDER | PBPIC_FLAG_CHERENKOV))
         for (int k = 0; k < 3; k++) chk(dalloc(&c->filt[k], nc * 8), "alloc filter planes");
     chk(dalloc(&c->acc, ACC_N * sizeof(double)), "alloc acc");
+    {
+        // two partials per workgroup of the widest reduction launch: the
+        // FDTD tiles (32 x 16 cells) or 4096 blocks
+        const int64_t nbf = (int64_t)((p.nx + 31) / 32) * ((p.ny + 15) / 16);
+        chk(dalloc(&c->part, (size_t)std::max<int64_t>(nbf, 4096) * 2 * sizeof(double)),
+            "alloc partials");
+    }
     chk(dalloc(&c->err, sizeof(int) * 4), "alloc err");
     {
         CellKey key;                       // counts are indexed by sort key
@@ -914,6 +957,8 @@ int pbpic_ctx_create(const pbpic_params *pp, pbpic_ctx **out)
             chk(hipMemsetAsync(c->B[k], 0, nc * 8, c->st), "zero B");
         }
         for (int k = 0; k < 4; k++) chk(hipMemsetAsync(c->J[k], 0, nc * c->rsz, c->st), "zero J");
+        for (int k = 0; k < 4 && c->J64[0]; k++)
+            chk(hipMemsetAsync(c->J64[k], 0, nc * 8, c->st), "zero J sums");
         chk(hipMemsetAsync(c->acc, 0, ACC_N * sizeof(double), c->st), "zero acc");
         chk(hipMemsetAsync(c->err, 0, sizeof(int) * 4, c->st), "zero err");
         chk(hipStreamSynchronize(c->st), "sync");
@@ -934,9 +979,9 @@ int pbpic_ctx_destroy(pbpic_ctx *c)
     if (c->st) (void)hipStreamSynchronize(c->st);
     auto f = [](void *p) { if (p) (void)hipFree(p); };
     for (int k = 0; k < 3; k++) { f(c->E[k]); f(c->B[k]); f(c->E2[k]); f(c->B2[k]); }
-    for (int k = 0; k < 4; k++) f(c->J[k]);
+    for (int k = 0; k < 4; k++) { f(c->J[k]); f(c->J64[k]); }
     f(c->NF);
-    f(c->acc); f(c->err); f(c->rho_scratch); f(c->gauss0); f(c->diag_counts);
+    f(c->acc); f(c->part); f(c->err); f(c->rho_scratch); f(c->gauss0); f(c->diag_counts);
     f(c->filt[0]); f(c->filt[1]); f(c->filt[2]); f(c->big); f(c->lmask);
     for (auto &S : c->sp)
         for (int b = 0; b < 2; b++) {
@@ -1192,6 +1237,11 @@ int pbpic_set_field(pbpic_ctx *c, int32_t which, const double *plane)
         } else {
             vf.assign(vd.begin(), vd.end());
             HIPCHK(c, hipMemcpy(dst, vf.data(), nc * 4, hipMemcpyHostToDevice));
+            // the fixed-point sums a following push adds to
+            const float sc = c->js[which - PBPIC_JX];
+            std::vector<long long> vq(nc);
+            for (size_t k = 0; k < nc; k++) vq[k] = llrint((double)vf[k] * (double)sc);
+            HIPCHK(c, hipMemcpy(c->J64[which - PBPIC_JX], vq.data(), nc * 8, hipMemcpyHostToDevice));
         }
     } else {
         return fail(c, PBPIC_ERR_ARG, "field %d cannot be set", which);
@@ -1247,6 +1297,9 @@ int pbpic_zero_current(pbpic_ctx *c)
 {
     if (!c) return PBPIC_ERR_ARG;
     HIPCHK(c, launch_zero_current(c->J[0], c->J[1], c->J[2], c->J[3], ncell(c), c->rsz, c->acc, 0, c->st));
+    if (c->J64[0])
+        HIPCHK(c, launch_zero_current(c->J64[0], c->J64[1], c->J64[2], c->J64[3], ncell(c), 8, c->acc,
+                                      0, c->st));
     return PBPIC_OK;
 }
 
@@ -1361,7 +1414,17 @@ int pbpic_diag(pbpic_ctx *c, double out[PBPIC_NDIAG])
     HIPCHK(c, launch_energy(fdtd_params(c), c->st));
     if ((rc = kinetic_impl(c))) return rc;
     const double cva2 = c->p.cva * c->p.cva;
-    HIPCHK(c, launch_gauss(push_params(c), c->p.precision, c->rho_scratch, c->gauss0,
+    // split form: the charge in fixed point, exact in any order; a particle
+    // adds at most 1 to a node, so ntot * scale stays below 2^62
+    double rscale = 0;
+    if (c->rsz == 4) {
+        int64_t ntot = 1;
+        for (auto &S : c->sp) ntot += S.n;
+        int e = 61;
+        while (e > 0 && (double)ntot > std::ldexp(1.0, 62 - e)) e--;
+        rscale = std::ldexp(1.0, std::min(e, 52));
+    }
+    HIPCHK(c, launch_gauss(push_params(c), c->p.precision, rscale, c->rho_scratch, c->gauss0,
                            c->gauss_first ? 1 : 0, c->p.rho_background, cva2 * c->p.in0, c->st));
     double a[ACC_N];
     HIPCHK(c, hipMemcpyAsync(a, c->acc, sizeof a, hipMemcpyDeviceToHost, c->st));

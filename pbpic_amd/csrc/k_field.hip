@@ -180,9 +180,9 @@ __global__ __launch_bounds__(FT_THREADS) void fdtd_kernel(FdtdParams P)
     if ((tid & 63) == 0) { jred[0][tid >> 6] = je; jred[1][tid >> 6] = jem; }
     __syncthreads();
     if (tid == 0) {
-        unsafeAtomicAdd(P.acc + ACC_JE_STEP, (jred[0][0] + jred[0][1]) + (jred[0][2] + jred[0][3]));
-        if (P.lmask)
-            unsafeAtomicAdd(P.acc + ACC_JEM_STEP, (jred[1][0] + jred[1][1]) + (jred[1][2] + jred[1][3]));
+        // per-workgroup partials, summed in a fixed order after the launch
+        P.part[blockIdx.x] = (jred[0][0] + jred[0][1]) + (jred[0][2] + jred[0][3]);
+        P.part[gridDim.x + blockIdx.x] = (jred[1][0] + jred[1][1]) + (jred[1][2] + jred[1][3]);
     }
 }
 
@@ -193,6 +193,31 @@ hipError_t launch_fdtd(const FdtdParams &p, int precision, hipStream_t st)
         hipLaunchKernelGGL((fdtd_kernel<double, false>), dim3(nb), dim3(FT_THREADS), 0, st, p);
     else
         hipLaunchKernelGGL((fdtd_kernel<float, true>), dim3(nb), dim3(FT_THREADS), 0, st, p);
+    PartSlots ps{p.lmask ? 2 : 1, {ACC_JE_STEP, ACC_JEM_STEP, 0, 0}, {1.0, 1.0, 0, 0}};
+    return launch_reduce_parts(p.part, nb, p.acc, ps, st);
+}
+
+// One workgroup per quantity: thread t sums part[q*nb + t + 256 k] for
+// k = 0, 1, ... in order, then a fixed tree over the threads.
+__global__ __launch_bounds__(256) void reduce_parts_kernel(const double *part, int nb, double *acc,
+                                                           PartSlots ps)
+{
+    __shared__ double w[4];
+    const int q = blockIdx.x;
+    const double *p = part + (size_t)q * nb;
+    double s = 0;
+    for (int b = threadIdx.x; b < nb; b += 256) s += p[b];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    if ((threadIdx.x & 63) == 0) w[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) acc[ps.slot[q]] += ps.scale[q] * ((w[0] + w[1]) + (w[2] + w[3]));
+}
+
+hipError_t launch_reduce_parts(const double *part, int nb, double *acc, const PartSlots &ps,
+                               hipStream_t st)
+{
+    hipLaunchKernelGGL(reduce_parts_kernel, dim3(ps.nq), dim3(256), 0, st, part, nb, acc, ps);
     return hipGetLastError();
 }
 
@@ -234,8 +259,8 @@ __global__ __launch_bounds__(256) void energy_kernel(FdtdParams P)
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        unsafeAtomicAdd(P.acc + ACC_WE, (red[0][0] + red[0][1]) + (red[0][2] + red[0][3]));
-        unsafeAtomicAdd(P.acc + ACC_WB, (red[1][0] + red[1][1]) + (red[1][2] + red[1][3]));
+        P.part[blockIdx.x] = (red[0][0] + red[0][1]) + (red[0][2] + red[0][3]);
+        P.part[gridDim.x + blockIdx.x] = (red[1][0] + red[1][1]) + (red[1][2] + red[1][3]);
     }
 }
 
@@ -245,7 +270,8 @@ hipError_t launch_energy(const FdtdParams &p, hipStream_t st)
     int nb = (int)((n + 255) / 256);
     if (nb > 4096) nb = 4096;
     hipLaunchKernelGGL(energy_kernel, dim3(nb), dim3(256), 0, st, p);
-    return hipGetLastError();
+    PartSlots ps{2, {ACC_WE, ACC_WB, 0, 0}, {1.0, 1.0, 0, 0}};
+    return launch_reduce_parts(p.part, nb, p.acc, ps, st);
 }
 
 // Gauss residual: rho by CIC with the gather's weights (:948-951) from the
@@ -257,10 +283,30 @@ hipError_t launch_energy(const FdtdParams &p, hipStream_t st)
 // global atomic per node; particles outside the box, or all of a tile whose
 // box exceeds the histogram, add to rho directly.
 constexpr int RB_NODES = 2048;
+// FX: the charge as 64-bit fixed point, each contribution rounded to
+// rscale (sums exact in any order), else FP64 atomics.
+__device__ __forceinline__ void rho_add(double *p, double v, double rscale, bool fx)
+{
+    if (fx)
+        atomicAdd(reinterpret_cast<unsigned long long *>(p),
+                  (unsigned long long)(long long)rint(v * rscale));
+    else
+        unsafeAtomicAdd(p, v);
+}
+__device__ __forceinline__ void rho_add_lds(double *p, double v, double rscale, bool fx)
+{
+    if (fx)
+        __hip_atomic_fetch_add(reinterpret_cast<unsigned long long *>(p),
+                               (unsigned long long)(long long)rint(v * rscale), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_WORKGROUP);
+    else
+        atomicAdd(p, v);
+}
 template <typename R>
 __global__ __launch_bounds__(256) void rho_cic_tiles_kernel(PushParams P, SpeciesDev S,
-                                                            double *rho)
+                                                            double *rho, double rscale)
 {
+    const bool fx = rscale > 0;
     __shared__ double h[RB_NODES];
     __shared__ int bb[4];
     const int nx = P.nx, ny = P.ny;
@@ -319,35 +365,40 @@ __global__ __launch_bounds__(256) void rho_cic_tiles_kernel(PushParams P, Specie
             if (ry < 0) ry += ny;
             if (rx <= b.y && ry <= b.w) {
                 double *n0 = h + ry * W + rx;
-                atomicAdd(n0, w00);
-                atomicAdd(n0 + 1, w10);
-                atomicAdd(n0 + W, w01);
-                atomicAdd(n0 + W + 1, w11);
+                rho_add_lds(n0, w00, rscale, fx);
+                rho_add_lds(n0 + 1, w10, rscale, fx);
+                rho_add_lds(n0 + W, w01, rscale, fx);
+                rho_add_lds(n0 + W + 1, w11, rscale, fx);
                 continue;
             }
         }
         const int cx1 = cx + 1 == nx ? 0 : cx + 1, cy1 = cy + 1 == ny ? 0 : cy + 1;
-        unsafeAtomicAdd(rho + (size_t)cy * nx + cx, w00);
-        unsafeAtomicAdd(rho + (size_t)cy * nx + cx1, w10);
-        unsafeAtomicAdd(rho + (size_t)cy1 * nx + cx, w01);
-        unsafeAtomicAdd(rho + (size_t)cy1 * nx + cx1, w11);
+        rho_add(rho + (size_t)cy * nx + cx, w00, rscale, fx);
+        rho_add(rho + (size_t)cy * nx + cx1, w10, rscale, fx);
+        rho_add(rho + (size_t)cy1 * nx + cx, w01, rscale, fx);
+        rho_add(rho + (size_t)cy1 * nx + cx1, w11, rscale, fx);
     }
     __syncthreads();
     if (boxed)
         for (int k = threadIdx.x; k < W * H; k += 256) {
             const double v = h[k];
-            if (v == 0.0) continue;
+            if (v == 0.0) continue;                  // (zero bits: zero in both forms)
             const int ky = k / W, kx = k - ky * W;
             int gx = b.x + kx, gy = b.z + ky;
             while (gx >= nx) gx -= nx;
             while (gy >= ny) gy -= ny;
-            unsafeAtomicAdd(rho + (size_t)gy * nx + gx, v);
+            double *g = rho + (size_t)gy * nx + gx;
+            if (fx)
+                atomicAdd(reinterpret_cast<unsigned long long *>(g),
+                          (unsigned long long)__double_as_longlong(v));
+            else
+                unsafeAtomicAdd(g, v);
         }
 }
 
 __global__ __launch_bounds__(256) void gauss_kernel(PushParams P, const double *rho,
                                                     double *g0, int first, double rho_bg,
-                                                    double cva2in0)
+                                                    double cva2in0, double rscale)
 {
     __shared__ double red[2][4];
     __shared__ unsigned long long mx[2][4];
@@ -362,7 +413,10 @@ __global__ __launch_bounds__(256) void gauss_kernel(PushParams P, const double *
         const int gxm = gx == 0 ? nx - 1 : gx - 1, gym = gy == 0 ? ny - 1 : gy - 1;
         const double div = (P.ex[k] - P.ex[(size_t)gy * nx + gxm]) * P.idx +
                            (P.ey[k] - P.ey[(size_t)gym * nx + gx]) * P.idy;
-        const double res = div - cva2in0 * (rho[k] + rho_bg);
+        const double rk = rscale > 0
+                              ? (double)__double_as_longlong(rho[k]) / rscale
+                              : rho[k];
+        const double res = div - cva2in0 * (rk + rho_bg);
         if (first) g0[k] = res;
         const double d = res - g0[k];
         l2 += res * res;
@@ -385,8 +439,8 @@ __global__ __launch_bounds__(256) void gauss_kernel(PushParams P, const double *
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        unsafeAtomicAdd(P.acc + ACC_G_L2, (red[0][0] + red[0][1]) + (red[0][2] + red[0][3]));
-        unsafeAtomicAdd(P.acc + ACC_G_DL2, (red[1][0] + red[1][1]) + (red[1][2] + red[1][3]));
+        P.part[blockIdx.x] = (red[0][0] + red[0][1]) + (red[0][2] + red[0][3]);
+        P.part[gridDim.x + blockIdx.x] = (red[1][0] + red[1][1]) + (red[1][2] + red[1][3]);
         unsigned long long a = mx[0][0], b = mx[1][0];
         for (int i = 1; i < 4; i++) { a = a > mx[0][i] ? a : mx[0][i]; b = b > mx[1][i] ? b : mx[1][i]; }
         atomicMax((unsigned long long *)(P.acc + ACC_G_LINF), a);
@@ -401,9 +455,9 @@ __global__ void zero_f64_kernel(double *p, int64_t n)
         p[k] = 0.0;
 }
 
-hipError_t launch_gauss(const PushParams &p, int precision, double *rho_scratch,
-                        double *gauss0, int first, double rho_bg, double cva2in0,
-                        hipStream_t st)
+hipError_t launch_gauss(const PushParams &p, int precision, double rho_scale,
+                        double *rho_scratch, double *gauss0, int first, double rho_bg,
+                        double cva2in0, hipStream_t st)
 {
     const int64_t n = (int64_t)p.nx * p.ny;
     hipLaunchKernelGGL(zero_f64_kernel, dim3(1024), dim3(256), 0, st, rho_scratch, n);
@@ -412,16 +466,17 @@ hipError_t launch_gauss(const PushParams &p, int precision, double *rho_scratch,
         const int nt = (int)((p.sp[s].n + PUSH_TILE - 1) / PUSH_TILE);
         if (precision == PBPIC_PREC_FP64)
             hipLaunchKernelGGL(rho_cic_tiles_kernel<double>, dim3(nt), dim3(256), 0, st, p,
-                               p.sp[s], rho_scratch);
+                               p.sp[s], rho_scratch, rho_scale);
         else
             hipLaunchKernelGGL(rho_cic_tiles_kernel<float>, dim3(nt), dim3(256), 0, st, p,
-                               p.sp[s], rho_scratch);
+                               p.sp[s], rho_scratch, rho_scale);
     }
     int nb = (int)((n + 255) / 256);
     if (nb > 4096) nb = 4096;
     hipLaunchKernelGGL(gauss_kernel, dim3(nb), dim3(256), 0, st, p, rho_scratch, gauss0,
-                       first, rho_bg, cva2in0);
-    return hipGetLastError();
+                       first, rho_bg, cva2in0, rho_scale);
+    PartSlots ps{2, {ACC_G_L2, ACC_G_DL2, 0, 0}, {1.0, 1.0, 0, 0}};
+    return launch_reduce_parts(p.part, nb, p.acc, ps, st);
 }
 
 // energyspectra (pic2d.f90:1957-1990): counts of particles by kinetic
@@ -606,6 +661,38 @@ hipError_t launch_fill_u32(uint32_t *p, int64_t n, uint32_t v, hipStream_t st)
     int64_t nb = (n + 255) / 256;
     if (nb > 8192) nb = 8192;
     hipLaunchKernelGGL(fill_u32_kernel, dim3((unsigned)nb), dim3(256), 0, st, p, n, v);
+    return hipGetLastError();
+}
+
+// Split form: the FP32 current planes from the fixed-point sums the push
+// accumulated (one rounding of an exact sum, the same for any order of the
+// contributions).
+__global__ void finalize_current_kernel(const long long *a, const long long *b,
+                                        const long long *c, const long long *d, float *ja,
+                                        float *jb, float *jc, float *jd, int64_t n, float4 inv)
+{
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
+         k += (int64_t)gridDim.x * blockDim.x) {
+        ja[k] = (float)((double)a[k] * (double)inv.x);
+        jb[k] = (float)((double)b[k] * (double)inv.y);
+        jc[k] = (float)((double)c[k] * (double)inv.z);
+        if (d) jd[k] = (float)((double)d[k] * (double)inv.w);
+    }
+}
+
+hipError_t launch_finalize_current(void *const *j64, void *const *j, int64_t ncell,
+                                   const float *scale, int with_rho, hipStream_t st)
+{
+    int64_t nb = (ncell + 255) / 256;
+    if (nb > 8192) nb = 8192;
+    if (nb < 1) nb = 1;
+    // 1/scale is a power of two: exact in float
+    const float4 inv = make_float4(1.0f / scale[0], 1.0f / scale[1], 1.0f / scale[2],
+                                   1.0f / scale[3]);
+    hipLaunchKernelGGL(finalize_current_kernel, dim3((unsigned)nb), dim3(256), 0, st,
+                       (const long long *)j64[0], (const long long *)j64[1],
+                       (const long long *)j64[2], with_rho ? (const long long *)j64[3] : nullptr,
+                       (float *)j[0], (float *)j[1], (float *)j[2], (float *)j[3], ncell, inv);
     return hipGetLastError();
 }
 

@@ -40,7 +40,19 @@ __device__ __forceinline__ void lds_add(R *p, R v)
     __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 __device__ __forceinline__ void gadd(double *p, double v) { unsafeAtomicAdd(p, v); }
-__device__ __forceinline__ void gadd(float *p, float v) { unsafeAtomicAdd(p, v); }
+// Split form, global planes: the integer sum itself, or one contribution
+// already multiplied by its plane's scale, rounded as the LDS window rounds
+// it (floor(v + 1/2), v_cvt_rpi_i32_f32).
+__device__ __forceinline__ void gfix(void *p, long long v)
+{
+    atomicAdd(reinterpret_cast<unsigned long long *>(p), (unsigned long long)v);
+}
+__device__ __forceinline__ void gfixf(void *p, float v)
+{
+    int qi;
+    asm("v_cvt_rpi_i32_f32 %0, %1" : "=v"(qi) : "v"(v));
+    gfix(p, (long long)qi);
+}
 
 // Node-averaged field at compact node (gx, gy): the emfgrid formulas of
 // pic2d.f90:837-845, evaluated in FP64 in the reference's order.
@@ -808,10 +820,11 @@ __device__ __forceinline__ void push_tile(const PushParams &P, int t)
       *uz_q = (R *)S.uz_o;
     R *gjx = (R *)P.jx, *gjy = (R *)P.jy, *gjz = (R *)P.jz, *grho = (R *)P.rho;
     constexpr bool want_rho = RHO;
-    // split form with an LDS window: deposit contributions come out of
-    // deposit_terms already multiplied by their plane's fixed-point scale
-    // (q*dx/dt, q*dy/dt and q pre-scaled by powers of two: exact)
-    const bool fxw = SPLIT && inlds;
+    // split form: deposit contributions come out of deposit_terms already
+    // multiplied by their plane's fixed-point scale (q*dx/dt, q*dy/dt and q
+    // pre-scaled by powers of two: exact), for the LDS window and for the
+    // global planes alike
+    const bool fxw = SPLIT;
     const R dqx = fxw ? qx * (R)S.jsx : qx, dqy = fxw ? qy * (R)S.jsy : qy,
             dqz = fxw ? qr * (R)S.jsz : qr, rsc = fxw ? (R)S.jsr / (R)S.jsz : (R)1;
     // window coordinates of the cells this push reads (o*) and writes (n*):
@@ -886,25 +899,29 @@ __device__ __forceinline__ void push_tile(const PushParams &P, int t)
                     int gy = ddy == 2 ? (yp + 1 == ny ? 0 : yp + 1) : ys[ddy + 1];
                     return (size_t)gy * nx + gx;
                 };
-                gadd(gjx + at(0, 0), d.jxa0);
-                gadd(gjx + at(0, 1), d.jxa1);
-                gadd(gjy + at(0, 0), d.jya0);
-                gadd(gjy + at(1, 0), d.jya1);
+                auto put = [&](R *plane, size_t g, R v) {
+                    if constexpr (SPLIT) gfixf(reinterpret_cast<long long *>(plane) + g, v);
+                    else gadd(plane + g, v);
+                };
+                put(gjx, at(0, 0), d.jxa0);
+                put(gjx, at(0, 1), d.jxa1);
+                put(gjy, at(0, 0), d.jya0);
+                put(gjy, at(1, 0), d.jya1);
                 if (d.a != 0 || d.b != 0) {
-                    gadd(gjx + at(d.a, d.b), d.jxb0);
-                    gadd(gjx + at(d.a, d.b + 1), d.jxb1);
-                    gadd(gjy + at(d.a, d.b), d.jyb0);
-                    gadd(gjy + at(d.a + 1, d.b), d.jyb1);
+                    put(gjx, at(d.a, d.b), d.jxb0);
+                    put(gjx, at(d.a, d.b + 1), d.jxb1);
+                    put(gjy, at(d.a, d.b), d.jyb0);
+                    put(gjy, at(d.a + 1, d.b), d.jyb1);
                 }
-                gadd(gjz + at(0, 0), d.jz0);
-                gadd(gjz + at(d.di, 0), d.jz1);
-                gadd(gjz + at(0, d.dj), d.jz2);
-                gadd(gjz + at(d.di, d.dj), d.jz3);
+                put(gjz, at(0, 0), d.jz0);
+                put(gjz, at(d.di, 0), d.jz1);
+                put(gjz, at(0, d.dj), d.jz2);
+                put(gjz, at(d.di, d.dj), d.jz3);
                 if (want_rho) {
-                    gadd(grho + at(0, 0), d.r0);
-                    gadd(grho + at(1, 0), d.r1);
-                    gadd(grho + at(0, 1), d.r2);
-                    gadd(grho + at(1, 1), d.r3);
+                    put(grho, at(0, 0), d.r0);
+                    put(grho, at(1, 0), d.r1);
+                    put(grho, at(0, 1), d.r2);
+                    put(grho, at(1, 1), d.r3);
                 }
             }
         }
@@ -1047,18 +1064,19 @@ __device__ __forceinline__ void push_tile(const PushParams &P, int t)
     // ---- D: flush the window, reduce the kinetic energy and error flags
     __syncthreads();
     if (DEP && inlds) {
-        const double isx = 1.0 / S.jsx, isy = 1.0 / S.jsy, isz = 1.0 / S.jsz, isr = 1.0 / S.jsr;
         for (int k = tid; k < WH; k += PUSH_THREADS) {
             const int wj = k / W, wi = k - wj * W;
             const int gx = wrap_index(x0 + wx0 + wi, nx), gy = wrap_index(y0 + wy0 + wj, ny);
             const size_t g = (size_t)gy * nx + gx;
             if constexpr (SPLIT) {
+                // integer sums into the fixed-point planes: exact, any order
                 const long long *jb = reinterpret_cast<const long long *>(jw);
                 long long v;
-                if ((v = jb[k]) != 0) gadd(gjx + g, (R)((double)v * isx));
-                if ((v = jb[WH + k]) != 0) gadd(gjy + g, (R)((double)v * isy));
-                if ((v = jb[2 * WH + k]) != 0) gadd(gjz + g, (R)((double)v * isz));
-                if (want_rho && (v = jb[3 * WH + k]) != 0) gadd(grho + g, (R)((double)v * isr));
+                if ((v = jb[k]) != 0) gfix(reinterpret_cast<long long *>(gjx) + g, v);
+                if ((v = jb[WH + k]) != 0) gfix(reinterpret_cast<long long *>(gjy) + g, v);
+                if ((v = jb[2 * WH + k]) != 0) gfix(reinterpret_cast<long long *>(gjz) + g, v);
+                if (want_rho && (v = jb[3 * WH + k]) != 0)
+                    gfix(reinterpret_cast<long long *>(grho) + g, v);
             } else {
                 double v;
                 if ((v = jw[k]) != 0.0) gadd(gjx + g, (R)v);
@@ -1142,38 +1160,45 @@ __global__ void deposit_batch_kernel(PushParams P, int64_t n, const int32_t *i1,
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     const int nx = P.nx, ny = P.ny;
-    const R qr = (R)q[k];
     // qx = q*dx/dt (:1237): sp[0].qx holds dx/dt; q is +-1 (checked on the
-    // host), so q*(dx/dt) equals (q*dx)/dt exactly.
-    const R qxx = (R)((double)q[k] * P.sp[0].qx), qyy = (R)((double)q[k] * P.sp[0].qy);
+    // host), so q*(dx/dt) equals (q*dx)/dt exactly.  Split form: the
+    // contributions pre-scaled to the fixed-point planes (powers of two).
+    const R sx = SPLIT ? (R)P.jsx : (R)1, sy = SPLIT ? (R)P.jsy : (R)1,
+            sz = SPLIT ? (R)P.jsz : (R)1, sr = SPLIT ? (R)P.jsr / (R)P.jsz : (R)1;
+    const R qr = (R)q[k] * sz;
+    const R qxx = (R)((double)q[k] * P.sp[0].qx) * sx, qyy = (R)((double)q[k] * P.sp[0].qy) * sy;
     Dep<R> d;
     const int a = i2[k] - i1[k], b = j2[k] - j1[k];
     deposit_terms<R, SPLIT>(i1[k], j1[k], a, b, (R)dx1[k], (R)dx2[k], (R)dy1[k],
                             (R)dy2[k], qxx, qyy, qr, (R)vz[k], P.compat,
                             P.deposit_rho != 0, d);
+    if (SPLIT) { d.r0 *= sr; d.r1 *= sr; d.r2 *= sr; d.r3 *= sr; }
     if ((P.compat & PBPIC_COMPAT_JY_DROP) && i2[k] == nx + 1) d.jyb1 = (R)0;
     const int cx = i1[k] - 1, cy = j1[k] - 1;
     auto at = [&](int ddx, int ddy) -> size_t {
         return (size_t)wrap_index(cy + ddy, ny) * nx + wrap_index(cx + ddx, nx);
     };
-    R *gjx = (R *)P.jx, *gjy = (R *)P.jy, *gjz = (R *)P.jz, *grho = (R *)P.rho;
-    gadd(gjx + at(0, 0), (double)d.jxa0);
-    gadd(gjx + at(0, 1), (double)d.jxa1);
-    gadd(gjy + at(0, 0), (double)d.jya0);
-    gadd(gjy + at(1, 0), (double)d.jya1);
-    gadd(gjx + at(a, b), (double)d.jxb0);
-    gadd(gjx + at(a, b + 1), (double)d.jxb1);
-    gadd(gjy + at(a, b), (double)d.jyb0);
-    gadd(gjy + at(a + 1, b), (double)d.jyb1);
-    gadd(gjz + at(0, 0), (double)d.jz0);
-    gadd(gjz + at(d.di, 0), (double)d.jz1);
-    gadd(gjz + at(0, d.dj), (double)d.jz2);
-    gadd(gjz + at(d.di, d.dj), (double)d.jz3);
+    auto put = [&](void *plane, size_t g, R v) {
+        if constexpr (SPLIT) gfixf(reinterpret_cast<long long *>(plane) + g, v);
+        else gadd((double *)plane + g, (double)v);
+    };
+    put(P.jx, at(0, 0), d.jxa0);
+    put(P.jx, at(0, 1), d.jxa1);
+    put(P.jy, at(0, 0), d.jya0);
+    put(P.jy, at(1, 0), d.jya1);
+    put(P.jx, at(a, b), d.jxb0);
+    put(P.jx, at(a, b + 1), d.jxb1);
+    put(P.jy, at(a, b), d.jyb0);
+    put(P.jy, at(a + 1, b), d.jyb1);
+    put(P.jz, at(0, 0), d.jz0);
+    put(P.jz, at(d.di, 0), d.jz1);
+    put(P.jz, at(0, d.dj), d.jz2);
+    put(P.jz, at(d.di, d.dj), d.jz3);
     if (P.deposit_rho) {
-        gadd(grho + at(0, 0), (double)d.r0);
-        gadd(grho + at(1, 0), (double)d.r1);
-        gadd(grho + at(0, 1), (double)d.r2);
-        gadd(grho + at(1, 1), (double)d.r3);
+        put(P.rho, at(0, 0), d.r0);
+        put(P.rho, at(1, 0), d.r1);
+        put(P.rho, at(0, 1), d.r2);
+        put(P.rho, at(1, 1), d.r3);
     }
 }
 
@@ -1192,7 +1217,7 @@ static int push_tiles(const PushParams &p, PushParams &q)
 // Kinetic energy tally (pic2d.f90:1061-1062): sum over a species of
 // (gamma - 1) * ms from the momenta as the last push left them, FP64 sums.
 template <typename R>
-__global__ __launch_bounds__(256) void kinetic_kernel(SpeciesDev S, double *acc)
+__global__ __launch_bounds__(256) void kinetic_kernel(SpeciesDev S, double *part)
 {
     __shared__ double red[4];
     double k = 0;
@@ -1204,21 +1229,21 @@ __global__ __launch_bounds__(256) void kinetic_kernel(SpeciesDev S, double *acc)
     for (int o = 32; o > 0; o >>= 1) k += __shfl_xor(k, o);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = k;
     __syncthreads();
-    if (threadIdx.x == 0)
-        unsafeAtomicAdd(acc + (S.q > 0 ? ACC_KE_I : ACC_KE_E),
-                        ((red[0] + red[1]) + (red[2] + red[3])) * S.ms);
+    if (threadIdx.x == 0) part[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
 }
 
-hipError_t launch_kinetic(const SpeciesDev &S, int precision, double *acc, hipStream_t st)
+hipError_t launch_kinetic(const SpeciesDev &S, int precision, double *acc, double *part,
+                          hipStream_t st)
 {
     if (S.n == 0) return hipSuccess;
     int64_t nb = (S.n + 255) / 256;
     if (nb > 4096) nb = 4096;
     if (precision == PBPIC_PREC_FP64)
-        hipLaunchKernelGGL(kinetic_kernel<double>, dim3((unsigned)nb), dim3(256), 0, st, S, acc);
+        hipLaunchKernelGGL(kinetic_kernel<double>, dim3((unsigned)nb), dim3(256), 0, st, S, part);
     else
-        hipLaunchKernelGGL(kinetic_kernel<float>, dim3((unsigned)nb), dim3(256), 0, st, S, acc);
-    return hipGetLastError();
+        hipLaunchKernelGGL(kinetic_kernel<float>, dim3((unsigned)nb), dim3(256), 0, st, S, part);
+    PartSlots ps{1, {S.q > 0 ? ACC_KE_I : ACC_KE_E, 0, 0, 0}, {S.ms, 0, 0, 0}};
+    return launch_reduce_parts(part, (int)nb, acc, ps, st);
 }
 
 size_t nodefield_bytes(int precision, int64_t ncell)

@@ -151,9 +151,15 @@ struct PushParams {
     int nspecies;
     double dt, cva, idx, idy, dtcva;   // dtcva = dt*cva (:1003)
     const double *ex, *ey, *ez, *bx, *by, *bz;
-    void *jx, *jy, *jz, *rho;  // unnormalized current, double or float
+    // unnormalized current: FP64 form, double planes; split form, 64-bit
+    // fixed-point planes (the sum of every contribution times the plane's
+    // scale jsx..jsr, each rounded to an integer: exact and independent of
+    // the order of the additions), converted by launch_finalize_current
+    void *jx, *jy, *jz, *rho;
+    float jsx, jsy, jsz, jsr;  // split form: fixed-point scales (as SpeciesDev's)
     void *nf;                  // node-field array (nodefield_bytes), push precision
     double *acc;
+    double *part;              // partial sums of the fixed-order reductions
     int *err;
     SpeciesDev sp[MAX_SPECIES];
 };
@@ -172,7 +178,20 @@ struct FdtdParams {
     const double *fex, *fey, *fez;     // filter increments added to E^{n+1} (or null)
     const double *lmask;               // loss-region weights per cell (or null)
     double *acc;
+    double *part;                      // per-workgroup partial sums (launch_reduce_parts)
 };
+
+// Reductions in a fixed order.  A reduction kernel writes one partial per
+// workgroup and quantity, part[q*nb + block]; launch_reduce_parts then adds
+// scale[q] * (the sum of part[q*nb .. q*nb+nb), taken in one fixed order) to
+// acc[slot[q]], so that the same inputs give the same bits every run.
+struct PartSlots {
+    int nq;
+    int slot[4];
+    double scale[4];
+};
+hipError_t launch_reduce_parts(const double *part, int nb, double *acc, const PartSlots &ps,
+                               hipStream_t st);
 
 // Marder / Cherenkov increments of E from E^n and rho^n (k_field.hip).
 struct FilterParams {
@@ -234,7 +253,8 @@ size_t nodefield_bytes(int precision, int64_t ncell);
 // Fills p.nf from the E and B planes; launch_push reads p.nf.
 hipError_t launch_nodefield(const PushParams &p, int precision, hipStream_t st);
 // Adds species S's kinetic energy to acc[ACC_KE_E] or acc[ACC_KE_I].
-hipError_t launch_kinetic(const SpeciesDev &S, int precision, double *acc, hipStream_t st);
+hipError_t launch_kinetic(const SpeciesDev &S, int precision, double *acc, double *part,
+                          hipStream_t st);
 hipError_t launch_deposit_batch(const PushParams &p, int precision, int64_t n,
                                 const int32_t *i1, const int32_t *i2,
                                 const int32_t *j1, const int32_t *j2,
@@ -244,7 +264,9 @@ hipError_t launch_deposit_batch(const PushParams &p, int precision, int64_t n,
                                 hipStream_t st);
 hipError_t launch_fdtd(const FdtdParams &p, int precision, hipStream_t st);
 hipError_t launch_energy(const FdtdParams &p, hipStream_t st);
-hipError_t launch_gauss(const PushParams &p, int precision, double *rho_scratch,
+// rho_scale > 0: the charge deposit as 64-bit fixed point (sums exact in any
+// order) with that scale; 0: FP64 atomics.
+hipError_t launch_gauss(const PushParams &p, int precision, double rho_scale, double *rho_scratch,
                         double *gauss0, int first, double rho_bg,
                         double cva2in0, hipStream_t st);
 hipError_t launch_fill_u32(uint32_t *p, int64_t n, uint32_t v, hipStream_t st);
@@ -263,6 +285,10 @@ struct HarrisDev {
 };
 hipError_t launch_init_harris(const SpeciesDev &ion, const SpeciesDev &ele, int precision,
                               int ny, const HarrisDev &h, hipStream_t st);
+// Split form: J (float planes) = fixed-point sums / scale, for the 3 current
+// planes and the charge plane too when with_rho.
+hipError_t launch_finalize_current(void *const *j64, void *const *j, int64_t ncell,
+                                   const float *scale, int with_rho, hipStream_t st);
 hipError_t launch_zero_current(void *jx, void *jy, void *jz, void *rho,
                                int64_t ncell, int elem_bytes, double *acc,
                                int reset_ke, hipStream_t st);
