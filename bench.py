@@ -38,11 +38,14 @@ def run_form(precision, cfg, steps, warmup, sort_every, seed, push_order=True, c
     """Times `steps` steps of one form.  The step counter is set (as a restart
     from a checkpoint at that step would set it, pbpic_set_counters) so that
     the timed window holds the sort of the reference cadence: its first sort
-    falls min(steps, sort_every) // 2 steps into the window.  A window of a
-    whole number of sort cycles holds exactly steps / sort_every sorts; any
-    other holds one more or one fewer, and ms_per_step then charges the
-    sorts at the cadence: the measured mean sort time x steps / sort_every in
-    place of the sorts that ran (ms_per_step_wall is the raw wall clock)."""
+    falls min(steps, sort_every) // 2 steps into the window and sort_every
+    steps after the previous sort (the untimed steps before the window are
+    the warm-up plus what that takes), so it sorts a full cycle of drift, as
+    in a long run.  A window of a whole number of sort cycles holds exactly
+    steps / sort_every sorts; any other holds one more or one fewer, and
+    ms_per_step then charges the sorts at the cadence: the measured mean sort
+    time x steps / sort_every in place of the sorts that ran
+    (ms_per_step_wall is the raw wall clock)."""
     import torch
     import pbpic_amd
 
@@ -56,10 +59,17 @@ def run_form(precision, cfg, steps, warmup, sort_every, seed, push_order=True, c
     ctx.init_maxwellian(k, sp["ppc"], sp["vth"], sp["vdrift"], counter_stream=sp["counter"],
                         seed=seed, stream=0)
     ctx.countingsort()                                  # pic2d.f90:161
-    ctx.step(warmup)
+    lead = min(steps, sort_every) // 2 + 1 if sort_every > 0 else 0
+    if sort_every > 0:
+        # untimed steps: at least `warmup`, and as many more as put the
+        # in-window sort sort_every steps after the last one (the initial
+        # sort or one in the warm-up), so that it sorts a full cycle's drift
+        pre = warmup + (sort_every - lead - warmup) % sort_every
+    else:
+        pre = warmup
+    ctx.step(pre)
     if sort_every > 0:
         nstep, je = ctx.counters()
-        lead = min(steps, sort_every) // 2 + 1          # the first in-window sort ends step `lead`
         base = nstep - nstep % sort_every + sort_every - lead
         ctx.set_counters(base, je)
     d0 = ctx.diag()
@@ -77,7 +87,7 @@ def run_form(precision, cfg, steps, warmup, sort_every, seed, push_order=True, c
     wall = t1 - t0
     sort_ms, nsorts = prof["sort"]
     out = dict(npart=npart, wall=wall, prof=prof, diag0=d0, diag1=d1, sorts_in_window=nsorts,
-               ms_per_step_wall=1e3 * wall / steps)
+               ms_per_step_wall=1e3 * wall / steps, untimed_steps=pre)
     if sort_every > 0:
         if nsorts == 0:
             raise SystemExit(f"timed window of {steps} steps holds no sort (sort_every="
@@ -94,16 +104,17 @@ def run_form(precision, cfg, steps, warmup, sort_every, seed, push_order=True, c
 # HBM bytes per push launch at 2048x2048 x 100 ppc, steady state: rocprofv3
 # FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (factors from a 4 GiB copy), from the
 # commands in profiles/push_kernel_traffic_two_stream_2048x2048_ppc100.md.
-TRAFFIC_2048_PPC100 = {1: 10.248e9 + 10.336e9, 0: 18.713e9 + 18.807e9}
+TRAFFIC_2048_PPC100 = {1: 10.263e9 + 10.412e9, 0: 18.714e9 + 18.807e9}
 TRAFFIC_SOURCE = "profiles/push_kernel_traffic_two_stream_2048x2048_ppc100.md"
 
 
 def roofline(precision, res, cfg):
     """Push kernel: algorithmic bytes per launch = particles x (read + write of
     one particle record) + cells x (6 staggered field planes read, FP64, + 3
-    current planes flushed)."""
+    current planes flushed: FP64 sums, or the split form's 64-bit fixed-point
+    sums)."""
     sp = 44 if precision == 0 else 24          # i32 cell + 5 x f64 | 5 x f32
-    wj = 8 if precision == 0 else 4
+    wj = 8
     ncell = cfg["nx"] * cfg["ny"]
     bytes_launch = res["npart"] * 2 * sp + ncell * (6 * 8 + 3 * wj)
     ms, cnt = res["prof"]["push"]

@@ -40,6 +40,12 @@ __host__ __device__ constexpr size_t adj_nf_bytes(size_t r, int maxw)
     return ((size_t)maxw * 6 * r + 7) & ~(size_t)7;
 }
 
+// Nodes of the current-adjoint window: the deposit targets of a cell reach
+// one node below/left and two above/right of the cell, so the window is the
+// node-field window grown by one node on each side; a box of maxw nodes or
+// fewer whose grown window exceeds this is handled as a wide box.
+__host__ __device__ constexpr int adj_jmax(int maxw) { return maxw + maxw / 2; }
+
 template <typename R>
 __device__ __forceinline__ void a_lds_add(R *p, R v)
 {
@@ -117,20 +123,25 @@ __global__ __launch_bounds__(ADJ_THREADS, 2) void push_adj_kernel(AdjParams P)
     while (s + 1 < P.nspecies && t >= P.sp[s + 1].tile0) s++;
     const AdjSpecies S = P.sp[s];
     const int64_t tstart = (int64_t)(t - S.tile0) * ADJ_TILE;
-    // Each thread takes 8 consecutive ranks of the sort order -- mostly one
-    // cell, so its node adjoints add up in registers and reach LDS once per
-    // cell.  In push order (full 4096-particle tiles laid out by tile_rank)
-    // this tile is half k of push tile T; ranks l*64 + 32k + [0, 32) of it,
-    // thread tid taking l = tid/4 and 8 ranks from 32k + 8(tid%4).
+    // Each thread takes 8 consecutive ranks of the sort order -- one cell at
+    // the last sort, so while the particles stay there its node adjoints add
+    // up in registers and reach LDS once per cell.  The 64 lanes of a wave
+    // start 32 or 64 ranks apart, in different cells: their flushes hit
+    // different LDS addresses, and their tape loads walk consecutive 16-byte
+    // groups.  In push order (full 4096-particle tiles laid out by
+    // tile_rank) this tile is half k of push tile T; ranks l*64 + 32k +
+    // [0, 32) of it, lane l of wave w taking the 8 ranks from 32k + 8w.
     const int64_t T0 = (tstart / PUSH_TILE) * PUSH_TILE;
     const bool porder = P.tv > 0 && T0 + PUSH_TILE <= S.n;
     // the push's vector width of this precision (P.tv is it or 0): a
     // constant, so the rank -> slot map compiles to shifts
     constexpr int TV = sizeof(R) == 8 ? push_vec(PBPIC_PREC_FP64) : push_vec(PBPIC_PREC_SPLIT32);
+    static_assert(ADJ_TILE == 64 * 32 && ADJ_THREADS == 64 * 4 && ADJ_ROUNDS == 8,
+                  "lane/wave rank map assumes 4 waves x 8 ranks per 32-rank row");
     auto slot_of = [&](int r) -> int64_t {
-        if (!porder) return tstart + (int64_t)tid * ADJ_ROUNDS + r;
+        if (!porder) return tstart + (int64_t)lane * 32 + wave * ADJ_ROUNDS + r;
         const int k = (int)((tstart - T0) / ADJ_TILE);
-        const int rank = (tid >> 2) * 64 + 32 * k + (tid & 3) * ADJ_ROUNDS + r;
+        const int rank = lane * 64 + 32 * k + wave * ADJ_ROUNDS + r;
         const int l = rank >> 6, c = (rank & 63) / TV, j = rank % TV;
         return T0 + (int64_t)c * 64 * TV + l * TV + j;
     };
@@ -174,11 +185,14 @@ __global__ __launch_bounds__(ADJ_THREADS, 2) void push_adj_kernel(AdjParams P)
     // nodes used: the cell's 4 corners -> window = box + 1 node
     const int wx0 = mnx, wy0 = mny;
     const int W = mxx - mnx + 2, H = mxy - mny + 2;
-    const bool inlds = mnx <= mxx && (int64_t)W * H <= P.maxw;
-    if (P.pass == 1 && !inlds && mnx <= mxx && (int64_t)W * H <= P.maxw_big) {
+    const bool inlds = mnx <= mxx && (int64_t)W * H <= P.maxw &&
+                       (int64_t)(W + 2) * (H + 2) <= adj_jmax(P.maxw);
+    if (P.pass == 1 && !inlds && mnx <= mxx && (int64_t)W * H <= P.maxw_big &&
+        (int64_t)(W + 2) * (H + 2) <= adj_jmax(P.maxw_big)) {
         if (tid == 0) P.big[1 + atomicAdd(P.big, 1)] = t;   // block-uniform exit
         return;
     }
+    const int JW = W + 2, JWH = inlds ? JW * (H + 2) : 0;
     const int WH = inlds ? W * H : 0;
     const bool interior = mnx <= mxx && x0 + mnx >= 1 && x0 + mxx <= nx - 3 && y0 + mny >= 1 &&
                           y0 + mxy <= ny - 3;
@@ -186,6 +200,9 @@ __global__ __launch_bounds__(ADJ_THREADS, 2) void push_adj_kernel(AdjParams P)
     // [6][WH] node adjoint accumulators, FP64 in both forms: LDS FP64 atomic
     // adds run at about ten times the rate of FP32 ones on this hardware
     double *na = (double *)(smem + adj_nf_bytes(sizeof(R), P.maxw));
+    // [3][JWH] current adjoint at the deposit targets, in the push precision
+    R *jw = (R *)(na + (size_t)6 * P.maxw);
+    const int jmax = adj_jmax(P.maxw);
     if (inlds) {
         for (int k = tid; k < WH; k += ADJ_THREADS) {
             const int wj = k / W, wi = k - wj * W;
@@ -195,6 +212,14 @@ __global__ __launch_bounds__(ADJ_THREADS, 2) void push_adj_kernel(AdjParams P)
             for (int q = 0; q < 6; q++) nf[(size_t)k * 6 + q] = (R)f[q];
         }
         for (int k = tid; k < 6 * WH; k += ADJ_THREADS) na[k] = 0.0;
+        for (int k = tid; k < JWH; k += ADJ_THREADS) {
+            const int wj = k / JW, wi = k - wj * JW;
+            const size_t g = (size_t)wrap_index(y0 + wy0 - 1 + wj, ny) * nx +
+                             wrap_index(x0 + wx0 - 1 + wi, nx);
+            jw[k] = (R)P.jadj[0][g];
+            jw[jmax + k] = (R)P.jadj[1][g];
+            jw[2 * jmax + k] = (R)P.jadj[2][g];
+        }
     }
     __syncthreads();
 
@@ -321,19 +346,30 @@ __global__ __launch_bounds__(ADJ_THREADS, 2) void push_adj_kernel(AdjParams P)
         // ---- reverse: deposit = gather of the current adjoint at the targets
         // deposit targets lie within one cell below/left and two above/right
         // of the cell: no periodic wrap for a tile away from every edge
-        const size_t g0 = (size_t)cy * nx + cx;
-        auto at = [&](int ddx, int ddy) -> size_t {
-            if (interior) return g0 + (ptrdiff_t)ddy * nx + ddx;
-            return (size_t)wrap_index(cy + ddy, ny) * nx + wrap_index(cx + ddx, nx);
-        };
-        const R l1 = (R)ja[at(0, 0)], l2 = (R)ja[at(0, 1)];
-        const R l3 = (R)jb[at(0, 0)], l4 = (R)jb[at(1, 0)];
-        const R l5 = (R)ja[at(a, b)], l6 = (R)ja[at(a, b + 1)];
-        const R l7 = (R)jb[at(a, b)];
         const bool drop = (P.compat & PBPIC_COMPAT_JY_DROP) && cx == nx - 1 && a == 1;
-        const R l8 = drop ? (R)0 : (R)jb[at(a + 1, b)];
-        const R z0 = (R)jc[at(0, 0)], z1 = (R)jc[at(di, 0)], z2 = (R)jc[at(0, dj)],
-                z3 = (R)jc[at(di, dj)];
+        R l1, l2, l3, l4, l5, l6, l7, l8, z0, z1, z2, z3;
+        if (inlds) {
+            // window node of target (cx + ddx, cy + ddy): (lx + 1 + ddx, ly + 1 + ddy)
+            const R *wa = jw + (ly + 1) * JW + lx + 1, *wb = wa + jmax, *wc = wb + jmax;
+            const int ab = b * JW + a;
+            l1 = wa[0]; l2 = wa[JW]; l3 = wb[0]; l4 = wb[1];
+            l5 = wa[ab]; l6 = wa[ab + JW]; l7 = wb[ab];
+            l8 = drop ? (R)0 : wb[ab + 1];
+            z0 = wc[0]; z1 = wc[di]; z2 = wc[dj * JW]; z3 = wc[dj * JW + di];
+        } else {
+            const size_t g0 = (size_t)cy * nx + cx;
+            auto at = [&](int ddx, int ddy) -> size_t {
+                if (interior) return g0 + (ptrdiff_t)ddy * nx + ddx;
+                return (size_t)wrap_index(cy + ddy, ny) * nx + wrap_index(cx + ddx, nx);
+            };
+            l1 = (R)ja[at(0, 0)]; l2 = (R)ja[at(0, 1)];
+            l3 = (R)jb[at(0, 0)]; l4 = (R)jb[at(1, 0)];
+            l5 = (R)ja[at(a, b)]; l6 = (R)ja[at(a, b + 1)];
+            l7 = (R)jb[at(a, b)];
+            l8 = drop ? (R)0 : (R)jb[at(a + 1, b)];
+            z0 = (R)jc[at(0, 0)]; z1 = (R)jc[at(di, 0)]; z2 = (R)jc[at(0, dj)];
+            z3 = (R)jc[at(di, dj)];
+        }
         const R bfx1 = l1 * (ONE - wy1) + l2 * wy1, bwy1 = fx1 * (l2 - l1);
         const R bfy1 = l3 * (ONE - wx1) + l4 * wx1, bwx1 = fy1 * (l4 - l3);
         const R bfx2 = l5 * (ONE - wy2) + l6 * wy2, bwy2 = fx2 * (l6 - l5);
@@ -654,7 +690,7 @@ __global__ void unpermute_kernel(const uint32_t *perm, int64_t n, Ptr5 post, Ptr
 size_t adj_lds_bytes(int precision, int maxw)
 {
     const size_t r = precision == PBPIC_PREC_FP64 ? 8 : 4;
-    return adj_nf_bytes(r, maxw) + (size_t)maxw * 6 * 8;
+    return adj_nf_bytes(r, maxw) + (size_t)maxw * 6 * 8 + 3 * r * (size_t)adj_jmax(maxw);
 }
 
 int64_t adj_tiles(int64_t n) { return (n + ADJ_TILE - 1) / ADJ_TILE; }
@@ -683,7 +719,12 @@ hipError_t launch_push_adj(const AdjParams &p0, int precision, hipStream_t st)
     }
     if (t == 0) return hipSuccess;
     // the second pass gets the window of one workgroup per CU (150 KiB)
-    p.maxw_big = (int)((150 * 1024) / adj_lds_bytes(precision, 1));
+    {
+        const size_t cap = 150 * 1024, r = precision == PBPIC_PREC_FP64 ? 8 : 4;
+        int m = (int)(cap / (6 * r + 48 + 3 * r * 3 / 2));
+        while (m > 0 && adj_lds_bytes(precision, m) > cap) m--;
+        p.maxw_big = m;
+    }
     p.pass = p.big && p.maxw_big > p.maxw ? 1 : 0;
     if (p.pass) {
         hipError_t e = launch_fill_u32((uint32_t *)p.big, 1, 0u, st);
