@@ -207,9 +207,11 @@ PushParams push_params(pbpic_ctx *c, bool recs = false, bool nodep = false)
     P.dny.init((uint32_t)c->p.ny);
     P.compat = c->p.compat;
     P.deposit_rho = (c->p.flags & PBPIC_FLAG_DEPOSIT_RHO) ? 1 : 0;
-    P.maxw = c->maxw;
-    P.maxw_mid = push_maxw_mid(c->p.precision, P.deposit_rho);
-    P.maxw_big = push_maxw_big(c->p.precision, P.deposit_rho);
+    // a push without deposit has no current window: its node-field window
+    // takes the LDS of both
+    P.maxw = nodep ? push_maxw_nodep(c->p.precision, P.deposit_rho, c->maxw) : c->maxw;
+    P.maxw_mid = push_maxw_mid(c->p.precision, P.deposit_rho, nodep ? 0 : 1);
+    P.maxw_big = push_maxw_big(c->p.precision, P.deposit_rho, nodep ? 0 : 1);
     P.big = c->big;
     P.pass = 0;
     P.nodep = nodep ? 1 : 0;

@@ -1261,10 +1261,12 @@ hipError_t launch_nodefield(const PushParams &p, int precision, hipStream_t st)
     return hipGetLastError();
 }
 
-size_t push_lds_bytes(int precision, int maxw, int deposit_rho)
+// dep = 0: a push without deposit (DEP = false) uses the node-field window only
+size_t push_lds_bytes(int precision, int maxw, int deposit_rho, int dep)
 {
     const size_t r = precision == PBPIC_PREC_FP64 ? 8 : 4;
     const size_t ns = precision == PBPIC_PREC_FP64 ? 6 : 8;
+    if (!dep) return (size_t)maxw * ns * r;
     const size_t windows = (size_t)maxw * (ns * r + (deposit_rho ? 4 : 3) * 8);
     return precision == PBPIC_PREC_FP64 ? windows
                                         : windows + (PUSH_THREADS / 64) * XQ_BYTES;
@@ -1319,21 +1321,28 @@ static void launch_push_f(const PushParams &q, int nt, size_t lds, hipStream_t s
 
 // Largest window whose LDS fits one workgroup per CU (150 KiB).
 // Window capacity (nodes) of a launch whose workgroups take `budget` bytes of LDS.
-static int push_maxw_for(int precision, int deposit_rho, size_t budget)
+static int push_maxw_for(int precision, int deposit_rho, int dep, size_t budget)
 {
-    const size_t per = push_lds_bytes(precision, 1, deposit_rho) -
-                       push_lds_bytes(precision, 0, deposit_rho);
-    return (int)((budget - push_lds_bytes(precision, 0, deposit_rho)) / per);
+    const size_t per = push_lds_bytes(precision, 1, deposit_rho, dep) -
+                       push_lds_bytes(precision, 0, deposit_rho, dep);
+    return (int)((budget - push_lds_bytes(precision, 0, deposit_rho, dep)) / per);
 }
 
 // one workgroup per CU (150 KiB) and two (78 KiB) of the 160 KiB
-int push_maxw_big(int precision, int deposit_rho)
+int push_maxw_big(int precision, int deposit_rho, int dep)
 {
-    return push_maxw_for(precision, deposit_rho, 150 * 1024);
+    return push_maxw_for(precision, deposit_rho, dep, 150 * 1024);
 }
-int push_maxw_mid(int precision, int deposit_rho)
+int push_maxw_mid(int precision, int deposit_rho, int dep)
 {
-    return push_maxw_for(precision, deposit_rho, 78 * 1024);
+    return push_maxw_for(precision, deposit_rho, dep, 78 * 1024);
+}
+// the first launch of a push without deposit: the node-field window that
+// fills the LDS the depositing push's window of maxw nodes takes (the same
+// workgroups per CU)
+int push_maxw_nodep(int precision, int deposit_rho, int maxw)
+{
+    return push_maxw_for(precision, deposit_rho, 0, push_lds_bytes(precision, maxw, deposit_rho, 1));
 }
 
 hipError_t launch_push(const PushParams &p, int precision, hipStream_t st)
@@ -1341,7 +1350,8 @@ hipError_t launch_push(const PushParams &p, int precision, hipStream_t st)
     PushParams q;
     const int nt = push_tiles(p, q);
     if (nt == 0) return hipSuccess;
-    const size_t lds = push_lds_bytes(precision, p.maxw, p.deposit_rho);
+    const int dep = p.nodep ? 0 : 1;
+    const size_t lds = push_lds_bytes(precision, p.maxw, p.deposit_rho, dep);
     q.pass = p.big && p.maxw_big > p.maxw ? 1 : 0;
     q.maxw_mid = std::max(p.maxw, std::min(p.maxw_mid, p.maxw_big));
     if (q.pass) {
@@ -1357,7 +1367,7 @@ hipError_t launch_push(const PushParams &p, int precision, hipStream_t st)
             b.pass = pass;
             b.maxw = pass == 2 ? q.maxw_mid : p.maxw_big;
             if (pass == 2 && b.maxw == p.maxw) continue;       // no middle tier
-            const size_t lb = push_lds_bytes(precision, b.maxw, p.deposit_rho);
+            const size_t lb = push_lds_bytes(precision, b.maxw, p.deposit_rho, dep);
             const int ng = std::max(1, (nt + 7) / 8);
             if (precision == PBPIC_PREC_FP64) launch_push_f<double, false>(b, ng, lb, st);
             else launch_push_f<float, true>(b, ng, lb, st);

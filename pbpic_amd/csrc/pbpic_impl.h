@@ -247,8 +247,9 @@ struct AdjFieldParams {
 // Launchers (one per translation unit family).
 hipError_t launch_push(const PushParams &p, int precision, hipStream_t st);
 // Window capacity of the second push pass (one workgroup per CU).
-int push_maxw_big(int precision, int deposit_rho);
-int push_maxw_mid(int precision, int deposit_rho);
+int push_maxw_big(int precision, int deposit_rho, int dep);
+int push_maxw_mid(int precision, int deposit_rho, int dep);
+int push_maxw_nodep(int precision, int deposit_rho, int maxw);
 size_t nodefield_bytes(int precision, int64_t ncell);
 // Fills p.nf from the E and B planes; launch_push reads p.nf.
 hipError_t launch_nodefield(const PushParams &p, int precision, hipStream_t st);
