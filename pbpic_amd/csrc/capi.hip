@@ -1740,9 +1740,11 @@ int pbpic_adjoint_backward(pbpic_ctx *c, double *const *fg, void *const *pg)
                 A.nyp = walls(c) ? nyp_of(c) : 0;
                 A.tv = (c->p.flags & PBPIC_FLAG_PUSH_ORDER) ? push_vec(c->p.precision) : 0;
                 A.dny.init((uint32_t)c->p.ny);
-                // windows of two workgroups per CU (90 or 132 B of LDS per
-                // node: node fields, node adjoints, 1.5 x the current adjoint)
-                A.maxw = c->p.precision == PBPIC_PREC_FP64 ? 608 : 896;
+                // windows of two workgroups per CU: node fields and node
+                // adjoints (72 or 96 B of LDS per node), the current adjoint
+                // in the rest of the 80 KiB
+                A.maxw = c->p.precision == PBPIC_PREC_FP64 ? 768 : 1024;
+                A.jmax = adj_jmax_for(c->p.precision, A.maxw, 80 * 1024);
                 A.nspecies = ns;
                 A.dtcva = c->dtcva; A.idx = c->idx; A.idy = c->idy; A.cva = c->p.cva;
                 A.ex = state_field(c, L, tstate[k], 0); A.ey = state_field(c, L, tstate[k], 1);

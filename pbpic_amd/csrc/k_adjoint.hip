@@ -40,11 +40,6 @@ __host__ __device__ constexpr size_t adj_nf_bytes(size_t r, int maxw)
     return ((size_t)maxw * 6 * r + 7) & ~(size_t)7;
 }
 
-// Nodes of the current-adjoint window: the deposit targets of a cell reach
-// one node below/left and two above/right of the cell, so the window is the
-// node-field window grown by one node on each side; a box of maxw nodes or
-// fewer whose grown window exceeds this is handled as a wide box.
-__host__ __device__ constexpr int adj_jmax(int maxw) { return maxw + maxw / 2; }
 
 template <typename R>
 __device__ __forceinline__ void a_lds_add(R *p, R v)
@@ -185,14 +180,17 @@ __global__ __launch_bounds__(ADJ_THREADS, 2) void push_adj_kernel(AdjParams P)
     // nodes used: the cell's 4 corners -> window = box + 1 node
     const int wx0 = mnx, wy0 = mny;
     const int W = mxx - mnx + 2, H = mxy - mny + 2;
-    const bool inlds = mnx <= mxx && (int64_t)W * H <= P.maxw &&
-                       (int64_t)(W + 2) * (H + 2) <= adj_jmax(P.maxw);
-    if (P.pass == 1 && !inlds && mnx <= mxx && (int64_t)W * H <= P.maxw_big &&
-        (int64_t)(W + 2) * (H + 2) <= adj_jmax(P.maxw_big)) {
+    const bool inlds = mnx <= mxx && (int64_t)W * H <= P.maxw;
+    if (P.pass == 1 && !inlds && mnx <= mxx && (int64_t)W * H <= P.maxw_big) {
         if (tid == 0) P.big[1 + atomicAdd(P.big, 1)] = t;   // block-uniform exit
         return;
     }
-    const int JW = W + 2, JWH = inlds ? JW * (H + 2) : 0;
+    // the current adjoint at the deposit targets, which reach one node
+    // below/left and two above/right of a cell: the box grown by one node on
+    // each side, staged when it fits the rest of the LDS (else read from L2)
+    const int JW = W + 2;
+    const bool jlds = inlds && (int64_t)JW * (H + 2) <= P.jmax;
+    const int JWH = jlds ? JW * (H + 2) : 0;
     const int WH = inlds ? W * H : 0;
     const bool interior = mnx <= mxx && x0 + mnx >= 1 && x0 + mxx <= nx - 3 && y0 + mny >= 1 &&
                           y0 + mxy <= ny - 3;
@@ -200,9 +198,9 @@ __global__ __launch_bounds__(ADJ_THREADS, 2) void push_adj_kernel(AdjParams P)
     // [6][WH] node adjoint accumulators, FP64 in both forms: LDS FP64 atomic
     // adds run at about ten times the rate of FP32 ones on this hardware
     double *na = (double *)(smem + adj_nf_bytes(sizeof(R), P.maxw));
-    // [3][JWH] current adjoint at the deposit targets, in the push precision
+    // [3][jmax] current adjoint at the deposit targets, in the push precision
     R *jw = (R *)(na + (size_t)6 * P.maxw);
-    const int jmax = adj_jmax(P.maxw);
+    const int jmax = P.jmax;
     if (inlds) {
         for (int k = tid; k < WH; k += ADJ_THREADS) {
             const int wj = k / W, wi = k - wj * W;
@@ -348,7 +346,7 @@ __global__ __launch_bounds__(ADJ_THREADS, 2) void push_adj_kernel(AdjParams P)
         // of the cell: no periodic wrap for a tile away from every edge
         const bool drop = (P.compat & PBPIC_COMPAT_JY_DROP) && cx == nx - 1 && a == 1;
         R l1, l2, l3, l4, l5, l6, l7, l8, z0, z1, z2, z3;
-        if (inlds) {
+        if (jlds) {
             // window node of target (cx + ddx, cy + ddy): (lx + 1 + ddx, ly + 1 + ddy)
             const R *wa = jw + (ly + 1) * JW + lx + 1, *wb = wa + jmax, *wc = wb + jmax;
             const int ab = b * JW + a;
@@ -687,17 +685,26 @@ __global__ void unpermute_kernel(const uint32_t *perm, int64_t n, Ptr5 post, Ptr
     for (int q = 0; q < 5; q++) ((R *)pre.p[q])[p] = ((const R *)post.p[q])[k];
 }
 
-size_t adj_lds_bytes(int precision, int maxw)
+size_t adj_lds_bytes(int precision, int maxw, int jmax)
 {
     const size_t r = precision == PBPIC_PREC_FP64 ? 8 : 4;
-    return adj_nf_bytes(r, maxw) + (size_t)maxw * 6 * 8 + 3 * r * (size_t)adj_jmax(maxw);
+    return adj_nf_bytes(r, maxw) + (size_t)maxw * 6 * 8 + 3 * r * (size_t)jmax;
+}
+
+// Current-adjoint window (nodes) that fills the LDS a workgroup may take
+// (`budget`, static arrays included) beside node windows of maxw nodes.
+int adj_jmax_for(int precision, int maxw, size_t budget)
+{
+    const size_t r = precision == PBPIC_PREC_FP64 ? 8 : 4, used = adj_lds_bytes(precision, maxw, 0);
+    const size_t stat = 64;                          // __shared__ red[4][4]
+    return used + stat >= budget ? 0 : (int)((budget - used - stat) / (3 * r));
 }
 
 int64_t adj_tiles(int64_t n) { return (n + ADJ_TILE - 1) / ADJ_TILE; }
 
 static void launch_push_adj_once(const AdjParams &p, int precision, int t, hipStream_t st)
 {
-    const size_t lds = adj_lds_bytes(precision, p.maxw);
+    const size_t lds = adj_lds_bytes(precision, p.maxw, p.jmax);
     if (precision == PBPIC_PREC_FP64) {
         (void)hipFuncSetAttribute((const void *)push_adj_kernel<double, false>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
@@ -719,12 +726,12 @@ hipError_t launch_push_adj(const AdjParams &p0, int precision, hipStream_t st)
     }
     if (t == 0) return hipSuccess;
     // the second pass gets the window of one workgroup per CU (150 KiB)
-    {
-        const size_t cap = 150 * 1024, r = precision == PBPIC_PREC_FP64 ? 8 : 4;
-        int m = (int)(cap / (6 * r + 48 + 3 * r * 3 / 2));
-        while (m > 0 && adj_lds_bytes(precision, m) > cap) m--;
-        p.maxw_big = m;
-    }
+    // the second pass: one workgroup per CU (150 KiB), node windows twice
+    // the first pass's, the current-adjoint window in the rest
+    const size_t cap = 150 * 1024;
+    p.maxw_big = 2 * p.maxw;
+    while (p.maxw_big > p.maxw && adj_lds_bytes(precision, p.maxw_big, 0) + 64 > cap) p.maxw_big--;
+    const int jmax_big = adj_jmax_for(precision, p.maxw_big, cap);
     p.pass = p.big && p.maxw_big > p.maxw ? 1 : 0;
     if (p.pass) {
         hipError_t e = launch_fill_u32((uint32_t *)p.big, 1, 0u, st);
@@ -735,6 +742,7 @@ hipError_t launch_push_adj(const AdjParams &p0, int precision, hipStream_t st)
         AdjParams b = p;
         b.pass = 2;
         b.maxw = p.maxw_big;
+        b.jmax = jmax_big;
         launch_push_adj_once(b, precision, t, st);
     }
     return hipGetLastError();

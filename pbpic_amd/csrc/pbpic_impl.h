@@ -223,6 +223,7 @@ struct AdjParams {
     // maxw_big are deferred to a second launch (big: count + tile list)
     int maxw_big, pass;
     int *big;
+    int jmax;                                    // current-adjoint window (nodes)
     double dtcva, idx, idy, cva;
     const double *ex, *ey, *ez, *bx, *by, *bz;   // fields at step n
     const double *jadj[3];                       // current adjoint (raw J)
@@ -329,6 +330,7 @@ struct SortScratch {
     int64_t rcap = 0;
 };
 hipError_t launch_push_adj(const AdjParams &p, int precision, hipStream_t st);
+int adj_jmax_for(int precision, int maxw, size_t budget);
 // Tiles of the adjoint push (2048 particles each) for n particles.
 int64_t adj_tiles(int64_t n);
 hipError_t launch_fdtd_adj(const AdjFieldParams &p, int precision, hipStream_t st);
