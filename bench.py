@@ -101,10 +101,11 @@ def run_form(precision, cfg, steps, warmup, sort_every, seed, push_order=True, c
     return out
 
 
-# HBM bytes per push launch at 2048x2048 x 100 ppc, steady state: rocprofv3
-# FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (factors from a 4 GiB copy), from the
-# commands in profiles/push_kernel_traffic_two_stream_2048x2048_ppc100.md.
-TRAFFIC_2048_PPC100 = {1: 10.263e9 + 10.412e9, 0: 18.714e9 + 18.807e9}
+# HBM bytes per push launch at 2048x2048 x 100 ppc, the mean over a 50-step
+# sort cycle: rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (factors from a
+# 4 GiB copy), from the commands in
+# profiles/push_kernel_traffic_two_stream_2048x2048_ppc100.md.
+TRAFFIC_2048_PPC100 = {1: 10.266e9 + 10.557e9, 0: 18.758e9 + 18.950e9}
 TRAFFIC_SOURCE = "profiles/push_kernel_traffic_two_stream_2048x2048_ppc100.md"
 
 
@@ -238,8 +239,9 @@ def adjoint_c4_leg(seed, nsteps=1000, ckpt=50, precision=1):
     (134,217,728 particles), 1000 steps, loss = J.E over the central 4 x 2 d_i
     diffusion region, gradient with respect to the initial Bz (and every
     other initial field and particle), checkpoints every 50 steps, split
-    precision.  Times the plain forward against the checkpointed forward and
-    the adjoint sweep, and reports the device memory the sweep needed."""
+    precision.  Times the plain forward against the checkpointed forward
+    (which ends by allocating the backward's tape) and the adjoint sweep,
+    and reports the device memory free before them."""
     import numpy as np
     import torch
     import pbpic_amd

@@ -430,7 +430,9 @@ int pbpic_set_field_device(pbpic_ctx *ctx, int32_t which, const double *plane);
  * 6 device pointers (nx*ny doubles: Ex, Ey, Ez, Bx, By, Bz), particle_grads
  * 5 device pointers per species in species order (dx, dy, ux, uy, uz, in the
  * particle precision, in the order the particles had before the forward).
- * Segments between checkpoints are recomputed.  The context ends in the state
+ * Segments between checkpoints are recomputed into a tape of ckpt_every + 1
+ * states, which the forward allocates at its end (the backward allocates it
+ * if that failed) and the backward frees.  The context ends in the state
  * after the forward.                                                         */
 int pbpic_adjoint_forward(pbpic_ctx *ctx, int32_t nsteps, int32_t ckpt_every,
                           double *loss);

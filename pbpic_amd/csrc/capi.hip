@@ -112,6 +112,9 @@ struct pbpic_ctx {
         // J after the push of every forward step (3 planes each), so that
         // the backward's replay need not deposit; null if it did not fit
         char *jstore = nullptr;
+        // the backward's tape and adjoint buffers, allocated at the end of
+        // the forward (or by the backward) and freed by the backward
+        char *tape = nullptr;
         double acc_after[ACC_N] = {0};
         double loss = 0;
     } adj;
@@ -602,7 +605,57 @@ void adj_free(pbpic_ctx *c)
     c->adj.final_state = nullptr;
     if (c->adj.jstore) (void)hipFree(c->adj.jstore);
     c->adj.jstore = nullptr;
+    if (c->adj.tape) (void)hipFree(c->adj.tape);
+    c->adj.tape = nullptr;
     c->adj.valid = false;
+}
+
+// Bytes of the backward's single allocation: the tape of ckpt + 1 states, the
+// replayed steps' J when the forward kept none, the sort permutations of a
+// segment, the field and particle adjoints, the wide-tile list.
+struct AdjTapeSizes {
+    size_t states, jtape, perm, fadj, padj, big;
+    int nperm;
+    size_t total(bool with_j) const { return states + (with_j ? jtape : 0) + perm + fadj + padj + big; }
+};
+static AdjTapeSizes adj_tape_sizes(pbpic_ctx *c, int K)
+{
+    const StateLayout L = state_layout(c);
+    const size_t nc = (size_t)ncell(c), rz = (size_t)c->rsz;
+    const int ns = (int)c->sp.size();
+    int64_t ntot = 0, nmax = 0, atiles = 0;
+    for (auto &S : c->sp) { ntot += S.n; nmax = std::max(nmax, S.n); atiles += adj_tiles(S.n); }
+    AdjTapeSizes z{};
+    z.states = (size_t)(K + 1) * L.total;
+    z.jtape = (size_t)K * 3 * al256(rz * nc);
+    // permutations only for the steps that sort: at most K/sort_every + 1 per segment
+    z.nperm = c->p.sort_every > 0 ? std::min(K, K / c->p.sort_every + 1) : 0;
+    z.perm = (size_t)z.nperm * al256(4 * (size_t)ntot + 256 * ns);
+    z.fadj = 18 * al256(8 * nc);                                 // Ebar, Bbar, E', Jbar, node x6
+    z.padj = 5 * al256(rz * (size_t)ntot + 256 * ns) + 5 * al256(rz * (size_t)nmax);
+    z.big = al256(sizeof(int32_t) * (size_t)(atiles + 1));
+    return z;
+}
+
+// Allocates the backward's buffers into c->adj.tape.  If the forward's kept J
+// leaves too little room, it is dropped and the replay deposits again,
+// keeping J in the tape instead.
+static hipError_t adj_tape_alloc(pbpic_ctx *c, int K)
+{
+    if (c->adj.tape) return hipSuccess;
+    const AdjTapeSizes z = adj_tape_sizes(c, K);
+    hipError_t e = hipMalloc((void **)&c->adj.tape, z.total(!c->adj.jstore));
+    if (e != hipSuccess && c->adj.jstore) {
+        (void)hipGetLastError();
+        (void)hipFree(c->adj.jstore);
+        c->adj.jstore = nullptr;
+        e = hipMalloc((void **)&c->adj.tape, z.total(true));
+    }
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        c->adj.tape = nullptr;
+    }
+    return e;
 }
 
 // Host particle I/O.  The caller's arrays are pageable memory: they are
@@ -994,7 +1047,7 @@ int pbpic_ctx_destroy(pbpic_ctx *c)
     f(c->sort.status); f(c->sort.dhist); f(c->sort.ticket); f(c->sort.counts);
     f(c->sort.tarea); f(c->sort.toff); f(c->sort.bsum); f(c->sort.ctl); f(c->sort.rbuf);
     for (auto p : c->adj.ckpts) f(p);
-    f(c->adj.final_state); f(c->adj.jstore);
+    f(c->adj.final_state); f(c->adj.jstore); f(c->adj.tape);
     for (auto e : c->ev_pool) (void)hipEventDestroy(e);
     for (int k = 0; k < 2; k++) {
         if (c->stage[k]) (void)hipHostFree(c->stage[k]);
@@ -1618,6 +1671,11 @@ int pbpic_adjoint_forward(pbpic_ctx *c, int32_t nsteps, int32_t ckpt, double *lo
     c->adj.loss = loss_total(c, a1) - loss_total(c, a0);
     c->adj.valid = true;
     if (loss) *loss = c->adj.loss;
+    // the backward's buffers, allocated here so that the backward is the
+    // sweep alone (an allocation of this size can take seconds while the
+    // driver clears memory another process freed); if they do not fit now,
+    // the backward tries again and reports the failure
+    (void)adj_tape_alloc(c, ckpt);
     return PBPIC_OK;
 }
 
@@ -1628,32 +1686,17 @@ int pbpic_adjoint_backward(pbpic_ctx *c, double *const *fg, void *const *pg)
     const StateLayout L = state_layout(c);
     const int K = c->adj.ckpt, N = c->adj.nsteps, ns = (int)c->sp.size();
     const size_t nc = (size_t)ncell(c), rz = (size_t)c->rsz;
-    int64_t ntot = 0, nmax = 0;
-    for (auto &S : c->sp) { ntot += S.n; nmax = std::max(nmax, S.n); }
-    // one allocation for the tape and the adjoint buffers
-    const size_t tape_states = (size_t)(K + 1) * L.total;
-    // J of each replayed step: kept by the forward, else recomputed into the tape
-    const size_t jplane = al256(rz * nc);
-    // permutations only for the steps that sort: at most K/sort_every + 1 per segment
-    const int nperm = c->p.sort_every > 0 ? std::min(K, K / c->p.sort_every + 1) : 0;
-    const size_t tape_perm = (size_t)nperm * al256(4 * (size_t)ntot + 256 * ns);
-    const size_t fadj = 18 * al256(8 * nc);                    // Ebar, Bbar, E', Jbar, node x6
-    const size_t padj = 5 * al256(rz * (size_t)ntot + 256 * ns) + 5 * al256(rz * (size_t)nmax);
-    int64_t atiles = 0;
-    for (auto &S : c->sp) atiles += adj_tiles(S.n);
-    const size_t abig = al256(sizeof(int32_t) * (size_t)(atiles + 1));
-    char *base = nullptr;
-    hipError_t e = hipMalloc(&base, tape_states + (c->adj.jstore ? 0 : (size_t)K * 3 * jplane) +
-                                        tape_perm + fadj + padj + abig);
-    if (e != hipSuccess && c->adj.jstore) {
-        // the forward's kept J leaves too little room: drop it and replay
-        // with the deposit, keeping J in the tape instead
-        (void)hipGetLastError();
-        (void)hipFree(c->adj.jstore);
-        c->adj.jstore = nullptr;
-        e = hipMalloc(&base, tape_states + (size_t)K * 3 * jplane + tape_perm + fadj + padj + abig);
-    }
+    int64_t nmax = 0, atiles = 0;
+    for (auto &S : c->sp) { nmax = std::max(nmax, S.n); atiles += adj_tiles(S.n); }
+    // one allocation for the tape and the adjoint buffers (normally made at
+    // the end of the forward); J of each replayed step is the forward's kept
+    // J, else recomputed into the tape
+    hipError_t e = adj_tape_alloc(c, K);
     if (e != hipSuccess) return fail(c, PBPIC_ERR_NOMEM, "adjoint tape allocation: %s", hipGetErrorString(e));
+    char *base = c->adj.tape;
+    c->adj.tape = nullptr;                     // freed at the end of this call
+    const int nperm = adj_tape_sizes(c, K).nperm;
+    const size_t jplane = al256(rz * nc);
     int rc = PBPIC_OK;
     char *p = base;
     auto take = [&](size_t b) { char *q = p; p += al256(b); return q; };

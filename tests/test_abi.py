@@ -51,8 +51,15 @@ def test_fortran_interface_binds_header_names():
     src = open(os.path.join(ROOT, "pbpic_amd", "fortran", "pbpic_mod.f90")).read()
     bound = set(re.findall(r'name="(pbpic_\w+)"', src))
     assert bound and bound <= set(declared_functions())
-    for sub in ("pusher", "fieldsolver", "countingsort", "currentdeposit", "fieldsenergy"):
+    # the reference's subroutine names: plain subroutines, or (currentdeposit,
+    # which takes real arguments of the caller's default kind) a generic
+    # interface over one module procedure per real kind
+    for sub in ("pusher", "fieldsolver", "countingsort", "fieldsenergy"):
         assert re.search(rf"^\s*subroutine {sub}\b", src, flags=re.M), sub
+    m = re.search(r"^\s*interface currentdeposit\s*\n\s*module procedure (.+)$", src, flags=re.M)
+    assert m, "currentdeposit"
+    for proc in (x.strip() for x in m.group(1).split(",")):
+        assert re.search(rf"^\s*subroutine {proc}\b", src, flags=re.M), proc
 
 
 def test_params_struct_layout(lib):
