@@ -351,12 +351,14 @@ int kinetic_impl(pbpic_ctx *c)
     return PBPIC_OK;
 }
 
-// Split form: the FP32 current planes from the fixed-point sums.
+// Split form: the FP32 current planes from the fixed-point sums, with the Jz
+// wall fold of conducting y walls applied to the sums.
 int finalize_current(pbpic_ctx *c)
 {
     if (c->rsz != 4) return PBPIC_OK;
     HIPCHK(c, launch_finalize_current(c->J64, c->J, ncell(c), c->js,
-                                      (c->p.flags & PBPIC_FLAG_DEPOSIT_RHO) ? 1 : 0, c->st));
+                                      (c->p.flags & PBPIC_FLAG_DEPOSIT_RHO) ? 1 : 0, c->p.nx,
+                                      walls(c) ? nyp_of(c) : 0, c->st));
     return PBPIC_OK;
 }
 
@@ -392,7 +394,8 @@ int push_impl(pbpic_ctx *c, bool recs = false, bool nodep = false)
         S.ux[b] = c->push_out[s][3]; S.uy[b] = c->push_out[s][4]; S.uz[b] = c->push_out[s][5];
     }
     c->push_out.clear();
-    if (walls(c) && !nodep)
+    // the split form folds Jz at the walls in finalize_current
+    if (walls(c) && !nodep && c->rsz == 8)
         HIPCHK(c, launch_wall_fold(c->J[2], c->p.nx, c->p.ny, P.nyp, c->rsz, c->st));
     for (auto &S : c->sp) S.tbox_ok = true;
     HIPCHK(c, prof_mark(c, PBPIC_PROF_PUSH, false));
@@ -402,7 +405,6 @@ int push_impl(pbpic_ctx *c, bool recs = false, bool nodep = false)
 
 int fieldsolve_impl(pbpic_ctx *c)
 {
-    HIPCHK(c, launch_acc_op(c->acc, 1, c->p.dt * c->p.mime / (c->p.cva * c->p.cva), c->st));
     HIPCHK(c, prof_mark(c, PBPIC_PROF_FIELD, true));
     if (c->filt[0]) {
         const double cva2 = c->p.cva * c->p.cva, dt = c->p.dt;
@@ -420,7 +422,9 @@ int fieldsolve_impl(pbpic_ctx *c)
         q.fex = c->filt[0]; q.fey = c->filt[1]; q.fez = c->filt[2];
         HIPCHK(c, launch_filters(q, c->p.precision, c->st));
     }
-    HIPCHK(c, launch_fdtd(fdtd_params(c), c->p.precision, c->st));
+    FdtdParams F = fdtd_params(c);
+    F.je_pre = c->p.dt * c->p.mime / (c->p.cva * c->p.cva);
+    HIPCHK(c, launch_fdtd(F, c->p.precision, c->st));
     for (int k = 0; k < 3; k++) {
         std::swap(c->E[k], c->E2[k]);
         std::swap(c->B[k], c->B2[k]);
@@ -954,9 +958,13 @@ int pbpic_ctx_create(const pbpic_params *pp, pbpic_ctx **out)
     pbpic_ctx *c = new pbpic_ctx();
     c->p = p;
     c->rsz = p.precision == PBPIC_PREC_FP64 ? 8 : 4;
-    // window capacity in nodes, sized so that three push workgroups share a
-    // CU's 160 KiB of LDS (push_lds_bytes without the rho plane)
-    c->maxw = p.precision == PBPIC_PREC_FP64 ? 752 : 768;
+    // window capacity in nodes: split form, three push workgroups share a
+    // CU's 160 KiB of LDS (push_lds_bytes without the rho plane); FP64 form,
+    // two (the FP64 push runs as fast at two workgroups per CU, and the wider
+    // window defers fewer tiles once the cells have spread)
+    c->maxw = p.precision == PBPIC_PREC_FP64
+                  ? push_maxw_mid(p.precision, (p.flags & PBPIC_FLAG_DEPOSIT_RHO) ? 1 : 0, 1)
+                  : 768;
     c->idx = 1.0 / p.dx;                   // pic2d.f90:16
     c->idy = 1.0 / p.dy;
     c->dth = p.dt / 2.0;                   // :43

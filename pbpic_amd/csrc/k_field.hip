@@ -193,7 +193,10 @@ hipError_t launch_fdtd(const FdtdParams &p, int precision, hipStream_t st)
         hipLaunchKernelGGL((fdtd_kernel<double, false>), dim3(nb), dim3(FT_THREADS), 0, st, p);
     else
         hipLaunchKernelGGL((fdtd_kernel<float, true>), dim3(nb), dim3(FT_THREADS), 0, st, p);
-    PartSlots ps{p.lmask ? 2 : 1, {ACC_JE_STEP, ACC_JEM_STEP, 0, 0}, {1.0, 1.0, 0, 0}};
+    // the previous step's J.E (and loss) values are integrated as they are
+    // replaced: JE_INT += JE_STEP * dt mime/c^2, then JE_STEP = this step's
+    PartSlots ps{2, {ACC_JE_STEP, ACC_JEM_STEP, 0, 0}, {1.0, 1.0, 0, 0},
+                 {ACC_JE_INT, ACC_JEM_INT, -1, -1}, p.je_pre};
     return launch_reduce_parts(p.part, nb, p.acc, ps, st);
 }
 
@@ -211,7 +214,15 @@ __global__ __launch_bounds__(256) void reduce_parts_kernel(const double *part, i
     for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
     if ((threadIdx.x & 63) == 0) w[threadIdx.x >> 6] = s;
     __syncthreads();
-    if (threadIdx.x == 0) acc[ps.slot[q]] += ps.scale[q] * ((w[0] + w[1]) + (w[2] + w[3]));
+    if (threadIdx.x == 0) {
+        const double v = ps.scale[q] * ((w[0] + w[1]) + (w[2] + w[3]));
+        if (ps.ints[q] >= 0) {
+            acc[ps.ints[q]] += acc[ps.slot[q]] * ps.pre;
+            acc[ps.slot[q]] = v;
+        } else {
+            acc[ps.slot[q]] += v;
+        }
+    }
 }
 
 hipError_t launch_reduce_parts(const double *part, int nb, double *acc, const PartSlots &ps,
@@ -667,21 +678,33 @@ hipError_t launch_fill_u32(uint32_t *p, int64_t n, uint32_t v, hipStream_t st)
 // Split form: the FP32 current planes from the fixed-point sums the push
 // accumulated (one rounding of an exact sum, the same for any order of the
 // contributions).
+// With conducting y walls (nyp > 0) Jz is folded on the way (as
+// wall_fold_kernel folds an FP64 plane): the guard rows nyp and nyp + 1 are
+// subtracted, as integers, from the domain rows nyp - 1 and 0 they mirror,
+// and read as zero.
 __global__ void finalize_current_kernel(const long long *a, const long long *b,
                                         const long long *c, const long long *d, float *ja,
-                                        float *jb, float *jc, float *jd, int64_t n, float4 inv)
+                                        float *jb, float *jc, float *jd, int64_t n, float4 inv,
+                                        int64_t nx, int64_t nyp)
 {
     for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
          k += (int64_t)gridDim.x * blockDim.x) {
         ja[k] = (float)((double)a[k] * (double)inv.x);
         jb[k] = (float)((double)b[k] * (double)inv.y);
-        jc[k] = (float)((double)c[k] * (double)inv.z);
+        long long z = c[k];
+        if (nyp > 0) {
+            if (k < nx) z -= c[k + (nyp + 1) * nx];                        // row 0 - row -1
+            else if (k >= (nyp - 1) * nx && k < nyp * nx) z -= c[k + nx];   // row nyp-1 - row nyp
+            else if (k >= nyp * nx) z = 0;                                  // the guard rows
+        }
+        jc[k] = (float)((double)z * (double)inv.z);
         if (d) jd[k] = (float)((double)d[k] * (double)inv.w);
     }
 }
 
 hipError_t launch_finalize_current(void *const *j64, void *const *j, int64_t ncell,
-                                   const float *scale, int with_rho, hipStream_t st)
+                                   const float *scale, int with_rho, int nx, int nyp,
+                                   hipStream_t st)
 {
     int64_t nb = (ncell + 255) / 256;
     if (nb > 8192) nb = 8192;
@@ -692,7 +715,8 @@ hipError_t launch_finalize_current(void *const *j64, void *const *j, int64_t nce
     hipLaunchKernelGGL(finalize_current_kernel, dim3((unsigned)nb), dim3(256), 0, st,
                        (const long long *)j64[0], (const long long *)j64[1],
                        (const long long *)j64[2], with_rho ? (const long long *)j64[3] : nullptr,
-                       (float *)j[0], (float *)j[1], (float *)j[2], (float *)j[3], ncell, inv);
+                       (float *)j[0], (float *)j[1], (float *)j[2], (float *)j[3], ncell, inv,
+                       (int64_t)nx, (int64_t)nyp);
     return hipGetLastError();
 }
 

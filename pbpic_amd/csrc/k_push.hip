@@ -85,6 +85,10 @@ __global__ __launch_bounds__(256) void nodefield_kernel(PushParams P, R *out)
     __shared__ double t[6][NF_T + 1][NF_T + 1];     // [plane][gy - Y0 + 1][gx - X0 + 1]
     const int nx = P.nx, ny = P.ny;
     const int X0 = blockIdx.x * NF_T, Y0 = blockIdx.y * NF_T;
+    // the push that follows appends deferred tiles to two lists: clear their
+    // counts here (the previous push's listed launches have finished reading
+    // them)
+    if (P.big && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x < 2) P.big[threadIdx.x] = 0;
     const double *pl[6] = {P.ex, P.ey, P.ez, P.bx, P.by, P.bz};
     for (int k = threadIdx.x; k < (NF_T + 1) * (NF_T + 1); k += 256) {
         const int r = k / (NF_T + 1), cc = k - r * (NF_T + 1);
@@ -1354,11 +1358,8 @@ hipError_t launch_push(const PushParams &p, int precision, hipStream_t st)
     const size_t lds = push_lds_bytes(precision, p.maxw, p.deposit_rho, dep);
     q.pass = p.big && p.maxw_big > p.maxw ? 1 : 0;
     q.maxw_mid = std::max(p.maxw, std::min(p.maxw_mid, p.maxw_big));
-    if (q.pass) {
-        // the two list counts
-        hipError_t e = launch_fill_u32((uint32_t *)p.big, 2, 0u, st);
-        if (e != hipSuccess) return e;
-    }
+    // the two list counts were cleared by the node-field launch before this
+    // one (launch_nodefield, which every push follows)
     if (precision == PBPIC_PREC_FP64) launch_push_f<double, false>(q, nt, lds, st);
     else launch_push_f<float, true>(q, nt, lds, st);
     if (q.pass) {

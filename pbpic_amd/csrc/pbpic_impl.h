@@ -179,6 +179,7 @@ struct FdtdParams {
     const double *lmask;               // loss-region weights per cell (or null)
     double *acc;
     double *part;                      // per-workgroup partial sums (launch_reduce_parts)
+    double je_pre;                     // dt mime/c^2: integrates the previous J.E step
 };
 
 // Reductions in a fixed order.  A reduction kernel writes one partial per
@@ -189,6 +190,10 @@ struct PartSlots {
     int nq;
     int slot[4];
     double scale[4];
+    // ints[q] >= 0: first acc[ints[q]] += acc[slot[q]] * pre (the time
+    // integral of the previous step's value), then acc[slot[q]] = the sum
+    int ints[4] = {-1, -1, -1, -1};
+    double pre = 0;
 };
 hipError_t launch_reduce_parts(const double *part, int nb, double *acc, const PartSlots &ps,
                                hipStream_t st);
@@ -290,7 +295,8 @@ hipError_t launch_init_harris(const SpeciesDev &ion, const SpeciesDev &ele, int 
 // Split form: J (float planes) = fixed-point sums / scale, for the 3 current
 // planes and the charge plane too when with_rho.
 hipError_t launch_finalize_current(void *const *j64, void *const *j, int64_t ncell,
-                                   const float *scale, int with_rho, hipStream_t st);
+                                   const float *scale, int with_rho, int nx, int nyp,
+                                   hipStream_t st);
 hipError_t launch_zero_current(void *jx, void *jy, void *jz, void *rho,
                                int64_t ncell, int elem_bytes, double *acc,
                                int reset_ke, hipStream_t st);
