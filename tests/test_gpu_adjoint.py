@@ -171,3 +171,28 @@ def test_adjoint_full_tiles(gpu, precision):
     for s in range(2):
         for k in PART:
             assert rel_norm(gPg[s][k], gP[s][k]) < tol, (s, k)
+
+
+@pytest.mark.parametrize("nx,ny", [(28, 24), (40, 36)])
+@pytest.mark.parametrize("precision", [0, 1])
+def test_adjoint_wide_windows(gpu, precision, nx, ny):
+    """Unsorted particles: every adjoint tile spans the whole domain until the
+    sort at step 5.  At 28 x 24 its node window (29 x 25 nodes) fits the first
+    launch (1,024 nodes split, 768 FP64) but the current-adjoint window grown
+    by one node on each side (31 x 27) does not fit the LDS left beside it
+    (677 and 338 nodes), so the deposit's transpose reads the current adjoint
+    from L2.  At 40 x 36 the box (41 x 37 nodes) goes to the second launch
+    (2,048 and 1,536 nodes), whose current adjoint is read from L2 as well.
+    Against torch's reverse mode."""
+    print(f"adjoint wide windows precision={precision} {nx}x{ny}", flush=True)
+    case = make_case(nx=nx, ny=ny, ppc=6, seed=4, efield=0.0, bfield=0.0, vth=(0.02, 0.1),
+                     shuffle=True)
+    L, gF, gP = _ref_grads(case, 0.01, 0.05, 10, 5, 7)
+    Lg, gFg, gPg = _gpu_grads(case, 0.01, 0.05, 10, 4, 5, 7, precision)
+    tol_l, tol = (1e-10, 1e-8) if precision == 0 else (1e-4, 1e-3)
+    assert abs(Lg - L) <= tol_l * abs(L)
+    for k in range(6):
+        assert rel_norm(gFg[k], gF[k]) < tol, k
+    for s in range(2):
+        for k in PART:
+            assert rel_norm(gPg[s][k], gP[s][k]) < tol, (s, k)
