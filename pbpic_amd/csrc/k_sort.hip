@@ -829,6 +829,11 @@ static hipError_t box_records(SortScratch &s, const SpeciesDev &sp, const CellKe
     uint16_t *local = reinterpret_cast<uint16_t *>(s.keys[0]);
     FastDiv dny;
     dny.init((uint32_t)ny);
+    // up to 64 KiB of per-wave counts (BX_MAX cells x 4 waves x 4 bytes) beside
+    // the static scan array: above the default dynamic LDS limit for the
+    // largest boxes
+    (void)hipFuncSetAttribute((const void *)tile_order_kernel,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, amax * 16);
     hipLaunchKernelGGL(tile_order_kernel, dim3((unsigned)nt), dim3(256), (size_t)amax * 16, st,
                        sp.cell, n, nx, ny, dny, (const int32_t *)sp.tbox, s.toff, loff, local, err);
     if ((e = hipGetLastError()) != hipSuccess) return e;
