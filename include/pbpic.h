@@ -78,15 +78,16 @@ extern "C" {
                                         (pic2d.f90:1283-1295)               */
 #define PBPIC_FLAG_TRACK_PID    0x2  /* keep a per-particle id array that the
                                         sort permutes (ptl%pid, :47-50)     */
-#define PBPIC_FLAG_PUSH_ORDER   0x4  /* sorts lay every full 4096-particle
-                                        tile out in the push kernel's lane
-                                        order: the tile holds the same
-                                        particles as the reference's cell
-                                        order (:1322-1327), permuted inside
-                                        the tile; the returned permutation
-                                        and counts describe that order.
+#define PBPIC_FLAG_PUSH_ORDER   0x4  /* sorts order the cells by 8 x 8
+                                        supertiles along a Hilbert curve
+                                        (cells x-major inside each) and lay
+                                        every full 4096-particle tile out in
+                                        the push kernel's lane order; the
+                                        returned permutation describes that
+                                        order, the counts stay per cell.
                                         Clear it for the reference's exact
-                                        order.  Set by pbpic_params_default */
+                                        order (:1322-1327).  Set by
+                                        pbpic_params_default               */
 #define PBPIC_FLAG_MARDER       0x8  /* divergenceclean after each E update
                                         (pic2d.f90:867-885, commented out at
                                         :754): E += d grad(div E^n -

@@ -300,7 +300,7 @@ int sort_one(pbpic_ctx *c, int s, int32_t *counts, int32_t *perm, uint32_t *perm
     HIPCHK(c, prof_mark(c, PBPIC_PROF_SORT, true));
     const bool po = (c->p.flags & PBPIC_FLAG_PUSH_ORDER) != 0;
     CellKey key;
-    key.init(c->p.nx, c->p.ny, po ? 1 : 0);
+    key.init(c->p.nx, c->p.ny, po ? 1 : 0, c->sort.strank, c->sort.strank_h.data());
     bool boxed = false;
     HIPCHK(c, sort_species(c->sort, src, c->p.precision, po ? 1 : 0, key, ncell(c), dst, c->err,
                            counts != nullptr, perm != nullptr || perm_dev != nullptr, S.rec,
@@ -1011,6 +1011,14 @@ int pbpic_ctx_create(const pbpic_params *pp, pbpic_ctx **out)
         key.init(p.nx, p.ny, 1);
         chk(dalloc(&c->sort.counts, (size_t)std::max<int64_t>(key.nkeys, nc) * sizeof(int32_t)),
             "alloc counts");
+        // the push order's supertile ranks (CellKey), host and device copies
+        const int nbx = (p.nx + 7) / 8, nby = (p.ny + 7) / 8;
+        supertile_ranks(nbx, nby, c->sort.strank_h);
+        chk(dalloc(&c->sort.strank, c->sort.strank_h.size() * sizeof(uint32_t)), "alloc ranks");
+        if (rc == PBPIC_OK)
+            chk(hipMemcpy(c->sort.strank, c->sort.strank_h.data(),
+                          c->sort.strank_h.size() * sizeof(uint32_t), hipMemcpyHostToDevice),
+                "copy ranks");
     }
     chk(dalloc(&c->sort.dhist, 4 * 256 * sizeof(uint32_t)), "alloc dhist");
     chk(dalloc(&c->sort.ticket, 8 * sizeof(uint32_t)), "alloc ticket");
@@ -1054,6 +1062,7 @@ int pbpic_ctx_destroy(pbpic_ctx *c)
     for (int b = 0; b < 2; b++) { f(c->sort.keys[b]); f(c->sort.vals[b]); }
     f(c->sort.status); f(c->sort.dhist); f(c->sort.ticket); f(c->sort.counts);
     f(c->sort.tarea); f(c->sort.toff); f(c->sort.bsum); f(c->sort.ctl); f(c->sort.rbuf);
+    f(c->sort.strank);
     for (auto p : c->adj.ckpts) f(p);
     f(c->adj.final_state); f(c->adj.jstore); f(c->adj.tape);
     for (auto e : c->ev_pool) (void)hipEventDestroy(e);

@@ -12,6 +12,8 @@
 // moves the particle payload once.  The result (permutation and per-cell
 // counts) is identical to the reference's: both are the unique stable sort
 // by cell.
+#include <cstdlib>
+
 #include "pbpic_impl.h"
 
 namespace pbpic {
@@ -740,6 +742,54 @@ __global__ __launch_bounds__(256) void expand_kernel(BoxRecs B, int64_t n, uint3
         const uint32_t l = (uint32_t)__shfl(lb, lo), t = (uint32_t)__shfl(tb, lo);
         if (d < D1) perm[d] = t + B.local[(uint32_t)(l + (uint32_t)d)];
     }
+}
+
+namespace {
+int sgn_i(int v) { return (v > 0) - (v < 0); }
+int floor_half(int v) { return v >= 0 ? v / 2 : -((-v + 1) / 2); }
+
+// One rectangle of the generalized Hilbert curve: origin (x, y), major axis
+// (ax, ay), minor axis (bx, by); cells are numbered from *next on.  A long
+// rectangle (major more than 1.5 x minor) is cut in two along its major axis;
+// otherwise it is walked as up / across / down, the Hilbert curve's three
+// parts, with the cuts moved by one cell where a half would be odd so that
+// the pieces chain by edge steps.
+void gilbert_walk(int x, int y, int ax, int ay, int bx, int by, int h,
+                  std::vector<uint32_t> &rank, uint32_t &next)
+{
+    const int w = std::abs(ax + ay), hh = std::abs(bx + by);
+    const int dax = sgn_i(ax), day = sgn_i(ay), dbx = sgn_i(bx), dby = sgn_i(by);
+    if (hh == 1) {
+        for (int i = 0; i < w; i++, x += dax, y += day) rank[(size_t)x * h + y] = next++;
+        return;
+    }
+    if (w == 1) {
+        for (int i = 0; i < hh; i++, x += dbx, y += dby) rank[(size_t)x * h + y] = next++;
+        return;
+    }
+    int ax2 = floor_half(ax), ay2 = floor_half(ay), bx2 = floor_half(bx), by2 = floor_half(by);
+    const int w2 = std::abs(ax2 + ay2), h2 = std::abs(bx2 + by2);
+    if (2 * w > 3 * hh) {
+        if ((w2 & 1) && w > 2) { ax2 += dax; ay2 += day; }
+        gilbert_walk(x, y, ax2, ay2, bx, by, h, rank, next);
+        gilbert_walk(x + ax2, y + ay2, ax - ax2, ay - ay2, bx, by, h, rank, next);
+    } else {
+        if ((h2 & 1) && hh > 2) { bx2 += dbx; by2 += dby; }
+        gilbert_walk(x, y, bx2, by2, ax2, ay2, h, rank, next);
+        gilbert_walk(x + bx2, y + by2, ax, ay, bx - bx2, by - by2, h, rank, next);
+        gilbert_walk(x + (ax - dax) + (bx2 - dbx), y + (ay - day) + (by2 - dby), -bx2, -by2,
+                     -(ax - ax2), -(ay - ay2), h, rank, next);
+    }
+}
+}  // namespace
+
+void supertile_ranks(int w, int h, std::vector<uint32_t> &rank)
+{
+    rank.assign((size_t)w * h, 0u);
+    uint32_t next = 0;
+    if (w <= 0 || h <= 0) return;
+    if (w >= h) gilbert_walk(0, 0, w, 0, 0, h, h, rank, next);
+    else gilbert_walk(0, 0, 0, h, w, 0, h, rank, next);
 }
 
 // The tile-box path of sort_species up to the records in key order (*ok), or

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <vector>
 
 #include "../../include/pbpic.h"
 
@@ -334,6 +335,9 @@ struct SortScratch {
     int *ctl = nullptr;
     uint32_t *rbuf = nullptr;
     int64_t rcap = 0;
+    // push-order supertile ranks (CellKey), device and host copies
+    uint32_t *strank = nullptr;
+    std::vector<uint32_t> strank_h;
 };
 hipError_t launch_push_adj(const AdjParams &p, int precision, hipStream_t st);
 int adj_jmax_for(int precision, int maxw, size_t budget);
@@ -348,30 +352,52 @@ hipError_t launch_unpermute(const uint32_t *perm, int64_t n, int precision,
                             void *const *post, void *const *pre, hipStream_t st);
 
 // Sort key of a cell.  Reference order: the compact cell id cx*ny + cy.
-// Push order: cells grouped in 8 x 8 supertiles, the supertiles x-major
-// (sx*nby + sy) and the cells x-major inside each, so that a run of 4096
-// particles (~41 cells at 100 per cell) covers a compact ~6 x 8 block
-// instead of a 41-cell column strip.
+// Push order: cells grouped in 8 x 8 supertiles, the supertiles in the order
+// of a generalized Hilbert curve over the nbx x nby supertile grid
+// (supertile_ranks) and the cells x-major inside each.  A run of 4096
+// particles (~41 cells at 100 per cell) covers a compact ~6 x 8 block; each
+// step of the curve goes to an edge neighbour, so a run that straddles two
+// supertiles stays compact too; and a supertile's neighbours in x and in y
+// mostly lie a few supertiles away in key order, so the sort's gather, whose
+// sources are the destinations' neighbours of a cycle before, finds them
+// still in L2.  The rank table (nbx * nby entries, index sx * nby + sy) has a
+// device copy for kernels and a host copy for host code.
 struct CellKey {
     FastDiv dny;
     uint32_t ny, nby;
     int tiled;
     int64_t nkeys;            // keys lie in [0, nkeys)
-    void init(int nx, int ny_, int tiled_)
+    const uint32_t *rank = nullptr;     // device table (tiled keys in kernels)
+    const uint32_t *rank_h = nullptr;   // host table (tiled keys in host code)
+    void init(int nx, int ny_, int tiled_, const uint32_t *rank_dev = nullptr,
+              const uint32_t *rank_host = nullptr)
     {
         dny.init((uint32_t)ny_);
         ny = (uint32_t)ny_;
         nby = (uint32_t)((ny_ + 7) / 8);
         tiled = tiled_;
         nkeys = tiled ? (int64_t)((nx + 7) / 8) * nby * 64 : (int64_t)nx * ny_;
+        rank = rank_dev;
+        rank_h = rank_host;
     }
     __host__ __device__ uint32_t operator()(uint32_t c) const
     {
         if (!tiled) return c;
         const uint32_t cx = dny.div31(c), cy = c - cx * ny;
-        return (((cx >> 3) * nby + (cy >> 3)) << 6) | ((cx & 7u) << 3) | (cy & 7u);
+        const uint32_t s = (cx >> 3) * nby + (cy >> 3);
+#if defined(__HIP_DEVICE_COMPILE__)
+        const uint32_t st = rank[s];
+#else
+        const uint32_t st = rank_h[s];
+#endif
+        return (st << 6) | ((cx & 7u) << 3) | (cy & 7u);
     }
 };
+// Position along the generalized Hilbert curve of every cell of a w x h grid,
+// rank[x * h + y] (J. Cerveny's "gilbert" construction for arbitrary
+// rectangles: every step goes to an edge neighbour, with at most one diagonal
+// step when the dimensions force it).
+void supertile_ranks(int w, int h, std::vector<uint32_t> &rank);
 
 // One particle of the split form as the sort's gather reads it: 24 bytes,
 // record i holding slot i.

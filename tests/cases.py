@@ -85,3 +85,62 @@ def rel_norm(a, b):
     b = np.asarray(b, np.float64)
     den = np.linalg.norm(b)
     return np.linalg.norm(a - b) / (den if den > 0 else 1.0)
+
+
+def gilbert_order(w, h):
+    """Cells (x, y) of a w x h grid in the order of the generalized Hilbert
+    curve (J. Cerveny's "gilbert" construction): a rectangle more than 1.5
+    times as long as it is wide is cut in two along its length, otherwise
+    walked up / across / down, with cuts moved by one where a half would be
+    odd."""
+    out = []
+
+    def sgn(v):
+        return (v > 0) - (v < 0)
+
+    def walk(x, y, ax, ay, bx, by):
+        w_, h_ = abs(ax + ay), abs(bx + by)
+        dax, day, dbx, dby = sgn(ax), sgn(ay), sgn(bx), sgn(by)
+        if h_ == 1:
+            for _ in range(w_):
+                out.append((x, y))
+                x, y = x + dax, y + day
+            return
+        if w_ == 1:
+            for _ in range(h_):
+                out.append((x, y))
+                x, y = x + dbx, y + dby
+            return
+        ax2, ay2, bx2, by2 = ax // 2, ay // 2, bx // 2, by // 2
+        w2, h2 = abs(ax2 + ay2), abs(bx2 + by2)
+        if 2 * w_ > 3 * h_:
+            if w2 % 2 and w_ > 2:
+                ax2, ay2 = ax2 + dax, ay2 + day
+            walk(x, y, ax2, ay2, bx, by)
+            walk(x + ax2, y + ay2, ax - ax2, ay - ay2, bx, by)
+        else:
+            if h2 % 2 and h_ > 2:
+                bx2, by2 = bx2 + dbx, by2 + dby
+            walk(x, y, bx2, by2, ax2, ay2)
+            walk(x + bx2, y + by2, ax, ay, bx - bx2, by - by2)
+            walk(x + (ax - dax) + (bx2 - dbx), y + (ay - day) + (by2 - dby),
+                 -bx2, -by2, -(ax - ax2), -(ay - ay2))
+
+    if w >= h:
+        walk(0, 0, w, 0, 0, h)
+    else:
+        walk(0, 0, 0, h, w, 0)
+    return out
+
+
+def supertile_key(cell, nx, ny):
+    """Push-order key of a 0-based cell id: 8 x 8 supertiles ranked along the
+    generalized Hilbert curve over the supertile grid, cells x-major inside
+    a supertile."""
+    cell = np.asarray(cell, dtype=np.int64)
+    cx, cy = cell // ny, cell % ny
+    nbx, nby = (nx + 7) // 8, (ny + 7) // 8
+    rank = np.zeros(nbx * nby, dtype=np.int64)
+    for r, (x, y) in enumerate(gilbert_order(nbx, nby)):
+        rank[x * nby + y] = r
+    return rank[(cx >> 3) * nby + (cy >> 3)] * 64 + (cx & 7) * 8 + (cy & 7)

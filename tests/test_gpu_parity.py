@@ -10,7 +10,8 @@ import numpy as np
 import pytest
 
 import oracle
-from cases import SPECIES_Q, gpu_for, make_case, oracle_for, oracle_species, rel_norm
+from cases import (SPECIES_Q, gpu_for, make_case, oracle_for, oracle_species, rel_norm,
+                   supertile_key)
 
 pytestmark = pytest.mark.gpu
 
@@ -68,14 +69,6 @@ def test_countingsort_bitexact(gpu, kind, nx, ny):
     assert np.array_equal(g["ux"], s["ux"][perm])
 
 
-def _supertile_key(cell, nx, ny):
-    """Push-order key of a 0-based cell id: 8 x 8 supertiles x-major, cells
-    x-major inside a supertile."""
-    cx, cy = cell // ny, cell % ny
-    nby = (ny + 7) // 8
-    return ((cx >> 3) * nby + (cy >> 3)) * 64 + (cx & 7) * 8 + (cy & 7)
-
-
 def _push_order(ref_perm, V):
     """The push order of a cell-order permutation: inside each full tile of
     4096 slots, slot q = c*64V + l*V + j (round c, lane l, element j) holds
@@ -91,13 +84,13 @@ def _push_order(ref_perm, V):
 
 @pytest.mark.parametrize("precision", [0, 1])
 @pytest.mark.parametrize("n", [4095, 4096, 20000, 3 * 4096 + 17])
-def test_countingsort_push_order(gpu, precision, n):
+@pytest.mark.parametrize("nx,ny", [(40, 37), (40, 130), (200, 20)])
+def test_countingsort_push_order(gpu, precision, n, nx, ny):
     """With the push order (the default) the particles are sorted stably by
     supertile key (8 x 8 cell blocks), every full 4096-particle tile is laid
     out so that the 64 lanes of a wave-wide load hold particles 64 ranks
     apart, and the per-cell counts are the reference's."""
-    _say(f"push-order sort precision={precision} n={n}")
-    nx, ny = 40, 37
+    _say(f"push-order sort precision={precision} n={n} nx={nx} ny={ny}")
     rng = np.random.default_rng(5 + n)
     grid = rng.integers(1, nx * ny + 1, n).astype(np.int32)
     ux = rng.standard_normal(n)
@@ -109,7 +102,7 @@ def test_countingsort_push_order(gpu, precision, n):
     z = np.zeros(n)
     ctx.set_particles(k, grid, z, z, ux, z, z, pid)
     counts, perm = ctx.countingsort(k, want=True)
-    ref = np.argsort(_supertile_key(grid - 1, nx, ny), kind="stable")
+    ref = np.argsort(supertile_key(grid - 1, nx, ny), kind="stable")
     assert np.array_equal(counts, np.bincount(grid - 1, minlength=nx * ny))
     for t0 in range(0, n, 4096):
         assert np.array_equal(np.sort(perm[t0:t0 + 4096]), np.sort(ref[t0:t0 + 4096]))
@@ -148,7 +141,7 @@ def test_countingsort_after_push_bitexact(gpu, precision, push_order, nx, ny, pp
         perm = ctx.countingsort(k, want="perm")
         after = ctx.get_particles(k, native=True)
         c = before["grid"] - 1
-        key = _supertile_key(c, nx, ny) if push_order else c
+        key = supertile_key(c, nx, ny) if push_order else c
         ref = np.argsort(key, kind="stable")
         if push_order:
             ref = _push_order(ref, V)
