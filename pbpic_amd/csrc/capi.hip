@@ -456,14 +456,19 @@ int step_impl(pbpic_ctx *c, void *const *jcopy, uint32_t *const *perm_dev, bool 
                       (c->steps + 1) % c->p.sort_every == 0;
     int rc = push_impl(c, recs, jsrc != nullptr);
     if (rc) return rc;
-    if (jsrc)
-        for (int k = 0; k < 3; k++)
-            HIPCHK(c, hipMemcpyAsync(c->J[k], jsrc[k], (size_t)c->rsz * ncell(c),
-                                     hipMemcpyDeviceToDevice, c->st));
-    if (jcopy)
-        for (int k = 0; k < 3; k++)
-            HIPCHK(c, hipMemcpyAsync(jcopy[k], c->J[k], (size_t)c->rsz * ncell(c),
-                                     hipMemcpyDeviceToDevice, c->st));
+    if (jsrc || jcopy) {
+        CopyList cl{};
+        for (int k = 0; k < 3; k++) {
+            if (jsrc) cl.add(c->J[k], jsrc[k], (size_t)c->rsz * ncell(c));
+            else cl.add(jcopy[k], c->J[k], (size_t)c->rsz * ncell(c));
+        }
+        HIPCHK(c, launch_copy_list(cl, c->st));
+    }
+    if (jsrc && jcopy) {
+        CopyList cl{};
+        for (int k = 0; k < 3; k++) cl.add(jcopy[k], c->J[k], (size_t)c->rsz * ncell(c));
+        HIPCHK(c, launch_copy_list(cl, c->st));
+    }
     if ((rc = fieldsolve_impl(c))) return rc;
     if (sorted) *sorted = false;
     if (c->p.sort_every > 0 && c->steps % c->p.sort_every == 0) {
@@ -524,19 +529,24 @@ int state_copy(pbpic_ctx *c, const StateLayout &L, char *buf, bool save, int wha
         const int b = S.cur;
         if (!save) S.tbox_ok = false;
         void *dev[7] = {S.cell[b], S.ox[b], S.oy[b], S.ux[b], S.uy[b], S.uz[b], S.pid[b]};
+        CopyList cl{};
         for (int part = 0; part < 7; part++) {
             if (!dev[part] || S.n == 0) continue;
             const size_t bytes = (part == 0 || part == 6 ? 4 : (size_t)c->rsz) * (size_t)S.n;
             void *st = state_ptr(c, L, buf, s, part);
-            HIPCHK(c, hipMemcpyAsync(save ? st : dev[part], save ? dev[part] : st, bytes,
-                                     hipMemcpyDeviceToDevice, c->st));
+            if (save) cl.add(st, dev[part], bytes);
+            else cl.add(dev[part], st, bytes);
         }
+        HIPCHK(c, launch_copy_list(cl, c->st));
     }
+    CopyList cl{};
     for (int f = 0; f < 6 && (what & STATE_FIELDS); f++) {
         double *dev = f < 3 ? c->E[f] : c->B[f - 3];
         double *st = state_field(c, L, buf, f);
-        HIPCHK(c, hipMemcpyAsync(save ? st : dev, save ? dev : st, nc * 8, hipMemcpyDeviceToDevice, c->st));
+        if (save) cl.add(st, dev, nc * 8);
+        else cl.add(dev, st, nc * 8);
     }
+    HIPCHK(c, launch_copy_list(cl, c->st));
     return PBPIC_OK;
 }
 
@@ -1771,12 +1781,20 @@ int pbpic_adjoint_backward(pbpic_ctx *c, double *const *fg, void *const *pg)
             guard.restore();
             for (int k = kk - 1; k >= 0; k--) {
                 if (sorted[k]) {
+                    const int po = (c->p.flags & PBPIC_FLAG_PUSH_ORDER) != 0;
                     for (int s = 0; s < ns; s++) {
                         HIPCHK(c, launch_unpermute(tperm[pslot[k]][s], c->sp[s].n, c->p.precision,
-                                                   pa[s].data(), tmp[0].data(), c->st));
-                        for (int q = 0; q < 5; q++)
-                            HIPCHK(c, hipMemcpyAsync(pa[s][q], tmp[0][q], rz * c->sp[s].n,
-                                                     hipMemcpyDeviceToDevice, c->st));
+                                                   po, pa[s].data(), tmp[0].data(), c->st));
+                        // the unpermuted arrays become the species' adjoint
+                        // arrays; a spare of the largest species' size stays
+                        // for the next species
+                        if (c->sp[s].n == nmax) {
+                            std::swap(pa[s], tmp[0]);
+                        } else {
+                            CopyList cl{};
+                            for (int q = 0; q < 5; q++) cl.add(pa[s][q], tmp[0][q], rz * c->sp[s].n);
+                            HIPCHK(c, launch_copy_list(cl, c->st));
+                        }
                     }
                 }
                 AdjFieldParams F{};

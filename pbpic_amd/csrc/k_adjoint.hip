@@ -675,11 +675,25 @@ struct Ptr5 {
     void *p[5];
 };
 
+// Lanes walk the sorted ranks, not the slots: in the push order (tv > 0)
+// consecutive slots of a full tile hold ranks 64 apart, in different cells,
+// while consecutive ranks of a cell came from nearby sources, so taking the
+// ranks in order makes neighbouring lanes' scattered stores land close
+// together.
 template <typename R>
-__global__ void unpermute_kernel(const uint32_t *perm, int64_t n, Ptr5 post, Ptr5 pre)
+__global__ void unpermute_kernel(const uint32_t *perm, int64_t n, int tv, Ptr5 post, Ptr5 pre)
 {
-    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
+    const int64_t r = (int64_t)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    int64_t k = r;
+    if (tv) {
+        const int64_t t0 = r & ~(int64_t)(PUSH_TILE - 1);
+        if (t0 + PUSH_TILE <= n) {
+            // the slot holding rank q of the tile (tile_rank's inverse)
+            const int q = (int)(r - t0), l = q >> 6, c = (q & 63) / tv, j = q % tv;
+            k = t0 + (int64_t)c * 64 * tv + l * tv + j;
+        }
+    }
     const uint32_t p = perm[k];
 #pragma unroll
     for (int q = 0; q < 5; q++) ((R *)pre.p[q])[p] = ((const R *)post.p[q])[k];
@@ -789,17 +803,19 @@ hipError_t launch_node_transpose(const AdjFieldParams &p, double *nadj, hipStrea
     return hipGetLastError();
 }
 
-hipError_t launch_unpermute(const uint32_t *perm, int64_t n, int precision, void *const *post,
-                            void *const *pre, hipStream_t st)
+hipError_t launch_unpermute(const uint32_t *perm, int64_t n, int precision, int push_order,
+                            void *const *post, void *const *pre, hipStream_t st)
 {
     if (n <= 0) return hipSuccess;
     const unsigned nb = (unsigned)((n + 255) / 256);
+    // the sort's layout: full tiles in the push kernel's lane order
+    const int tv = push_order && n >= PUSH_TILE ? push_vec(precision) : 0;
     Ptr5 a, b;
     for (int q = 0; q < 5; q++) { a.p[q] = post[q]; b.p[q] = pre[q]; }
     if (precision == PBPIC_PREC_FP64)
-        hipLaunchKernelGGL(unpermute_kernel<double>, dim3(nb), dim3(256), 0, st, perm, n, a, b);
+        hipLaunchKernelGGL(unpermute_kernel<double>, dim3(nb), dim3(256), 0, st, perm, n, tv, a, b);
     else
-        hipLaunchKernelGGL(unpermute_kernel<float>, dim3(nb), dim3(256), 0, st, perm, n, a, b);
+        hipLaunchKernelGGL(unpermute_kernel<float>, dim3(nb), dim3(256), 0, st, perm, n, tv, a, b);
     return hipGetLastError();
 }
 

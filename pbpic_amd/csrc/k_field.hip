@@ -675,6 +675,43 @@ hipError_t launch_fill_u32(uint32_t *p, int64_t n, uint32_t v, hipStream_t st)
     return hipGetLastError();
 }
 
+// Several device-to-device copies in one launch (field planes, current
+// planes, particle arrays of a state): each workgroup moves 16-byte chunks
+// of one copy, the copies laid end to end over the grid.
+__global__ __launch_bounds__(256) void copy_list_kernel(CopyList L)
+{
+    const int64_t g = (int64_t)xcd_remap(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
+    int k = 0;
+    while (k + 1 < L.n && g >= L.first[k + 1]) k++;
+    const int64_t i = g - L.first[k];
+    if (i < (int64_t)(L.bytes[k] >> 4))
+        reinterpret_cast<uint4 *>(L.dst[k])[i] = reinterpret_cast<const uint4 *>(L.src[k])[i];
+}
+
+hipError_t launch_copy_list(CopyList L, hipStream_t st)
+{
+    // chunk offsets of the copies that are whole 16-byte chunks on 16-byte
+    // boundaries; the others go through hipMemcpyAsync
+    CopyList K{};
+    int64_t tot = 0;
+    for (int k = 0; k < L.n; k++) {
+        if (L.bytes[k] == 0) continue;
+        const bool ok = L.bytes[k] % 16 == 0 && ((uintptr_t)L.src[k] | (uintptr_t)L.dst[k]) % 16 == 0;
+        if (!ok) {
+            hipError_t e = hipMemcpyAsync(L.dst[k], L.src[k], L.bytes[k], hipMemcpyDeviceToDevice, st);
+            if (e != hipSuccess) return e;
+            continue;
+        }
+        K.src[K.n] = L.src[k]; K.dst[K.n] = L.dst[k]; K.bytes[K.n] = L.bytes[k];
+        K.first[K.n] = tot;
+        tot += (int64_t)((L.bytes[k] >> 4) + 255) / 256 * 256;   // whole workgroups per copy
+        K.n++;
+    }
+    if (K.n == 0) return hipSuccess;
+    hipLaunchKernelGGL(copy_list_kernel, dim3((unsigned)(tot / 256)), dim3(256), 0, st, K);
+    return hipGetLastError();
+}
+
 // Split form: the FP32 current planes from the fixed-point sums the push
 // accumulated (one rounding of an exact sum, the same for any order of the
 // contributions).

@@ -278,6 +278,18 @@ hipError_t launch_gauss(const PushParams &p, int precision, double rho_scale, do
                         double *gauss0, int first, double rho_bg,
                         double cva2in0, hipStream_t st);
 hipError_t launch_fill_u32(uint32_t *p, int64_t n, uint32_t v, hipStream_t st);
+// Up to COPY_MAX device-to-device copies in one launch (copies that are not
+// whole 16-byte chunks on 16-byte boundaries fall back to hipMemcpyAsync).
+constexpr int COPY_MAX = 8;
+struct CopyList {
+    const void *src[COPY_MAX];
+    void *dst[COPY_MAX];
+    size_t bytes[COPY_MAX];
+    int64_t first[COPY_MAX];    // first global thread of each copy (set by the launcher)
+    int n;
+    void add(void *d, const void *s, size_t b) { src[n] = s; dst[n] = d; bytes[n] = b; n++; }
+};
+hipError_t launch_copy_list(CopyList L, hipStream_t st);
 // Conducting y walls: folds the Jz deposited in the guard rows back into the
 // domain (image sign -1) and clears them; sets the guard rows of the field
 // planes selected by mask (bit k: Ex, Ey, Ez, Bx, By, Bz) to their images.
@@ -348,7 +360,9 @@ hipError_t launch_fdtd_adj(const AdjFieldParams &p, int precision, hipStream_t s
 // adjoints (exa..bza) in place.
 hipError_t launch_wall_images_adj(const AdjFieldParams &p, hipStream_t st);
 hipError_t launch_node_transpose(const AdjFieldParams &p, double *nadj, hipStream_t st);
-hipError_t launch_unpermute(const uint32_t *perm, int64_t n, int precision,
+// Reverse of a sort for the five adjoint arrays: pre[perm[k]] = post[k];
+// push_order: the sort laid full tiles out in the push kernel's lane order.
+hipError_t launch_unpermute(const uint32_t *perm, int64_t n, int precision, int push_order,
                             void *const *post, void *const *pre, hipStream_t st);
 
 // Sort key of a cell.  Reference order: the compact cell id cx*ny + cy.
