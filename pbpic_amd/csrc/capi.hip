@@ -1830,6 +1830,17 @@ int pbpic_adjoint_backward(pbpic_ctx *c, double *const *fg, void *const *pg)
                 A.by = state_field(c, L, tstate[k], 4); A.bz = state_field(c, L, tstate[k], 5);
                 A.jadj[0] = fa[9]; A.jadj[1] = fa[10]; A.jadj[2] = fa[11];
                 A.nadj = fa[12];
+                {
+                    // node fields of step k for the adjoint push's windows
+                    PushParams Pn{};
+                    Pn.nx = c->p.nx; Pn.ny = c->p.ny;
+                    Pn.ex = A.ex; Pn.ey = A.ey; Pn.ez = A.ez;
+                    Pn.bx = A.bx; Pn.by = A.by; Pn.bz = A.bz;
+                    Pn.nf = c->NF;
+                    Pn.big = nullptr;
+                    HIPCHK(c, launch_nodefield(Pn, c->p.precision, c->st));
+                    A.nf = c->NF;
+                }
                 A.big = adj_big;
                 for (int s = 0; s < ns; s++) {
                     Species &S = c->sp[s];

@@ -202,12 +202,22 @@ __global__ __launch_bounds__(ADJ_THREADS, 2) void push_adj_kernel(AdjParams P)
     R *jw = (R *)(na + (size_t)6 * P.maxw);
     const int jmax = P.jmax;
     if (inlds) {
+        // node fields from the node-field array (y fastest: lanes walk a
+        // window column, reading consecutive node records); the same values
+        // as a_node_fields rounded to R
         for (int k = tid; k < WH; k += ADJ_THREADS) {
-            const int wj = k / W, wi = k - wj * W;
-            double f[6];
-            a_node_fields(P, wrap_index(x0 + wx0 + wi, nx), wrap_index(y0 + wy0 + wj, ny), f);
-#pragma unroll
-            for (int q = 0; q < 6; q++) nf[(size_t)k * 6 + q] = (R)f[q];
+            const int wi = k / H, wj = k - wi * H;
+            const size_t g = (size_t)wrap_index(x0 + wx0 + wi, nx) * ny + wrap_index(y0 + wy0 + wj, ny);
+            R *o = nf + (size_t)(wj * W + wi) * 6;
+            if constexpr (sizeof(R) == 4) {
+                const float4 *src = reinterpret_cast<const float4 *>(P.nf) + 2 * g;
+                const float4 a = src[0], b = src[1];
+                o[0] = a.x; o[1] = a.y; o[2] = a.z; o[3] = a.w; o[4] = b.x; o[5] = b.y;
+            } else {
+                const double2 *src = reinterpret_cast<const double2 *>(P.nf) + 3 * g;
+                const double2 a = src[0], b = src[1], cc = src[2];
+                o[0] = a.x; o[1] = a.y; o[2] = b.x; o[3] = b.y; o[4] = cc.x; o[5] = cc.y;
+            }
         }
         for (int k = tid; k < 6 * WH; k += ADJ_THREADS) na[k] = 0.0;
         for (int k = tid; k < JWH; k += ADJ_THREADS) {

@@ -232,6 +232,10 @@ struct AdjParams {
     int jmax;                                    // current-adjoint window (nodes)
     double dtcva, idx, idy, cva;
     const double *ex, *ey, *ez, *bx, *by, *bz;   // fields at step n
+    // node fields of step n (launch_nodefield's array: per node (cx*ny + cy)
+    // 8 floats (split) or 6 doubles (FP64), the first six E, B averaged as
+    // from the planes above), read by the LDS window staging
+    const void *nf;
     const double *jadj[3];                       // current adjoint (raw J)
     double *nadj;                                // 6 node-adjoint planes
     void *xa[MAX_SPECIES], *ya[MAX_SPECIES], *uxa[MAX_SPECIES], *uya[MAX_SPECIES],
