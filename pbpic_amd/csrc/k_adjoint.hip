@@ -33,6 +33,8 @@ namespace {
 constexpr int ADJ_THREADS = 256;
 constexpr int ADJ_ROUNDS = 8;
 constexpr int ADJ_TILE = ADJ_THREADS * ADJ_ROUNDS;      // 2048, half a push tile
+// the second launch (one workgroup per CU) runs eight waves of four ranks each
+constexpr int ADJ_THREADS_BIG = 2 * ADJ_THREADS;
 
 // LDS bytes of the node-field window, rounded up to 8-byte alignment.
 __host__ __device__ constexpr size_t adj_nf_bytes(size_t r, int maxw)
@@ -101,11 +103,12 @@ __device__ __forceinline__ void zig_axis(int a, R o1, R o2, R qf, R &f1, R &f2, 
 // One step of the particle adjoint.  State n of the particle (cell, offsets,
 // momenta) is read from the tape; the adjoint of the state at n+1 is read
 // from, and the adjoint at n written to, the xa..ua arrays (in place).
-template <typename R, bool SPLIT>
-__global__ __launch_bounds__(ADJ_THREADS, 2) void push_adj_kernel(AdjParams P)
+template <typename R, bool SPLIT, int NT>
+__global__ __launch_bounds__(NT, NT == ADJ_THREADS ? 2 : 1) void push_adj_kernel(AdjParams P)
 {
+    constexpr int NW = NT / 64, ROUNDS = ADJ_TILE / NT;   // waves, ranks per thread
     extern __shared__ __align__(16) unsigned char smem[];
-    __shared__ int red[4][4];
+    __shared__ int red[NW][4];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     int t;
     if (P.pass == 2) {
@@ -131,12 +134,12 @@ __global__ __launch_bounds__(ADJ_THREADS, 2) void push_adj_kernel(AdjParams P)
     // the push's vector width of this precision (P.tv is it or 0): a
     // constant, so the rank -> slot map compiles to shifts
     constexpr int TV = sizeof(R) == 8 ? push_vec(PBPIC_PREC_FP64) : push_vec(PBPIC_PREC_SPLIT32);
-    static_assert(ADJ_TILE == 64 * 32 && ADJ_THREADS == 64 * 4 && ADJ_ROUNDS == 8,
-                  "lane/wave rank map assumes 4 waves x 8 ranks per 32-rank row");
+    static_assert(ADJ_TILE == 64 * 32 && NW * ROUNDS == 32,
+                  "lane/wave rank map: the waves' ranks fill each 32-rank row");
     auto slot_of = [&](int r) -> int64_t {
-        if (!porder) return tstart + (int64_t)lane * 32 + wave * ADJ_ROUNDS + r;
+        if (!porder) return tstart + (int64_t)lane * 32 + wave * ROUNDS + r;
         const int k = (int)((tstart - T0) / ADJ_TILE);
-        const int rank = lane * 64 + 32 * k + wave * ADJ_ROUNDS + r;
+        const int rank = lane * 64 + 32 * k + wave * ROUNDS + r;
         const int l = rank >> 6, c = (rank & 63) / TV, j = rank % TV;
         return T0 + (int64_t)c * 64 * TV + l * TV + j;
     };
@@ -146,10 +149,10 @@ __global__ __launch_bounds__(ADJ_THREADS, 2) void push_adj_kernel(AdjParams P)
     const bool c00ok = (unsigned)c00 < (unsigned)(nx * ny);
     const int x0 = c00ok ? (int)P.dny.div31((uint32_t)c00) : 0;
     const int y0 = c00ok ? c00 - x0 * ny : 0;
-    int cl[ADJ_ROUNDS];
+    int cl[ROUNDS];
     int mnx = 1 << 30, mxx = -(1 << 30), mny = 1 << 30, mxy = -(1 << 30);
 #pragma unroll
-    for (int r = 0; r < ADJ_ROUNDS; r++) {
+    for (int r = 0; r < ROUNDS; r++) {
         const int64_t i = slot_of(r);
         cl[r] = -1;
         if (i < S.n) {
@@ -173,10 +176,12 @@ __global__ __launch_bounds__(ADJ_THREADS, 2) void push_adj_kernel(AdjParams P)
     }
     if (lane == 0) { red[wave][0] = mnx; red[wave][1] = mxx; red[wave][2] = mny; red[wave][3] = mxy; }
     __syncthreads();
-    mnx = min(min(red[0][0], red[1][0]), min(red[2][0], red[3][0]));
-    mxx = max(max(red[0][1], red[1][1]), max(red[2][1], red[3][1]));
-    mny = min(min(red[0][2], red[1][2]), min(red[2][2], red[3][2]));
-    mxy = max(max(red[0][3], red[1][3]), max(red[2][3], red[3][3]));
+    mnx = red[0][0]; mxx = red[0][1]; mny = red[0][2]; mxy = red[0][3];
+#pragma unroll
+    for (int w = 1; w < NW; w++) {
+        mnx = min(mnx, red[w][0]); mxx = max(mxx, red[w][1]);
+        mny = min(mny, red[w][2]); mxy = max(mxy, red[w][3]);
+    }
     // nodes used: the cell's 4 corners -> window = box + 1 node
     const int wx0 = mnx, wy0 = mny;
     const int W = mxx - mnx + 2, H = mxy - mny + 2;
@@ -205,7 +210,7 @@ __global__ __launch_bounds__(ADJ_THREADS, 2) void push_adj_kernel(AdjParams P)
         // node fields from the node-field array (y fastest: lanes walk a
         // window column, reading consecutive node records); the same values
         // as a_node_fields rounded to R
-        for (int k = tid; k < WH; k += ADJ_THREADS) {
+        for (int k = tid; k < WH; k += NT) {
             const int wi = k / H, wj = k - wi * H;
             const size_t g = (size_t)wrap_index(x0 + wx0 + wi, nx) * ny + wrap_index(y0 + wy0 + wj, ny);
             R *o = nf + (size_t)(wj * W + wi) * 6;
@@ -219,8 +224,8 @@ __global__ __launch_bounds__(ADJ_THREADS, 2) void push_adj_kernel(AdjParams P)
                 o[0] = a.x; o[1] = a.y; o[2] = b.x; o[3] = b.y; o[4] = cc.x; o[5] = cc.y;
             }
         }
-        for (int k = tid; k < 6 * WH; k += ADJ_THREADS) na[k] = 0.0;
-        for (int k = tid; k < JWH; k += ADJ_THREADS) {
+        for (int k = tid; k < 6 * WH; k += NT) na[k] = 0.0;
+        for (int k = tid; k < JWH; k += NT) {
             const int wj = k / JW, wi = k - wj * JW;
             const size_t g = (size_t)wrap_index(y0 + wy0 - 1 + wj, ny) * nx +
                              wrap_index(x0 + wx0 - 1 + wi, nx);
@@ -263,7 +268,7 @@ __global__ __launch_bounds__(ADJ_THREADS, 2) void push_adj_kernel(AdjParams P)
         acell = -1;
     };
 #pragma unroll 1
-    for (int r = 0; r < ADJ_ROUNDS; r++) {
+    for (int r = 0; r < ROUNDS; r++) {
         const int c = cl[r];
         if (c < 0) continue;
         const int64_t i = slot_of(r);
@@ -493,7 +498,7 @@ __global__ __launch_bounds__(ADJ_THREADS, 2) void push_adj_kernel(AdjParams P)
     flush();
     __syncthreads();
     if (inlds) {
-        for (int k = tid; k < WH; k += ADJ_THREADS) {
+        for (int k = tid; k < WH; k += NT) {
             const int wj = k / W, wi = k - wj * W;
             const size_t g = (size_t)wrap_index(y0 + wy0 + wj, ny) * nx + wrap_index(x0 + wx0 + wi, nx);
 #pragma unroll
@@ -720,24 +725,33 @@ size_t adj_lds_bytes(int precision, int maxw, int jmax)
 int adj_jmax_for(int precision, int maxw, size_t budget)
 {
     const size_t r = precision == PBPIC_PREC_FP64 ? 8 : 4, used = adj_lds_bytes(precision, maxw, 0);
-    const size_t stat = 64;                          // __shared__ red[4][4]
+    const size_t stat = 128;                         // __shared__ red[8][4] at most
     return used + stat >= budget ? 0 : (int)((budget - used - stat) / (3 * r));
 }
 
 int64_t adj_tiles(int64_t n) { return (n + ADJ_TILE - 1) / ADJ_TILE; }
 
+template <int NT>
+static void launch_push_adj_nt(const AdjParams &p, int precision, int t, size_t lds, hipStream_t st)
+{
+    if (precision == PBPIC_PREC_FP64) {
+        (void)hipFuncSetAttribute((const void *)push_adj_kernel<double, false, NT>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        hipLaunchKernelGGL((push_adj_kernel<double, false, NT>), dim3(t), dim3(NT), lds, st, p);
+    } else {
+        (void)hipFuncSetAttribute((const void *)push_adj_kernel<float, true, NT>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        hipLaunchKernelGGL((push_adj_kernel<float, true, NT>), dim3(t), dim3(NT), lds, st, p);
+    }
+}
+
+// the first launch at two workgroups of four waves per CU, the second (pass
+// 2, one workgroup per CU) with eight waves
 static void launch_push_adj_once(const AdjParams &p, int precision, int t, hipStream_t st)
 {
     const size_t lds = adj_lds_bytes(precision, p.maxw, p.jmax);
-    if (precision == PBPIC_PREC_FP64) {
-        (void)hipFuncSetAttribute((const void *)push_adj_kernel<double, false>,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        hipLaunchKernelGGL((push_adj_kernel<double, false>), dim3(t), dim3(ADJ_THREADS), lds, st, p);
-    } else {
-        (void)hipFuncSetAttribute((const void *)push_adj_kernel<float, true>,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        hipLaunchKernelGGL((push_adj_kernel<float, true>), dim3(t), dim3(ADJ_THREADS), lds, st, p);
-    }
+    if (p.pass == 2) launch_push_adj_nt<ADJ_THREADS_BIG>(p, precision, t, lds, st);
+    else launch_push_adj_nt<ADJ_THREADS>(p, precision, t, lds, st);
 }
 
 hipError_t launch_push_adj(const AdjParams &p0, int precision, hipStream_t st)

@@ -707,14 +707,17 @@ __device__ __forceinline__ void deposit_split_fast(double *jw, int WH, int W, in
 // as the sort's records at S.rec_o in place of the five value arrays; DEP:
 // deposit the current (false: the adjoint's replay, whose J comes from the
 // checkpointed forward).
-template <typename R, bool SPLIT, bool OLDU, bool RHO, bool REC, bool DEP>
+template <typename R, bool SPLIT, bool OLDU, bool RHO, bool REC, bool DEP, int NT>
 __device__ __forceinline__ void push_tile(const PushParams &P, int t)
 {
     constexpr int V = Vw<R>::V;
     constexpr int CH = 64 * V;                                   // slots per wave-wide load
-    constexpr int ROUNDS = PUSH_TILE / (CH * (PUSH_THREADS / 64));   // 8 (FP64) or 4 (FP32)
+    constexpr int NW = NT / 64;                                  // waves per workgroup
+    // rounds per wave: 8 (FP64) or 4 (FP32) with four waves, half with eight
+    constexpr int ROUNDS = PUSH_TILE / (CH * NW);
+    static_assert(ROUNDS * CH * NW == PUSH_TILE, "a tile is whole rounds of every wave");
     extern __shared__ __align__(16) unsigned char smem[];
-    __shared__ int red[4][5];
+    __shared__ int red[NW][5];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     int s = 0;
     while (s + 1 < P.nspecies && t >= P.sp[s + 1].tile0) s++;
@@ -774,11 +777,14 @@ __device__ __forceinline__ void push_tile(const PushParams &P, int t)
         red[wave][4] = aerr;
     }
     __syncthreads();
-    aerr = (red[0][4] | red[1][4]) | (red[2][4] | red[3][4]);
-    mnx = min(min(red[0][0], red[1][0]), min(red[2][0], red[3][0]));
-    mxx = max(max(red[0][1], red[1][1]), max(red[2][1], red[3][1]));
-    mny = min(min(red[0][2], red[1][2]), min(red[2][2], red[3][2]));
-    mxy = max(max(red[0][3], red[1][3]), max(red[2][3], red[3][3]));
+    aerr = 0;
+    mnx = 1 << 30; mxx = -(1 << 30); mny = 1 << 30; mxy = -(1 << 30);
+#pragma unroll
+    for (int w = 0; w < NW; w++) {
+        aerr |= red[w][4];
+        mnx = min(mnx, red[w][0]); mxx = max(mxx, red[w][1]);
+        mny = min(mny, red[w][2]); mxy = max(mxy, red[w][3]);
+    }
     }
     const int wx0 = mnx - 1, wy0 = mny - 1;
     const int W = mxx - mnx + 4, H = mxy - mny + 4;
@@ -806,13 +812,13 @@ __device__ __forceinline__ void push_tile(const PushParams &P, int t)
     // node-field array), zero the current window
     if (inlds) {
         const R *gnf = (const R *)P.nf;
-        for (int k = tid; k < WH; k += PUSH_THREADS) {
+        for (int k = tid; k < WH; k += NT) {
             const int wi = k / H, wj = k - wi * H;
             const int gx = wrap_index(x0 + wx0 + wi, nx), gy = wrap_index(y0 + wy0 + wj, ny);
             copy_node<R>(nf + (size_t)k * NS, gnf + ((size_t)gx * ny + gy) * NS);
         }
         if constexpr (DEP)
-            for (int k = tid; k < njc * WH; k += PUSH_THREADS) jw[k] = 0.0;
+            for (int k = tid; k < njc * WH; k += NT) jw[k] = 0.0;
     }
     __syncthreads();
 
@@ -1068,7 +1074,7 @@ __device__ __forceinline__ void push_tile(const PushParams &P, int t)
     // ---- D: flush the window, reduce the kinetic energy and error flags
     __syncthreads();
     if (DEP && inlds) {
-        for (int k = tid; k < WH; k += PUSH_THREADS) {
+        for (int k = tid; k < WH; k += NT) {
             const int wj = k / W, wi = k - wj * W;
             const int gx = wrap_index(x0 + wx0 + wi, nx), gy = wrap_index(y0 + wy0 + wj, ny);
             const size_t g = (size_t)gy * nx + gx;
@@ -1101,14 +1107,18 @@ __device__ __forceinline__ void push_tile(const PushParams &P, int t)
     for (int o = 32; o > 0; o >>= 1)
 #pragma unroll
         for (int q = 0; q < 8; q++) v[q] = min(v[q], __shfl_xor(v[q], o));
-    __shared__ int bred[4][8];
+    __shared__ int bred[NW][8];
     if (lane == 0)
 #pragma unroll
         for (int q = 0; q < 8; q++) bred[wave][q] = v[q];
     __syncthreads();
     if (tid == 0) {
 #pragma unroll
-        for (int q = 0; q < 8; q++) v[q] = min(min(bred[0][q], bred[1][q]), min(bred[2][q], bred[3][q]));
+        for (int q = 0; q < 8; q++) {
+            v[q] = bred[0][q];
+#pragma unroll
+            for (int w = 1; w < NW; w++) v[q] = min(v[q], bred[w][q]);
+        }
         const bool any = v[4] <= -v[5];            // at least one valid particle
         // cells read must lie inside the window's interior [1, W-3] x [1, H-3]
         if (any && (v[0] < 1 || -v[1] > W - 3 || v[2] < 1 || -v[3] > H - 3))
@@ -1132,7 +1142,7 @@ __device__ __forceinline__ void push_tile(const PushParams &P, int t)
 template <typename R, bool SPLIT, bool OLDU, bool RHO, bool REC, bool DEP>
 __global__ __launch_bounds__(PUSH_THREADS, 3) void push_kernel(PushParams P)
 {
-    push_tile<R, SPLIT, OLDU, RHO, REC, DEP>(P, xcd_remap(blockIdx.x, gridDim.x));
+    push_tile<R, SPLIT, OLDU, RHO, REC, DEP, PUSH_THREADS>(P, xcd_remap(blockIdx.x, gridDim.x));
 }
 
 // Passes 2 and 3: the tiles the first launch deferred, list entry k taken by
@@ -1141,13 +1151,16 @@ __global__ __launch_bounds__(PUSH_THREADS, 3) void push_kernel(PushParams P)
 // workgroups takes one per workgroup in list order (the hardware's dispatch
 // order; taking them from a shared counter instead was 6% slower on a
 // Harris sheet).  Their LDS windows allow at most two workgroups per CU, so
-// the register budget is that of two.
-template <typename R, bool SPLIT, bool OLDU, bool RHO, bool REC, bool DEP>
-__global__ __launch_bounds__(PUSH_THREADS, 2) void push_kernel_listed(PushParams P)
+// the register budget is that of two.  The last pass, whose window takes a
+// CU's LDS, runs workgroups of eight waves (NT = 512, two per SIMD) on the
+// tile's rounds, where four would leave one wave per SIMD to hide every
+// memory and LDS latency.
+template <typename R, bool SPLIT, bool OLDU, bool RHO, bool REC, bool DEP, int NT>
+__global__ __launch_bounds__(NT, NT == PUSH_THREADS ? 2 : 1) void push_kernel_listed(PushParams P)
 {
     const int which = P.pass - 2, cnt = P.big[which];
     for (int k = blockIdx.x; k < cnt; k += gridDim.x) {
-        push_tile<R, SPLIT, OLDU, RHO, REC, DEP>(P, P.big[2 + which * P.ntiles + k]);
+        push_tile<R, SPLIT, OLDU, RHO, REC, DEP, NT>(P, P.big[2 + which * P.ntiles + k]);
         __syncthreads();
     }
 }
@@ -1265,25 +1278,37 @@ hipError_t launch_nodefield(const PushParams &p, int precision, hipStream_t st)
     return hipGetLastError();
 }
 
-// dep = 0: a push without deposit (DEP = false) uses the node-field window only
-size_t push_lds_bytes(int precision, int maxw, int deposit_rho, int dep)
+// dep = 0: a push without deposit (DEP = false) uses the node-field window
+// only; nwaves: waves per workgroup (the split form's crossing queues)
+size_t push_lds_bytes(int precision, int maxw, int deposit_rho, int dep, int nwaves)
 {
     const size_t r = precision == PBPIC_PREC_FP64 ? 8 : 4;
     const size_t ns = precision == PBPIC_PREC_FP64 ? 6 : 8;
     if (!dep) return (size_t)maxw * ns * r;
     const size_t windows = (size_t)maxw * (ns * r + (deposit_rho ? 4 : 3) * 8);
-    return precision == PBPIC_PREC_FP64 ? windows
-                                        : windows + (PUSH_THREADS / 64) * XQ_BYTES;
+    return precision == PBPIC_PREC_FP64 ? windows : windows + (size_t)nwaves * XQ_BYTES;
 }
+
+// threads per workgroup of a push pass: eight waves for the last listed pass
+constexpr int push_threads_of_pass(int pass) { return pass == 3 ? 2 * PUSH_THREADS : PUSH_THREADS; }
 
 template <typename R, bool SPLIT, bool OLDU, bool RHO, bool REC, bool DEP>
 static void launch_push_t(const PushParams &q, int nt, size_t lds, hipStream_t st)
 {
-    if (q.pass >= 2) {
-        (void)hipFuncSetAttribute((const void *)push_kernel_listed<R, SPLIT, OLDU, RHO, REC, DEP>,
+    if (q.pass == 3) {
+        constexpr int NT = push_threads_of_pass(3);
+        (void)hipFuncSetAttribute((const void *)push_kernel_listed<R, SPLIT, OLDU, RHO, REC, DEP, NT>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        hipLaunchKernelGGL((push_kernel_listed<R, SPLIT, OLDU, RHO, REC, DEP>), dim3(nt),
-                           dim3(PUSH_THREADS), lds, st, q);
+        hipLaunchKernelGGL((push_kernel_listed<R, SPLIT, OLDU, RHO, REC, DEP, NT>), dim3(nt),
+                           dim3(NT), lds, st, q);
+        return;
+    }
+    if (q.pass == 2) {
+        constexpr int NT = push_threads_of_pass(2);
+        (void)hipFuncSetAttribute((const void *)push_kernel_listed<R, SPLIT, OLDU, RHO, REC, DEP, NT>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        hipLaunchKernelGGL((push_kernel_listed<R, SPLIT, OLDU, RHO, REC, DEP, NT>), dim3(nt),
+                           dim3(NT), lds, st, q);
         return;
     }
     (void)hipFuncSetAttribute((const void *)push_kernel<R, SPLIT, OLDU, RHO, REC, DEP>,
@@ -1325,17 +1350,17 @@ static void launch_push_f(const PushParams &q, int nt, size_t lds, hipStream_t s
 
 // Largest window whose LDS fits one workgroup per CU (150 KiB).
 // Window capacity (nodes) of a launch whose workgroups take `budget` bytes of LDS.
-static int push_maxw_for(int precision, int deposit_rho, int dep, size_t budget)
+static int push_maxw_for(int precision, int deposit_rho, int dep, size_t budget, int nwaves = 4)
 {
-    const size_t per = push_lds_bytes(precision, 1, deposit_rho, dep) -
-                       push_lds_bytes(precision, 0, deposit_rho, dep);
-    return (int)((budget - push_lds_bytes(precision, 0, deposit_rho, dep)) / per);
+    const size_t per = push_lds_bytes(precision, 1, deposit_rho, dep, nwaves) -
+                       push_lds_bytes(precision, 0, deposit_rho, dep, nwaves);
+    return (int)((budget - push_lds_bytes(precision, 0, deposit_rho, dep, nwaves)) / per);
 }
 
 // one workgroup per CU (150 KiB) and two (78 KiB) of the 160 KiB
 int push_maxw_big(int precision, int deposit_rho, int dep)
 {
-    return push_maxw_for(precision, deposit_rho, dep, 150 * 1024);
+    return push_maxw_for(precision, deposit_rho, dep, 150 * 1024, push_threads_of_pass(3) / 64);
 }
 int push_maxw_mid(int precision, int deposit_rho, int dep)
 {
@@ -1346,7 +1371,7 @@ int push_maxw_mid(int precision, int deposit_rho, int dep)
 // workgroups per CU)
 int push_maxw_nodep(int precision, int deposit_rho, int maxw)
 {
-    return push_maxw_for(precision, deposit_rho, 0, push_lds_bytes(precision, maxw, deposit_rho, 1));
+    return push_maxw_for(precision, deposit_rho, 0, push_lds_bytes(precision, maxw, deposit_rho, 1, 4));
 }
 
 hipError_t launch_push(const PushParams &p, int precision, hipStream_t st)
@@ -1355,7 +1380,7 @@ hipError_t launch_push(const PushParams &p, int precision, hipStream_t st)
     const int nt = push_tiles(p, q);
     if (nt == 0) return hipSuccess;
     const int dep = p.nodep ? 0 : 1;
-    const size_t lds = push_lds_bytes(precision, p.maxw, p.deposit_rho, dep);
+    const size_t lds = push_lds_bytes(precision, p.maxw, p.deposit_rho, dep, PUSH_THREADS / 64);
     q.pass = p.big && p.maxw_big > p.maxw ? 1 : 0;
     q.maxw_mid = std::max(p.maxw, std::min(p.maxw_mid, p.maxw_big));
     // the two list counts were cleared by the node-field launch before this
@@ -1368,7 +1393,8 @@ hipError_t launch_push(const PushParams &p, int precision, hipStream_t st)
             b.pass = pass;
             b.maxw = pass == 2 ? q.maxw_mid : p.maxw_big;
             if (pass == 2 && b.maxw == p.maxw) continue;       // no middle tier
-            const size_t lb = push_lds_bytes(precision, b.maxw, p.deposit_rho, dep);
+            const size_t lb = push_lds_bytes(precision, b.maxw, p.deposit_rho, dep,
+                                             push_threads_of_pass(pass) / 64);
             const int ng = std::max(1, (nt + 7) / 8);
             if (precision == PBPIC_PREC_FP64) launch_push_f<double, false>(b, ng, lb, st);
             else launch_push_f<float, true>(b, ng, lb, st);
