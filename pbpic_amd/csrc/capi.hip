@@ -969,12 +969,13 @@ int pbpic_ctx_create(const pbpic_params *pp, pbpic_ctx **out)
     c->p = p;
     c->rsz = p.precision == PBPIC_PREC_FP64 ? 8 : 4;
     // window capacity in nodes: split form, three push workgroups share a
-    // CU's 160 KiB of LDS (push_lds_bytes without the rho plane); FP64 form,
-    // two (the FP64 push runs as fast at two workgroups per CU, and the wider
-    // window defers fewer tiles once the cells have spread)
+    // CU's 160 KiB of LDS (896 nodes of 48 bytes and four crossing queues:
+    // 52 KiB; push_lds_bytes without the rho plane); FP64 form, two (the
+    // FP64 push runs as fast at two workgroups per CU, and the wider window
+    // defers fewer tiles once the cells have spread)
     c->maxw = p.precision == PBPIC_PREC_FP64
                   ? push_maxw_mid(p.precision, (p.flags & PBPIC_FLAG_DEPOSIT_RHO) ? 1 : 0, 1)
-                  : 768;
+                  : 896;
     c->idx = 1.0 / p.dx;                   // pic2d.f90:16
     c->idy = 1.0 / p.dy;
     c->dth = p.dt / 2.0;                   // :43

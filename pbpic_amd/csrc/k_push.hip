@@ -303,10 +303,14 @@ __device__ __forceinline__ void wall_reflect(int nyp, int cy, int &dcy, R &oy, R
     }
 }
 
-// LDS node record: 6 field components, padded to 8 floats in FP32 so that
-// every record starts 16-byte aligned (48-byte FP64 records already are).
+// Node-field array record: 6 field components, padded to 8 floats in FP32
+// so that every record starts 16-byte aligned (48-byte FP64 records already
+// are).  The LDS window packs the 6 components (24 bytes in FP32): a third
+// more nodes in the same LDS, and a node and its +y neighbour, adjacent in
+// the y-fastest window, read as six 8-byte pairs.
 template <typename R> struct NodeStride { static constexpr int v = 6; };
 template <> struct NodeStride<float> { static constexpr int v = 8; };
+constexpr int LDS_NS = 6;
 
 // Split-precision form of push_math: the same scheme in FP32 with one
 // reciprocal square root shared by every 1/gamma and contracted multiply-adds.
@@ -367,15 +371,23 @@ __device__ __forceinline__ float kin_energy(float ux, float uy, float uz)
     return u2 * __builtin_amdgcn_rcpf(gam + 1.0f);
 }
 
-// Copies one NS-value node record with 16-byte moves (records are 16-byte
-// aligned in both the node-field array and the LDS window).
+// Copies the 6 components of one node record from the node-field array
+// (16-byte aligned records) into the LDS window (LDS_NS values per node,
+// 8-byte aligned).
 template <typename R>
 __device__ __forceinline__ void copy_node(R *dst, const R *src)
 {
-    constexpr int NS = NodeStride<R>::v;
+    if constexpr (sizeof(R) == 4) {
+        const float4 a = *reinterpret_cast<const float4 *>(src);
+        const float2 b = *reinterpret_cast<const float2 *>(src + 4);
+        reinterpret_cast<float2 *>(dst)[0] = make_float2(a.x, a.y);
+        reinterpret_cast<float2 *>(dst)[1] = make_float2(a.z, a.w);
+        reinterpret_cast<float2 *>(dst)[2] = b;
+    } else {
 #pragma unroll
-    for (int q = 0; q < NS; q += 16 / (int)sizeof(R))
-        *reinterpret_cast<float4 *>(dst + q) = *reinterpret_cast<const float4 *>(src + q);
+        for (int q = 0; q < 6; q += 2)
+            *reinterpret_cast<double2 *>(dst + q) = *reinterpret_cast<const double2 *>(src + q);
+    }
 }
 
 // Vector width of the particle streams: each lane moves 16 bytes per array
@@ -639,12 +651,12 @@ struct CrossQueue {
     }
 };
 
-// LDS layout of a push workgroup: node-field window [maxw][NS], current
-// window [3 or 4 planes][maxw] of 8-byte sums, then (split form) the four
-// waves' crossing queues.  Split-form offset of the queues:
+// LDS layout of a push workgroup: node-field window [maxw][LDS_NS], current
+// window [3 or 4 planes][maxw] of 8-byte sums, then (split form) the waves'
+// crossing queues.  Split-form offset of the queues:
 __host__ __device__ constexpr size_t push_queue_offset(int maxw, int deposit_rho)
 {
-    return (size_t)maxw * (8 * 4 + (deposit_rho ? 4 : 3) * 8);
+    return (size_t)maxw * (LDS_NS * 4 + (deposit_rho ? 4 : 3) * 8);
 }
 
 // One particle's deposit in the split form on an interior tile (contributions
@@ -802,7 +814,8 @@ __device__ __forceinline__ void push_tile(const PushParams &P, int t)
     const bool fast = inlds && aerr == 0 && tstart + PUSH_TILE <= n && x0 + mnx >= 1 &&
                       x0 + mxx <= nx - 2 && y0 + mny >= 1 && y0 + mxy <= ny - 2;
     const int njc = RHO ? 4 : 3;
-    constexpr int NS = NodeStride<R>::v;
+    constexpr int NS = LDS_NS;
+    constexpr int GNS = NodeStride<R>::v;    // node-field array record
     R *nf = (R *)smem;                       // [W][H][NS] node fields, y fastest
     // current window [njc][H][W], x fastest: FP64 sums (FP64 form) or 64-bit
     // fixed-point sums (split form)
@@ -815,7 +828,7 @@ __device__ __forceinline__ void push_tile(const PushParams &P, int t)
         for (int k = tid; k < WH; k += NT) {
             const int wi = k / H, wj = k - wi * H;
             const int gx = wrap_index(x0 + wx0 + wi, nx), gy = wrap_index(y0 + wy0 + wj, ny);
-            copy_node<R>(nf + (size_t)k * NS, gnf + ((size_t)gx * ny + gy) * NS);
+            copy_node<R>(nf + (size_t)k * NS, gnf + ((size_t)gx * ny + gy) * GNS);
         }
         if constexpr (DEP)
             for (int k = tid; k < njc * WH; k += NT) jw[k] = 0.0;
@@ -1283,7 +1296,7 @@ hipError_t launch_nodefield(const PushParams &p, int precision, hipStream_t st)
 size_t push_lds_bytes(int precision, int maxw, int deposit_rho, int dep, int nwaves)
 {
     const size_t r = precision == PBPIC_PREC_FP64 ? 8 : 4;
-    const size_t ns = precision == PBPIC_PREC_FP64 ? 6 : 8;
+    const size_t ns = LDS_NS;
     if (!dep) return (size_t)maxw * ns * r;
     const size_t windows = (size_t)maxw * (ns * r + (deposit_rho ? 4 : 3) * 8);
     return precision == PBPIC_PREC_FP64 ? windows : windows + (size_t)nwaves * XQ_BYTES;

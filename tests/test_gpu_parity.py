@@ -341,15 +341,15 @@ def test_one_step_full_tiles(gpu, precision, compat, nx, ny):
 
 
 @pytest.mark.parametrize("precision", [0, 1])
-@pytest.mark.parametrize("nx,ny,path", [(28, 24, "middle pass"), (40, 36, "last pass"),
+@pytest.mark.parametrize("nx,ny,path", [(30, 26, "middle pass"), (40, 36, "last pass"),
                                         (72, 70, "global atomics")])
 def test_one_step_oversized_windows(gpu, precision, nx, ny, path):
     """Unsorted particles: every 4096-particle tile spans the whole domain, a
-    window of (nx+3) x (ny+3) nodes.  At 28 x 24 (837 nodes) it exceeds the
-    split form's first window (768) and fits its middle one (1088 with the
+    window of (nx+3) x (ny+3) nodes.  At 30 x 26 (957 nodes) it exceeds the
+    split form's first window (896) and fits its middle one (1243 with the
     rho plane); the FP64 form's first launch, at two workgroups per CU, holds
     it (998 with the rho plane).  At 40 x 36 (1677) only the last launch's
-    window holds it (2240, 1920); at 72 x 70 none, and the tiles deposit with
+    window holds it (2377, 1920); at 72 x 70 none, and the tiles deposit with
     global atomics.  All against the oracle."""
     _say(f"oversized windows precision={precision} {nx}x{ny} ({path})")
     case = make_case(nx=nx, ny=ny, ppc=12, seed=13, shuffle=True)
