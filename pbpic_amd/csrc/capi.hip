@@ -87,7 +87,27 @@ struct pbpic_ctx {
     double *rho_scratch = nullptr, *gauss0 = nullptr;
     int32_t *diag_counts = nullptr;   // spectrum / number-density scratch
     double *filt[3] = {nullptr, nullptr, nullptr};   // filter increments of E
-    int32_t *big = nullptr;    // deferred push tiles (count + list)
+    int32_t *big = nullptr;    // deferred push tiles (counts + lists)
+    // The split form's plain push has two builds of its first launch, at
+    // four and at three workgroups per CU, which give the same bits.  Which
+    // is faster depends on how far the tiles' cells have spread since the
+    // last sort: four per CU wins while every tile is compact (a uniform
+    // plasma all cycle long, a Harris sheet early in each cycle); three
+    // wins once many tiles need wider windows than four per CU leave, and
+    // mixing them with the compact ones in one launch overlaps their work.
+    // So the choice is kept per tenth of the sort cycle (OCC_BUCKETS): each
+    // tenth times one push of each build every OCC_REPROBE cycles (events
+    // around the push's launches; the host waits for the second push of the
+    // pair, a short stall once per tenth and probing cycle) and runs the
+    // faster.  Until a tenth has been timed it runs three per CU.
+    static constexpr int OCC_BUCKETS = 10, OCC_REPROBE = 8;
+    struct OccBucket {
+        hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+        int64_t last_probe = -(1 << 30);   // cycle of the last timed pair
+        int stage = 0;                      // 0 idle, 1 three timed, 2 both timed
+        bool use4 = false;
+    };
+    OccBucket occ[OCC_BUCKETS];
     double *lmask = nullptr;   // loss-region weights of the adjoint (nx*ny) or null
     int64_t big_cap = 0;
     bool gauss_first = true;
@@ -213,6 +233,9 @@ PushParams push_params(pbpic_ctx *c, bool recs = false, bool nodep = false)
     // a push without deposit has no current window: its node-field window
     // takes the LDS of both
     P.maxw = nodep ? push_maxw_nodep(c->p.precision, P.deposit_rho, c->maxw) : c->maxw;
+    // the split form's plain push may run its first launch at four
+    // workgroups per CU (push_impl decides; launch_push sets the windows)
+    P.occ4 = c->rsz == 4 && !nodep && !recs && !P.deposit_rho;
     P.maxw_mid = push_maxw_mid(c->p.precision, P.deposit_rho, nodep ? 0 : 1);
     P.maxw_big = push_maxw_big(c->p.precision, P.deposit_rho, nodep ? 0 : 1);
     P.big = c->big;
@@ -367,20 +390,57 @@ int push_impl(pbpic_ctx *c, bool recs = false, bool nodep = false)
 {
     int64_t ntiles = 0;
     for (auto &S : c->sp) ntiles += (S.n + PUSH_TILE - 1) / PUSH_TILE;
-    if (2 * ntiles + 2 > c->big_cap) {
-        // the deferred-tile lists of the tiered push: two counts, one entry
+    const int64_t bneed = 3 * ntiles + BIG_LISTS;
+    if (bneed > c->big_cap) {
+        // the deferred-tile lists of the tiered push: three counts, one entry
         // per tile in each list
         if (c->big) { HIPCHK(c, hipStreamSynchronize(c->st)); (void)hipFree(c->big); c->big = nullptr; }
-        HIPCHK(c, dalloc(&c->big, sizeof(int32_t) * (size_t)(2 * ntiles + 2)));
-        c->big_cap = 2 * ntiles + 2;
+        HIPCHK(c, dalloc(&c->big, sizeof(int32_t) * (size_t)bneed));
+        c->big_cap = bneed;
     }
-    const PushParams P = push_params(c, recs, nodep);
+    PushParams P = push_params(c, recs, nodep);
+    // the first launch's build: the faster for this tenth of the sort cycle,
+    // or one of a timed pair
+    pbpic_ctx::OccBucket *ob = nullptr;
+    int probe = -1;
+    if (P.occ4 && c->occ[0].ev[0]) {
+        const int K = c->p.sort_every > 0 ? c->p.sort_every : 50;
+        const int64_t phase = c->steps % K, cycle = c->steps / K;
+        ob = &c->occ[phase * pbpic_ctx::OCC_BUCKETS / K];
+        if (ob->stage == 2 && hipEventQuery(ob->ev[3]) == hipSuccess) {
+            float t3 = 0, t4 = 0;
+            if (hipEventElapsedTime(&t3, ob->ev[0], ob->ev[1]) == hipSuccess &&
+                hipEventElapsedTime(&t4, ob->ev[2], ob->ev[3]) == hipSuccess)
+                ob->use4 = t4 < t3;
+            ob->stage = 0;
+        }
+        (void)hipGetLastError();
+        if (ob->stage == 1) probe = 1;                 // the pair's second push
+        else if (ob->stage == 0 && cycle - ob->last_probe >= pbpic_ctx::OCC_REPROBE) probe = 0;
+        P.occ4 = probe >= 0 ? probe == 1 : ob->use4;
+    }
     // node fields (emfgrid, pic2d.f90:837-845) are field work: timed as such
     HIPCHK(c, prof_mark(c, PBPIC_PROF_FIELD, true));
     HIPCHK(c, launch_nodefield(P, c->p.precision, c->st));
     HIPCHK(c, prof_mark(c, PBPIC_PROF_FIELD, false));
     HIPCHK(c, prof_mark(c, PBPIC_PROF_PUSH, true));
+    if (probe >= 0) HIPCHK(c, hipEventRecord(ob->ev[2 * probe], c->st));
     HIPCHK(c, launch_push(P, c->p.precision, c->st));
+    if (probe >= 0) {
+        HIPCHK(c, hipEventRecord(ob->ev[2 * probe + 1], c->st));
+        ob->stage = probe + 1;
+        if (probe == 0) ob->last_probe = c->steps / (c->p.sort_every > 0 ? c->p.sort_every : 50);
+        if (probe == 1) {
+            // the pair complete: decide now, for the rest of this tenth
+            float t3 = 0, t4 = 0;
+            if (hipEventSynchronize(ob->ev[3]) == hipSuccess &&
+                hipEventElapsedTime(&t3, ob->ev[0], ob->ev[1]) == hipSuccess &&
+                hipEventElapsedTime(&t4, ob->ev[2], ob->ev[3]) == hipSuccess)
+                ob->use4 = t4 < t3;
+            (void)hipGetLastError();
+            ob->stage = 0;
+        }
+    }
     if (!nodep) {
         int rc = finalize_current(c);
         if (rc) return rc;
@@ -1017,6 +1077,9 @@ int pbpic_ctx_create(const pbpic_params *pp, pbpic_ctx **out)
             "alloc partials");
     }
     chk(dalloc(&c->err, sizeof(int) * 4), "alloc err");
+    for (auto &b : c->occ)
+        for (auto &e : b.ev) chk(hipEventCreate(&e), "create push timing events");
+
     {
         CellKey key;                       // counts are indexed by sort key
         key.init(p.nx, p.ny, 1);
@@ -1077,6 +1140,8 @@ int pbpic_ctx_destroy(pbpic_ctx *c)
     for (auto p : c->adj.ckpts) f(p);
     f(c->adj.final_state); f(c->adj.jstore); f(c->adj.tape);
     for (auto e : c->ev_pool) (void)hipEventDestroy(e);
+    for (auto &b : c->occ)
+        for (auto e : b.ev) if (e) (void)hipEventDestroy(e);
     for (int k = 0; k < 2; k++) {
         if (c->stage[k]) (void)hipHostFree(c->stage[k]);
         if (c->stage_ev[k]) (void)hipEventDestroy(c->stage_ev[k]);

@@ -85,10 +85,10 @@ __global__ __launch_bounds__(256) void nodefield_kernel(PushParams P, R *out)
     __shared__ double t[6][NF_T + 1][NF_T + 1];     // [plane][gy - Y0 + 1][gx - X0 + 1]
     const int nx = P.nx, ny = P.ny;
     const int X0 = blockIdx.x * NF_T, Y0 = blockIdx.y * NF_T;
-    // the push that follows appends deferred tiles to two lists: clear their
-    // counts here (the previous push's listed launches have finished reading
-    // them)
-    if (P.big && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x < 2) P.big[threadIdx.x] = 0;
+    // the push that follows appends deferred tiles to its lists: clear
+    // their counts here (the previous push's listed launches have finished
+    // reading them)
+    if (P.big && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x < 3) P.big[threadIdx.x] = 0;
     const double *pl[6] = {P.ex, P.ey, P.ez, P.bx, P.by, P.bz};
     for (int k = threadIdx.x; k < (NF_T + 1) * (NF_T + 1); k += 256) {
         const int r = k / (NF_T + 1), cc = k - r * (NF_T + 1);
@@ -719,9 +719,13 @@ __device__ __forceinline__ void deposit_split_fast(double *jw, int WH, int W, in
 // as the sort's records at S.rec_o in place of the five value arrays; DEP:
 // deposit the current (false: the adjoint's replay, whose J comes from the
 // checkpointed forward).
-template <typename R, bool SPLIT, bool OLDU, bool RHO, bool REC, bool DEP, int NT>
+// OCC: workgroups per CU the launch is built for.  At four the rounds are
+// not prefetched (the registers of the next round's particles would spill;
+// the fourth wave per SIMD hides the load latency instead).
+template <typename R, bool SPLIT, bool OLDU, bool RHO, bool REC, bool DEP, int NT, int OCC>
 __device__ __forceinline__ void push_tile(const PushParams &P, int t)
 {
+    constexpr bool PF = OCC < 4;
     constexpr int V = Vw<R>::V;
     constexpr int CH = 64 * V;                                   // slots per wave-wide load
     constexpr int NW = NT / 64;                                  // waves per workgroup
@@ -802,9 +806,12 @@ __device__ __forceinline__ void push_tile(const PushParams &P, int t)
     const int W = mxx - mnx + 4, H = mxy - mny + 4;
     const bool inlds = mnx <= mxx && (int64_t)W * H <= P.maxw;
     if (P.pass == 1 && !inlds && mnx <= mxx && (int64_t)W * H <= P.maxw_big) {
-        // the box is block-uniform: the whole workgroup leaves together
-        const int which = (int64_t)W * H <= P.maxw_mid ? 0 : 1;
-        if (tid == 0) P.big[2 + which * P.ntiles + atomicAdd(P.big + which, 1)] = t;
+        // the box is block-uniform: the whole workgroup leaves together.
+        // Lists: 2 the three-per-CU tier (after a four-per-CU first launch),
+        // 0 the middle tier, 1 the last
+        const int64_t wh = (int64_t)W * H;
+        const int which = wh <= P.maxw_occ3 ? 2 : wh <= P.maxw_mid ? 0 : 1;
+        if (tid == 0) P.big[BIG_LISTS + which * P.ntiles + atomicAdd(P.big + which, 1)] = t;
         return;
     }
     if (!inlds && mnx <= mxx && tid == 0) unsafeAtomicAdd(P.acc + ACC_SPILL, 1.0);
@@ -1015,10 +1022,16 @@ __device__ __forceinline__ void push_tile(const PushParams &P, int t)
 #pragma unroll 1
         for (int r = 0; r < ROUNDS; r++) {
             const int o = lo + r * CH;
-            if (r + 1 < ROUNDS) {
-                ldu(cp + o + CH, cnxt);
-                ldu(op + o + CH, nxt[0]); ldu(pp + o + CH, nxt[1]); ldu(up + o + CH, nxt[2]);
-                ldu(vp + o + CH, nxt[3]); ldu(wp + o + CH, nxt[4]);
+            if constexpr (PF) {
+                if (r + 1 < ROUNDS) {
+                    ldu(cp + o + CH, cnxt);
+                    ldu(op + o + CH, nxt[0]); ldu(pp + o + CH, nxt[1]); ldu(up + o + CH, nxt[2]);
+                    ldu(vp + o + CH, nxt[3]); ldu(wp + o + CH, nxt[4]);
+                }
+            } else if (r > 0) {
+                ldu(cp + o, ccur);
+                ldu(op + o, cur[0]); ldu(pp + o, cur[1]); ldu(up + o, cur[2]);
+                ldu(vp + o, cur[3]); ldu(wp + o, cur[4]);
             }
             int nc[V];
 #pragma unroll
@@ -1031,11 +1044,13 @@ __device__ __forceinline__ void push_tile(const PushParams &P, int t)
                 stu(oq + o, cur[0]); stu(pq + o, cur[1]); stu(uq + o, cur[2]);
                 stu(vq + o, cur[3]); stu(wq + o, cur[4]);
             }
+            if constexpr (PF) {
 #pragma unroll
-            for (int j = 0; j < V; j++) {
-                ccur[j] = cnxt[j];
+                for (int j = 0; j < V; j++) {
+                    ccur[j] = cnxt[j];
 #pragma unroll
-                for (int q = 0; q < 5; q++) cur[q][j] = nxt[q][j];
+                    for (int q = 0; q < 5; q++) cur[q][j] = nxt[q][j];
+                }
             }
         }
         if constexpr (SPLIT && DEP) Q.drain(jw, WH, W, dqx, dqy, P.compat, Q.tail - Q.head);
@@ -1052,11 +1067,17 @@ __device__ __forceinline__ void push_tile(const PushParams &P, int t)
 #pragma unroll 1
     for (int r = 0; r < ROUNDS; r++) {
         const int64_t i0 = pbase + r * CH;
-        if (r + 1 < ROUNDS) {
-            const int64_t i1 = i0 + CH;
-            vld(S.cell, i1, n, cnxt);
-            vld(ox_p, i1, n, nxt[0]); vld(oy_p, i1, n, nxt[1]); vld(ux_p, i1, n, nxt[2]);
-            vld(uy_p, i1, n, nxt[3]); vld(uz_p, i1, n, nxt[4]);
+        if constexpr (PF) {
+            if (r + 1 < ROUNDS) {
+                const int64_t i1 = i0 + CH;
+                vld(S.cell, i1, n, cnxt);
+                vld(ox_p, i1, n, nxt[0]); vld(oy_p, i1, n, nxt[1]); vld(ux_p, i1, n, nxt[2]);
+                vld(uy_p, i1, n, nxt[3]); vld(uz_p, i1, n, nxt[4]);
+            }
+        } else if (r > 0) {
+            vld(S.cell, i0, n, ccur);
+            vld(ox_p, i0, n, cur[0]); vld(oy_p, i0, n, cur[1]); vld(ux_p, i0, n, cur[2]);
+            vld(uy_p, i0, n, cur[3]); vld(uz_p, i0, n, cur[4]);
         }
         if (i0 < n) {
             int nc[V];
@@ -1075,11 +1096,13 @@ __device__ __forceinline__ void push_tile(const PushParams &P, int t)
                 vst(uy_q, i0, n, cur[3]); vst(uz_q, i0, n, cur[4]);
             }
         }
+        if constexpr (PF) {
 #pragma unroll
-        for (int j = 0; j < V; j++) {
-            ccur[j] = cnxt[j];
+            for (int j = 0; j < V; j++) {
+                ccur[j] = cnxt[j];
 #pragma unroll
-            for (int q = 0; q < 5; q++) cur[q][j] = nxt[q][j];
+                for (int q = 0; q < 5; q++) cur[q][j] = nxt[q][j];
+            }
         }
     }
     }
@@ -1152,28 +1175,29 @@ __device__ __forceinline__ void push_tile(const PushParams &P, int t)
     }
 }
 
-template <typename R, bool SPLIT, bool OLDU, bool RHO, bool REC, bool DEP>
-__global__ __launch_bounds__(PUSH_THREADS, 3) void push_kernel(PushParams P)
+template <typename R, bool SPLIT, bool OLDU, bool RHO, bool REC, bool DEP, int OCC>
+__global__ __launch_bounds__(PUSH_THREADS, OCC) void push_kernel(PushParams P)
 {
-    push_tile<R, SPLIT, OLDU, RHO, REC, DEP, PUSH_THREADS>(P, xcd_remap(blockIdx.x, gridDim.x));
+    push_tile<R, SPLIT, OLDU, RHO, REC, DEP, PUSH_THREADS, OCC>(P, xcd_remap(blockIdx.x, gridDim.x));
 }
 
-// Passes 2 and 3: the tiles the first launch deferred, list entry k taken by
-// workgroup k mod gridDim.x.  The grid is an eighth of the tiles, so a pass
-// with nothing to do costs little, and one with fewer deferred tiles than
-// workgroups takes one per workgroup in list order (the hardware's dispatch
-// order; taking them from a shared counter instead was 6% slower on a
-// Harris sheet).  Their LDS windows allow at most two workgroups per CU, so
-// the register budget is that of two.  The last pass, whose window takes a
-// CU's LDS, runs workgroups of eight waves (NT = 512, two per SIMD) on the
-// tile's rounds, where four would leave one wave per SIMD to hide every
-// memory and LDS latency.
-template <typename R, bool SPLIT, bool OLDU, bool RHO, bool REC, bool DEP, int NT>
-__global__ __launch_bounds__(NT, NT == PUSH_THREADS ? 2 : 1) void push_kernel_listed(PushParams P)
+// Passes 4, 2 and 3: the tiles the first launch deferred, list entry k taken
+// by workgroup k mod gridDim.x.  The grid is an eighth of the tiles, so a
+// pass with nothing to do costs little, and one with fewer deferred tiles
+// than workgroups takes one per workgroup in list order (the hardware's
+// dispatch order; taking them from a shared counter instead was 6% slower on
+// a Harris sheet).  OCC is the workgroups per CU the pass's LDS window
+// allows (its register budget): pass 4 (after a first launch at four per CU)
+// three, pass 2 two, pass 3 one.  The last pass, whose window takes a CU's
+// LDS, runs workgroups of eight waves (NT = 512, two per SIMD) on the tile's
+// rounds, where four would leave one wave per SIMD to hide every memory and
+// LDS latency.
+template <typename R, bool SPLIT, bool OLDU, bool RHO, bool REC, bool DEP, int NT, int OCC>
+__global__ __launch_bounds__(NT, OCC) void push_kernel_listed(PushParams P)
 {
-    const int which = P.pass - 2, cnt = P.big[which];
+    const int which = P.pass == 4 ? 2 : P.pass - 2, cnt = P.big[which];
     for (int k = blockIdx.x; k < cnt; k += gridDim.x) {
-        push_tile<R, SPLIT, OLDU, RHO, REC, DEP, NT>(P, P.big[2 + which * P.ntiles + k]);
+        push_tile<R, SPLIT, OLDU, RHO, REC, DEP, NT, OCC>(P, P.big[BIG_LISTS + which * P.ntiles + k]);
         __syncthreads();
     }
 }
@@ -1310,23 +1334,43 @@ static void launch_push_t(const PushParams &q, int nt, size_t lds, hipStream_t s
 {
     if (q.pass == 3) {
         constexpr int NT = push_threads_of_pass(3);
-        (void)hipFuncSetAttribute((const void *)push_kernel_listed<R, SPLIT, OLDU, RHO, REC, DEP, NT>,
+        (void)hipFuncSetAttribute((const void *)push_kernel_listed<R, SPLIT, OLDU, RHO, REC, DEP, NT, 1>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        hipLaunchKernelGGL((push_kernel_listed<R, SPLIT, OLDU, RHO, REC, DEP, NT>), dim3(nt),
+        hipLaunchKernelGGL((push_kernel_listed<R, SPLIT, OLDU, RHO, REC, DEP, NT, 1>), dim3(nt),
                            dim3(NT), lds, st, q);
         return;
     }
     if (q.pass == 2) {
         constexpr int NT = push_threads_of_pass(2);
-        (void)hipFuncSetAttribute((const void *)push_kernel_listed<R, SPLIT, OLDU, RHO, REC, DEP, NT>,
+        (void)hipFuncSetAttribute((const void *)push_kernel_listed<R, SPLIT, OLDU, RHO, REC, DEP, NT, 2>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        hipLaunchKernelGGL((push_kernel_listed<R, SPLIT, OLDU, RHO, REC, DEP, NT>), dim3(nt),
+        hipLaunchKernelGGL((push_kernel_listed<R, SPLIT, OLDU, RHO, REC, DEP, NT, 2>), dim3(nt),
                            dim3(NT), lds, st, q);
         return;
     }
-    (void)hipFuncSetAttribute((const void *)push_kernel<R, SPLIT, OLDU, RHO, REC, DEP>,
+    if constexpr (SPLIT && DEP && !REC) {
+        if (q.pass == 4) {
+            (void)hipFuncSetAttribute(
+                (const void *)push_kernel_listed<R, SPLIT, OLDU, RHO, REC, DEP, PUSH_THREADS, 3>,
+                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+            hipLaunchKernelGGL((push_kernel_listed<R, SPLIT, OLDU, RHO, REC, DEP, PUSH_THREADS, 3>),
+                               dim3(nt), dim3(PUSH_THREADS), lds, st, q);
+            return;
+        }
+    }
+    if (q.pass == 4) return;       // only the plain split push has this tier
+    if constexpr (SPLIT && DEP && !REC) {
+        if (q.occ4) {
+            (void)hipFuncSetAttribute((const void *)push_kernel<R, SPLIT, OLDU, RHO, REC, DEP, 4>,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+            hipLaunchKernelGGL((push_kernel<R, SPLIT, OLDU, RHO, REC, DEP, 4>), dim3(nt),
+                               dim3(PUSH_THREADS), lds, st, q);
+            return;
+        }
+    }
+    (void)hipFuncSetAttribute((const void *)push_kernel<R, SPLIT, OLDU, RHO, REC, DEP, 3>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    hipLaunchKernelGGL((push_kernel<R, SPLIT, OLDU, RHO, REC, DEP>), dim3(nt), dim3(PUSH_THREADS), lds, st,
+    hipLaunchKernelGGL((push_kernel<R, SPLIT, OLDU, RHO, REC, DEP, 3>), dim3(nt), dim3(PUSH_THREADS), lds, st,
                        q);
 }
 
@@ -1393,22 +1437,38 @@ hipError_t launch_push(const PushParams &p, int precision, hipStream_t st)
     const int nt = push_tiles(p, q);
     if (nt == 0) return hipSuccess;
     const int dep = p.nodep ? 0 : 1;
-    const size_t lds = push_lds_bytes(precision, p.maxw, p.deposit_rho, dep, PUSH_THREADS / 64);
-    q.pass = p.big && p.maxw_big > p.maxw ? 1 : 0;
-    q.maxw_mid = std::max(p.maxw, std::min(p.maxw_mid, p.maxw_big));
-    // the two list counts were cleared by the node-field launch before this
-    // one (launch_nodefield, which every push follows)
+    // the split form's plain push: a first launch at four workgroups per CU
+    // (PUSH_MAXW_OCC4-node windows), the tiles it cannot hold deferred to a
+    // launch at three per CU with the context's window (pass 4), and wider
+    // ones to the middle and last tiers as usual
+    if (q.occ4) {
+        q.maxw_occ3 = p.maxw;
+        q.maxw = PUSH_MAXW_OCC4;
+    } else {
+        q.maxw_occ3 = 0;
+    }
+    const size_t lds = push_lds_bytes(precision, q.maxw, p.deposit_rho, dep, PUSH_THREADS / 64);
+    q.pass = p.big && p.maxw_big > q.maxw ? 1 : 0;
+    q.maxw_mid = std::max(q.maxw_occ3 ? q.maxw_occ3 : q.maxw, std::min(p.maxw_mid, p.maxw_big));
+    // the list counts were cleared by the node-field launch before this one
+    // (launch_nodefield, which every push follows)
     if (precision == PBPIC_PREC_FP64) launch_push_f<double, false>(q, nt, lds, st);
     else launch_push_f<float, true>(q, nt, lds, st);
     if (q.pass) {
-        for (int pass = 2; pass <= 3; pass++) {
+        const int ng = std::max(1, (nt + 7) / 8);
+        for (int pass : {4, 2, 3}) {
             PushParams b = q;
             b.pass = pass;
-            b.maxw = pass == 2 ? q.maxw_mid : p.maxw_big;
-            if (pass == 2 && b.maxw == p.maxw) continue;       // no middle tier
+            if (pass == 4) {
+                if (!q.occ4) continue;
+                b.maxw = q.maxw_occ3;
+            } else {
+                b.maxw = pass == 2 ? q.maxw_mid : p.maxw_big;
+                // no middle tier when the window before it is as wide
+                if (pass == 2 && b.maxw == (q.occ4 ? q.maxw_occ3 : q.maxw)) continue;
+            }
             const size_t lb = push_lds_bytes(precision, b.maxw, p.deposit_rho, dep,
                                              push_threads_of_pass(pass) / 64);
-            const int ng = std::max(1, (nt + 7) / 8);
             if (precision == PBPIC_PREC_FP64) launch_push_f<double, false>(b, ng, lb, st);
             else launch_push_f<float, true>(b, ng, lb, st);
         }

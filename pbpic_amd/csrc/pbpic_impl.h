@@ -15,6 +15,14 @@ namespace pbpic {
 // push_vec(precision) consecutive particles (16 bytes per array) per round.
 constexpr int PUSH_THREADS = 256;
 constexpr int PUSH_TILE = 4096;
+// big[]: three deferred-tile counts and a pad, then the three lists (one
+// list per tier for the whole device: per-XCD lists left the XCDs that hold
+// the fast-spreading species with most of a listed launch's work, 12%
+// slower on a Harris sheet)
+constexpr int BIG_LISTS = 4;
+// Split-form first-launch window at four workgroups per CU: 632 nodes of 48
+// bytes and four crossing queues (40 KiB of a CU's 160); at three, 896.
+constexpr int PUSH_MAXW_OCC4 = 632;
 __host__ __device__ constexpr int push_vec(int precision) { return precision == 0 ? 2 : 4; }
 
 // Push order.  Inside every full tile the sort can lay the particles out so
@@ -143,9 +151,17 @@ struct PushParams {
     // (pass 1): to a second launch with windows of maxw_mid nodes (pass 2,
     // two workgroups per CU) if they fit it, else to a third with maxw_big
     // (pass 3, one per CU).  big[0] and big[1] count the two lists, which
-    // start at big + 2 and big + 2 + ntiles.  pass 0: one launch; tiles that
-    // do not fit deposit with global atomics, as do those wider than maxw_big.
+    // start at big + BIG_LISTS and big + BIG_LISTS + ntiles (big[2] and the
+    // list at big + BIG_LISTS + 2 ntiles: pass 4, below).  pass 0: one
+    // launch; tiles that do not fit deposit with global atomics, as do those
+    // wider than maxw_big.
     int maxw_mid, maxw_big, pass;
+    // occ4: the split form's plain push runs its first launch at four
+    // workgroups per CU with windows of PUSH_MAXW_OCC4 nodes; the tiles
+    // wider than that, up to maxw_occ3 nodes (the context's window), go to a
+    // launch at three per CU (pass 4, list big[2]).  maxw_occ3 = 0: no such
+    // tier.
+    int occ4, maxw_occ3;
     int nodep;                 // 1: push without depositing J (adjoint replay)
     int *big;
     int ntiles;
