@@ -104,7 +104,7 @@ struct pbpic_ctx {
     struct OccBucket {
         hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
         int64_t last_probe = -(1 << 30);   // cycle of the last timed pair
-        int stage = 0;                      // 0 idle, 1 three timed, 2 both timed
+        int stage = 0;                      // 0 idle, 1 the three-per-CU push timed
         bool use4 = false;
     };
     OccBucket occ[OCC_BUCKETS];
@@ -407,15 +407,13 @@ int push_impl(pbpic_ctx *c, bool recs = false, bool nodep = false)
         const int K = c->p.sort_every > 0 ? c->p.sort_every : 50;
         const int64_t phase = c->steps % K, cycle = c->steps / K;
         ob = &c->occ[phase * pbpic_ctx::OCC_BUCKETS / K];
-        if (ob->stage == 2 && hipEventQuery(ob->ev[3]) == hipSuccess) {
-            float t3 = 0, t4 = 0;
-            if (hipEventElapsedTime(&t3, ob->ev[0], ob->ev[1]) == hipSuccess &&
-                hipEventElapsedTime(&t4, ob->ev[2], ob->ev[3]) == hipSuccess)
-                ob->use4 = t4 < t3;
-            ob->stage = 0;
-        }
-        (void)hipGetLastError();
-        if (ob->stage == 1) probe = 1;                 // the pair's second push
+        // no timing under stream capture (a graph replays without the host)
+        hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+        const hipError_t ce = hipStreamIsCapturing(c->st, &cap);
+        if (ce != hipSuccess) (void)hipGetLastError();
+        const bool capturing = ce != hipSuccess || cap != hipStreamCaptureStatusNone;
+        if (capturing) probe = -1;
+        else if (ob->stage == 1) probe = 1;            // the pair's second push
         else if (ob->stage == 0 && cycle - ob->last_probe >= pbpic_ctx::OCC_REPROBE) probe = 0;
         P.occ4 = probe >= 0 ? probe == 1 : ob->use4;
     }
@@ -428,7 +426,7 @@ int push_impl(pbpic_ctx *c, bool recs = false, bool nodep = false)
     HIPCHK(c, launch_push(P, c->p.precision, c->st));
     if (probe >= 0) {
         HIPCHK(c, hipEventRecord(ob->ev[2 * probe + 1], c->st));
-        ob->stage = probe + 1;
+        ob->stage = probe == 0 ? 1 : 0;
         if (probe == 0) ob->last_probe = c->steps / (c->p.sort_every > 0 ? c->p.sort_every : 50);
         if (probe == 1) {
             // the pair complete: decide now, for the rest of this tenth
@@ -437,7 +435,8 @@ int push_impl(pbpic_ctx *c, bool recs = false, bool nodep = false)
                 hipEventElapsedTime(&t3, ob->ev[0], ob->ev[1]) == hipSuccess &&
                 hipEventElapsedTime(&t4, ob->ev[2], ob->ev[3]) == hipSuccess)
                 ob->use4 = t4 < t3;
-            (void)hipGetLastError();
+            else
+                (void)hipGetLastError();
             ob->stage = 0;
         }
     }
