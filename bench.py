@@ -105,7 +105,7 @@ def run_form(precision, cfg, steps, warmup, sort_every, seed, push_order=True, c
 # sort cycle: rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (factors from a
 # 4 GiB copy), from the commands in
 # profiles/push_kernel_traffic_two_stream_2048x2048_ppc100.md.
-TRAFFIC_2048_PPC100 = {1: 10.234e9 + 10.535e9, 0: 18.699e9 + 18.924e9}
+TRAFFIC_2048_PPC100 = {1: 10.267e9 + 10.576e9, 0: 18.699e9 + 18.924e9}
 TRAFFIC_SOURCE = "profiles/push_kernel_traffic_two_stream_2048x2048_ppc100.md"
 
 
