@@ -248,8 +248,8 @@ PushParams push_params(pbpic_ctx *c, bool recs = false, bool nodep = false)
     P.dtcva = c->dtcva;
     P.ex = c->E[0]; P.ey = c->E[1]; P.ez = c->E[2];
     P.bx = c->B[0]; P.by = c->B[1]; P.bz = c->B[2];
-    void *const *jp = c->rsz == 4 ? c->J64 : c->J;
-    P.jx = jp[0]; P.jy = jp[1]; P.jz = jp[2]; P.rho = jp[3];
+    P.jx = c->J64[0]; P.jy = c->J64[1]; P.jz = c->J64[2]; P.rho = c->J64[3];
+    P.jlo = c->rsz == 8 ? ncell(c) : 0;
     P.jsx = c->js[0]; P.jsy = c->js[1]; P.jsz = c->js[2]; P.jsr = c->js[3];
     P.nf = c->NF;
     P.acc = c->acc;
@@ -374,14 +374,13 @@ int kinetic_impl(pbpic_ctx *c)
     return PBPIC_OK;
 }
 
-// Split form: the FP32 current planes from the fixed-point sums, with the Jz
-// wall fold of conducting y walls applied to the sums.
+// The current planes (FP32 split form, FP64) from the fixed-point sums, with
+// the Jz wall fold of conducting y walls applied to the sums.
 int finalize_current(pbpic_ctx *c)
 {
-    if (c->rsz != 4) return PBPIC_OK;
     HIPCHK(c, launch_finalize_current(c->J64, c->J, ncell(c), c->js,
                                       (c->p.flags & PBPIC_FLAG_DEPOSIT_RHO) ? 1 : 0, c->p.nx,
-                                      walls(c) ? nyp_of(c) : 0, c->st));
+                                      walls(c) ? nyp_of(c) : 0, c->rsz, c->st));
     return PBPIC_OK;
 }
 
@@ -453,9 +452,6 @@ int push_impl(pbpic_ctx *c, bool recs = false, bool nodep = false)
         S.ux[b] = c->push_out[s][3]; S.uy[b] = c->push_out[s][4]; S.uz[b] = c->push_out[s][5];
     }
     c->push_out.clear();
-    // the split form folds Jz at the walls in finalize_current
-    if (walls(c) && !nodep && c->rsz == 8)
-        HIPCHK(c, launch_wall_fold(c->J[2], c->p.nx, c->p.ny, P.nyp, c->rsz, c->st));
     for (auto &S : c->sp) S.tbox_ok = true;
     HIPCHK(c, prof_mark(c, PBPIC_PROF_PUSH, false));
     return PBPIC_OK;
@@ -503,10 +499,12 @@ int step_impl(pbpic_ctx *c, void *const *jcopy, uint32_t *const *perm_dev, bool 
               void *const *jsrc = nullptr)
 {
     HIPCHK(c, prof_mark(c, PBPIC_PROF_ZERO, true));
-    // the split form's push writes its FP32 planes whole from the sums
-    void *const *jz = c->rsz == 4 ? c->J64 : c->J;
+    // the push writes the current planes whole from the fixed-point sums
+    // (FP64 form: two words per node)
+    void *const *jz = c->J64;
     if (!jsrc)
-        HIPCHK(c, launch_zero_current(jz[0], jz[1], jz[2], jz[3], ncell(c), 8, c->acc, 1, c->st));
+        HIPCHK(c, launch_zero_current(jz[0], jz[1], jz[2], jz[3], ncell(c) * (c->rsz / 4), 8,
+                                      c->acc, 1, c->st));
     HIPCHK(c, prof_mark(c, PBPIC_PROF_ZERO, false));
     // split form: when this step sorts, the push writes the particles as the
     // sort's records, so the sort skips its packing pass.  Until the sort
@@ -1055,8 +1053,11 @@ int pbpic_ctx_create(const pbpic_params *pp, pbpic_ctx **out)
         chk(dalloc(&c->B2[k], nc * 8), "alloc B");
     }
     for (int k = 0; k < 4; k++) chk(dalloc(&c->J[k], nc * c->rsz), "alloc J");
-    if (c->rsz == 4) {
-        for (int k = 0; k < 4; k++) chk(dalloc(&c->J64[k], nc * 8), "alloc J sums");
+    {
+        // the fixed-point sums: one word per node (split form), two (FP64
+        // form: high words, then low words)
+        const size_t words = c->rsz / 4;
+        for (int k = 0; k < 4; k++) chk(dalloc(&c->J64[k], nc * 8 * words), "alloc J sums");
         // one contribution of each plane stays below 2^30 (bounds as in
         // species_dev, |q| = 1): |jx| <= dx/dt, |jy| <= dy/dt, |jz| <= cva, |rho| <= 1
         c->js[0] = fx_scale(p.dx / p.dt);
@@ -1102,7 +1103,7 @@ int pbpic_ctx_create(const pbpic_params *pp, pbpic_ctx **out)
         }
         for (int k = 0; k < 4; k++) chk(hipMemsetAsync(c->J[k], 0, nc * c->rsz, c->st), "zero J");
         for (int k = 0; k < 4 && c->J64[0]; k++)
-            chk(hipMemsetAsync(c->J64[k], 0, nc * 8, c->st), "zero J sums");
+            chk(hipMemsetAsync(c->J64[k], 0, nc * 8 * (c->rsz / 4), c->st), "zero J sums");
         chk(hipMemsetAsync(c->acc, 0, ACC_N * sizeof(double), c->st), "zero acc");
         chk(hipMemsetAsync(c->err, 0, sizeof(int) * 4, c->st), "zero err");
         chk(hipStreamSynchronize(c->st), "sync");
@@ -1379,15 +1380,24 @@ int pbpic_set_field(pbpic_ctx *c, int32_t which, const double *plane)
         std::vector<float> vf;
         for (size_t k = 0; k < nc; k++) vd[k] = plane[k] / c->p.in0;
         void *dst = c->J[which - PBPIC_JX];
+        const double sc = c->js[which - PBPIC_JX];
         if (c->rsz == 8) {
             HIPCHK(c, hipMemcpy(dst, vd.data(), nc * 8, hipMemcpyHostToDevice));
+            // the two-word fixed-point sums a following push adds to: the
+            // scaled value x as floor(x) and rint((x - floor(x)) 2^FX2_LOBITS)
+            std::vector<long long> vq(2 * nc);
+            for (size_t k = 0; k < nc; k++) {
+                const double x = vd[k] * sc, h = std::floor(x);
+                vq[k] = (long long)h;
+                vq[nc + k] = llrint((x - h) * (double)(1 << FX2_LOBITS));
+            }
+            HIPCHK(c, hipMemcpy(c->J64[which - PBPIC_JX], vq.data(), 2 * nc * 8, hipMemcpyHostToDevice));
         } else {
             vf.assign(vd.begin(), vd.end());
             HIPCHK(c, hipMemcpy(dst, vf.data(), nc * 4, hipMemcpyHostToDevice));
             // the fixed-point sums a following push adds to
-            const float sc = c->js[which - PBPIC_JX];
             std::vector<long long> vq(nc);
-            for (size_t k = 0; k < nc; k++) vq[k] = llrint((double)vf[k] * (double)sc);
+            for (size_t k = 0; k < nc; k++) vq[k] = llrint((double)vf[k] * sc);
             HIPCHK(c, hipMemcpy(c->J64[which - PBPIC_JX], vq.data(), nc * 8, hipMemcpyHostToDevice));
         }
     } else {
@@ -1445,7 +1455,7 @@ int pbpic_zero_current(pbpic_ctx *c)
     if (!c) return PBPIC_ERR_ARG;
     HIPCHK(c, launch_zero_current(c->J[0], c->J[1], c->J[2], c->J[3], ncell(c), c->rsz, c->acc, 0, c->st));
     if (c->J64[0])
-        HIPCHK(c, launch_zero_current(c->J64[0], c->J64[1], c->J64[2], c->J64[3], ncell(c), 8, c->acc,
+        HIPCHK(c, launch_zero_current(c->J64[0], c->J64[1], c->J64[2], c->J64[3], ncell(c) * (c->rsz / 4), 8, c->acc,
                                       0, c->st));
     return PBPIC_OK;
 }
@@ -1553,7 +1563,7 @@ int pbpic_diag(pbpic_ctx *c, double out[PBPIC_NDIAG])
     for (int k = 0; k < PBPIC_NDIAG; k++) out[k] = 0;
     const size_t nc = (size_t)ncell(c);
     if (!c->rho_scratch) {
-        HIPCHK(c, dalloc(&c->rho_scratch, nc * 8));
+        HIPCHK(c, dalloc(&c->rho_scratch, nc * 8 * (c->rsz / 4)));   // FP64 form: two words
         HIPCHK(c, dalloc(&c->gauss0, nc * 8));
         c->gauss_first = true;
     }
@@ -1561,8 +1571,9 @@ int pbpic_diag(pbpic_ctx *c, double out[PBPIC_NDIAG])
     HIPCHK(c, launch_energy(fdtd_params(c), c->st));
     if ((rc = kinetic_impl(c))) return rc;
     const double cva2 = c->p.cva * c->p.cva;
-    // split form: the charge in fixed point, exact in any order; a particle
-    // adds at most 1 to a node, so ntot * scale stays below 2^62
+    // the charge in fixed point, exact in any order.  Split form: one word,
+    // a particle adds at most 1 to a node, so ntot * scale stays below 2^62;
+    // FP64 form: two words (launch_gauss)
     double rscale = 0;
     if (c->rsz == 4) {
         int64_t ntot = 1;

@@ -294,31 +294,35 @@ hipError_t launch_energy(const FdtdParams &p, hipStream_t st)
 // global atomic per node; particles outside the box, or all of a tile whose
 // box exceeds the histogram, add to rho directly.
 constexpr int RB_NODES = 2048;
-// FX: the charge as 64-bit fixed point, each contribution rounded to
-// rscale (sums exact in any order), else FP64 atomics.
-__device__ __forceinline__ void rho_add(double *p, double v, double rscale, bool fx)
+// The charge as 64-bit fixed point, sums exact in any order.  Split form
+// (TW false): each weight times rscale, rounded.  FP64 form (TW): each weight
+// times rscale = 2^30 and 2^FX2_LOBITS, rounded (fx2_quant), the global plane
+// in two words (fx2_add, the low words lo entries on).  A tile adds at most
+// PUSH_TILE weights below 2^49 to an LDS node: below 2^61.
+template <bool TW>
+__device__ __forceinline__ long long rho_quant(double v, double rscale)
 {
-    if (fx)
-        atomicAdd(reinterpret_cast<unsigned long long *>(p),
-                  (unsigned long long)(long long)rint(v * rscale));
-    else
-        unsafeAtomicAdd(p, v);
+    if (TW) return fx2_quant(v * rscale * (double)(1 << FX2_LOBITS));
+    return (long long)rint(v * rscale);
 }
-__device__ __forceinline__ void rho_add_lds(double *p, double v, double rscale, bool fx)
+template <bool TW>
+__device__ __forceinline__ void rho_add(long long *p, long long q, int64_t lo)
 {
-    if (fx)
-        __hip_atomic_fetch_add(reinterpret_cast<unsigned long long *>(p),
-                               (unsigned long long)(long long)rint(v * rscale), __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_WORKGROUP);
-    else
-        atomicAdd(p, v);
+    if (TW) fx2_add(p, lo, q);
+    else atomicAdd(reinterpret_cast<unsigned long long *>(p), (unsigned long long)q);
+}
+__device__ __forceinline__ void rho_add_lds(long long *p, long long q)
+{
+    __hip_atomic_fetch_add(reinterpret_cast<unsigned long long *>(p), (unsigned long long)q,
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 template <typename R>
 __global__ __launch_bounds__(256) void rho_cic_tiles_kernel(PushParams P, SpeciesDev S,
-                                                            double *rho, double rscale)
+                                                            long long *rho, double rscale)
 {
-    const bool fx = rscale > 0;
-    __shared__ double h[RB_NODES];
+    constexpr bool TW = sizeof(R) == 8;
+    const int64_t lo = TW ? (int64_t)P.nx * P.ny : 0;
+    __shared__ long long h[RB_NODES];
     __shared__ int bb[4];
     const int nx = P.nx, ny = P.ny;
     const int64_t t0 = (int64_t)blockIdx.x * PUSH_TILE;
@@ -357,7 +361,7 @@ __global__ __launch_bounds__(256) void rho_cic_tiles_kernel(PushParams P, Specie
     const int W = b.y + 2, H = b.w + 2;                 // nodes of the box's cells
     const bool boxed = b.y >= 0 && b.w >= 0 && (int64_t)W * H <= RB_NODES;
     if (boxed)
-        for (int k = threadIdx.x; k < W * H; k += 256) h[k] = 0.0;
+        for (int k = threadIdx.x; k < W * H; k += 256) h[k] = 0;
     __syncthreads();
     const double q = (double)S.q;
     for (int u = threadIdx.x; u < PUSH_TILE; u += 256) {
@@ -375,41 +379,38 @@ __global__ __launch_bounds__(256) void rho_cic_tiles_kernel(PushParams P, Specie
             if (rx < 0) rx += nx;
             if (ry < 0) ry += ny;
             if (rx <= b.y && ry <= b.w) {
-                double *n0 = h + ry * W + rx;
-                rho_add_lds(n0, w00, rscale, fx);
-                rho_add_lds(n0 + 1, w10, rscale, fx);
-                rho_add_lds(n0 + W, w01, rscale, fx);
-                rho_add_lds(n0 + W + 1, w11, rscale, fx);
+                long long *n0 = h + ry * W + rx;
+                rho_add_lds(n0, rho_quant<TW>(w00, rscale));
+                rho_add_lds(n0 + 1, rho_quant<TW>(w10, rscale));
+                rho_add_lds(n0 + W, rho_quant<TW>(w01, rscale));
+                rho_add_lds(n0 + W + 1, rho_quant<TW>(w11, rscale));
                 continue;
             }
         }
         const int cx1 = cx + 1 == nx ? 0 : cx + 1, cy1 = cy + 1 == ny ? 0 : cy + 1;
-        rho_add(rho + (size_t)cy * nx + cx, w00, rscale, fx);
-        rho_add(rho + (size_t)cy * nx + cx1, w10, rscale, fx);
-        rho_add(rho + (size_t)cy1 * nx + cx, w01, rscale, fx);
-        rho_add(rho + (size_t)cy1 * nx + cx1, w11, rscale, fx);
+        rho_add<TW>(rho + (size_t)cy * nx + cx, rho_quant<TW>(w00, rscale), lo);
+        rho_add<TW>(rho + (size_t)cy * nx + cx1, rho_quant<TW>(w10, rscale), lo);
+        rho_add<TW>(rho + (size_t)cy1 * nx + cx, rho_quant<TW>(w01, rscale), lo);
+        rho_add<TW>(rho + (size_t)cy1 * nx + cx1, rho_quant<TW>(w11, rscale), lo);
     }
     __syncthreads();
     if (boxed)
         for (int k = threadIdx.x; k < W * H; k += 256) {
-            const double v = h[k];
-            if (v == 0.0) continue;                  // (zero bits: zero in both forms)
+            const long long v = h[k];
+            if (v == 0) continue;
             const int ky = k / W, kx = k - ky * W;
             int gx = b.x + kx, gy = b.z + ky;
             while (gx >= nx) gx -= nx;
             while (gy >= ny) gy -= ny;
-            double *g = rho + (size_t)gy * nx + gx;
-            if (fx)
-                atomicAdd(reinterpret_cast<unsigned long long *>(g),
-                          (unsigned long long)__double_as_longlong(v));
-            else
-                unsafeAtomicAdd(g, v);
+            rho_add<TW>(rho + (size_t)gy * nx + gx, v, lo);
         }
 }
 
-__global__ __launch_bounds__(256) void gauss_kernel(PushParams P, const double *rho,
+// rho: the fixed-point charge of rho_cic_tiles_kernel, one word (lo = 0) or
+// two (the low words lo entries after the high ones).
+__global__ __launch_bounds__(256) void gauss_kernel(PushParams P, const long long *rho,
                                                     double *g0, int first, double rho_bg,
-                                                    double cva2in0, double rscale)
+                                                    double cva2in0, double rscale, int64_t lo)
 {
     __shared__ double red[2][4];
     __shared__ unsigned long long mx[2][4];
@@ -424,9 +425,8 @@ __global__ __launch_bounds__(256) void gauss_kernel(PushParams P, const double *
         const int gxm = gx == 0 ? nx - 1 : gx - 1, gym = gy == 0 ? ny - 1 : gy - 1;
         const double div = (P.ex[k] - P.ex[(size_t)gy * nx + gxm]) * P.idx +
                            (P.ey[k] - P.ey[(size_t)gym * nx + gx]) * P.idy;
-        const double rk = rscale > 0
-                              ? (double)__double_as_longlong(rho[k]) / rscale
-                              : rho[k];
+        const double rk = lo ? ((double)rho[k] + (double)rho[lo + k] * FX2_LO) / rscale
+                             : (double)rho[k] / rscale;
         const double res = div - cva2in0 * (rk + rho_bg);
         if (first) g0[k] = res;
         const double d = res - g0[k];
@@ -471,21 +471,27 @@ hipError_t launch_gauss(const PushParams &p, int precision, double rho_scale,
                         double cva2in0, hipStream_t st)
 {
     const int64_t n = (int64_t)p.nx * p.ny;
-    hipLaunchKernelGGL(zero_f64_kernel, dim3(1024), dim3(256), 0, st, rho_scratch, n);
+    // FP64 form: two words per node (rho_scratch holds 2 n), scale 2^30 (a
+    // weight, at most 1, times 2^30 2^FX2_LOBITS stays below 2^51)
+    const bool tw = precision == PBPIC_PREC_FP64;
+    const int64_t lo = tw ? n : 0;
+    if (tw) rho_scale = 1073741824.0;
+    long long *rho = reinterpret_cast<long long *>(rho_scratch);
+    hipLaunchKernelGGL(zero_f64_kernel, dim3(1024), dim3(256), 0, st, rho_scratch, n + lo);
     for (int s = 0; s < p.nspecies; s++) {
         if (p.sp[s].n == 0) continue;
         const int nt = (int)((p.sp[s].n + PUSH_TILE - 1) / PUSH_TILE);
-        if (precision == PBPIC_PREC_FP64)
+        if (tw)
             hipLaunchKernelGGL(rho_cic_tiles_kernel<double>, dim3(nt), dim3(256), 0, st, p,
-                               p.sp[s], rho_scratch, rho_scale);
+                               p.sp[s], rho, rho_scale);
         else
             hipLaunchKernelGGL(rho_cic_tiles_kernel<float>, dim3(nt), dim3(256), 0, st, p,
-                               p.sp[s], rho_scratch, rho_scale);
+                               p.sp[s], rho, rho_scale);
     }
     int nb = (int)((n + 255) / 256);
     if (nb > 4096) nb = 4096;
-    hipLaunchKernelGGL(gauss_kernel, dim3(nb), dim3(256), 0, st, p, rho_scratch, gauss0,
-                       first, rho_bg, cva2in0, rho_scale);
+    hipLaunchKernelGGL(gauss_kernel, dim3(nb), dim3(256), 0, st, p, rho, gauss0,
+                       first, rho_bg, cva2in0, rho_scale, lo);
     PartSlots ps{2, {ACC_G_L2, ACC_G_DL2, 0, 0}, {1.0, 1.0, 0, 0}};
     return launch_reduce_parts(p.part, nb, p.acc, ps, st);
 }
@@ -599,31 +605,6 @@ hipError_t launch_filters(const FilterParams &p, int precision, hipStream_t st)
 }
 
 // ------------------------------------------------------ conducting y walls
-template <typename R>
-__global__ void wall_fold_kernel(R *jz, int nx, int nyp)
-{
-    const int gx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (gx >= nx) return;
-    R *top = jz + (size_t)nyp * nx, *bot = jz + (size_t)(nyp + 1) * nx;
-    // the image of charge q at -y carries -q v_z: Jz beyond a wall folds back
-    // with sign -1 into the domain row it mirrors
-    jz[gx] -= bot[gx];
-    jz[(size_t)(nyp - 1) * nx + gx] -= top[gx];
-    top[gx] = (R)0;
-    bot[gx] = (R)0;
-}
-
-hipError_t launch_wall_fold(void *jz, int nx, int ny, int nyp, int elem_bytes, hipStream_t st)
-{
-    (void)ny;
-    const int nb = (nx + 255) / 256;
-    if (elem_bytes == 8)
-        hipLaunchKernelGGL(wall_fold_kernel<double>, dim3(nb), dim3(256), 0, st, (double *)jz, nx, nyp);
-    else
-        hipLaunchKernelGGL(wall_fold_kernel<float>, dim3(nb), dim3(256), 0, st, (float *)jz, nx, nyp);
-    return hipGetLastError();
-}
-
 // Guard rows of the field planes: half-row components (Ey, Bz even; Ez, By
 // odd) of row nyp mirror row nyp-1 and of row nyp+1 (= row -1) mirror row 0;
 // Ex is zero on the wall rows 0 and nyp; Ex, Bx of row nyp+1 (y = -dy) mirror
@@ -715,10 +696,11 @@ hipError_t launch_copy_list(CopyList L, hipStream_t st)
 // Split form: the FP32 current planes from the fixed-point sums the push
 // accumulated (one rounding of an exact sum, the same for any order of the
 // contributions).
-// With conducting y walls (nyp > 0) Jz is folded on the way (as
-// wall_fold_kernel folds an FP64 plane): the guard rows nyp and nyp + 1 are
-// subtracted, as integers, from the domain rows nyp - 1 and 0 they mirror,
-// and read as zero.
+// With conducting y walls (nyp > 0) Jz is folded on the way: the image of
+// charge q at -y carries -q v_z, so the Jz a particle deposits beyond a wall
+// folds back with sign -1 into the domain row it mirrors.  The guard rows nyp
+// and nyp + 1 are subtracted, as integers, from the domain rows nyp - 1 and 0
+// they mirror, and read as zero.
 __global__ void finalize_current_kernel(const long long *a, const long long *b,
                                         const long long *c, const long long *d, float *ja,
                                         float *jb, float *jc, float *jd, int64_t n, float4 inv,
@@ -739,13 +721,54 @@ __global__ void finalize_current_kernel(const long long *a, const long long *b,
     }
 }
 
+// FP64 form: each plane's sums in two words (fx2_add), hi at k and lo at
+// n + k, the value (hi + lo 2^-FX2_LOBITS) / scale.  Both conversions to
+// double are exact (below 2^53 for fewer than 2^23 full-size contributions a
+// node), their sum is one rounding, the scale a power of two: the result
+// depends on the integer sums only.  Walls: the guard rows' sums are
+// subtracted as integers, as above (the low word of a difference may leave
+// [0, 2^FX2_LOBITS); the value is the same).
+__global__ void finalize_current2_kernel(const long long *a, const long long *b,
+                                         const long long *c, const long long *d, double *ja,
+                                         double *jb, double *jc, double *jd, int64_t n,
+                                         double4 inv, int64_t nx, int64_t nyp)
+{
+    constexpr double L = FX2_LO;
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
+         k += (int64_t)gridDim.x * blockDim.x) {
+        ja[k] = ((double)a[k] + (double)a[n + k] * L) * inv.x;
+        jb[k] = ((double)b[k] + (double)b[n + k] * L) * inv.y;
+        long long zh = c[k], zl = c[n + k];
+        if (nyp > 0) {
+            int64_t m = -1;
+            if (k < nx) m = k + (nyp + 1) * nx;                              // row 0 - row -1
+            else if (k >= (nyp - 1) * nx && k < nyp * nx) m = k + nx;         // row nyp-1 - row nyp
+            else if (k >= nyp * nx) zh = zl = 0;                             // the guard rows
+            if (m >= 0) { zh -= c[m]; zl -= c[n + m]; }
+        }
+        jc[k] = ((double)zh + (double)zl * L) * inv.z;
+        if (d) jd[k] = ((double)d[k] + (double)d[n + k] * L) * inv.w;
+    }
+}
+
 hipError_t launch_finalize_current(void *const *j64, void *const *j, int64_t ncell,
                                    const float *scale, int with_rho, int nx, int nyp,
-                                   hipStream_t st)
+                                   int elem_bytes, hipStream_t st)
 {
     int64_t nb = (ncell + 255) / 256;
     if (nb > 8192) nb = 8192;
     if (nb < 1) nb = 1;
+    if (elem_bytes == 8) {
+        const double4 inv = make_double4(1.0 / scale[0], 1.0 / scale[1], 1.0 / scale[2],
+                                         1.0 / scale[3]);
+        hipLaunchKernelGGL(finalize_current2_kernel, dim3((unsigned)nb), dim3(256), 0, st,
+                           (const long long *)j64[0], (const long long *)j64[1],
+                           (const long long *)j64[2],
+                           with_rho ? (const long long *)j64[3] : nullptr, (double *)j[0],
+                           (double *)j[1], (double *)j[2], (double *)j[3], ncell, inv,
+                           (int64_t)nx, (int64_t)nyp);
+        return hipGetLastError();
+    }
     // 1/scale is a power of two: exact in float
     const float4 inv = make_float4(1.0f / scale[0], 1.0f / scale[1], 1.0f / scale[2],
                                    1.0f / scale[3]);

@@ -34,12 +34,6 @@
 
 namespace pbpic {
 
-template <typename R>
-__device__ __forceinline__ void lds_add(R *p, R v)
-{
-    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-}
-__device__ __forceinline__ void gadd(double *p, double v) { unsafeAtomicAdd(p, v); }
 // Split form, global planes: the integer sum itself, or one contribution
 // already multiplied by its plane's scale, rounded as the LDS window rounds
 // it (floor(v + 1/2), v_cvt_rpi_i32_f32).
@@ -540,13 +534,20 @@ __device__ __forceinline__ void vst_rec4(void *rec, int64_t i0, int64_t n, const
     }
 }
 
-// One contribution to the LDS current window.  FP64 form: FP64 atomic add.
-// Split form: the contribution arrives pre-multiplied by its plane's scale
+// One contribution to the LDS current window.  Split form: the contribution arrives pre-multiplied by its plane's scale
 // (a power of two, so the product is exact) and below 2^30 in magnitude; it
 // is rounded to the nearest integer and added as a 64-bit integer, which
 // gfx950's LDS sustains at a higher rate than floating-point atomics and
 // which makes the window sums exact and independent of the lane order.
-__device__ __forceinline__ void jadd(double *p, double v) { lds_add(p, v); }
+// FP64 form: the contribution (pre-scaled by 2^FX2_LOBITS as well) rounded
+// to an integer (fx2_quant); one tile puts at most 2 PUSH_TILE contributions
+// below 2^49 on a node of a plane, so the window sums stay below 2^63.
+__device__ __forceinline__ void jadd(double *p, double v)
+{
+    __hip_atomic_fetch_add(reinterpret_cast<unsigned long long *>(p),
+                           (unsigned long long)fx2_quant(v), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_WORKGROUP);
+}
 __device__ __forceinline__ void jadd(double *p, float v)
 {
     // v_cvt_rpi_i32_f32: floor(v + 0.5) in one instruction (the compiler
@@ -824,8 +825,7 @@ __device__ __forceinline__ void push_tile(const PushParams &P, int t)
     constexpr int NS = LDS_NS;
     constexpr int GNS = NodeStride<R>::v;    // node-field array record
     R *nf = (R *)smem;                       // [W][H][NS] node fields, y fastest
-    // current window [njc][H][W], x fastest: FP64 sums (FP64 form) or 64-bit
-    // fixed-point sums (split form)
+    // current window [njc][H][W], x fastest: 64-bit fixed-point sums
     double *jw = (double *)(nf + (size_t)NS * P.maxw);
 
     // ---- B: stage node fields (contiguous column runs of the y-fastest
@@ -850,13 +850,13 @@ __device__ __forceinline__ void push_tile(const PushParams &P, int t)
       *uz_q = (R *)S.uz_o;
     R *gjx = (R *)P.jx, *gjy = (R *)P.jy, *gjz = (R *)P.jz, *grho = (R *)P.rho;
     constexpr bool want_rho = RHO;
-    // split form: deposit contributions come out of deposit_terms already
-    // multiplied by their plane's fixed-point scale (q*dx/dt, q*dy/dt and q
-    // pre-scaled by powers of two: exact), for the LDS window and for the
-    // global planes alike
-    const bool fxw = SPLIT;
-    const R dqx = fxw ? qx * (R)S.jsx : qx, dqy = fxw ? qy * (R)S.jsy : qy,
-            dqz = fxw ? qr * (R)S.jsz : qr, rsc = fxw ? (R)S.jsr / (R)S.jsz : (R)1;
+    // deposit contributions come out of deposit_terms already multiplied by
+    // their plane's fixed-point scale (q*dx/dt, q*dy/dt and q pre-scaled by
+    // powers of two, and by 2^FX2_LOBITS in the FP64 form: exact), for the
+    // LDS window and for the global planes alike
+    const R fxs = SPLIT ? (R)1 : (R)(1 << FX2_LOBITS);
+    const R dqx = qx * (R)S.jsx * fxs, dqy = qy * (R)S.jsy * fxs, dqz = qr * (R)S.jsz * fxs,
+            rsc = (R)S.jsr / (R)S.jsz;
     // window coordinates of the cells this push reads (o*) and writes (n*):
     // the first check the recorded box, the second become the next one
     int omnx = 1 << 30, omxx = -(1 << 30), omny = 1 << 30, omxy = -(1 << 30);
@@ -912,7 +912,7 @@ __device__ __forceinline__ void push_tile(const PushParams &P, int t)
             Dep<R> d;
             deposit_terms<R, SPLIT>(cx + 1, cy + 1, dcx, dcy, dx0, ox, dy0, oy, dqx, dqy,
                                     dqz, vz, P.compat, want_rho, d);
-            if (fxw && want_rho) { d.r0 *= rsc; d.r1 *= rsc; d.r2 *= rsc; d.r3 *= rsc; }
+            if (want_rho) { d.r0 *= rsc; d.r1 *= rsc; d.r2 *= rsc; d.r3 *= rsc; }
             // jy at column mx+2 is lost by the reference's fold (:1156, :1159)
             if ((P.compat & PBPIC_COMPAT_JY_DROP) && cx == nx - 1 && dcx == 1) d.jyb1 = (R)0;
             if (d.a == 0 && d.b == 0) {
@@ -930,8 +930,9 @@ __device__ __forceinline__ void push_tile(const PushParams &P, int t)
                     return (size_t)gy * nx + gx;
                 };
                 auto put = [&](R *plane, size_t g, R v) {
-                    if constexpr (SPLIT) gfixf(reinterpret_cast<long long *>(plane) + g, v);
-                    else gadd(plane + g, v);
+                    long long *p = reinterpret_cast<long long *>(plane) + g;
+                    if constexpr (SPLIT) gfixf(p, v);
+                    else fx2_add(p, P.jlo, fx2_quant(v));
                 };
                 put(gjx, at(0, 0), d.jxa0);
                 put(gjx, at(0, 1), d.jxa1);
@@ -1006,6 +1007,7 @@ __device__ __forceinline__ void push_tile(const PushParams &P, int t)
                 Dep<R> d;
                 deposit_terms<R, SPLIT>(cx + 1, cy + 1, dcx, dcy, dx0, ox, dy0, oy, dqx, dqy,
                                         dqz, vz, P.compat, want_rho, d);
+                if (want_rho) { d.r0 *= rsc; d.r1 *= rsc; d.r2 *= rsc; d.r3 *= rsc; }
                 if (d.a == 0 && d.b == 0) {
                     d.jxa0 += d.jxb0; d.jxa1 += d.jxb1; d.jya0 += d.jyb0; d.jya1 += d.jyb1;
                 }
@@ -1114,21 +1116,17 @@ __device__ __forceinline__ void push_tile(const PushParams &P, int t)
             const int wj = k / W, wi = k - wj * W;
             const int gx = wrap_index(x0 + wx0 + wi, nx), gy = wrap_index(y0 + wy0 + wj, ny);
             const size_t g = (size_t)gy * nx + gx;
-            if constexpr (SPLIT) {
-                // integer sums into the fixed-point planes: exact, any order
-                const long long *jb = reinterpret_cast<const long long *>(jw);
-                long long v;
-                if ((v = jb[k]) != 0) gfix(reinterpret_cast<long long *>(gjx) + g, v);
-                if ((v = jb[WH + k]) != 0) gfix(reinterpret_cast<long long *>(gjy) + g, v);
-                if ((v = jb[2 * WH + k]) != 0) gfix(reinterpret_cast<long long *>(gjz) + g, v);
-                if (want_rho && (v = jb[3 * WH + k]) != 0)
-                    gfix(reinterpret_cast<long long *>(grho) + g, v);
-            } else {
-                double v;
-                if ((v = jw[k]) != 0.0) gadd(gjx + g, (R)v);
-                if ((v = jw[WH + k]) != 0.0) gadd(gjy + g, (R)v);
-                if ((v = jw[2 * WH + k]) != 0.0) gadd(gjz + g, (R)v);
-                if (want_rho && (v = jw[3 * WH + k]) != 0.0) gadd(grho + g, (R)v);
+            // integer sums into the fixed-point planes: exact, any order
+            const long long *jb = reinterpret_cast<const long long *>(jw);
+            R *const planes[4] = {gjx, gjy, gjz, grho};
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                if (q == 3 && !want_rho) break;
+                long long *gp = reinterpret_cast<long long *>(planes[q]) + g;
+                const long long v = jb[q * WH + k];
+                if (v == 0) continue;
+                if constexpr (SPLIT) gfix(gp, v);
+                else fx2_add(gp, P.jlo, v);
             }
         }
     }
@@ -1215,10 +1213,11 @@ __global__ void deposit_batch_kernel(PushParams P, int64_t n, const int32_t *i1,
     if (k >= n) return;
     const int nx = P.nx, ny = P.ny;
     // qx = q*dx/dt (:1237): sp[0].qx holds dx/dt; q is +-1 (checked on the
-    // host), so q*(dx/dt) equals (q*dx)/dt exactly.  Split form: the
-    // contributions pre-scaled to the fixed-point planes (powers of two).
-    const R sx = SPLIT ? (R)P.jsx : (R)1, sy = SPLIT ? (R)P.jsy : (R)1,
-            sz = SPLIT ? (R)P.jsz : (R)1, sr = SPLIT ? (R)P.jsr / (R)P.jsz : (R)1;
+    // host), so q*(dx/dt) equals (q*dx)/dt exactly.  The contributions are
+    // pre-scaled to the fixed-point planes (powers of two).
+    const R fxs = SPLIT ? (R)1 : (R)(1 << FX2_LOBITS);
+    const R sx = (R)P.jsx * fxs, sy = (R)P.jsy * fxs, sz = (R)P.jsz * fxs,
+            sr = (R)P.jsr / (R)P.jsz;
     const R qr = (R)q[k] * sz;
     const R qxx = (R)((double)q[k] * P.sp[0].qx) * sx, qyy = (R)((double)q[k] * P.sp[0].qy) * sy;
     Dep<R> d;
@@ -1226,15 +1225,16 @@ __global__ void deposit_batch_kernel(PushParams P, int64_t n, const int32_t *i1,
     deposit_terms<R, SPLIT>(i1[k], j1[k], a, b, (R)dx1[k], (R)dx2[k], (R)dy1[k],
                             (R)dy2[k], qxx, qyy, qr, (R)vz[k], P.compat,
                             P.deposit_rho != 0, d);
-    if (SPLIT) { d.r0 *= sr; d.r1 *= sr; d.r2 *= sr; d.r3 *= sr; }
+    d.r0 *= sr; d.r1 *= sr; d.r2 *= sr; d.r3 *= sr;
     if ((P.compat & PBPIC_COMPAT_JY_DROP) && i2[k] == nx + 1) d.jyb1 = (R)0;
     const int cx = i1[k] - 1, cy = j1[k] - 1;
     auto at = [&](int ddx, int ddy) -> size_t {
         return (size_t)wrap_index(cy + ddy, ny) * nx + wrap_index(cx + ddx, nx);
     };
     auto put = [&](void *plane, size_t g, R v) {
-        if constexpr (SPLIT) gfixf(reinterpret_cast<long long *>(plane) + g, v);
-        else gadd((double *)plane + g, (double)v);
+        long long *p = reinterpret_cast<long long *>(plane) + g;
+        if constexpr (SPLIT) gfixf(p, v);
+        else fx2_add(p, P.jlo, fx2_quant((double)v));
     };
     put(P.jx, at(0, 0), d.jxa0);
     put(P.jx, at(0, 1), d.jxa1);

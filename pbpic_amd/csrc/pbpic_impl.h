@@ -75,6 +75,28 @@ __device__ __forceinline__ int wrap_index(int g, int n)
     return g;
 }
 
+// FP64 form's fixed point.  A contribution x, times its plane's power-of-two
+// scale and 2^FX2_LOBITS (|x| < 2^49), becomes the integer rint(x): x + 1.5
+// 2^52 lies in [2^52, 2^53), where doubles are the integers, so the addition
+// rounds x to an integer (to nearest even) and the difference of the bit
+// patterns is that integer.  Sums of such integers are exact in any order.  A
+// global plane holds its sum Q in two 64-bit words, floor(Q / 2^FX2_LOBITS)
+// and Q mod 2^FX2_LOBITS (the second plane lo entries on), exact up to 2^33
+// contributions a node; the value is (hi + lo 2^-FX2_LOBITS) / scale.
+constexpr int FX2_LOBITS = 19;
+constexpr double FX2_LO = 1.0 / (double)(1 << FX2_LOBITS);   // exact
+__device__ __forceinline__ long long fx2_quant(double x)
+{
+    constexpr double M = 6755399441055744.0;    // 1.5 * 2^52
+    return __double_as_longlong(x + M) - __double_as_longlong(M);
+}
+__device__ __forceinline__ void fx2_add(long long *hi, int64_t lo, long long q)
+{
+    const long long h = q >> FX2_LOBITS, l = q & ((1LL << FX2_LOBITS) - 1);
+    if (h) atomicAdd(reinterpret_cast<unsigned long long *>(hi), (unsigned long long)h);
+    if (l) atomicAdd(reinterpret_cast<unsigned long long *>(hi + lo), (unsigned long long)l);
+}
+
 // Workgroups are dispatched round-robin over the 8 XCDs.  Block b of nb is
 // given work item xcd_remap(b, nb) so that each XCD takes a contiguous range
 // (a bijection of [0, nb)): neighbouring items, which share field halos or
@@ -130,9 +152,9 @@ struct SpeciesDev {
     double ms;                 // mass: mime for q > 0, 1 otherwise (:964-968)
     double c0, c1;             // q*dt*mime/ms/2 and c0/cva (:969-970)
     double qx, qy;             // q*dx/dt, q*dy/dt (:1237-1238)
-    // Split form: the LDS current window holds 64-bit fixed-point sums of the
-    // FP32 contributions times these powers of two (jx, jy, jz, rho), each
-    // chosen so that one contribution stays below 2^30 in magnitude.
+    // The LDS current window holds 64-bit fixed-point sums of the
+    // contributions times these powers of two (jx, jy, jz, rho), each chosen
+    // so that one contribution stays below 2^30 in magnitude.
     float jsx, jsy, jsz, jsr;
     // Per push tile: bounding box of the tile's cells as the last push left
     // them, {x0, x extent, y0, y extent} with x0 in [0, nx), y0 in [0, ny)
@@ -168,12 +190,15 @@ struct PushParams {
     int nspecies;
     double dt, cva, idx, idy, dtcva;   // dtcva = dt*cva (:1003)
     const double *ex, *ey, *ez, *bx, *by, *bz;
-    // unnormalized current: FP64 form, double planes; split form, 64-bit
-    // fixed-point planes (the sum of every contribution times the plane's
-    // scale jsx..jsr, each rounded to an integer: exact and independent of
-    // the order of the additions), converted by launch_finalize_current
+    // unnormalized current as 64-bit fixed-point planes: the sum of every
+    // contribution times the plane's scale jsx..jsr, rounded to an integer
+    // (split form), or times 2^FX2_LOBITS as well, rounded, and held in two
+    // words (FP64 form, fx2_add; the low-word plane jlo entries after the
+    // high one).  Integer sums are exact and independent of the order of the
+    // additions; launch_finalize_current converts them.
     void *jx, *jy, *jz, *rho;
-    float jsx, jsy, jsz, jsr;  // split form: fixed-point scales (as SpeciesDev's)
+    int64_t jlo;               // FP64 form: ncell; split form: 0
+    float jsx, jsy, jsz, jsr;  // fixed-point scales (as SpeciesDev's)
     void *nf;                  // node-field array (nodefield_bytes), push precision
     double *acc;
     double *part;              // partial sums of the fixed-order reductions
@@ -292,8 +317,9 @@ hipError_t launch_deposit_batch(const PushParams &p, int precision, int64_t n,
                                 hipStream_t st);
 hipError_t launch_fdtd(const FdtdParams &p, int precision, hipStream_t st);
 hipError_t launch_energy(const FdtdParams &p, hipStream_t st);
-// rho_scale > 0: the charge deposit as 64-bit fixed point (sums exact in any
-// order) with that scale; 0: FP64 atomics.
+// The charge deposit as 64-bit fixed point (sums exact in any order): split
+// form, one word per node with scale rho_scale; FP64 form, two words per node
+// (rho_scratch holds 2 ncell), scale 2^30 (rho_scale is not used).
 hipError_t launch_gauss(const PushParams &p, int precision, double rho_scale, double *rho_scratch,
                         double *gauss0, int first, double rho_bg,
                         double cva2in0, hipStream_t st);
@@ -310,10 +336,9 @@ struct CopyList {
     void add(void *d, const void *s, size_t b) { src[n] = s; dst[n] = d; bytes[n] = b; n++; }
 };
 hipError_t launch_copy_list(CopyList L, hipStream_t st);
-// Conducting y walls: folds the Jz deposited in the guard rows back into the
-// domain (image sign -1) and clears them; sets the guard rows of the field
-// planes selected by mask (bit k: Ex, Ey, Ez, Bx, By, Bz) to their images.
-hipError_t launch_wall_fold(void *jz, int nx, int ny, int nyp, int elem_bytes, hipStream_t st);
+// Conducting y walls: sets the guard rows of the field planes selected by
+// mask (bit k: Ex, Ey, Ez, Bx, By, Bz) to their images (the Jz of the guard
+// rows is folded back by launch_finalize_current).
 hipError_t launch_wall_images(const FdtdParams &p, int mask, hipStream_t st);
 // Harris-sheet loader (k_field.hip); see pbpic_init_harris.
 struct HarrisDev {
@@ -325,11 +350,13 @@ struct HarrisDev {
 };
 hipError_t launch_init_harris(const SpeciesDev &ion, const SpeciesDev &ele, int precision,
                               int ny, const HarrisDev &h, hipStream_t st);
-// Split form: J (float planes) = fixed-point sums / scale, for the 3 current
-// planes and the charge plane too when with_rho.
+// J = fixed-point sums / scale, for the 3 current planes and the charge
+// plane too when with_rho: float planes from one word (elem_bytes 4, split
+// form), double planes from two (elem_bytes 8, FP64 form; low words ncell
+// entries after the high ones).
 hipError_t launch_finalize_current(void *const *j64, void *const *j, int64_t ncell,
                                    const float *scale, int with_rho, int nx, int nyp,
-                                   hipStream_t st);
+                                   int elem_bytes, hipStream_t st);
 hipError_t launch_zero_current(void *jx, void *jy, void *jz, void *rho,
                                int64_t ncell, int elem_bytes, double *acc,
                                int reset_ke, hipStream_t st);

@@ -1,9 +1,10 @@
 """pbpic_deposit_noalloc: the deposit entry point that allocates no device
 memory (its inputs go through a fixed per-GPU scratch array) against
 pbpic_deposit, through the C ABI, on two calls in a row with the second
-smaller, in both precision forms.  Bar: the global current sums of both
-entry points are unordered atomics, so J agrees norm-wise to rounding (FP64
-1e-13, split FP32 atomics 1e-6); rho likewise."""
+smaller, in both precision forms.  Bar: both entry points add every
+contribution as a 64-bit fixed-point integer (one word in the split form,
+two in the FP64 form), exact in any order of the atomics, so J and rho are
+bit-identical."""
 import numpy as np
 import pytest
 
@@ -37,7 +38,6 @@ def test_deposit_noalloc_matches_deposit(gpu, precision):
     print(f"deposit_noalloc precision={precision}", flush=True)
     nx, ny = 64, 48
     rng = np.random.default_rng(11 + precision)
-    tol = 1e-13 if precision == 0 else 1e-6
     ref = pbpic_amd.Context(nx, ny, 0.1, 0.12, 0.003, 15.0, 25.0, 0.05, precision=precision)
     new = pbpic_amd.Context(nx, ny, 0.1, 0.12, 0.003, 15.0, 25.0, 0.05, precision=precision)
     for n in (pbpic_amd.DEPOSIT_NOALLOC_MAX, 1237):
@@ -47,8 +47,9 @@ def test_deposit_noalloc_matches_deposit(gpu, precision):
         ref.currentdeposit(*args)
         new.currentdeposit(*args, noalloc=True)
         for f in ("jx", "jy", "jz", "rho"):
-            err = _rel(new.get_field(f), ref.get_field(f))
-            assert err < tol, (n, f, err)
+            a, b = new.get_field(f), ref.get_field(f)
+            assert np.array_equal(a, b), (n, f, _rel(a, b))
+            assert np.abs(b).max() > 0, (n, f)
     ref.close()
     new.close()
 
