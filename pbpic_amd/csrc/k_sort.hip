@@ -554,39 +554,6 @@ __device__ __forceinline__ int box_bin(int c, int4 b, int nx, int ny, const Fast
     return rx * (b.w + 1) + ry;
 }
 
-// One push tile per workgroup: the count of every box cell (an LDS
-// histogram), its sort key and (tile, cell) as the tile's records at toff[t].
-__global__ __launch_bounds__(256) void tile_hist_kernel(const int32_t *cell, int64_t n, int nx,
-                                                        int ny, CellKey key, const int32_t *tbox,
-                                                        const uint32_t *toff, uint32_t *thist,
-                                                        uint32_t *rkey, uint32_t *rinfo, int *ctl)
-{
-    extern __shared__ uint32_t h[];
-    const int64_t t = blockIdx.x;
-    const int4 b = *reinterpret_cast<const int4 *>(tbox + 4 * t);
-    const int H = b.w + 1, A = (b.y + 1) * H;
-    for (int k = threadIdx.x; k < A; k += 256) h[k] = 0;
-    __syncthreads();
-    for (int q = threadIdx.x; q < PUSH_TILE; q += 256) {
-        const int64_t i = t * PUSH_TILE + q;
-        if (i >= n) break;
-        const int c = cell[i];
-        const int bin = (unsigned)c < (unsigned)(nx * ny) ? box_bin(c, b, nx, ny, key.dny) : -1;
-        if (bin < 0) { atomicOr(ctl, BXF_OUTSIDE); continue; }
-        atomicAdd(&h[bin], 1u);
-    }
-    __syncthreads();
-    const uint32_t o = toff[t];
-    for (int k = threadIdx.x; k < A; k += 256) {
-        int gx = b.x + k / H, gy = b.z + k % H;
-        if (gx >= nx) gx -= nx;
-        if (gy >= ny) gy -= ny;
-        thist[o + k] = h[k];
-        rkey[o + k] = key((uint32_t)(gx * ny + gy));
-        rinfo[o + k] = (uint32_t)(t << 12) | (uint32_t)k;       // tile, box cell (< 2^12)
-    }
-}
-
 // Digit histograms of all passes over the record keys.
 __global__ __launch_bounds__(256) void rec_digit_hist_kernel(const uint32_t *k, int64_t n,
                                                              int npass, uint32_t *dhist)
@@ -604,18 +571,22 @@ __global__ __launch_bounds__(256) void rec_digit_hist_kernel(const uint32_t *k, 
         if (h[q >> 8][q & 255]) atomicAdd(dhist + q, h[q >> 8][q & 255]);
 }
 
-// One push tile per workgroup: the tile's particles in order of box cell,
-// ties in slot order (a stable in-tile counting sort).  Each wave counts its
-// quarter of the tile per box cell; the counts become per-wave bases (the
-// cell's offset in the tile, an exclusive scan over the box, plus the earlier
-// waves' counts); each wave then walks its quarter again in slot order,
-// ranks equal box cells within a round by ballots and writes the slot (in the
-// tile) at its position, local[t*PUSH_TILE + pos].  loff receives each box
-// cell's offset in the tile, at the cell's record.
+// One push tile per workgroup: the tile's records and its particles in order
+// of box cell, ties in slot order (a stable in-tile counting sort).  Each wave
+// counts its quarter of the tile per box cell (an LDS histogram); the sums
+// are the records at toff[t] (count, sort key, tile and box cell), and the
+// counts become per-wave bases (the cell's offset in the tile, an exclusive
+// scan over the box, plus the earlier waves' counts); each wave then walks
+// its quarter again in slot order, ranks equal box cells within a round by
+// ballots and writes the slot (in the tile) at its position,
+// local[t*PUSH_TILE + pos].  loff receives each box cell's offset in the
+// tile, at the cell's record.  A particle outside its tile's box sets
+// BXF_OUTSIDE in ctl (the caller then takes the radix path).
 __global__ __launch_bounds__(256) void tile_order_kernel(const int32_t *cell, int64_t n, int nx,
-                                                         int ny, FastDiv dny, const int32_t *tbox,
-                                                         const uint32_t *toff, uint32_t *loff,
-                                                         uint16_t *local, int *err)
+                                                         int ny, CellKey key, const int32_t *tbox,
+                                                         const uint32_t *toff, uint32_t *thist,
+                                                         uint32_t *rkey, uint32_t *rinfo,
+                                                         uint32_t *loff, uint16_t *local, int *ctl)
 {
     extern __shared__ uint32_t cnt[];    // [4][A]
     __shared__ uint32_t part[256];
@@ -627,13 +598,23 @@ __global__ __launch_bounds__(256) void tile_order_kernel(const int32_t *cell, in
     __syncthreads();
     const int q0 = wave * (PUSH_TILE / 4);
     uint32_t *mine = cnt + wave * A;
+    const FastDiv &dny = key.dny;
+    // the tile's cells, kept in registers for the ranking pass
+    int bins[PUSH_TILE / 256];
+    bool outside = false;
+#pragma unroll
     for (int r = 0; r < PUSH_TILE / 256; r++) {
         const int64_t i = t * PUSH_TILE + q0 + r * 64 + lane;
+        bins[r] = -1;
         if (i < n) {
-            const int bin = box_bin(cell[i], b, nx, ny, dny);
+            const int c = cell[i];
+            const int bin = (unsigned)c < (unsigned)(nx * ny) ? box_bin(c, b, nx, ny, dny) : -1;
+            bins[r] = bin;
             if (bin >= 0) atomicAdd(&mine[bin], 1u);
+            else outside = true;
         }
     }
+    if (outside) atomicOr(ctl, BXF_OUTSIDE);
     __syncthreads();
     // in-tile offset of every box cell: each thread scans a contiguous chunk
     const int per = (A + 255) / 256, k0 = tid * per, k1 = min(A, k0 + per);
@@ -656,8 +637,15 @@ __global__ __launch_bounds__(256) void tile_order_kernel(const int32_t *cell, in
     __syncthreads();
     uint32_t run = part[tid];
     const uint32_t o = toff[t];
+    const int H = b.w + 1;
     for (int k = k0; k < k1; k++) {
         const uint32_t c0 = cnt[k], c1 = cnt[A + k], c2 = cnt[2 * A + k], c3 = cnt[3 * A + k];
+        int gx = b.x + k / H, gy = b.z + k % H;
+        if (gx >= nx) gx -= nx;
+        if (gy >= ny) gy -= ny;
+        thist[o + k] = c0 + c1 + c2 + c3;
+        rkey[o + k] = key((uint32_t)(gx * ny + gy));
+        rinfo[o + k] = (uint32_t)(t << 12) | (uint32_t)k;       // tile, box cell (< 2^12)
         loff[o + k] = run;
         cnt[k] = run;
         cnt[A + k] = run + c0;
@@ -667,19 +655,17 @@ __global__ __launch_bounds__(256) void tile_order_kernel(const int32_t *cell, in
     }
     __syncthreads();
     const uint64_t lt = (1ull << lane) - 1;
+    const int nbits = A > 1 ? 32 - __clz(A - 1) : 0;     // bins < 2^nbits <= BX_MAX = 2^12
+#pragma unroll
     for (int r = 0; r < PUSH_TILE / 256; r++) {
         const int q = q0 + r * 64 + lane;
-        const int64_t i = t * PUSH_TILE + q;
-        int bin = i < n ? box_bin(cell[i], b, nx, ny, dny) : 0;
-        // every cell lies in its box (tile_hist_kernel checked it)
-        const bool valid = i < n && bin >= 0;
-        if (bin < 0) {
-            atomicOr(err, ERRF_STATE);
-            bin = 0;
-        }
+        // a particle outside the box is left out (the caller falls back)
+        const bool valid = bins[r] >= 0;
+        const int bin = valid ? bins[r] : 0;
         uint64_t m = __ballot(valid);
 #pragma unroll
-        for (int j = 0; j < 12; j++) {               // bins < BX_MAX = 2^12
+        for (int j = 0; j < 12; j++) {
+            if (j >= nbits) break;                       // block-uniform
             const uint64_t bb = __ballot((bin >> j) & 1);
             m &= ((bin >> j) & 1) ? bb : ~bb;
         }
@@ -830,9 +816,17 @@ static hipError_t box_records(SortScratch &s, const SpeciesDev &sp, const CellKe
     uint32_t *rinfo = s.rbuf + 3 * s.rcap, *loff = s.rbuf + 4 * s.rcap;
     uint32_t *kb[2] = {s.rbuf + 5 * s.rcap, s.rbuf + 6 * s.rcap};
     uint32_t *vb[2] = {s.rbuf + 7 * s.rcap, s.rbuf + 8 * s.rcap};
-    hipLaunchKernelGGL(tile_hist_kernel, dim3((unsigned)nt), dim3(256), (size_t)amax * 4, st,
+    // the tiles' records, and their particles by box cell (u16 slots, in the
+    // first key buffer)
+    uint16_t *local = reinterpret_cast<uint16_t *>(s.keys[0]);
+    // up to 64 KiB of per-wave counts (BX_MAX cells x 4 waves x 4 bytes) beside
+    // the static scan array: above the default dynamic LDS limit for the
+    // largest boxes
+    (void)hipFuncSetAttribute((const void *)tile_order_kernel,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, amax * 16);
+    hipLaunchKernelGGL(tile_order_kernel, dim3((unsigned)nt), dim3(256), (size_t)amax * 16, st,
                        sp.cell, n, nx, ny, key, (const int32_t *)sp.tbox, s.toff, thist, rkey, rinfo,
-                       s.ctl);
+                       loff, local, s.ctl);
     if ((e = hipMemcpyAsync(ctl, s.ctl, sizeof(int), hipMemcpyDeviceToHost, st)) != hipSuccess) return e;
     if ((e = hipStreamSynchronize(st)) != hipSuccess) return e;
     if (ctl[0]) return hipSuccess;
@@ -875,18 +869,6 @@ static hipError_t box_records(SortScratch &s, const SpeciesDev &sp, const CellKe
     if ((e = exclusive_scan(LdGather{thist, vin}, StArr{sbase}, R, s.bsum, nullptr, st)) !=
         hipSuccess)
         return e;
-    // the tiles' particles by box cell (u16 slots, in the first key buffer)
-    uint16_t *local = reinterpret_cast<uint16_t *>(s.keys[0]);
-    FastDiv dny;
-    dny.init((uint32_t)ny);
-    // up to 64 KiB of per-wave counts (BX_MAX cells x 4 waves x 4 bytes) beside
-    // the static scan array: above the default dynamic LDS limit for the
-    // largest boxes
-    (void)hipFuncSetAttribute((const void *)tile_order_kernel,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, amax * 16);
-    hipLaunchKernelGGL(tile_order_kernel, dim3((unsigned)nt), dim3(256), (size_t)amax * 16, st,
-                       sp.cell, n, nx, ny, dny, (const int32_t *)sp.tbox, s.toff, loff, local, err);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
     *out = BoxRecs{vin, sbase, thist, rinfo, loff, local, R};
     *ok = true;
     return hipSuccess;
