@@ -105,15 +105,15 @@ def run_form(precision, cfg, steps, warmup, sort_every, seed, push_order=True, c
 # sort cycle: rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (factors from a
 # 4 GiB copy), from the commands in
 # profiles/push_kernel_traffic_two_stream_2048x2048_ppc100.md.
-TRAFFIC_2048_PPC100 = {1: 10.267e9 + 10.576e9, 0: 18.699e9 + 18.924e9}
+TRAFFIC_2048_PPC100 = {1: 10.266e9 + 10.576e9, 0: 18.721e9 + 19.391e9}
 TRAFFIC_SOURCE = "profiles/push_kernel_traffic_two_stream_2048x2048_ppc100.md"
 
 
 def roofline(precision, res, cfg):
     """Push kernel: algorithmic bytes per launch = particles x (read + write of
     one particle record) + cells x (6 staggered field planes read, FP64, + 3
-    current planes flushed: FP64 sums, or the split form's 64-bit fixed-point
-    sums)."""
+    current planes flushed as 64-bit fixed-point sums; the FP64 form's second
+    word per node is left out, as overhead of its exact summation)."""
     sp = 44 if precision == 0 else 24          # i32 cell + 5 x f64 | 5 x f32
     wj = 8
     ncell = cfg["nx"] * cfg["ny"]
