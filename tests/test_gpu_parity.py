@@ -298,6 +298,35 @@ def _by_pid(sp):
     return {k: v[order] for k, v in sp.items()}
 
 
+@pytest.mark.parametrize("precision", [0, 1])
+def test_push_adds_to_uploaded_current(gpu, precision):
+    """A push adds its deposit to the current as it stands: current uploaded
+    with set_field, then one push without zero_current, gives the upload plus
+    the deposit the oracle's push makes from zero.  The upload goes into the
+    fixed-point sums the push adds to (two words a node in the FP64 form).
+    (The library holds the current unnormalized and applies in0 at its
+    boundary, so a push adds in0 times its raw deposit.  The reference's
+    pusher multiplies the whole planes by in0 after its loop, :1065-1069,
+    and its time loop always starts it from zero planes, :173-176.)"""
+    _say(f"push onto uploaded current precision={precision}")
+    case = make_case(nx=40, ny=36, ppc=16, seed=41, efield=0.05, bfield=0.5)
+    ctx = gpu_for(case, precision=precision, compat=0)
+    o = oracle_for(case, compat=0)
+    o.zero_current()
+    o.pusher()
+    rng = np.random.default_rng(8)
+    up = {}
+    for f in ("jx", "jy", "jz", "rho"):
+        up[f] = np.abs(o.get_field(f)).max() * rng.standard_normal((36, 40))
+        ctx.set_field(f, up[f])
+    ctx.pusher()
+    tol = FP64_TOL if precision == 0 else 1e-5
+    for f in ("jx", "jy", "jz", "rho"):
+        g, dep = ctx.get_field(f), o.get_field(f)
+        assert rel_norm(g, up[f] + dep) < tol, f
+        assert rel_norm(g - up[f], dep) < 10 * tol, f
+
+
 @pytest.mark.parametrize("nx,ny", [(32, 64), (45, 37)])
 @pytest.mark.parametrize("precision,compat", [(0, 7), (0, 0), (1, 7)])
 def test_one_step_full_tiles(gpu, precision, compat, nx, ny):
