@@ -4,7 +4,7 @@ allows (the loop of pic2d.f90:171-188 with the countingsort every 50 steps,
 
   two-stream, 2048 x 2048 x 100 ppc, 500 steps, both forms (compat 0, so the
       deposit conserves charge):  the split form's total energy and its drift
-      against the FP64 form's (within 1%), the
+      against the FP64 form's (the split form's accuracy bar: within 1%), the
       discrete Poynting balance of the Yee scheme at full size, Gauss-residual
       drift at rounding, integrated J.E of the two forms;
   two-stream at 512 x 512 x 100 ppc, 60 steps (a sort at 50, and in the split
@@ -102,7 +102,7 @@ def test_two_stream_2048_full_size(gpu):
     es0, es1, ed0, ed1 = _total(hs[0]), _total(hs[-1]), _total(hd[0]), _total(hd[-1])
     drift_s, drift_d = (es1 - es0) / es0, (ed1 - ed0) / ed0
     _say(f"energy drift over 500 steps: split {drift_s:.6e}, fp64 {drift_d:.6e}")
-    # the split form's energy drift within 1% of the FP64
+    # the split form's accuracy bar: its energy drift within 1% of the FP64
     # drift (the grid heating of dx = 25 Debye lengths raises both by ~4%)
     assert abs(drift_s - drift_d) <= 0.01 * abs(drift_d)
     for a, b in zip(hs, hd):
