@@ -200,6 +200,42 @@ def test_currentdeposit_batch(gpu, compat):
         assert rel_norm(ctx.get_field(f), o.get_field(f)) < 1e-13, f
 
 
+@pytest.mark.parametrize("precision", [0, 1])
+def test_currentdeposit_hot_cell(gpu, precision):
+    """300,000 same-sign contributions into one cell and its neighbours: the
+    fixed-point sums (one word split, two words FP64) stay exact far beyond
+    the few hundred contributions a node gets in a step, and agree with the
+    oracle's sequential FP64 sums (FP64 1e-12, split 1e-5)."""
+    _say(f"deposit hot cell precision={precision}")
+    nx, ny, n = 10, 9, 300000
+    rng = np.random.default_rng(17)
+    i1 = np.full(n, 5)
+    j1 = np.full(n, 4)
+    a = rng.integers(-1, 2, n)
+    b = rng.integers(-1, 2, n)
+    dx1 = rng.random(n) - 0.5
+    dy1 = rng.random(n) - 0.5
+    dx2 = np.where(a == 0, np.clip(dx1 + 0.3 * (rng.random(n) - 0.5), -0.5, 0.5),
+                   -a * 0.5 + a * 0.4 * rng.random(n))
+    dy2 = np.where(b == 0, np.clip(dy1 + 0.3 * (rng.random(n) - 0.5), -0.5, 0.5),
+                   -b * 0.5 + b * 0.4 * rng.random(n))
+    q = np.ones(n, dtype=np.int64)
+    vz = np.abs(rng.standard_normal(n)) + 1.0
+    args = (i1, i1 + a, j1, j1 + b, dx1, dx2, dy1, dy2, q, vz)
+    import pbpic_amd
+    ctx = pbpic_amd.Context(nx, ny, 0.1, 0.12, 0.003, 15.0, 25.0, 0.05, precision=precision,
+                            compat=0)
+    ctx.currentdeposit(*args)
+    o = oracle.Oracle(nx, ny, 0.1, 0.12, 0.003, 15.0, 25.0, 0.05, compat=0)
+    o.deposit_batch(*args)
+    o.finish_current()
+    tol = 1e-12 if precision == 0 else 1e-5
+    for f in ("jx", "jy", "jz", "rho"):
+        g, r = ctx.get_field(f), o.get_field(f)
+        assert np.abs(r).max() > 1e3, f
+        assert rel_norm(g, r) < tol, f
+
+
 # ------------------------------------------------------------- fields
 def test_fieldsolver_fp64(gpu):
     """fieldsolver (pic2d.f90:678-860) with random E, B, J: agrees to rounding."""
