@@ -335,6 +335,37 @@ def _by_pid(sp):
 
 
 @pytest.mark.parametrize("precision", [0, 1])
+def test_one_step_crowded_cell(gpu, precision):
+    """Every particle of each species in one cell (15,360 per species, four
+    push tiles each): each tile's LDS window takes up to 8,192 contributions
+    on a node of a plane, the most a tile can put there, so the window sums
+    reach their largest values (below 2^62 in the FP64 form's fixed point).
+    One push against the oracle."""
+    _say(f"one step crowded cell precision={precision}")
+    case = make_case(nx=32, ny=30, ppc=16, seed=19, efield=0.05, bfield=0.5)
+    for k, sp in enumerate(case["species"]):
+        sp["grid"][:] = 30 * (11 + k) + 10          # cell (12 + k, 10), 1-based
+    ctx = gpu_for(case, precision=precision, compat=0)
+    o = oracle_for(case, compat=0)
+    ctx.zero_current(); ctx.pusher()
+    o.zero_current(); o.pusher()
+    tol = FP64_TOL if precision == 0 else 1e-5
+    for f in ("jx", "jy", "jz", "rho"):
+        r = o.get_field(f)
+        assert np.abs(r).max() > 0, f
+        assert rel_norm(ctx.get_field(f), r) < tol, f
+    assert ctx.diag()["tiles_spilled"] == 0
+    if precision == 0:
+        osp = oracle_species(o, case)
+        for k in range(2):
+            g, r = _by_pid(ctx.get_particles(k)), _by_pid(osp[k])
+            assert np.array_equal(g["grid"], r["grid"])
+            for f in ("dx", "dy", "ux", "uy", "uz"):
+                err = np.max(np.abs(g[f] - r[f]) / np.maximum(np.abs(r[f]), 1e-300))
+                assert err < FP64_TOL, (k, f, err)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
 def test_push_adds_to_uploaded_current(gpu, precision):
     """A push adds its deposit to the current as it stands: current uploaded
     with set_field, then one push without zero_current, gives the upload plus
